@@ -1,0 +1,104 @@
+"""Host-side main loop over a hydro engine, mirroring the reference driver.
+
+``run`` restates the part of ``program hormone`` (src/main/hormone.f90:61-179)
+that touches the hot path: initial state (initialcondition + conserve +
+pressure), then per step ``timestep`` -> [``gravity``] -> ``hydro_step`` ->
+``phidamp`` (when mag_on and dim>1) -> ``time += dt`` with the reference's
+end and output bookkeeping (endstyle/outstyle, t_out += dt_out).
+
+An engine is any object with the methods of :class:`hormone_amd.native.Hydro`
+(upload, download, timestep, hydro_step, phidamp, ufnmax); the HIP library
+is the product engine.
+"""
+from __future__ import annotations
+
+import math
+import time as _time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import problems as P
+from .grid import Grid
+
+F_D, F_V1, F_V2, F_V3, F_E, F_B1, F_B2, F_B3, F_PHI, F_EINT, F_P = range(11)
+F_GRVPHI, F_GRVPSI, F_U = 11, 12, 16
+
+
+def desc_fields(prm: P.Params, g: Grid) -> dict:
+    """Keyword values for the C descriptor (hg_desc)."""
+    return dict(grid=g, mag_on=int(prm.mag_on),
+                flux_limiter=P.LIMITERS[prm.flux_limiter], rktype=prm.rktype,
+                bc=prm.effective_bc(), gamma=prm.gamma, courant=prm.courant,
+                alpha9wave=prm.alpha9wave, gravswitch=prm.gravswitch,
+                grktype=prm.grktype, HGfac=prm.HGfac, hgcfl=prm.hgcfl,
+                alphagrv=prm.alphagrv, G=prm.G, maxtngrv=prm.maxtngrv)
+
+
+def load_initial_state(eng, prm: P.Params, g: Grid, st: P.State):
+    """initialcondition (energy) + conserve; the engine recomputes eint/p
+    from e, d, v, B exactly as pressure() does before the first step."""
+    eint, e = P.initial_energy(st, prm.gamma)
+    u = P.conserve(st, e, prm.mag_on, g.dim)
+    prims = {F_D: st.d, F_V1: st.v[0], F_V2: st.v[1], F_V3: st.v[2], F_E: e,
+             F_B1: st.b[0], F_B2: st.b[1], F_B3: st.b[2], F_PHI: st.phi}
+    for f, a in prims.items():
+        eng.upload(f, g.pad(a))
+    for n, a in enumerate(u):
+        eng.upload(F_U + n, g.pad(a))
+
+
+@dataclass
+class RunResult:
+    tn: int
+    time: float
+    wall_s: float
+    steps_timed: int
+    dts: list = field(default_factory=list)
+
+
+def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
+        record_dt: bool = False) -> RunResult:
+    """Main loop of hormone.f90:124-179 restricted to the hydro path."""
+    t_end, dt_out, t_out = prm.times()
+    tim = 0.0
+    tn = 0
+    damp = math.exp(-prm.alpha9wave * prm.courant)  # source.f90:253
+    dts = []
+    t0 = _time.perf_counter()
+    while True:
+        dt, ch = eng.timestep(tim, t_out, t_end, prm.outstyle, prm.endstyle,
+                              prm.fixed_dt)
+        if record_dt:
+            dts.append(dt)
+        eng.hydro_step(dt, ch)
+        if prm.mag_on and g.dim > 1:
+            eng.phidamp(damp)
+        tim = tim + dt
+        tn += 1
+        if prm.endstyle == 1:
+            if tim >= t_end:
+                break
+        elif prm.endstyle == 2:
+            if tn >= prm.tnlim:
+                break
+        if prm.outstyle == 1:
+            if tim >= t_out:
+                t_out = t_out + dt_out
+        if max_steps is not None and tn >= max_steps:
+            break
+    wall = _time.perf_counter() - t0
+    return RunResult(tn, tim, wall, tn, dts)
+
+
+def fetch(eng, g: Grid, fields) -> dict:
+    out = {}
+    for f in fields:
+        a = np.zeros(g.padded_shape)
+        eng.download(f, a)
+        out[f] = Grid.interior(a).copy()
+    return out
+
+
+SNAP_FIELDS = (F_D, F_V1, F_V2, F_V3, F_E)
+SNAP_FIELDS_MAG = SNAP_FIELDS + (F_B1, F_B2, F_B3, F_PHI)

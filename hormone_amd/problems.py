@@ -1,0 +1,239 @@
+"""Problem set-ups: parameters and initial conditions.
+
+Parameters mirror the reference's namelist files (para/parameters_base plus
+para/parameters_<simtype>); initial conditions restate src/init/*.f90 and
+the energy initialisation of src/main/initialcondition.f90:133-150.  The
+time unit conversion follows checksetup.f90:181-208.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from .grid import Grid, cartesian_grid
+
+LIMITERS = {"flat": 0, "minmod": 1, "modified_mc": 2}
+UNIT_SEC = {"yr": 3600.0 * 24.0 * 365.25, "d": 3600 * 24.0, "hr": 3600.0,
+            "ks": 1000.0, "min": 60.0, "s": 1.0, "ms": 1e-3, "ns": 1e-9}
+
+
+@dataclass
+class Params:
+    """Settings read by the hot path (defaults: para/parameters_base)."""
+    n: tuple = (10, 1, 1)
+    lo: tuple = (0.0, 0.0, 0.0)
+    hi: tuple = (1.0, 1.0, 1.0)
+    outstyle: int = 1
+    endstyle: int = 1
+    tnlim: int = 0
+    t_end: float = 0.0       # in dt_unit
+    dt_out: float = 1.0      # in dt_unit
+    dt_unit: str = "s"
+    gamma: float = 1.6666666666666667
+    courant: float = 0.9
+    rktype: int = 3
+    mag_on: bool = False
+    flux_limiter: str = "modified_mc"
+    alpha9wave: float = 0.1
+    fixed_dt: float = -1.0
+    # bc1is bc1os bc2is bc2os bc3is bc3os bc1iv bc1ov bc2iv bc2ov bc3iv bc3ov
+    bc: tuple = (0,) * 12
+    gravswitch: int = 0
+    HGfac: float = 5.0
+    hgcfl: float = 0.9
+    maxtngrv: int = -1
+    grktype: int = 2
+    alphagrv: float = 0.1
+    G: float = 6.67430e-8
+    is_test: bool = True
+
+    @property
+    def solve(self):
+        return tuple(m > 1 for m in self.n)
+
+    def effective_bc(self):
+        """checksetup.f90:126-136: forced codes for unsolved directions."""
+        bc = list(self.bc)
+        s = self.solve
+        if not s[0]:
+            for q in (0, 1, 6, 7):
+                bc[q] = 2
+        if not s[1]:
+            for q in (2, 3, 8, 9):
+                bc[q] = 3
+        if not s[2]:
+            for q in (4, 5, 10, 11):
+                bc[q] = 3
+        return tuple(bc)
+
+    def times(self):
+        """(t_end, dt_out, t_out) in seconds, checksetup.f90:201-208."""
+        u = UNIT_SEC[self.dt_unit]
+        dt_out = self.dt_out * u
+        t_end = self.t_end * u
+        t_out = min(dt_out, t_end)
+        if self.is_test:
+            t_out = t_end
+        return t_end, dt_out, t_out
+
+
+# --- parameter files of the reference's Cartesian test problems ----------
+# para/parameters_sodshock_{x,y,z}, _briowushock_{x,y,z}, _orszagtang_{..}
+
+def _axis_n(direction: str, n: int):
+    return tuple(n if c in direction else 1 for c in "xyz")
+
+
+def _bc_for(direction: str, code: int):
+    bc = [0] * 12
+    for d, c in enumerate("xyz"):
+        if c in direction:
+            for q in (2 * d, 2 * d + 1, 6 + 2 * d, 7 + 2 * d):
+                bc[q] = code
+    return tuple(bc)
+
+
+def sodshock_params(direction: str, n: int = 500, **kw) -> Params:
+    p = Params(n=_axis_n(direction, n), t_end=200.0, dt_out=200.0, dt_unit="ms",
+               gamma=1.4, bc=_bc_for(direction, 3), tnlim=100000)
+    return replace(p, **kw)
+
+
+def briowu_params(direction: str, n: int = 500, **kw) -> Params:
+    p = Params(n=_axis_n(direction, n), t_end=200.0, dt_out=200.0, dt_unit="ms",
+               gamma=2.0, mag_on=True, bc=_bc_for(direction, 3), tnlim=100000)
+    return replace(p, **kw)
+
+
+def orszagtang_params(plane: str, n: int = 128, **kw) -> Params:
+    p = Params(n=_axis_n(plane, n), t_end=500.0, dt_out=500.0, dt_unit="ms",
+               mag_on=True, bc=(0,) * 12, tnlim=100000)
+    return replace(p, **kw)
+
+
+# --- initial conditions ----------------------------------------------------
+
+@dataclass
+class State:
+    """Interior primitive fields in C order (nz, ny, nx)."""
+    d: np.ndarray
+    p: np.ndarray
+    v: list
+    b: list
+    phi: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+
+def _zeros(g: Grid):
+    return np.zeros(g.shape)
+
+
+def shocktube(g: Grid, direction: str, kind: str) -> State:
+    """src/init/shocktube.f90:14-192 for 'sodshock' and 'briowushock'."""
+    dirn = {"x": 1, "y": 2, "z": 3}[direction]
+    dl, dr, pl, pr = 1.0, 0.125, 1.0, 0.1
+    v1l = v1r = v2l = v2r = v3l = v3r = 0.0
+    b1l = b1r = b2l = b2r = b3l = b3r = 0.0
+    if kind == "briowushock":
+        if dirn == 1:
+            b1l, b1r, b2l, b2r, b3l, b3r = 0.75, 0.75, 1.0, -1.0, 0.0, 0.0
+        elif dirn == 2:
+            b1l, b1r, b2l, b2r, b3l, b3r = 0.0, 0.0, 0.75, 0.75, 1.0, -1.0
+        else:
+            b1l, b1r, b2l, b2r, b3l, b3r = 1.0, -1.0, 0.0, 0.0, 0.75, 0.75
+    ax = g.ax[dirn - 1]
+    idx = np.arange(ax.s, ax.e + 1)
+    left = idx <= (ax.s + ax.e) // 2
+    shape = [1, 1, 1]
+    shape[3 - dirn] = ax.n
+    left = left.reshape(shape)
+
+    def fill(a, b_):
+        return np.broadcast_to(np.where(left, a, b_), g.shape).astype(float).copy()
+
+    return State(d=fill(dl, dr), p=fill(pl, pr),
+                 v=[fill(v1l, v1r), fill(v2l, v2r), fill(v3l, v3r)],
+                 b=[fill(b1l, b1r), fill(b2l, b2r), fill(b3l, b3r)],
+                 phi=_zeros(g))
+
+
+def orszagtang(g: Grid, gamma: float) -> State:
+    """src/init/orszagtang.f90:14-83 (xy, xz or yz plane)."""
+    pi = math.acos(-1.0)
+    n = [a.n for a in g.ax]
+    if n[2] == 1:
+        which = 1
+    if n[1] == 1:
+        which = 2
+    if n[0] == 1:
+        which = 3
+    nz, ny, nx = g.shape
+    d = np.full(g.shape, gamma * gamma / (4.0 * pi))
+    p = np.full(g.shape, gamma / (4.0 * pi))
+    s4 = math.sqrt(4.0 * pi)
+    v = [_zeros(g), _zeros(g), _zeros(g)]
+    b = [_zeros(g), _zeros(g), _zeros(g)]
+    # (first, second) in-plane axes and the Fortran loops' coordinates
+    if which == 1:
+        ax1, ax2 = 0, 1
+    elif which == 2:
+        ax1, ax2 = 0, 2
+    else:
+        ax1, ax2 = 1, 2
+    c1 = g.centres(ax1)
+    c2 = g.centres(ax2)
+    s2a = np.array([-math.sin(2.0 * pi * t) for t in c2])   # -sin(2 pi x_2nd)
+    s1a = np.array([math.sin(2.0 * pi * t) for t in c1])    # sin(2 pi x_1st)
+    s4a = np.array([math.sin(4.0 * pi * t) / s4 for t in c1])
+    bs2 = np.array([-math.sin(2.0 * pi * t) / s4 for t in c2])
+
+    def along(arr, axis):
+        shape = [1, 1, 1]
+        shape[2 - axis] = arr.size
+        return np.broadcast_to(arr.reshape(shape), g.shape).copy()
+
+    v[ax1] = along(s2a, ax2)
+    v[ax2] = along(s1a, ax1)
+    b[ax1] = along(bs2, ax2)
+    b[ax2] = along(s4a, ax1)
+    return State(d=d, p=p, v=v, b=b, phi=_zeros(g))
+
+
+def initial_energy(st: State, gamma: float):
+    """initialcondition.f90:133-150 (eostype 0): eint = p/(gamma-1),
+    e = eint + d*0.5*|v|^2 + 0.5*|B|^2, evaluated left to right."""
+    v1, v2, v3 = st.v
+    b1, b2, b3 = st.b
+    eint = st.p / (gamma - 1.0)
+    e = (eint + (st.d * 0.5) * ((v1 * v1 + v2 * v2) + v3 * v3)) \
+        + 0.5 * ((b1 * b1 + b2 * b2) + b3 * b3)
+    return eint, e
+
+
+def conserve(st: State, e: np.ndarray, mag_on: bool, dim: int):
+    """conserve.f90:13-49: the conserved variables in equation order."""
+    u = [st.d.copy(), st.d * st.v[0], st.d * st.v[1], st.d * st.v[2], e.copy()]
+    if mag_on:
+        u += [st.b[0].copy(), st.b[1].copy(), st.b[2].copy()]
+        if dim > 1:
+            u.append(st.phi.copy())
+    return u
+
+
+def make_problem(name: str, **kw):
+    """(Params, Grid, State) for a reference test problem name."""
+    if name.startswith("sodshock_"):
+        prm = sodshock_params(name[-1], **kw)
+        g = cartesian_grid(prm.n, prm.lo, prm.hi)
+        return prm, g, shocktube(g, name[-1], "sodshock")
+    if name.startswith("briowushock_"):
+        prm = briowu_params(name[-1], **kw)
+        g = cartesian_grid(prm.n, prm.lo, prm.hi)
+        return prm, g, shocktube(g, name[-1], "briowushock")
+    if name.startswith("orszagtang_"):
+        prm = orszagtang_params(name.split("_")[1], **kw)
+        g = cartesian_grid(prm.n, prm.lo, prm.hi)
+        return prm, g, orszagtang(g, prm.gamma)
+    raise ValueError("unknown problem " + name)

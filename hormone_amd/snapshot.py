@@ -1,0 +1,52 @@
+"""Reader for the reference's binary snapshot format.
+
+Layout (src/main/output.f90:697-795, io.F90:345-363; stream access,
+little-endian, legacy int4(-1) record markers): [-1][tn i4][time f8][-1]
+[-1] d v1 v2 v3 e [-1] then optional groups; for mag_on [-1] b1 b2 b3 phi [-1].
+Each field is nx*ny*nz float64, x fastest.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def read_bin(path: str, shape_zyx, mag_on: bool, eostype: int = 0,
+             gravswitch: int = 0, compswitch: int = 0):
+    raw = open(path, "rb").read()
+    n = int(np.prod(shape_zyx))
+    off = 0
+
+    def i4():
+        nonlocal off
+        v = int(np.frombuffer(raw, "<i4", 1, off)[0])
+        off += 4
+        return v
+
+    def f8(cnt):
+        nonlocal off
+        v = np.frombuffer(raw, "<f8", cnt, off).copy()
+        off += 8 * cnt
+        return v
+
+    out = {}
+    assert i4() == -1
+    out["tn"] = i4()
+    out["time"] = float(f8(1)[0])
+    assert i4() == -1
+    assert i4() == -1
+    for nm in ("d", "v1", "v2", "v3", "e"):
+        out[nm] = f8(n).reshape(shape_zyx)
+    assert i4() == -1
+    if eostype >= 1:
+        assert i4() == -1
+        out["T"] = f8(n).reshape(shape_zyx)
+        assert i4() == -1
+    if gravswitch >= 2 or gravswitch == 3 or compswitch:
+        raise NotImplementedError("only hydro/MHD snapshots are read here")
+    if mag_on:
+        assert i4() == -1
+        for nm in ("b1", "b2", "b3", "phi"):
+            out[nm] = f8(n).reshape(shape_zyx)
+        assert i4() == -1
+    assert off == len(raw), (off, len(raw))
+    return out

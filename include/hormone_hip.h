@@ -1,0 +1,163 @@
+/*
+ * hormone_hip.h -- C ABI of the MI355X (gfx950) explicit hydro step for
+ * HORMONE's Cartesian grid.
+ *
+ * The reference has no plugin API: its hot path is four argument-less
+ * Fortran subroutines working on module globals, called from the main loop
+ * (src/main/hormone.f90:126,130,136,140):
+ *     timestep   src/main/timestep.f90:15-105
+ *     gravity    src/main/gravity.f90:18-64 -> gravity_hyperbolic.f90:19-50
+ *     hydro_step src/main/hydro.f90:15-52
+ *     phidamp    src/main/source.f90:239-260
+ * Each entry point below replaces one of them (cited per function).  A
+ * Fortran driver binds them with iso_c_binding (hormone_amd/fortran/
+ * hormone_hip_mod.f90).  All pointers are host pointers; the library owns
+ * the device copies, keeps the state resident between calls and retains no
+ * host pointer after a call returns.  Every function returns an HG_* status.
+ *
+ * Array convention: a "padded" array is one 3D field laid out exactly like
+ * the reference's allocations (src/main/allocations.f90:56-57,212): Fortran
+ * order, x fastest, index ranges is-2:ie+2, js-2:je+2, ks-2:ke+2 (ghost width
+ * 2 in every dimension, active or not).  Pass c_loc(a(is-2,js-2,ks-2)).
+ */
+#ifndef HORMONE_HIP_H
+#define HORMONE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mirror the reference's stop sites) ------------------ */
+enum {
+  HG_OK = 0,
+  HG_ERR_ARG = 1,         /* bad argument / descriptor                       */
+  HG_ERR_UNSUPPORTED = 2, /* crdnt/=0, eostype/=0, compswitch/=0, bc 4/5/9/10 */
+  HG_ERR_DEVICE = 3,      /* a HIP runtime call failed                       */
+  HG_ERR_FACE_EOS = 4,    /* numflux.f90:579-593 error_flux: rho<=0 or
+                             e_int<=0 at a reconstructed face state          */
+  HG_ERR_NEG_EINT = 5,    /* eos.f90:833-839 internalenergy: e_int<=0 in a
+                             cell after the RK update                        */
+  HG_ERR_NOMEM = 6,       /* device allocation failed                        */
+  HG_ERR_STATE = 7        /* call order violated (e.g. step before upload)   */
+};
+
+/* flux_limiter (settings: 'flat', 'minmod', 'modified_mc';
+   src/main/fluxlimiter.f90:27-35, interpolation.f90:42-45) */
+enum { HG_LIMITER_FLAT = 0, HG_LIMITER_MINMOD = 1, HG_LIMITER_MODIFIED_MC = 2 };
+
+/* precision of the arithmetic on the path */
+enum { HG_PREC_FP64 = 0, HG_PREC_FP32MIX = 1 };
+
+/* boundary codes accepted (boundarycondition.f90:30-38): 0 periodic,
+   1 reflective, 2 outgoing, 3 free */
+
+/* fields for hg_upload / hg_download.  Primitive fields are the reference's
+   physval arrays (modules.f90:106-138); HG_F_U + n (n = 0..ufnmax-1) is the
+   conserved variable u(:,:,:,n+1) in the reference's equation order
+   icnt,imo1,imo2,imo3,iene[,img1,img2,img3[,i9wv]] (checksetup.f90:62-75). */
+enum {
+  HG_F_D = 0, HG_F_V1 = 1, HG_F_V2 = 2, HG_F_V3 = 3, HG_F_E = 4,
+  HG_F_B1 = 5, HG_F_B2 = 6, HG_F_B3 = 7, HG_F_PHI = 8,
+  HG_F_EINT = 9,      /* download only: internal energy                   */
+  HG_F_P = 10,        /* download only: gas pressure (gamma-1)*eint       */
+  HG_F_GRVPHI = 11,   /* gravitational potential (gravswitch 3)           */
+  HG_F_GRVPSI = 12,   /* its time derivative variable psi                 */
+  HG_F_U = 16         /* + n: conserved variable n                        */
+};
+
+/* Everything the hot path reads from the reference's settings, grid and
+   physval modules (modules.f90:8-184), gathered into one struct. */
+typedef struct hg_desc {
+  /* interior index ranges (grid module: is,ie,js,je,ks,ke).  A dimension is
+     solved when its range holds more than one cell (checksetup.f90:41-44). */
+  int32_t is, ie, js, je, ks, ke;
+  /* Cartesian coordinate arrays of the grid module (gridset.f90:43-158),
+     each covering indices is-2..ie+2 (resp. j, k): pass c_loc(x1(is-2)). */
+  const double *x1, *xi1, *dx1, *dxi1, *idx1, *idxi1;
+  const double *x2, *xi2, *dx2, *dxi2, *idx2, *idxi2;
+  const double *x3, *xi3, *dx3, *dxi3, *idx3, *idxi3;
+  int32_t crdnt;        /* must be 0 (Cartesian)                            */
+  int32_t eostype;      /* must be 0 (ideal gas)                            */
+  int32_t compswitch;   /* must be 0 (uniform composition)                  */
+  int32_t mag_on;       /* 0/1                                              */
+  int32_t flux_limiter; /* HG_LIMITER_*                                     */
+  int32_t rktype;       /* 1, 2 or 3                                        */
+  int32_t bc[12];       /* bc1is bc1os bc2is bc2os bc3is bc3os
+                           bc1iv bc1ov bc2iv bc2ov bc3iv bc3ov              */
+  double gamma;         /* physval: gamma                                   */
+  double courant;       /* settings: courant                                */
+  double alpha9wave;    /* settings: alpha9wave                             */
+  int32_t gravswitch;   /* 0 (off) or 3 (hyperbolic self-gravity)           */
+  int32_t grktype;      /* RK order of the gravity wave update (1 or 2)     */
+  double HGfac, hgcfl, alphagrv;
+  double G;             /* gravitational constant (constants: G)            */
+  int32_t maxtngrv;     /* settings: maxtngrv (<=0: no limit)               */
+  int32_t precision;    /* HG_PREC_*                                        */
+  int32_t device;       /* HIP device ordinal                               */
+  int32_t reserved;
+} hg_desc;
+
+typedef struct hg_ctx hg_ctx;
+
+/* Library version; no reference counterpart. */
+const char *hg_version(void);
+
+/* Validate a descriptor and create a device-resident context (replaces the
+   hot-path part of allocations.f90:14-335 + metric.f90:26-51 for crdnt=0).
+   On error *out is NULL and the status says why. */
+int hg_create(const hg_desc *desc, hg_ctx **out);
+
+/* Release the context and its device memory. */
+int hg_destroy(hg_ctx *ctx);
+
+/* Number of conserved variables (checksetup.f90:62-75: ufnmax). */
+int hg_ufnmax(const hg_ctx *ctx);
+
+/* Copy the interior of one padded host field into the device state
+   (HG_F_D..HG_F_PHI, HG_F_GRVPHI, HG_F_GRVPSI, HG_F_U+n).  After primitive
+   fields change, e_int and p are recomputed on the device from e, rho, v, B
+   before the next step, exactly as pressure() does (eos.f90:742-842). */
+int hg_upload(hg_ctx *ctx, int32_t field, const double *padded);
+
+/* Copy the interior of one device field into a padded host array; ghost
+   cells of the host array are left untouched.  Blocks until done. */
+int hg_download(hg_ctx *ctx, int32_t field, double *padded);
+
+/* Replaces timestep (timestep.f90:15-105) for fixed_dt<=0: per-cell CFL
+   (dti_cell, timestep.f90:125-183), max-reduction of the fast speed and
+   min-reduction of dti, dt=courant*min(dti), clipping to t_out/t_end
+   (timestep.f90:74-85, outstyle/endstyle as in the reference), ch=cfmax,
+   and for gravswitch 3: cgrav, dtgrav (timestep.f90:92-97).
+   fixed_dt>0 returns dt=fixed_dt and leaves ch untouched, like the reference.
+   Any output pointer may be NULL. */
+int hg_timestep(hg_ctx *ctx, double time, double t_out, double t_end,
+                int32_t outstyle, int32_t endstyle, double fixed_dt,
+                double *dt, double *ch, double *cgrav, double *dtgrav);
+
+/* Replaces hydro_step (hydro.f90:15-52) for the serial Cartesian case: for
+   rungen = 1..rktype: boundary conditions (boundarycondition.f90),
+   MUSCL slopes + e_int guard (interpolation.f90), face states + HLLD/GLM flux
+   (numflux.f90, hlldflux.f90), gravity source (source.f90:14-85),
+   SSP-RK update (rungekutta.f90:16-98) and primitive recovery
+   (rungekutta.f90:193-232, eos.f90:742-842).  dt and ch are the values the
+   reference holds in its grid/physval modules when it calls hydro_step.
+   On HG_ERR_FACE_EOS / HG_ERR_NEG_EINT, hg_last_error reports where. */
+int hg_hydro_step(hg_ctx *ctx, double dt, double ch);
+
+/* Replaces phidamp (source.f90:239-260): multiplies the primitive psi (not
+   the conserved u(:,:,:,i9wv)) by factor = exp(-alpha9wave*courant), which
+   the caller evaluates on the host. */
+int hg_phidamp(hg_ctx *ctx, double factor);
+
+/* Cell (Fortran indices) and direction (1..3, 0 for a cell error) of the
+   last HG_ERR_FACE_EOS / HG_ERR_NEG_EINT. */
+int hg_last_error(const hg_ctx *ctx, int32_t *i, int32_t *j, int32_t *k,
+                  int32_t *dir);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HORMONE_HIP_H */
