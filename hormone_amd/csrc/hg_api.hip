@@ -1,0 +1,397 @@
+// hg_api.hip -- the C ABI of include/hormone_hip.h over a device context.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/hormone_hip.h"
+#include "hg_kernels.hpp"
+
+using namespace hg;
+
+struct hg_ctx {
+  hg_desc desc;
+  Dev D;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int is, js, ks;        // Fortran index of the first interior cell
+  int nxg, nyg, nzg;     // padded extents
+  std::vector<void *> allocs;
+  double *coord = nullptr;  // device coordinate block
+  Scalars *h_sc = nullptr;  // pinned host mirror of the device scalars
+  bool eint_stale = true;
+  double ch = 0.0;
+  int err_i = 0, err_j = 0, err_k = 0, err_dir = 0;
+  double *h_stage = nullptr;  // pinned staging for field transfers
+};
+
+#define HIPCHK(x)                                   \
+  do {                                              \
+    hipError_t e_ = (x);                            \
+    if (e_ != hipSuccess) return HG_ERR_DEVICE;     \
+  } while (0)
+
+static const char kVersion[] = "hormone_hip 0.1 (gfx950)";
+
+extern "C" const char *hg_version(void) { return kVersion; }
+
+static int dev_alloc(hg_ctx *c, size_t bytes, void **p) {
+  if (hipMalloc(p, bytes) != hipSuccess) return HG_ERR_NOMEM;
+  c->allocs.push_back(*p);
+  return HG_OK;
+}
+
+static int validate(const hg_desc *d) {
+  if (!d) return HG_ERR_ARG;
+  if (d->crdnt != 0 || d->eostype != 0 || d->compswitch != 0) return HG_ERR_UNSUPPORTED;
+  if (d->ie < d->is || d->je < d->js || d->ke < d->ks) return HG_ERR_ARG;
+  if (d->rktype < 1 || d->rktype > 3) return HG_ERR_ARG;
+  if (d->flux_limiter < 0 || d->flux_limiter > 2) return HG_ERR_ARG;
+  for (int b = 0; b < 12; ++b)
+    if (d->bc[b] < 0 || d->bc[b] > 3) return HG_ERR_UNSUPPORTED;
+  // checksetup.f90:82-107: periodic must pair with periodic
+  for (int q = 0; q < 6; ++q) {
+    int a = d->bc[2 * q], b = d->bc[2 * q + 1];
+    if (a * b == 0 && a + b != 0) return HG_ERR_ARG;
+  }
+  if (d->gravswitch != 0) return HG_ERR_UNSUPPORTED;
+  if (d->precision != HG_PREC_FP64) return HG_ERR_UNSUPPORTED;
+  const double *const *arr[] = {&d->x1, &d->xi1, &d->dx1, &d->dxi1, &d->idx1, &d->idxi1,
+                                &d->x2, &d->xi2, &d->dx2, &d->dxi2, &d->idx2, &d->idxi2,
+                                &d->x3, &d->xi3, &d->dx3, &d->dxi3, &d->idx3, &d->idxi3};
+  for (auto a : arr)
+    if (!*a) return HG_ERR_ARG;
+  return HG_OK;
+}
+
+extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
+  if (!out) return HG_ERR_ARG;
+  *out = nullptr;
+  int st = validate(desc);
+  if (st) return st;
+  hg_ctx *c = new (std::nothrow) hg_ctx();
+  if (!c) return HG_ERR_NOMEM;
+  c->desc = *desc;
+  c->device = desc->device;
+  auto fail = [&](int code) {
+    hg_destroy(c);
+    return code;
+  };
+  if (hipSetDevice(c->device) != hipSuccess) return fail(HG_ERR_DEVICE);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(HG_ERR_DEVICE);
+  Dev &D = c->D;
+  std::memset(&D, 0, sizeof(D));
+  c->is = desc->is;
+  c->js = desc->js;
+  c->ks = desc->ks;
+  D.n1 = desc->ie - desc->is + 1;
+  D.n2 = desc->je - desc->js + 1;
+  D.n3 = desc->ke - desc->ks + 1;
+  c->nxg = D.n1 + 4;
+  c->nyg = D.n2 + 4;
+  c->nzg = D.n3 + 4;
+  D.ncell = (long)D.n1 * D.n2 * D.n3;
+  D.solve[0] = D.n1 > 1;
+  D.solve[1] = D.n2 > 1;
+  D.solve[2] = D.n3 > 1;
+  D.dim = D.solve[0] + D.solve[1] + D.solve[2];
+  D.mag = desc->mag_on ? 1 : 0;
+  D.limiter = desc->flux_limiter;
+  D.gamma = desc->gamma;
+  D.nv = D.mag ? 8 : 5;
+  D.i9wv = -1;
+  if (D.mag && D.dim > 1) {
+    D.nv = 9;
+    D.i9wv = 8;
+  }
+  D.nw = D.mag ? 10 : 6;
+  // coordinates: 6 arrays x 3 axes, each n+4 long
+  const int nn[3] = {D.n1, D.n2, D.n3};
+  size_t tot = 0;
+  for (int a = 0; a < 3; ++a) tot += 6 * (size_t)(nn[a] + 4);
+  if ((st = dev_alloc(c, tot * sizeof(double), (void **)&c->coord))) return fail(st);
+  {
+    std::vector<double> h(tot);
+    const double *src[3][6] = {
+        {desc->x1, desc->xi1, desc->dx1, desc->dxi1, desc->idx1, desc->idxi1},
+        {desc->x2, desc->xi2, desc->dx2, desc->dxi2, desc->idx2, desc->idxi2},
+        {desc->x3, desc->xi3, desc->dx3, desc->dxi3, desc->idx3, desc->idxi3}};
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+      Axis &A = D.ax[a];
+      const double *dp[6];
+      for (int q = 0; q < 6; ++q) {
+        std::memcpy(&h[off], src[a][q], sizeof(double) * (nn[a] + 4));
+        dp[q] = c->coord + off + 2;  // element 2 holds interior cell 0
+        off += nn[a] + 4;
+      }
+      A.x = dp[0];
+      A.xi = dp[1];
+      A.dx = dp[2];
+      A.dxi = dp[3];
+      A.idx = dp[4];
+      A.idxi = dp[5];
+      A.n = nn[a];
+      A.stride = a == 0 ? 1 : (a == 1 ? (long)D.n1 : (long)D.n1 * D.n2);
+      A.sc_lo = desc->bc[2 * a];
+      A.sc_hi = desc->bc[2 * a + 1];
+      A.vc_lo = desc->bc[6 + 2 * a];
+      A.vc_hi = desc->bc[6 + 2 * a + 1];
+    }
+    if (hipMemcpy(c->coord, h.data(), tot * sizeof(double), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return fail(HG_ERR_DEVICE);
+  }
+  const size_t fb = sizeof(double) * (size_t)D.ncell;
+  for (int w = 0; w < D.nw; ++w)
+    if ((st = dev_alloc(c, fb, (void **)&D.W[w]))) return fail(st);
+  for (int v = 0; v < D.nv; ++v) {
+    if ((st = dev_alloc(c, fb, (void **)&D.U[v]))) return fail(st);
+    if ((st = dev_alloc(c, fb, (void **)&D.U0[v]))) return fail(st);
+    for (int a = 0; a < 3; ++a)
+      if (D.solve[a])
+        if ((st = dev_alloc(c, fb, (void **)&D.DU[a][v]))) return fail(st);
+  }
+  for (int w = 0; w < D.nw; ++w)
+    if (hipMemset(D.W[w], 0, fb) != hipSuccess) return fail(HG_ERR_DEVICE);
+  D.nblk_cfl = (int)((D.ncell + 255) / 256);
+  if ((st = dev_alloc(c, sizeof(double) * 2 * D.nblk_cfl, (void **)&D.part))) return fail(st);
+  if ((st = dev_alloc(c, sizeof(Scalars), (void **)&D.sc))) return fail(st);
+  if (hipMemset(D.sc, 0, sizeof(Scalars)) != hipSuccess) return fail(HG_ERR_DEVICE);
+  if (hipHostMalloc((void **)&c->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess)
+    return fail(HG_ERR_NOMEM);
+  if (hipHostMalloc((void **)&c->h_stage, fb, hipHostMallocDefault) != hipSuccess)
+    return fail(HG_ERR_NOMEM);
+  *out = c;
+  return HG_OK;
+}
+
+extern "C" int hg_destroy(hg_ctx *c) {
+  if (!c) return HG_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) hipFree(p);
+  if (c->h_sc) hipHostFree(c->h_sc);
+  if (c->h_stage) hipHostFree(c->h_stage);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return HG_OK;
+}
+
+extern "C" int hg_ufnmax(const hg_ctx *c) { return c ? c->D.nv : 0; }
+
+static double *field_dev(hg_ctx *c, int field, bool upload) {
+  Dev &D = c->D;
+  switch (field) {
+    case HG_F_D: return D.W[WD];
+    case HG_F_V1: return D.W[WV1];
+    case HG_F_V2: return D.W[WV2];
+    case HG_F_V3: return D.W[WV3];
+    case HG_F_E: return D.W[WE];
+    case HG_F_B1: return D.mag ? D.W[WB1] : nullptr;
+    case HG_F_B2: return D.mag ? D.W[WB2] : nullptr;
+    case HG_F_B3: return D.mag ? D.W[WB3] : nullptr;
+    case HG_F_PHI: return D.mag ? D.W[WPHI] : nullptr;
+    case HG_F_EINT: return upload ? nullptr : D.W[WEINT];
+    case HG_F_P: return upload ? nullptr : D.W[WEINT];
+    default:
+      if (field >= HG_F_U && field < HG_F_U + D.nv) return D.U[field - HG_F_U];
+      return nullptr;
+  }
+}
+
+extern "C" int hg_upload(hg_ctx *c, int32_t field, const double *a) {
+  if (!c || !a) return HG_ERR_ARG;
+  if (!c->D.mag && field >= HG_F_B1 && field <= HG_F_PHI) return HG_OK;  // B == 0
+  double *dst = field_dev(c, field, true);
+  if (!dst) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  const Dev &D = c->D;
+  double *h = c->h_stage;
+  HIPCHK(hipStreamSynchronize(c->stream));  // staging buffer reuse
+  long q = 0;
+  for (int k = 0; k < D.n3; ++k)
+    for (int j = 0; j < D.n2; ++j) {
+      const double *row = a + ((size_t)(k + 2) * c->nyg + (j + 2)) * c->nxg + 2;
+      std::memcpy(h + q, row, sizeof(double) * D.n1);
+      q += D.n1;
+    }
+  HIPCHK(hipMemcpyAsync(dst, h, sizeof(double) * D.ncell, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (field < HG_F_EINT) c->eint_stale = true;
+  return HG_OK;
+}
+
+static int check_error(hg_ctx *c) {
+  HIPCHK(hipMemcpyAsync(c->h_sc, c->D.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  unsigned long long key = c->h_sc->err;
+  if (!key) return HG_OK;
+  const Dev &D = c->D;
+  unsigned long long cell = (key - 1) & ((1ull << 42) - 1);
+  unsigned long long cls = (key - 1) >> 42;
+  int dir = (int)(cls % 4);
+  int phase = (int)((cls / 4) % 2);
+  long n1 = D.n1 + 4, n2 = D.n2 + 4;
+  c->err_i = (int)(cell % n1) - 2 + c->is;
+  c->err_j = (int)((cell / n1) % n2) - 2 + c->js;
+  c->err_k = (int)(cell / (n1 * n2)) - 2 + c->ks;
+  c->err_dir = dir;
+  // leave the device flag set: the state is invalid from here on
+  return phase == 0 ? HG_ERR_FACE_EOS : HG_ERR_NEG_EINT;
+}
+
+static int ensure_eint(hg_ctx *c) {
+  if (!c->eint_stale) return HG_OK;
+  launch_pressure(c->D, c->stream);
+  HIPCHK(hipGetLastError());
+  c->eint_stale = false;
+  return check_error(c);
+}
+
+extern "C" int hg_download(hg_ctx *c, int32_t field, double *a) {
+  if (!c || !a) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  const Dev &D = c->D;
+  if (!D.mag && field >= HG_F_B1 && field <= HG_F_PHI) {
+    for (int k = 0; k < D.n3; ++k)
+      for (int j = 0; j < D.n2; ++j) {
+        double *row = a + ((size_t)(k + 2) * c->nyg + (j + 2)) * c->nxg + 2;
+        std::memset(row, 0, sizeof(double) * D.n1);
+      }
+    return HG_OK;
+  }
+  double *src = field_dev(c, field, false);
+  if (!src) return HG_ERR_ARG;
+  int st = ensure_eint(c);
+  if (st && st != HG_ERR_NEG_EINT) return st;
+  double *h = c->h_stage;
+  HIPCHK(hipMemcpyAsync(h, src, sizeof(double) * D.ncell, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double gm1 = c->desc.gamma - 1.0;
+  long q = 0;
+  for (int k = 0; k < D.n3; ++k)
+    for (int j = 0; j < D.n2; ++j) {
+      double *row = a + ((size_t)(k + 2) * c->nyg + (j + 2)) * c->nxg + 2;
+      if (field == HG_F_P)
+        for (int i = 0; i < D.n1; ++i) row[i] = gm1 * h[q + i];  // eos_p
+      else
+        std::memcpy(row, h + q, sizeof(double) * D.n1);
+      q += D.n1;
+    }
+  return HG_OK;
+}
+
+static inline double fmin2(double a, double b) { return a < b ? a : b; }
+
+extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
+                           int32_t outstyle, int32_t endstyle, double fixed_dt, double *dt_out,
+                           double *ch_out, double *cgrav_out, double *dtgrav_out) {
+  if (!c) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  if (fixed_dt > 0.0) {
+    if (dt_out) *dt_out = fixed_dt;
+    if (ch_out) *ch_out = c->ch;
+    return HG_OK;
+  }
+  int st = ensure_eint(c);
+  if (st) return st;
+  launch_cfl(c->D, c->stream);
+  HIPCHK(hipGetLastError());
+  if ((st = check_error(c))) return st;
+  const double dtimin = c->h_sc->dtimin, cfmax = c->h_sc->cfmax;
+  // timestep.f90:74-89
+  double dt = c->desc.courant * dtimin;
+  if (outstyle == 1) {
+    double t_ref = t_out;  // the reference leaves t_ref unset unless endstyle==1
+    if (endstyle == 1) t_ref = fmin2(t_out, t_end);
+    if (t_ref - time - dt > 0.0 && t_ref - time - 2.0 * dt < 0.0)
+      dt = 0.5 * (t_ref - time);
+    else
+      dt = fmin2(dt, t_out - time);
+  }
+  if (endstyle == 1) dt = fmin2(dt, t_end - time);
+  c->ch = cfmax;
+  if (dt_out) *dt_out = dt;
+  if (ch_out) *ch_out = cfmax;
+  if (cgrav_out) *cgrav_out = 0.0;
+  if (dtgrav_out) *dtgrav_out = 0.0;
+  return HG_OK;
+}
+
+extern "C" int hg_hydro_step(hg_ctx *c, double dt, double ch) {
+  if (!c) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  int st = ensure_eint(c);
+  if (st) return st;
+  const Dev &D = c->D;
+  launch_set_scalars(D, dt, ch, c->stream);
+  const int rk = c->desc.rktype;
+  for (int stage = 1; stage <= rk; ++stage) {
+    double faco, fact, facn;  // get_runge_coeff, rungekutta.f90:146-183
+    if (rk == 3) {
+      if (stage == 1) { faco = 1.0; fact = 1.0; facn = 0.0; }
+      else if (stage == 2) { faco = 0.75; fact = 0.25; facn = fact; }
+      else { faco = 1.0 / 3.0; fact = 2.0 / 3.0; facn = fact; }
+    } else if (rk == 2) {
+      if (stage == 1) { faco = 1.0; fact = 1.0; facn = 0.0; }
+      else { faco = 0.5; fact = 0.5; facn = fact; }
+    } else {
+      faco = 1.0; fact = 1.0; facn = 0.0;
+    }
+    for (int a = 0; a < 3; ++a)
+      if (D.solve[a]) launch_sweep(D, a, stage, c->stream);
+    launch_rk(D, stage, faco, fact, facn, c->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return check_error(c);
+}
+
+extern "C" int hg_phidamp(hg_ctx *c, double factor) {
+  if (!c) return HG_ERR_ARG;
+  if (!c->D.mag) return HG_OK;
+  HIPCHK(hipSetDevice(c->device));
+  launch_phidamp(c->D, factor, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return HG_OK;
+}
+
+extern "C" int hg_last_error(const hg_ctx *c, int32_t *i, int32_t *j, int32_t *k,
+                             int32_t *dir) {
+  if (!c) return HG_ERR_ARG;
+  if (i) *i = c->err_i;
+  if (j) *j = c->err_j;
+  if (k) *k = c->err_k;
+  if (dir) *dir = c->err_dir;
+  return HG_OK;
+}
+
+extern "C" int hg_kat(int32_t which, int32_t n, const double *in, double *out) {
+  static const int nin[5] = {25, 5, 8, 5, 2}, nout[5] = {9, 1, 1, 1, 2};
+  if (which < 0 || which > 4 || n < 0 || !in || !out) return HG_ERR_ARG;
+  if (n == 0) return HG_OK;
+  double *din = nullptr, *dout = nullptr;
+  const size_t bi = sizeof(double) * (size_t)n * nin[which];
+  const size_t bo = sizeof(double) * (size_t)n * nout[which];
+  if (hipMalloc((void **)&din, bi) != hipSuccess) return HG_ERR_NOMEM;
+  if (hipMalloc((void **)&dout, bo) != hipSuccess) {
+    hipFree(din);
+    return HG_ERR_NOMEM;
+  }
+  int st = HG_OK;
+  if (hipMemcpy(din, in, bi, hipMemcpyHostToDevice) != hipSuccess) st = HG_ERR_DEVICE;
+  if (!st) {
+    launch_kat(which, n, din, dout, nullptr);
+    if (hipGetLastError() != hipSuccess) st = HG_ERR_DEVICE;
+  }
+  if (!st && hipMemcpy(out, dout, bo, hipMemcpyDeviceToHost) != hipSuccess) st = HG_ERR_DEVICE;
+  hipFree(din);
+  hipFree(dout);
+  return st;
+}

@@ -1,0 +1,516 @@
+// hg_kernels.hip -- FP64 kernels of the explicit hydro step on gfx950.
+//
+// One RK stage is: one "sweep" kernel per solved direction (MUSCL slopes,
+// e_int guard, face states, HLLD/GLM flux and the per-direction flux
+// difference), then one "rk" kernel (SSP-RK combination, primitive recovery,
+// e_int check).  The timestep kernel reduces the CFL condition.
+//
+// Sweep mapping: a wavefront owns a line segment of 64 consecutive cells
+// along the sweep direction (cells c0-2 .. c0+61); lane l holds cell c0-2+l.
+// Neighbour values travel by cross-lane shuffles: slopes need l-1 and l+1,
+// the face between l and l+1 needs the right state from l+1, and the flux
+// difference of cell l needs the face flux of l-1.  60 of the 64 lanes
+// produce a result.  Boundary ghosts are evaluated on the fly.
+#include <hip/hip_runtime.h>
+
+#include "hg_kernels.hpp"
+#include "hg_physics.hpp"
+
+namespace hg {
+
+static constexpr int SEG = 60;  // outputs per 64-lane wavefront
+
+struct Cell {
+  double d, v[3], e, eint, b[3], phi;
+};
+
+__device__ __forceinline__ int ghost_src(int code, bool lo, int c, int n) {
+  // boundarycondition.f90: 0 periodic, 1 reflective, 2/3 outgoing/free
+  int s;
+  if (code == 0) s = lo ? c + n : c - n;
+  else if (code == 1) s = lo ? -1 - c : 2 * n - 1 - c;
+  else s = lo ? 0 : n - 1;
+  return s < 0 ? 0 : (s > n - 1 ? n - 1 : s);  // lanes past the ghosts: unused
+}
+
+template <int DIR, bool MAG>
+__device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base, int c) {
+  Cell q;
+  const int n = A.n;
+  if (c >= 0 && c < n) {
+    long m = base + (long)c * A.stride;
+    q.d = D.W[WD][m];
+    q.v[0] = D.W[WV1][m];
+    q.v[1] = D.W[WV2][m];
+    q.v[2] = D.W[WV3][m];
+    q.e = D.W[WE][m];
+    q.eint = D.W[WEINT][m];
+    if (MAG) {
+      q.b[0] = D.W[WB1][m];
+      q.b[1] = D.W[WB2][m];
+      q.b[2] = D.W[WB3][m];
+      q.phi = D.W[WPHI][m];
+    } else {
+      q.b[0] = q.b[1] = q.b[2] = q.phi = 0.0;
+    }
+    return q;
+  }
+  // ghost cell (boundarycondition.f90, codes 0..3): scalars d, p, phi from
+  // the scalar-code source, v and B from the vector-code source with the
+  // normal components transformed; then ptot/eint/e rebuilt (:221-243).
+  const bool lo = c < 0;
+  const int sc = lo ? A.sc_lo : A.sc_hi, vc = lo ? A.vc_lo : A.vc_hi;
+  const long ms = base + (long)ghost_src(sc, lo, c, n) * A.stride;
+  const long mv = base + (long)ghost_src(vc, lo, c, n) * A.stride;
+  const double gm1 = D.gamma - 1.0;
+  q.d = D.W[WD][ms];
+  const double p = gm1 * D.W[WEINT][ms];
+  q.phi = MAG ? D.W[WPHI][ms] : 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double v = D.W[WV1 + a][mv];
+    double b = MAG ? D.W[WB1 + a][mv] : 0.0;
+    if (a == DIR) {
+      if (vc == 1) {
+        v = -v;
+        b = -b;
+      } else if (vc == 2) {
+        if (DIR == 0 && lo) v = mn2(0.0, v);
+        else v = v * (lo ? 0.5 - sgn(0.5, v) : 0.5 + sgn(0.5, v));
+      }
+    }
+    q.v[a] = v;
+    q.b[a] = b;
+  }
+  q.eint = p / gm1;
+  q.e = q.eint + 0.5 * (q.d * (q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]) +
+                        q.b[0] * q.b[0] + q.b[1] * q.b[1] + q.b[2] * q.b[2]);
+  return q;
+}
+
+__device__ __forceinline__ double up1(double x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ double dn1(double x) { return __shfl_down(x, 1, 64); }
+
+template <int DIR, bool MAG, int LIM>
+__global__ __launch_bounds__(256) void k_sweep(Dev D, int stage) {
+  const Axis &A = D.ax[DIR];
+  const int lane = threadIdx.x & 63;
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int n = A.n;
+  const int nseg = (n + SEG - 1) / SEG;
+  const long nlines = D.ncell / n;
+  if (wave >= (long)nseg * nlines) return;  // uniform per wavefront
+  const long line = wave % nlines;
+  const int seg = (int)(wave / nlines);
+  long base;
+  int ci, cj, ck;  // cell-0 coordinates of the line (for error reports)
+  if (DIR == 0) {
+    base = line * D.n1;
+    ci = 0; cj = (int)(line % D.n2); ck = (int)(line / D.n2);
+  } else if (DIR == 1) {
+    const long i = line % D.n1, k = line / D.n1;
+    base = i + k * (long)D.n1 * D.n2;
+    ci = (int)i; cj = 0; ck = (int)k;
+  } else {
+    base = line;
+    ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
+  }
+  const int c = seg * SEG - 2 + lane;
+  const Cell q = load_cell<DIR, MAG>(D, A, base, c);
+  const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
+
+  // ---- slopes of cell c (interpolation.f90, loop c = -1 .. n) -----------
+  double S[8];  // dd de dm1 dm2 dm3 db1 db2 db3 (dphi is always 0)
+  const int cc = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
+  if (LIM == LIM_FLAT) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) S[s] = 0.0;
+  } else {
+    const double qv[9] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2], q.eint};
+    double ql[9], qr[9];
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+      ql[s] = up1(qv[s]);
+      qr[s] = dn1(qv[s]);
+    }
+    const int cl = cc > -1 ? cc : -1, cr = cc < n ? cc : n;  // keep indices in range
+    const double xm = A.x[cl - 1], x0 = A.x[cl], xp = A.x[cr + 1];
+    const double xim = A.xi[cl - 1], xi0 = A.xi[cl];
+    const double ida = A.idx[cl], idb = A.idx[cr + 1];
+    const double hl = x0 - xim, hr = xi0 - x0;
+    double fl[8], fr[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      if (!MAG && s >= 5) {
+        S[s] = 0.0;
+        fl[s] = fr[s] = 0.0;
+        continue;
+      }
+      const double du = (LIM == LIM_MINMOD) ? minmod(ql[s], qv[s], qr[s], ida, idb)
+                                            : modified_mc(ql[s], qv[s], qr[s], xm, x0, xp, xim, xi0);
+      S[s] = du;
+      fl[s] = qv[s] - hl * du;
+      fr[s] = qv[s] + hr * du;
+    }
+    // e_int guard (interpolation.f90:143-163)
+    double eintl = fl[1] - 0.5 * (fl[2] * fl[2] + fl[3] * fl[3] + fl[4] * fl[4]) / fl[0];
+    double eintr = fr[1] - 0.5 * (fr[2] * fr[2] + fr[3] * fr[3] + fr[4] * fr[4]) / fr[0];
+    if (MAG) {
+      eintl = eintl - 0.5 * (fl[5] * fl[5] + fl[6] * fl[6] + fl[7] * fl[7]);
+      eintr = eintr - 0.5 * (fr[5] * fr[5] + fr[6] * fr[6] + fr[7] * fr[7]);
+    }
+    const double el_ = ql[8], em_ = q.eint, er_ = qr[8];
+    if (eintl >= mx2(el_, em_) || eintl <= mn2(el_, em_) || eintr >= mx2(em_, er_) ||
+        eintr <= mn2(em_, er_)) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) S[s] = 0.0;
+    }
+  }
+
+  // ---- face c+1/2 (numflux.f90, faces c = -1 .. n-1) ---------------------
+  // right cell's primitives and slopes from lane+1
+  const double rd = dn1(q.d), re = dn1(q.e);
+  const double rm[3] = {dn1(m0), dn1(m1), dn1(m2)};
+  double rb[3] = {0.0, 0.0, 0.0}, rphi = 0.0;
+  if (MAG) {
+    rb[0] = dn1(q.b[0]);
+    rb[1] = dn1(q.b[1]);
+    rb[2] = dn1(q.b[2]);
+    rphi = dn1(q.phi);
+  }
+  double RS[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) RS[s] = (!MAG && s >= 5) ? 0.0 : dn1(S[s]);
+
+  const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;  // clamp face index
+  const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
+  // cyclic rotation: normal a = DIR, then b, c (numflux.f90:73-93,246-266,419-439)
+  constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
+  const double lm[3] = {m0, m1, m2};
+  const double dl = q.d + dx1 * S[0];
+  const double dr = rd - dx2 * RS[0];
+  const double el = q.e + dx1 * S[1];
+  const double er = re - dx2 * RS[1];
+  const double v1l = (lm[a_] + dx1 * S[2 + a_]) / dl;
+  const double v1r = (rm[a_] - dx2 * RS[2 + a_]) / dr;
+  const double v2l = (lm[b_] + dx1 * S[2 + b_]) / dl;
+  const double v2r = (rm[b_] - dx2 * RS[2 + b_]) / dr;
+  const double v3l = (lm[c_] + dx1 * S[2 + c_]) / dl;
+  const double v3r = (rm[c_] - dx2 * RS[2 + c_]) / dr;
+  double b1l = 0, b1r = 0, b2l = 0, b2r = 0, b3l = 0, b3r = 0, phil = 0, phir = 0;
+  if (MAG) {
+    b1l = q.b[a_] + dx1 * S[5 + a_];
+    b1r = rb[a_] - dx2 * RS[5 + a_];
+    b2l = q.b[b_] + dx1 * S[5 + b_];
+    b2r = rb[b_] - dx2 * RS[5 + b_];
+    b3l = q.b[c_] + dx1 * S[5 + c_];
+    b3r = rb[c_] - dx2 * RS[5 + c_];
+    // dphi is identically zero: phi + dx*0 == phi + 0
+    phil = q.phi + 0.0;
+    phir = rphi + 0.0;
+  }
+  const double gam = D.gamma;
+  const double eil = get_eint(MAG, el, dl, v1l, v2l, v3l, b1l, b2l, b3l);
+  const double eir = get_eint(MAG, er, dr, v1r, v2r, v3r, b1r, b2r, b3r);
+  // faces between lanes l and l+1 have valid slopes on both sides for l = 1..61
+  const bool face_valid = (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
+  if (face_valid && (eil <= 0.0 || dl <= 0.0 || eir <= 0.0 || dr <= 0.0)) {
+    int ii = ci, jj = cj, kk = ck;
+    if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
+    atomicMin(&D.sc->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
+  }
+  // eos_p_cs (eos.f90:195-242); the erad term 4*0/(9d) adds +0 to a
+  // positive cs2 and is omitted.
+  double ptl = (gam - 1.0) * eil;
+  double ptr = (gam - 1.0) * eir;
+  const double csl = sqrt(gam * ptl / dl);
+  const double csr = sqrt(gam * ptr / dr);
+  if (MAG) {
+    ptl = ptl + 0.5 * (b1l * b1l + b2l * b2l + b3l * b3l);
+    ptr = ptr + 0.5 * (b1r * b1r + b2r * b2r + b3r * b3r);
+  }
+  const double cfl = get_cf(dl, csl, b1l, b2l, b3l);
+  const double cfr = get_cf(dr, csr, b1r, b2r, b3r);
+  double t[9];
+  hlld(t, cfl, cfr, v1l, v1r, v2l, v2r, v3l, v3r, dl, dr, el, er, ptl, ptr, b1l, b1r,
+       b2l, b2r, b3l, b3r, phil, phir, D.sc->ch);
+
+  // ---- flux difference of cell c (rungekutta.f90:42-55, one direction) --
+  double tm[9];
+#pragma unroll
+  for (int s = 0; s < 9; ++s) tm[s] = up1(t[s]);
+  if (lane >= 2 && lane <= 61 && c >= 0 && c < n) {
+    const long m = base + (long)c * A.stride;
+    const double ix = A.idxi[c];
+    // equation order: icnt imo1 imo2 imo3 iene img1 img2 img3 i9wv
+    double *const *DU = D.DU[DIR];
+    DU[0][m] = ix * (tm[0] - t[0]);
+    DU[1 + a_][m] = ix * (tm[1] - t[1]);
+    DU[1 + b_][m] = ix * (tm[2] - t[2]);
+    DU[1 + c_][m] = ix * (tm[3] - t[3]);
+    DU[4][m] = ix * (tm[4] - t[4]);
+    if (MAG) {
+      DU[5 + a_][m] = ix * (tm[5] - t[5]);
+      DU[5 + b_][m] = ix * (tm[6] - t[6]);
+      DU[5 + c_][m] = ix * (tm[7] - t[7]);
+      if (D.i9wv >= 0) DU[8][m] = ix * (tm[8] - t[8]);
+    }
+  }
+}
+
+// ---- RK update + primitive recovery ------------------------------------
+template <bool MAG>
+__global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
+                                            double facn) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= D.ncell) return;
+  const double ft = fact * D.sc->dt;
+  double un[NVMAX];
+#pragma unroll
+  for (int v = 0; v < NVMAX; ++v) {
+    if (v >= D.nv) break;
+    // inactive directions contribute idxi*(0-0) = +0 (allocations.f90:214-240)
+    const double a = D.solve[0] ? D.DU[0][v][m] : 0.0;
+    const double b = D.solve[1] ? D.DU[1][v][m] : 0.0;
+    const double c = D.solve[2] ? D.DU[2][v][m] : 0.0;
+    const double acc = ((a + b) + c) + 0.0;  // + src (zero without gravity)
+    const double u = D.U[v][m];
+    double u0;
+    if (stage == 1) {
+      u0 = u;
+      D.U0[v][m] = u;
+    } else {
+      u0 = D.U0[v][m];
+    }
+    un[v] = faco * u0 + facn * u + ft * acc;
+    D.U[v][m] = un[v];
+  }
+  // primitive (rungekutta.f90:193-232) + internalenergy (eos.f90:805-842)
+  const double d = un[0];
+  const double v1 = un[1] / d, v2 = un[2] / d, v3 = un[3] / d;
+  const double e = un[4];
+  D.W[WD][m] = d;
+  D.W[WV1][m] = v1;
+  D.W[WV2][m] = v2;
+  D.W[WV3][m] = v3;
+  D.W[WE][m] = e;
+  double b1 = 0, b2 = 0, b3 = 0;
+  if (MAG) {
+    b1 = un[5];
+    b2 = un[6];
+    b3 = un[7];
+    D.W[WB1][m] = b1;
+    D.W[WB2][m] = b2;
+    D.W[WB3][m] = b3;
+    if (D.i9wv >= 0) D.W[WPHI][m] = un[8];
+  }
+  const double eint = get_eint(MAG, e, d, v1, v2, v3, b1, b2, b3);
+  D.W[WEINT][m] = eint;
+  if (eint <= 0.0) {
+    const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
+    atomicMin(&D.sc->err, errkey(stage, 1, 0, i, j, k, D));
+  }
+}
+
+// ---- pressure(): eint from e, d, v, B after an upload ---------------------
+template <bool MAG>
+__global__ __launch_bounds__(256) void k_pressure(Dev D) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= D.ncell) return;
+  double b1 = 0, b2 = 0, b3 = 0;
+  if (MAG) {
+    b1 = D.W[WB1][m];
+    b2 = D.W[WB2][m];
+    b3 = D.W[WB3][m];
+  }
+  const double eint = get_eint(MAG, D.W[WE][m], D.W[WD][m], D.W[WV1][m], D.W[WV2][m],
+                               D.W[WV3][m], b1, b2, b3);
+  D.W[WEINT][m] = eint;
+  if (eint <= 0.0) {
+    const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
+    atomicMin(&D.sc->err, errkey(0, 1, 0, i, j, k, D));
+  }
+}
+
+// ---- CFL condition (timestep.f90:35-57, dti_cell :125-183) ----------------
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = mn2(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = mx2(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+template <bool MAG>
+__global__ __launch_bounds__(256) void k_cfl(Dev D) {
+  __shared__ double smin[4], smax[4];
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double dti = __builtin_inf(), cfm = 0.0;
+  if (m < D.ncell) {
+    const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
+    const double d = D.W[WD][m], eint = D.W[WEINT][m];
+    const double gam = D.gamma;
+    const double p = (gam - 1.0) * eint;
+    const double cs = sqrt(gam * p / d);
+    const double av1 = fabs(D.W[WV1][m]), av2 = fabs(D.W[WV2][m]), av3 = fabs(D.W[WV3][m]);
+    double cf1, cf2, cf3;
+    if (MAG) {
+      const double b1 = D.W[WB1][m], b2 = D.W[WB2][m], b3 = D.W[WB3][m];
+      cf1 = get_cf(d, cs, b1, b2, b3) + av1;
+      cf2 = get_cf(d, cs, b2, b1, b3) + av2;
+      cf3 = get_cf(d, cs, b3, b1, b2) + av3;
+    } else {
+      cf1 = cs + av1;
+      cf2 = cs + av2;
+      cf3 = cs + av3;
+    }
+    const double x1 = D.ax[0].dxi[i], x2 = D.ax[1].dxi[j], x3 = D.ax[2].dxi[k];
+    const double dvol = x1 * x2 * x3;  // metric.f90:44-47
+    const double sa1 = x2 * x3, sa2 = x1 * x3, sa3 = x1 * x2;
+    const double o1 = D.solve[0] ? 1.0 : 0.0, o2 = D.solve[1] ? 1.0 : 0.0,
+                 o3 = D.solve[2] ? 1.0 : 0.0;
+    dti = dvol / (cf1 * o1 * (sa1 + sa1) + cf2 * o2 * (sa2 + sa2) + cf3 * o3 * (sa3 + sa3));
+    cfm = mx3(cf1, cf2, cf3);
+    if (eint <= 0.0 || d <= 0.0) atomicMin(&D.sc->err, errkey(0, 1, 0, i, j, k, D));
+  }
+  dti = wave_min(dti);
+  cfm = wave_max(cfm);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    smin[w] = dti;
+    smax[w] = cfm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = smin[0], b = smax[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+      a = mn2(a, smin[q]);
+      b = mx2(b, smax[q]);
+    }
+    D.part[2 * blockIdx.x] = a;
+    D.part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cfl_final(Dev D) {
+  __shared__ double smin[4], smax[4];
+  double a = __builtin_inf(), b = 0.0;
+  for (int q = threadIdx.x; q < D.nblk_cfl; q += blockDim.x) {
+    a = mn2(a, D.part[2 * q]);
+    b = mx2(b, D.part[2 * q + 1]);
+  }
+  a = wave_min(a);
+  b = wave_max(b);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    smin[w] = a;
+    smax[w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = smin[0];
+    b = smax[0];
+    for (int q = 1; q < 4; ++q) {
+      a = mn2(a, smin[q]);
+      b = mx2(b, smax[q]);
+    }
+    D.sc->dtimin = a;
+    D.sc->cfmax = b;
+  }
+}
+
+__global__ void k_set_scalars(Scalars *sc, double dt, double ch) {
+  sc->dt = dt;
+  sc->ch = ch;
+}
+
+__global__ __launch_bounds__(256) void k_phidamp(Dev D, double factor) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= D.ncell) return;
+  D.W[WPHI][m] = D.W[WPHI][m] * factor;
+}
+
+// ---- known-answer evaluation of the device arithmetic -----------------------
+__global__ void k_kat(int which, int n, const double *in, double *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (which == 0) {  // hlldflux: 25 inputs (hlldflux.f90:19-21 order), 9 outputs
+    const double *a = in + 25 * (long)t;
+    double f[9];
+    hlld(f, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11],
+         a[12], a[13], a[14], a[15], a[16], a[17], a[18], a[19], a[20], a[21], a[22]);
+    for (int s = 0; s < 9; ++s) out[9 * (long)t + s] = f[s];
+  } else if (which == 1) {  // get_cf: d cs b1 b2 b3
+    const double *a = in + 5 * (long)t;
+    out[t] = get_cf(a[0], a[1], a[2], a[3], a[4]);
+  } else if (which == 2) {  // modified_mc: u0 u1 u2 xm x0 xp xim xi0
+    const double *a = in + 8 * (long)t;
+    out[t] = modified_mc(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]);
+  } else if (which == 3) {  // minmod: u0 u1 u2 dxa dxb
+    const double *a = in + 5 * (long)t;
+    out[t] = minmod(a[0], a[1], a[2], a[3], a[4]);
+  } else if (which == 4) {  // division and square root: a b -> a/b, sqrt(a)
+    const double *a = in + 2 * (long)t;
+    out[2 * (long)t] = a[0] / a[1];
+    out[2 * (long)t + 1] = sqrt(a[0]);
+  }
+}
+
+// ---- launchers ----------------------------------------------------------------
+static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_pressure(const Dev &D, hipStream_t s) {
+  if (D.mag) k_pressure<true><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
+  else k_pressure<false><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
+}
+
+void launch_set_scalars(const Dev &D, double dt, double ch, hipStream_t s) {
+  k_set_scalars<<<1, 1, 0, s>>>(D.sc, dt, ch);
+}
+
+template <int DIR>
+static void sweep_dir(const Dev &D, int stage, hipStream_t s) {
+  const int n = D.ax[DIR].n;
+  const long waves = (long)((n + SEG - 1) / SEG) * (D.ncell / n);
+  const unsigned g = nblk(waves, 4);
+  if (D.mag) {
+    if (D.limiter == LIM_FLAT) k_sweep<DIR, true, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
+    else if (D.limiter == LIM_MINMOD) k_sweep<DIR, true, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
+    else k_sweep<DIR, true, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+  } else {
+    if (D.limiter == LIM_FLAT) k_sweep<DIR, false, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
+    else if (D.limiter == LIM_MINMOD) k_sweep<DIR, false, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
+    else k_sweep<DIR, false, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+  }
+}
+
+void launch_sweep(const Dev &D, int dir, int stage, hipStream_t s) {
+  if (dir == 0) sweep_dir<0>(D, stage, s);
+  else if (dir == 1) sweep_dir<1>(D, stage, s);
+  else sweep_dir<2>(D, stage, s);
+}
+
+void launch_rk(const Dev &D, int stage, double faco, double fact, double facn,
+               hipStream_t s) {
+  if (D.mag) k_rk<true><<<nblk(D.ncell, 256), 256, 0, s>>>(D, stage, faco, fact, facn);
+  else k_rk<false><<<nblk(D.ncell, 256), 256, 0, s>>>(D, stage, faco, fact, facn);
+}
+
+void launch_cfl(const Dev &D, hipStream_t s) {
+  if (D.mag) k_cfl<true><<<D.nblk_cfl, 256, 0, s>>>(D);
+  else k_cfl<false><<<D.nblk_cfl, 256, 0, s>>>(D);
+  k_cfl_final<<<1, 256, 0, s>>>(D);
+}
+
+void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
+  k_phidamp<<<nblk(D.ncell, 256), 256, 0, s>>>(D, factor);
+}
+
+void launch_kat(int which, int n, const double *in, double *out, hipStream_t s) {
+  k_kat<<<nblk(n, 128), 128, 0, s>>>(which, n, in, out);
+}
+
+}  // namespace hg

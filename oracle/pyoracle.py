@@ -69,6 +69,12 @@ class Oracle:
         if not self.h:
             raise HgError(st.value, "or_create")
         self._dirty = True
+        g = desc_kw["grid"]
+        # small grids run faster on one thread than through OpenMP barriers
+        if g.ncells < 65536:
+            set_threads(1)
+        else:
+            set_threads(0)
 
     def close(self):
         if self.h:

@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Generate the golden fixtures under tests/golden/ (run in the build
+container, where the reference tree is mounted read-only).
+
+Two sources, both data produced by the reference itself:
+
+1. The reference's own committed golden snapshots (tests/NAME.bin) for the
+   Cartesian hydro/MHD problems, read with hormone_amd.snapshot.read_bin and
+   re-stored in our format: small 1D states as float64 arrays, 2D states as a
+   SHA-256 digest of each field's little-endian bytes plus norms.
+2. Runs of the reference program built from source by oracle/build_ref.py
+   (oracle/_ref/hormone) at the benchmark sizes: Sod 1024 cells at
+   courant 0.4 to t=0.2, and Orszag-Tang 512^2 (a 20-step prefix and the
+   full run to t=0.5).  The binary runs in a scratch directory under /tmp
+   whose ../para points at the reference's parameter files.
+
+Usage: python tests/golden/make_fixtures.py [--skip-long]
+"""
+from __future__ import annotations
+
+import glob
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from hormone_amd.snapshot import read_bin  # noqa: E402
+
+REF = os.environ.get("HORMONE_REF", "/root/reference")
+BIN = os.path.join(ROOT, "oracle", "_ref", "hormone")
+FIELDS = ("d", "v1", "v2", "v3", "e")
+FIELDS_MAG = FIELDS + ("b1", "b2", "b3", "phi")
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def summary(snap: dict, names) -> dict:
+    out = {"tn": snap["tn"], "time": snap["time"], "fields": {}}
+    for nm in names:
+        a = snap[nm]
+        out["fields"][nm] = {"sha256": digest(a), "sum": float(a.sum()),
+                             "abs_sum": float(np.abs(a).sum())}
+    return out
+
+
+def shape_of(n):
+    return (n[2], n[1], n[0])
+
+
+CASES_1D = {
+    "sodshock": (False, 500), "briowushock": (True, 500)}
+
+
+def convert_reference_goldens(meta: dict):
+    for kind, (mag, n) in CASES_1D.items():
+        for d in "xyz":
+            for flat in ("", "_flat"):
+                name = f"{kind}_{d}{flat}"
+                nn = tuple(n if c == d else 1 for c in "xyz")
+                snap = read_bin(os.path.join(REF, "tests", name + ".bin"), shape_of(nn), mag)
+                names = FIELDS_MAG if mag else FIELDS
+                np.savez_compressed(os.path.join(HERE, name + ".npz"), tn=snap["tn"],
+                                    time=snap["time"],
+                                    **{k: snap[k].reshape(-1) for k in names})
+                meta[name] = summary(snap, names)
+                meta[name]["source"] = f"reference tests/{name}.bin"
+    for plane in ("xy", "xz", "yz"):
+        for flat in ("", "_flat"):
+            name = f"orszagtang_{plane}{flat}"
+            nn = tuple(128 if c in plane else 1 for c in "xyz")
+            snap = read_bin(os.path.join(REF, "tests", name + ".bin"), shape_of(nn), True)
+            meta[name] = summary(snap, FIELDS_MAG)
+            meta[name]["source"] = f"reference tests/{name}.bin"
+
+
+def run_reference(simtype: str, params: str, threads: int = 8) -> str:
+    run = "/tmp/hormone_fixture_run"
+    shutil.rmtree(run, ignore_errors=True)
+    os.makedirs(os.path.join(run, "work", "data"))
+    os.symlink(os.path.join(REF, "para"), os.path.join(run, "para"))
+    os.symlink(os.path.join(REF, "tests"), os.path.join(run, "tests"))
+    with open(os.path.join(run, "work", "startfile"), "w") as f:
+        f.write(f"&startcon\n start=0\n simtype='{simtype}'\n parafile='parameters'\n/\n")
+    with open(os.path.join(run, "work", "parameters"), "w") as f:
+        f.write(params)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    with open(os.path.join(run, "log.txt"), "w") as log:
+        subprocess.run([BIN], cwd=os.path.join(run, "work"), env=env, check=True,
+                       stdout=log, stderr=subprocess.STDOUT)
+    bins = glob.glob(os.path.join(run, "work", "data", "bin*.dat"))
+    # the final snapshot is the one with the largest step count
+    return max(bins, key=lambda b: read_header(b)[0])
+
+
+def read_header(path):
+    raw = open(path, "rb").read(20)
+    return int(np.frombuffer(raw, "<i4", 1, 4)[0]), float(np.frombuffer(raw, "<f8", 1, 8)[0])
+
+
+def reference_runs(meta: dict, long: bool):
+    # Sod 1024 cells, courant 0.4, to t = 0.2 (200 ms)
+    p = ("&gridcon ie=1024 /\n&out_con output_ascii=.false. /\n"
+         "&simucon courant=0.4d0 /\n")
+    b = run_reference("sodshock_x", p)
+    snap = read_bin(b, (1, 1, 1024), False)
+    np.savez_compressed(os.path.join(HERE, "sod1024_c04.npz"), tn=snap["tn"], time=snap["time"],
+                        **{k: snap[k].reshape(-1) for k in FIELDS})
+    meta["sod1024_c04"] = summary(snap, FIELDS)
+    meta["sod1024_c04"]["source"] = "reference run: sodshock_x ie=1024 courant=0.4"
+    # Orszag-Tang 512^2, first 20 steps
+    p = ("&gridcon ie=512 je=512 /\n&out_con outstyle=2 endstyle=2 tnlim=20 tn_out=20 "
+         "output_ascii=.false. /\n")
+    b = run_reference("orszagtang_xy", p)
+    snap = read_bin(b, (1, 512, 512), True)
+    meta["ot512_20steps"] = summary(snap, FIELDS_MAG)
+    meta["ot512_20steps"]["source"] = "reference run: orszagtang_xy ie=je=512, tnlim=20"
+    if long:
+        p = ("&gridcon ie=512 je=512 /\n&out_con output_ascii=.false. /\n")
+        b = run_reference("orszagtang_xy", p)
+        snap = read_bin(b, (1, 512, 512), True)
+        meta["ot512_t05"] = summary(snap, FIELDS_MAG)
+        meta["ot512_t05"]["source"] = "reference run: orszagtang_xy ie=je=512 to t=0.5"
+
+
+def main():
+    long = "--skip-long" not in sys.argv
+    path = os.path.join(HERE, "golden_meta.json")
+    meta = json.load(open(path)) if os.path.exists(path) else {}
+    convert_reference_goldens(meta)
+    reference_runs(meta, long)
+    with open(path, "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()

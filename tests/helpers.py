@@ -1,0 +1,58 @@
+"""Shared helpers for the parity tests."""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from hormone_amd import driver as D
+from hormone_amd import problems as P
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NAMES = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
+
+
+def meta():
+    with open(os.path.join(GOLDEN, "golden_meta.json")) as f:
+        return json.load(f)
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def run_problem(engine_cls, name, flat=False, max_steps=None, **kw):
+    prm, g, st = P.make_problem(name, flux_limiter="flat" if flat else "modified_mc", **kw)
+    eng = engine_cls(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, max_steps=max_steps)
+    fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+    out = D.fetch(eng, g, fields)
+    state = {NAMES[i]: out[f] for i, f in enumerate(fields)}
+    return res, state, eng
+
+
+def compare_to_meta(res, state, entry):
+    """Bitwise comparison against a digest entry of golden_meta.json."""
+    assert res.tn == entry["tn"], (res.tn, entry["tn"])
+    assert res.time == entry["time"], (res.time, entry["time"])
+    bad = [k for k, v in entry["fields"].items() if digest(state[k]) != v["sha256"]]
+    assert not bad, f"fields differing from the reference: {bad}"
+
+
+def compare_to_npz(res, state, name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    assert res.tn == int(z["tn"])
+    assert res.time == float(z["time"])
+    for k in state:
+        a = state[k].reshape(-1)
+        b = z[k]
+        nbad = int(np.sum(a.view(np.uint64) != b.view(np.uint64)))
+        assert nbad == 0, f"{name}: field {k} differs from the reference in {nbad} cells"
+
+
+def rel_l1(a, b):
+    den = np.abs(b).sum()
+    return float(np.abs(a - b).sum() / den) if den > 0 else float(np.abs(a - b).sum())

@@ -1,0 +1,111 @@
+"""HIP path vs the reference: bit-exact FP64 parity on the GPU.
+
+Each case runs the same initial state through the C-ABI (hg_*) and compares
+the final state with the reference's own golden snapshot (or a reference run
+at the benchmark size) byte for byte, and with the CPU oracle on the same
+inputs.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from hormone_amd.native import Hydro
+from oracle.pyoracle import Oracle, lib as oracle_lib
+from tests.helpers import compare_to_meta, compare_to_npz, meta, run_problem
+
+pytestmark = pytest.mark.gpu
+
+ONE_D = [f"{k}_{d}" for k in ("sodshock", "briowushock") for d in "xyz"]
+
+
+def _kat(hiplib, which, rows, nout):
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    out = np.zeros((rows.shape[0], nout))
+    dp = ctypes.POINTER(ctypes.c_double)
+    hiplib.hg_kat.argtypes = [ctypes.c_int32, ctypes.c_int32, dp, dp]
+    st = hiplib.hg_kat(which, rows.shape[0], rows.ctypes.data_as(dp), out.ctypes.data_as(dp))
+    assert st == 0
+    return out
+
+
+def _random_riemann(n, rng):
+    """Left/right states covering sub/supersonic flow, B_n = 0, psi != 0."""
+    dl, dr = rng.lognormal(0, 1, n), rng.lognormal(0, 1, n)
+    pl, pr = rng.lognormal(0, 1, n), rng.lognormal(0, 1, n)
+    v = rng.uniform(-3, 3, (6, n))
+    b = rng.uniform(-1, 1, (6, n))
+    b[0:2, : n // 8] = 0.0  # B_n = 0
+    phi = rng.uniform(-0.5, 0.5, (2, n))
+    gam = 5.0 / 3.0
+    cs_l, cs_r = np.sqrt(gam * pl / dl), np.sqrt(gam * pr / dr)
+    ol = oracle_lib()
+    cfl = np.array([ol.or_kat_get_cf(dl[i], cs_l[i], b[0, i], b[2, i], b[4, i]) for i in range(n)])
+    cfr = np.array([ol.or_kat_get_cf(dr[i], cs_r[i], b[1, i], b[3, i], b[5, i]) for i in range(n)])
+    el = pl / (gam - 1) + 0.5 * dl * (v[0] ** 2 + v[2] ** 2 + v[4] ** 2) + 0.5 * (b[0] ** 2 + b[2] ** 2 + b[4] ** 2)
+    er = pr / (gam - 1) + 0.5 * dr * (v[1] ** 2 + v[3] ** 2 + v[5] ** 2) + 0.5 * (b[1] ** 2 + b[3] ** 2 + b[5] ** 2)
+    ptl = pl + 0.5 * (b[0] ** 2 + b[2] ** 2 + b[4] ** 2)
+    ptr = pr + 0.5 * (b[1] ** 2 + b[3] ** 2 + b[5] ** 2)
+    ch = np.maximum(cfl, cfr) + 1.0
+    rows = np.stack([cfl, cfr, v[0], v[1], v[2], v[3], v[4], v[5], dl, dr, el, er, ptl, ptr,
+                     b[0], b[1], b[2], b[3], b[4], b[5], phi[0], phi[1], ch,
+                     np.zeros(n), np.zeros(n)], axis=1)
+    return rows
+
+
+def test_kat_hlld_bitexact_vs_oracle(hiplib):
+    rng = np.random.default_rng(20261202)
+    rows = _random_riemann(4096, rng)
+    gpu = _kat(hiplib, 0, rows, 9)
+    ol = oracle_lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ref = np.zeros((rows.shape[0], 10))
+    for i in range(rows.shape[0]):
+        r = np.ascontiguousarray(rows[i])
+        ol.or_kat_hlldflux(r.ctypes.data_as(dp), ref[i].ctypes.data_as(dp))
+    assert np.array_equal(gpu.view(np.uint64), ref[:, :9].view(np.uint64))
+
+
+def test_kat_div_sqrt_correctly_rounded(hiplib):
+    rng = np.random.default_rng(7)
+    a = np.exp(rng.uniform(-300, 300, 20000))
+    b = np.exp(rng.uniform(-300, 300, 20000)) * rng.choice([-1, 1], 20000)
+    out = _kat(hiplib, 4, np.stack([a, b], 1), 2)
+    assert np.array_equal(out[:, 0].view(np.uint64), (a / b).view(np.uint64))
+    assert np.array_equal(out[:, 1].view(np.uint64), np.sqrt(a).view(np.uint64))
+
+
+@pytest.mark.parametrize("flat", [False, True], ids=["muscl", "flat"])
+@pytest.mark.parametrize("name", ONE_D)
+def test_gpu_1d_goldens(hiplib, name, flat):
+    print(f"case {name} flat={flat}", flush=True)
+    res, state, _ = run_problem(Hydro, name, flat=flat)
+    compare_to_npz(res, state, name + ("_flat" if flat else ""))
+
+
+@pytest.mark.parametrize("flat", [False, True], ids=["muscl", "flat"])
+@pytest.mark.parametrize("plane", ["xy", "xz", "yz"])
+def test_gpu_orszagtang_goldens(hiplib, plane, flat):
+    print(f"case orszagtang_{plane} flat={flat}", flush=True)
+    res, state, _ = run_problem(Hydro, f"orszagtang_{plane}", flat=flat)
+    compare_to_meta(res, state, meta()[f"orszagtang_{plane}" + ("_flat" if flat else "")])
+
+
+def test_gpu_sod1024_courant04(hiplib):
+    res, state, _ = run_problem(Hydro, "sodshock_x", n=1024, courant=0.4)
+    compare_to_npz(res, state, "sod1024_c04")
+
+
+def test_gpu_ot512_prefix(hiplib):
+    res, state, _ = run_problem(Hydro, "orszagtang_xy", n=512, outstyle=2, endstyle=2,
+                                tnlim=20)
+    compare_to_meta(res, state, meta()["ot512_20steps"])
+
+
+def test_gpu_matches_oracle_midrun(hiplib):
+    """Same inputs, 37 steps of OT 96^2: every field equal to the oracle."""
+    r1, s1, _ = run_problem(Hydro, "orszagtang_xy", n=96, max_steps=37)
+    r2, s2, _ = run_problem(Oracle, "orszagtang_xy", n=96, max_steps=37)
+    assert r1.time == r2.time
+    for k in s2:
+        assert np.array_equal(s1[k].view(np.uint64), s2[k].view(np.uint64)), k
