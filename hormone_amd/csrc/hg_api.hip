@@ -27,6 +27,13 @@ struct hg_ctx {
   double ch = 0.0;
   int err_i = 0, err_j = 0, err_k = 0, err_dir = 0;
   double *h_stage = nullptr;  // pinned staging for field transfers
+  // profiling (the reference's profiler categories, profiler.f90:5-51)
+  bool prof = false;
+  double prof_ms[HG_PROF_N] = {0};
+  long long prof_n[HG_PROF_N] = {0};
+  std::vector<hipEvent_t> ev;   // event pool
+  std::vector<int> ev_kind;     // kind of each recorded pair
+  int ev_used = 0;
 };
 
 #define HIPCHK(x)                                   \
@@ -103,6 +110,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
   D.mag = desc->mag_on ? 1 : 0;
   D.limiter = desc->flux_limiter;
   D.gamma = desc->gamma;
+  D.courant = desc->courant;
   D.nv = D.mag ? 8 : 5;
   D.i9wv = -1;
   if (D.mag && D.dim > 1) {
@@ -160,6 +168,8 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
   for (int w = 0; w < D.nw; ++w)
     if (hipMemset(D.W[w], 0, fb) != hipSuccess) return fail(HG_ERR_DEVICE);
   D.nblk_cfl = (int)((D.ncell + 255) / 256);
+  for (int a = 0; a < 3; ++a)
+    D.waves[a] = D.solve[a] ? (long)((nn[a] + 59) / 60) * (D.ncell / nn[a]) : 0;
   if ((st = dev_alloc(c, sizeof(double) * 2 * D.nblk_cfl, (void **)&D.part))) return fail(st);
   if ((st = dev_alloc(c, sizeof(Scalars), (void **)&D.sc))) return fail(st);
   if (hipMemset(D.sc, 0, sizeof(Scalars)) != hipSuccess) return fail(HG_ERR_DEVICE);
@@ -175,6 +185,7 @@ extern "C" int hg_destroy(hg_ctx *c) {
   if (!c) return HG_OK;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  for (hipEvent_t e : c->ev) hipEventDestroy(e);
   for (void *p : c->allocs) hipFree(p);
   if (c->h_sc) hipHostFree(c->h_sc);
   if (c->h_stage) hipHostFree(c->h_stage);
@@ -287,8 +298,79 @@ extern "C" int hg_download(hg_ctx *c, int32_t field, double *a) {
   return HG_OK;
 }
 
-static inline double fmin2(double a, double b) { return a < b ? a : b; }
+// ---- profiling -----------------------------------------------------------------
+static void prof_begin(hg_ctx *c, int kind) {
+  if (!c->prof) return;
+  if ((size_t)(2 * c->ev_used + 2) > c->ev.size()) {
+    for (int q = 0; q < 64; ++q) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      c->ev.push_back(e);
+    }
+  }
+  if ((size_t)c->ev_used >= c->ev_kind.size()) c->ev_kind.resize(c->ev_used + 64);
+  c->ev_kind[c->ev_used] = kind;
+  hipEventRecord(c->ev[2 * c->ev_used], c->stream);
+}
 
+static void prof_end(hg_ctx *c) {
+  if (!c->prof) return;
+  hipEventRecord(c->ev[2 * c->ev_used + 1], c->stream);
+  ++c->ev_used;
+}
+
+// after the stream has been synchronised: fold recorded pairs into totals
+static void prof_collect(hg_ctx *c) {
+  for (int q = 0; q < c->ev_used; ++q) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[2 * q], c->ev[2 * q + 1]) == hipSuccess) {
+      c->prof_ms[c->ev_kind[q]] += ms;
+      c->prof_n[c->ev_kind[q]] += 1;
+    }
+  }
+  c->ev_used = 0;
+}
+
+extern "C" int hg_profile(hg_ctx *c, int32_t enable) {
+  if (!c) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  c->prof = enable != 0;
+  for (int k = 0; k < HG_PROF_N; ++k) {
+    c->prof_ms[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
+  return HG_OK;
+}
+
+extern "C" int hg_profile_read(const hg_ctx *c, int32_t kind, double *ms_total,
+                               int64_t *launches, double *bytes_per_launch) {
+  if (!c || kind < 0 || kind >= HG_PROF_N) return HG_ERR_ARG;
+  const Dev &D = c->D;
+  const double cell = (double)D.ncell * sizeof(double);
+  double bytes = 0.0;
+  switch (kind) {
+    case HG_PROF_SWEEP:  // read the primitives once and write one divergence per direction
+      bytes = cell * (double)D.dim * (D.nw + D.nv);
+      break;
+    case HG_PROF_RK: {  // read U, U0, DU_d; write U, W (stage 1 writes U0, reads no U0)
+      bytes = cell * (double)(D.nv * (2 + D.dim) + D.nw);
+      break;
+    }
+    case HG_PROF_CFL:
+      bytes = cell * (double)(D.nw - (D.mag ? 2 : 1));
+      break;
+    default:
+      bytes = 0.0;
+  }
+  if (ms_total) *ms_total = c->prof_ms[kind];
+  if (launches) *launches = c->prof_n[kind];
+  if (bytes_per_launch) *bytes_per_launch = bytes;
+  return HG_OK;
+}
+
+// ---- the hot-path entry points ------------------------------------------------
 extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
                            int32_t outstyle, int32_t endstyle, double fixed_dt, double *dt_out,
                            double *ch_out, double *cgrav_out, double *dtgrav_out) {
@@ -301,27 +383,38 @@ extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
   }
   int st = ensure_eint(c);
   if (st) return st;
-  launch_cfl(c->D, c->stream);
+  const Dev &D = c->D;
+  launch_set_clock(D, time, t_out, t_end, 0.0, outstyle, endstyle, 0, 0, 1, 1, c->stream);
+  prof_begin(c, HG_PROF_CFL);
+  launch_cfl(D, c->stream);
+  prof_end(c);
+  launch_cfl_final(D, c->stream);
   HIPCHK(hipGetLastError());
   if ((st = check_error(c))) return st;
-  const double dtimin = c->h_sc->dtimin, cfmax = c->h_sc->cfmax;
-  // timestep.f90:74-89
-  double dt = c->desc.courant * dtimin;
-  if (outstyle == 1) {
-    double t_ref = t_out;  // the reference leaves t_ref unset unless endstyle==1
-    if (endstyle == 1) t_ref = fmin2(t_out, t_end);
-    if (t_ref - time - dt > 0.0 && t_ref - time - 2.0 * dt < 0.0)
-      dt = 0.5 * (t_ref - time);
-    else
-      dt = fmin2(dt, t_out - time);
-  }
-  if (endstyle == 1) dt = fmin2(dt, t_end - time);
-  c->ch = cfmax;
-  if (dt_out) *dt_out = dt;
-  if (ch_out) *ch_out = cfmax;
+  prof_collect(c);
+  c->ch = c->h_sc->ch;
+  if (dt_out) *dt_out = c->h_sc->dt;
+  if (ch_out) *ch_out = c->h_sc->ch;
   if (cgrav_out) *cgrav_out = 0.0;
   if (dtgrav_out) *dtgrav_out = 0.0;
   return HG_OK;
+}
+
+static void enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
+  const Dev &D = c->D;
+  const int rk = c->desc.rktype;
+  for (int stage = 1; stage <= rk; ++stage) {
+    double faco, fact, facn;
+    runge_coeff(stage, rk, &faco, &fact, &facn);
+    prof_begin(c, HG_PROF_SWEEP);
+    launch_sweep(D, stage, c->stream);
+    prof_end(c);
+    const bool last = stage == rk;
+    prof_begin(c, HG_PROF_RK);
+    launch_rk(D, stage, faco, fact, facn, fuse_tail && last, fuse_tail && last && damp > 0.0,
+              damp, c->stream);
+    prof_end(c);
+  }
 }
 
 extern "C" int hg_hydro_step(hg_ctx *c, double dt, double ch) {
@@ -329,27 +422,12 @@ extern "C" int hg_hydro_step(hg_ctx *c, double dt, double ch) {
   HIPCHK(hipSetDevice(c->device));
   int st = ensure_eint(c);
   if (st) return st;
-  const Dev &D = c->D;
-  launch_set_scalars(D, dt, ch, c->stream);
-  const int rk = c->desc.rktype;
-  for (int stage = 1; stage <= rk; ++stage) {
-    double faco, fact, facn;  // get_runge_coeff, rungekutta.f90:146-183
-    if (rk == 3) {
-      if (stage == 1) { faco = 1.0; fact = 1.0; facn = 0.0; }
-      else if (stage == 2) { faco = 0.75; fact = 0.25; facn = fact; }
-      else { faco = 1.0 / 3.0; fact = 2.0 / 3.0; facn = fact; }
-    } else if (rk == 2) {
-      if (stage == 1) { faco = 1.0; fact = 1.0; facn = 0.0; }
-      else { faco = 0.5; fact = 0.5; facn = fact; }
-    } else {
-      faco = 1.0; fact = 1.0; facn = 0.0;
-    }
-    for (int a = 0; a < 3; ++a)
-      if (D.solve[a]) launch_sweep(D, a, stage, c->stream);
-    launch_rk(D, stage, faco, fact, facn, c->stream);
-  }
+  launch_set_step(c->D, dt, ch, c->stream);
+  enqueue_stages(c, false, 0.0);
   HIPCHK(hipGetLastError());
-  return check_error(c);
+  st = check_error(c);
+  prof_collect(c);
+  return st;
 }
 
 extern "C" int hg_phidamp(hg_ctx *c, double factor) {
@@ -359,6 +437,46 @@ extern "C" int hg_phidamp(hg_ctx *c, double factor) {
   launch_phidamp(c->D, factor, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
+  return HG_OK;
+}
+
+extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, double dt_out,
+                      double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
+                      int64_t tn_out, int64_t max_steps, double damp_factor,
+                      int32_t *stop_reason) {
+  if (!c || !time || !tn || !t_out || max_steps < 1) return HG_ERR_ARG;
+  if (outstyle == 2 && tn_out < 1) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  int st = ensure_eint(c);
+  if (st) return st;
+  const Dev &D = c->D;
+  const bool damp = D.mag && D.dim > 1 && damp_factor > 0.0;
+  launch_set_clock(D, *time, *t_out, t_end, dt_out, outstyle, endstyle, *tn, tnlim, tn_out,
+                   max_steps, c->stream);
+  prof_begin(c, HG_PROF_CFL);
+  launch_cfl(D, c->stream);
+  prof_end(c);
+  const int batch = 16;
+  int reason = STOP_NONE;
+  for (;;) {
+    for (int s = 0; s < batch; ++s) {
+      launch_cfl_final(D, c->stream);
+      enqueue_stages(c, true, damp ? damp_factor : 0.0);
+      launch_step_end(D, c->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_sc, D.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    prof_collect(c);
+    reason = c->h_sc->stop;
+    if (reason != STOP_NONE) break;
+  }
+  *time = c->h_sc->time;
+  *tn = c->h_sc->tn;
+  *t_out = c->h_sc->t_out;
+  c->ch = c->h_sc->ch;
+  if (stop_reason) *stop_reason = reason;
+  if (reason == STOP_ERROR) return check_error(c);
   return HG_OK;
 }
 

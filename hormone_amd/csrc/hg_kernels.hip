@@ -92,14 +92,10 @@ __device__ __forceinline__ double up1(double x) { return __shfl_up(x, 1, 64); }
 __device__ __forceinline__ double dn1(double x) { return __shfl_down(x, 1, 64); }
 
 template <int DIR, bool MAG, int LIM>
-__global__ __launch_bounds__(256) void k_sweep(Dev D, int stage) {
+__device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage) {
   const Axis &A = D.ax[DIR];
-  const int lane = threadIdx.x & 63;
-  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int n = A.n;
-  const int nseg = (n + SEG - 1) / SEG;
   const long nlines = D.ncell / n;
-  if (wave >= (long)nseg * nlines) return;  // uniform per wavefront
   const long line = wave % nlines;
   const int seg = (int)(wave / nlines);
   long base;
@@ -258,12 +254,101 @@ __global__ __launch_bounds__(256) void k_sweep(Dev D, int stage) {
   }
 }
 
-// ---- RK update + primitive recovery ------------------------------------
+// One launch covers every solved direction: wavefronts [0, waves[0]) sweep
+// x, the next waves[1] sweep y, then z.  Blocks are dispatched in order, so
+// a compute unit mostly runs one direction's code at a time.
+template <bool MAG, int LIM>
+__global__ __launch_bounds__(256) void k_sweep(Dev D, int stage) {
+  if (D.sc->stop) return;
+  const int lane = threadIdx.x & 63;
+  long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave < D.waves[0]) {
+    sweep_line<0, MAG, LIM>(D, wave, lane, stage);
+    return;
+  }
+  wave -= D.waves[0];
+  if (wave < D.waves[1]) {
+    sweep_line<1, MAG, LIM>(D, wave, lane, stage);
+    return;
+  }
+  wave -= D.waves[1];
+  if (wave < D.waves[2]) sweep_line<2, MAG, LIM>(D, wave, lane, stage);
+}
+
+// ---- CFL condition of one cell (timestep.f90:35-57, dti_cell :125-183) ----
 template <bool MAG>
+__device__ __forceinline__ void cfl_cell(const Dev &D, long m, double d, double eint,
+                                         double v1, double v2, double v3, double b1, double b2,
+                                         double b3, double &dti, double &cfm) {
+  const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2),
+            k = (int)(m / ((long)D.n1 * D.n2));
+  const double gam = D.gamma;
+  const double p = (gam - 1.0) * eint;
+  const double cs = sqrt(gam * p / d);
+  double cf1, cf2, cf3;
+  if (MAG) {
+    cf1 = get_cf(d, cs, b1, b2, b3) + fabs(v1);
+    cf2 = get_cf(d, cs, b2, b1, b3) + fabs(v2);
+    cf3 = get_cf(d, cs, b3, b1, b2) + fabs(v3);
+  } else {
+    cf1 = cs + fabs(v1);
+    cf2 = cs + fabs(v2);
+    cf3 = cs + fabs(v3);
+  }
+  const double x1 = D.ax[0].dxi[i], x2 = D.ax[1].dxi[j], x3 = D.ax[2].dxi[k];
+  const double dvol = x1 * x2 * x3;  // metric.f90:44-47
+  const double sa1 = x2 * x3, sa2 = x1 * x3, sa3 = x1 * x2;
+  const double o1 = D.solve[0] ? 1.0 : 0.0, o2 = D.solve[1] ? 1.0 : 0.0,
+               o3 = D.solve[2] ? 1.0 : 0.0;
+  dti = dvol / (cf1 * o1 * (sa1 + sa1) + cf2 * o2 * (sa2 + sa2) + cf3 * o3 * (sa3 + sa3));
+  cfm = mx3(cf1, cf2, cf3);
+  if (eint <= 0.0 || d <= 0.0) atomicMin(&D.sc->err, errkey(0, 1, 0, i, j, k, D));
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = mn2(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = mx2(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// block (256 threads) reduction of (min dti, max cf) into D.part[blockIdx]
+__device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, double cfm) {
+  __shared__ double smin[4], smax[4];
+  dti = wave_min(dti);
+  cfm = wave_max(cfm);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    smin[w] = dti;
+    smax[w] = cfm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = smin[0], b = smax[0];
+    for (int q = 1; q < 4; ++q) {
+      a = mn2(a, smin[q]);
+      b = mx2(b, smax[q]);
+    }
+    D.part[2 * blockIdx.x] = a;
+    D.part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// ---- RK update + primitive recovery ------------------------------------
+// CFL: also evaluate the next step's CFL condition from the new primitives
+// (the reference's timestep reads exactly these values at the start of the
+// next step).  DAMP: apply phidamp (source.f90:239-260) to the new psi.
+template <bool MAG, bool CFL, bool DAMP>
 __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
-                                            double facn) {
+                                            double facn, double damp) {
+  if (D.sc->stop) return;
   const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= D.ncell) return;
+  double dti = __builtin_inf(), cfm = 0.0;
+  if (m < D.ncell) {
   const double ft = fact * D.sc->dt;
   double un[NVMAX];
 #pragma unroll
@@ -302,7 +387,7 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
     D.W[WB1][m] = b1;
     D.W[WB2][m] = b2;
     D.W[WB3][m] = b3;
-    if (D.i9wv >= 0) D.W[WPHI][m] = un[8];
+    if (D.i9wv >= 0) D.W[WPHI][m] = DAMP ? un[8] * damp : un[8];
   }
   const double eint = get_eint(MAG, e, d, v1, v2, v3, b1, b2, b3);
   D.W[WEINT][m] = eint;
@@ -310,6 +395,9 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
     const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
     atomicMin(&D.sc->err, errkey(stage, 1, 0, i, j, k, D));
   }
+  if (CFL) cfl_cell<MAG>(D, m, d, eint, v1, v2, v3, b1, b2, b3, dti, cfm);
+  }
+  if (CFL) cfl_block_reduce(D, dti, cfm);
 }
 
 // ---- pressure(): eint from e, d, v, B after an upload ---------------------
@@ -332,70 +420,29 @@ __global__ __launch_bounds__(256) void k_pressure(Dev D) {
   }
 }
 
-// ---- CFL condition (timestep.f90:35-57, dti_cell :125-183) ----------------
-__device__ __forceinline__ double wave_min(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = mn2(x, __shfl_xor(x, o, 64));
-  return x;
-}
-__device__ __forceinline__ double wave_max(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = mx2(x, __shfl_xor(x, o, 64));
-  return x;
-}
-
+// ---- CFL condition as its own pass (the reference's timestep) ------------
 template <bool MAG>
 __global__ __launch_bounds__(256) void k_cfl(Dev D) {
-  __shared__ double smin[4], smax[4];
   const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double dti = __builtin_inf(), cfm = 0.0;
   if (m < D.ncell) {
-    const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
-    const double d = D.W[WD][m], eint = D.W[WEINT][m];
-    const double gam = D.gamma;
-    const double p = (gam - 1.0) * eint;
-    const double cs = sqrt(gam * p / d);
-    const double av1 = fabs(D.W[WV1][m]), av2 = fabs(D.W[WV2][m]), av3 = fabs(D.W[WV3][m]);
-    double cf1, cf2, cf3;
+    double b1 = 0, b2 = 0, b3 = 0;
     if (MAG) {
-      const double b1 = D.W[WB1][m], b2 = D.W[WB2][m], b3 = D.W[WB3][m];
-      cf1 = get_cf(d, cs, b1, b2, b3) + av1;
-      cf2 = get_cf(d, cs, b2, b1, b3) + av2;
-      cf3 = get_cf(d, cs, b3, b1, b2) + av3;
-    } else {
-      cf1 = cs + av1;
-      cf2 = cs + av2;
-      cf3 = cs + av3;
+      b1 = D.W[WB1][m];
+      b2 = D.W[WB2][m];
+      b3 = D.W[WB3][m];
     }
-    const double x1 = D.ax[0].dxi[i], x2 = D.ax[1].dxi[j], x3 = D.ax[2].dxi[k];
-    const double dvol = x1 * x2 * x3;  // metric.f90:44-47
-    const double sa1 = x2 * x3, sa2 = x1 * x3, sa3 = x1 * x2;
-    const double o1 = D.solve[0] ? 1.0 : 0.0, o2 = D.solve[1] ? 1.0 : 0.0,
-                 o3 = D.solve[2] ? 1.0 : 0.0;
-    dti = dvol / (cf1 * o1 * (sa1 + sa1) + cf2 * o2 * (sa2 + sa2) + cf3 * o3 * (sa3 + sa3));
-    cfm = mx3(cf1, cf2, cf3);
-    if (eint <= 0.0 || d <= 0.0) atomicMin(&D.sc->err, errkey(0, 1, 0, i, j, k, D));
+    cfl_cell<MAG>(D, m, D.W[WD][m], D.W[WEINT][m], D.W[WV1][m], D.W[WV2][m], D.W[WV3][m],
+                  b1, b2, b3, dti, cfm);
   }
-  dti = wave_min(dti);
-  cfm = wave_max(cfm);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    smin[w] = dti;
-    smax[w] = cfm;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = smin[0], b = smax[0];
-    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
-      a = mn2(a, smin[q]);
-      b = mx2(b, smax[q]);
-    }
-    D.part[2 * blockIdx.x] = a;
-    D.part[2 * blockIdx.x + 1] = b;
-  }
+  cfl_block_reduce(D, dti, cfm);
 }
 
+// Final CFL reduction and dt (timestep.f90:74-89): dt = courant*min(dti),
+// clipped to t_out / t_end as the reference does; ch = max fast speed.
 __global__ __launch_bounds__(256) void k_cfl_final(Dev D) {
+  Scalars *sc = D.sc;
+  if (sc->stop) return;
   __shared__ double smin[4], smax[4];
   double a = __builtin_inf(), b = 0.0;
   for (int q = threadIdx.x; q < D.nblk_cfl; q += blockDim.x) {
@@ -417,14 +464,82 @@ __global__ __launch_bounds__(256) void k_cfl_final(Dev D) {
       a = mn2(a, smin[q]);
       b = mx2(b, smax[q]);
     }
-    D.sc->dtimin = a;
-    D.sc->cfmax = b;
+    sc->dtimin = a;
+    sc->cfmax = b;
+    double dt = D.courant * a;
+    const double time = sc->time, t_out = sc->t_out, t_end = sc->t_end;
+    if (sc->outstyle == 1) {
+      double t_ref = t_out;  // left unset by the reference unless endstyle == 1
+      if (sc->endstyle == 1) t_ref = mn2(t_out, t_end);
+      if (t_ref - time - dt > 0.0 && t_ref - time - 2.0 * dt < 0.0)
+        dt = 0.5 * (t_ref - time);
+      else
+        dt = mn2(dt, t_out - time);
+    }
+    if (sc->endstyle == 1) dt = mn2(dt, t_end - time);
+    sc->dt = dt;
+    sc->ch = b;
   }
 }
 
-__global__ void k_set_scalars(Scalars *sc, double dt, double ch) {
+// End of one main-loop iteration (hormone.f90:147-177): advance the clock,
+// then the end and output checks in the reference's order.
+__global__ void k_step_end(Dev D) {
+  Scalars *sc = D.sc;
+  if (sc->stop) return;
+  if (sc->err) {
+    sc->stop = STOP_ERROR;
+    return;
+  }
+  sc->time = sc->time + sc->dt;
+  sc->tn = sc->tn + 1;
+  sc->steps_left = sc->steps_left - 1;
+  if (sc->endstyle == 1) {
+    if (sc->time >= sc->t_end) {
+      sc->stop = STOP_END;
+      return;
+    }
+  } else if (sc->endstyle == 2) {
+    if (sc->tn >= sc->tnlim) {
+      sc->stop = STOP_END;
+      return;
+    }
+  }
+  if (sc->outstyle == 1) {
+    if (sc->time >= sc->t_out) {
+      sc->t_out = sc->t_out + sc->dt_out;
+      sc->stop = STOP_OUTPUT;
+      return;
+    }
+  } else if (sc->outstyle == 2) {
+    if (sc->tn != 0 && sc->tn % sc->tn_out == 0) {
+      sc->stop = STOP_OUTPUT;
+      return;
+    }
+  }
+  if (sc->steps_left <= 0) sc->stop = STOP_MAXSTEPS;
+}
+
+__global__ void k_set_step(Scalars *sc, double dt, double ch) {
   sc->dt = dt;
   sc->ch = ch;
+  sc->stop = STOP_NONE;
+}
+
+__global__ void k_set_clock(Scalars *sc, double time, double t_out, double t_end,
+                            double dt_out, int outstyle, int endstyle, long long tn,
+                            long long tnlim, long long tn_out, long long steps) {
+  sc->time = time;
+  sc->t_out = t_out;
+  sc->t_end = t_end;
+  sc->dt_out = dt_out;
+  sc->outstyle = outstyle;
+  sc->endstyle = endstyle;
+  sc->tn = tn;
+  sc->tnlim = tnlim;
+  sc->tn_out = tn_out;
+  sc->steps_left = steps;
+  sc->stop = STOP_NONE;
 }
 
 __global__ __launch_bounds__(256) void k_phidamp(Dev D, double factor) {
@@ -462,48 +577,72 @@ __global__ void k_kat(int which, int n, const double *in, double *out) {
 // ---- launchers ----------------------------------------------------------------
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+void runge_coeff(int stage, int rk, double *faco, double *fact, double *facn) {
+  if (rk == 3) {
+    if (stage == 1) { *faco = 1.0; *fact = 1.0; *facn = 0.0; }
+    else if (stage == 2) { *faco = 0.75; *fact = 0.25; *facn = *fact; }
+    else { *faco = 1.0 / 3.0; *fact = 2.0 / 3.0; *facn = *fact; }
+  } else if (rk == 2) {
+    if (stage == 1) { *faco = 1.0; *fact = 1.0; *facn = 0.0; }
+    else { *faco = 0.5; *fact = 0.5; *facn = *fact; }
+  } else {
+    *faco = 1.0; *fact = 1.0; *facn = 0.0;
+  }
+}
+
 void launch_pressure(const Dev &D, hipStream_t s) {
   if (D.mag) k_pressure<true><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
   else k_pressure<false><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
 }
 
-void launch_set_scalars(const Dev &D, double dt, double ch, hipStream_t s) {
-  k_set_scalars<<<1, 1, 0, s>>>(D.sc, dt, ch);
+void launch_set_step(const Dev &D, double dt, double ch, hipStream_t s) {
+  k_set_step<<<1, 1, 0, s>>>(D.sc, dt, ch);
 }
 
-template <int DIR>
-static void sweep_dir(const Dev &D, int stage, hipStream_t s) {
-  const int n = D.ax[DIR].n;
-  const long waves = (long)((n + SEG - 1) / SEG) * (D.ncell / n);
-  const unsigned g = nblk(waves, 4);
+void launch_set_clock(const Dev &D, double time, double t_out, double t_end, double dt_out,
+                      int outstyle, int endstyle, long long tn, long long tnlim,
+                      long long tn_out, long long steps, hipStream_t s) {
+  k_set_clock<<<1, 1, 0, s>>>(D.sc, time, t_out, t_end, dt_out, outstyle, endstyle, tn,
+                              tnlim, tn_out, steps);
+}
+
+void launch_sweep(const Dev &D, int stage, hipStream_t s) {
+  const unsigned g = nblk(D.waves[0] + D.waves[1] + D.waves[2], 4);
   if (D.mag) {
-    if (D.limiter == LIM_FLAT) k_sweep<DIR, true, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
-    else if (D.limiter == LIM_MINMOD) k_sweep<DIR, true, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
-    else k_sweep<DIR, true, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+    if (D.limiter == LIM_FLAT) k_sweep<true, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
+    else if (D.limiter == LIM_MINMOD) k_sweep<true, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
+    else k_sweep<true, LIM_MC><<<g, 256, 0, s>>>(D, stage);
   } else {
-    if (D.limiter == LIM_FLAT) k_sweep<DIR, false, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
-    else if (D.limiter == LIM_MINMOD) k_sweep<DIR, false, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
-    else k_sweep<DIR, false, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+    if (D.limiter == LIM_FLAT) k_sweep<false, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
+    else if (D.limiter == LIM_MINMOD) k_sweep<false, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
+    else k_sweep<false, LIM_MC><<<g, 256, 0, s>>>(D, stage);
   }
 }
 
-void launch_sweep(const Dev &D, int dir, int stage, hipStream_t s) {
-  if (dir == 0) sweep_dir<0>(D, stage, s);
-  else if (dir == 1) sweep_dir<1>(D, stage, s);
-  else sweep_dir<2>(D, stage, s);
+template <bool MAG>
+static void rk_mag(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
+                   bool damp, double df, hipStream_t s) {
+  const unsigned g = nblk(D.ncell, 256);
+  if (cfl && damp) k_rk<MAG, true, true><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
+  else if (cfl) k_rk<MAG, true, false><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
+  else if (damp) k_rk<MAG, false, true><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
+  else k_rk<MAG, false, false><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
 }
 
-void launch_rk(const Dev &D, int stage, double faco, double fact, double facn,
-               hipStream_t s) {
-  if (D.mag) k_rk<true><<<nblk(D.ncell, 256), 256, 0, s>>>(D, stage, faco, fact, facn);
-  else k_rk<false><<<nblk(D.ncell, 256), 256, 0, s>>>(D, stage, faco, fact, facn);
+void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
+               bool damp, double damp_factor, hipStream_t s) {
+  if (D.mag) rk_mag<true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s);
+  else rk_mag<false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s);
 }
 
 void launch_cfl(const Dev &D, hipStream_t s) {
   if (D.mag) k_cfl<true><<<D.nblk_cfl, 256, 0, s>>>(D);
   else k_cfl<false><<<D.nblk_cfl, 256, 0, s>>>(D);
-  k_cfl_final<<<1, 256, 0, s>>>(D);
 }
+
+void launch_cfl_final(const Dev &D, hipStream_t s) { k_cfl_final<<<1, 256, 0, s>>>(D); }
+
+void launch_step_end(const Dev &D, hipStream_t s) { k_step_end<<<1, 1, 0, s>>>(D); }
 
 void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
   k_phidamp<<<nblk(D.ncell, 256), 256, 0, s>>>(D, factor);

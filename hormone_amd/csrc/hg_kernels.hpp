@@ -20,38 +20,45 @@ namespace hg {
 enum { WD = 0, WV1, WV2, WV3, WE, WEINT, WB1, WB2, WB3, WPHI, NWMAX };
 enum { NVMAX = 9 };
 
-// device-resident scalars shared by the kernels of one context
+// stop reasons of the device-resident main loop
+enum { STOP_NONE = 0, STOP_END = 1, STOP_OUTPUT = 2, STOP_MAXSTEPS = 3, STOP_ERROR = 4 };
+
+// Device-resident scalars of one context: the hydro step inputs (dt, ch),
+// the CFL reduction results and the main-loop clock of hormone.f90:124-179.
 struct Scalars {
-  double dt, ch;          // hydro step inputs
-  double dtimin, cfmax;   // CFL reduction results
-  unsigned long long err; // first error key (0 = none), see errkey()
-  double pad[3];
+  double dt, ch;
+  double dtimin, cfmax;
+  double time, t_out, t_end, dt_out;
+  long long tn, tnlim, tn_out, steps_left;
+  int outstyle, endstyle, stop, pad0;
+  unsigned long long err;  // first error key (0 = none), see errkey()
 };
 
 // geometry of one sweep direction, indices 0-based interior (ghosts -2,-1,n,n+1)
 struct Axis {
   const double *x, *xi, *idx, *idxi, *dx, *dxi;  // base-shifted: valid for c = -2..n+1
   int n;
-  long stride;      // element stride of this axis in the SoA arrays
-  int sc_lo, sc_hi; // scalar BC codes (bcNis, bcNos)
-  int vc_lo, vc_hi; // vector BC codes (bcNiv, bcNov)
+  long stride;       // element stride of this axis in the SoA arrays
+  int sc_lo, sc_hi;  // scalar BC codes (bcNis, bcNos)
+  int vc_lo, vc_hi;  // vector BC codes (bcNiv, bcNov)
 };
 
 struct Dev {
-  int n1, n2, n3;          // interior extents
+  int n1, n2, n3;  // interior extents
   long ncell;
-  int nv, nw;              // conserved / primitive field counts
+  int nv, nw;      // conserved / primitive field counts
   int mag, dim, limiter;
   int solve[3];
-  int i9wv;                // index of the GLM variable or -1
-  double gamma;
+  int i9wv;        // index of the GLM variable or -1
+  double gamma, courant;
   Axis ax[3];
   double *W[NWMAX];
   double *U[NVMAX], *U0[NVMAX];
   double *DU[3][NVMAX];
-  double *part;            // CFL partials: 2 doubles per block
+  double *part;    // CFL partials: 2 doubles per block of 256 cells
   Scalars *sc;
   int nblk_cfl;
+  long waves[3];   // sweep wavefronts per direction (0 if not solved)
 };
 
 // Error keys: the smallest key is the first error in the reference's serial
@@ -66,12 +73,20 @@ __host__ __device__ inline unsigned long long errkey(int stage, int phase, int d
   return (key << 42) + cell + 1;
 }
 
+// RK coefficients (faco, fact, facn) of get_runge_coeff, rungekutta.f90:146-183
+void runge_coeff(int stage, int rktype, double *faco, double *fact, double *facn);
+
 void launch_pressure(const Dev &D, hipStream_t s);
-void launch_set_scalars(const Dev &D, double dt, double ch, hipStream_t s);
-void launch_sweep(const Dev &D, int dir, int stage, hipStream_t s);
-void launch_rk(const Dev &D, int stage, double faco, double fact, double facn,
-               hipStream_t s);
+void launch_set_step(const Dev &D, double dt, double ch, hipStream_t s);
+void launch_set_clock(const Dev &D, double time, double t_out, double t_end, double dt_out,
+                      int outstyle, int endstyle, long long tn, long long tnlim,
+                      long long tn_out, long long steps, hipStream_t s);
+void launch_sweep(const Dev &D, int stage, hipStream_t s);
+void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
+               bool damp, double damp_factor, hipStream_t s);
 void launch_cfl(const Dev &D, hipStream_t s);
+void launch_cfl_final(const Dev &D, hipStream_t s);
+void launch_step_end(const Dev &D, hipStream_t s);
 void launch_phidamp(const Dev &D, double factor, hipStream_t s);
 void launch_kat(int which, int n, const double *in, double *out, hipStream_t s);
 

@@ -58,9 +58,15 @@ class RunResult:
 
 
 def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
-        record_dt: bool = False) -> RunResult:
-    """Main loop of hormone.f90:124-179 restricted to the hydro path."""
+        record_dt: bool = False, device_loop: bool = False) -> RunResult:
+    """Main loop of hormone.f90:124-179 restricted to the hydro path.
+
+    ``device_loop`` runs it on the device (hg_run) when the engine offers it;
+    the state is the same bit for bit.
+    """
     t_end, dt_out, t_out = prm.times()
+    if device_loop and hasattr(eng, "run_loop") and prm.fixed_dt <= 0 and not record_dt:
+        return _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out)
     tim = 0.0
     tn = 0
     damp = math.exp(-prm.alpha9wave * prm.courant)  # source.f90:253
@@ -89,6 +95,22 @@ def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
             break
     wall = _time.perf_counter() - t0
     return RunResult(tn, tim, wall, tn, dts)
+
+
+def _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out) -> RunResult:
+    damp = math.exp(-prm.alpha9wave * prm.courant) if (prm.mag_on and g.dim > 1) else 0.0
+    tim, tn = 0.0, 0
+    budget = max_steps if max_steps is not None else 1 << 62
+    t0 = _time.perf_counter()
+    while True:
+        tim, tn, t_out, reason = eng.run_loop(tim, tn, t_out, dt_out, t_end, prm.outstyle,
+                                              prm.endstyle, prm.tnlim, prm.tn_out, budget - tn,
+                                              damp)
+        # reason 2: an output is due (none is written here); 1: end; 3: max steps
+        if reason != 2 or tn >= budget:
+            break
+    wall = _time.perf_counter() - t0
+    return RunResult(tn, tim, wall, tn)
 
 
 def fetch(eng, g: Grid, fields) -> dict:

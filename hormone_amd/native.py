@@ -99,6 +99,12 @@ def _load():
     lib.hg_hydro_step.argtypes = [vp, C.c_double, C.c_double]
     lib.hg_phidamp.argtypes = [vp, C.c_double]
     lib.hg_last_error.argtypes = [vp] + [C.POINTER(C.c_int32)] * 4
+    lib.hg_run.argtypes = [vp, _dp, C.POINTER(C.c_int64), _dp, C.c_double, C.c_double,
+                           C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int64,
+                           C.c_double, C.POINTER(C.c_int32)]
+    lib.hg_profile.argtypes = [vp, C.c_int32]
+    lib.hg_profile_read.argtypes = [vp, C.c_int32, _dp, C.POINTER(C.c_int64), _dp]
+    lib.hg_kat.argtypes = [C.c_int32, C.c_int32, _dp, _dp]
     return lib
 
 
@@ -116,7 +122,11 @@ def exported_symbols():
     """Entry points declared in include/hormone_hip.h."""
     return ["hg_version", "hg_create", "hg_destroy", "hg_ufnmax", "hg_upload",
             "hg_download", "hg_timestep", "hg_hydro_step", "hg_phidamp",
-            "hg_last_error"]
+            "hg_last_error", "hg_run", "hg_profile", "hg_profile_read", "hg_kat"]
+
+
+STOP_END, STOP_OUTPUT, STOP_MAXSTEPS, STOP_ERROR = 1, 2, 3, 4
+PROF_SWEEP, PROF_RK, PROF_CFL = 0, 1, 2
 
 
 class Hydro:
@@ -172,3 +182,21 @@ class Hydro:
 
     def phidamp(self, factor):
         self._check(self._lib.hg_phidamp(self.h, factor), "hg_phidamp")
+
+    def run_loop(self, time, tn, t_out, dt_out, t_end, outstyle, endstyle, tnlim, tn_out,
+                 max_steps, damp_factor):
+        """Device-resident main loop (hg_run); returns (time, tn, t_out, reason)."""
+        t, n, to, r = C.c_double(time), C.c_int64(tn), C.c_double(t_out), C.c_int32(0)
+        self._check(self._lib.hg_run(self.h, C.byref(t), C.byref(n), C.byref(to), dt_out,
+                                     t_end, outstyle, endstyle, tnlim, tn_out, max_steps,
+                                     damp_factor, C.byref(r)), "hg_run")
+        return t.value, n.value, to.value, r.value
+
+    def profile(self, enable=True):
+        self._check(self._lib.hg_profile(self.h, int(enable)), "hg_profile")
+
+    def profile_read(self, kind):
+        ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+        self._check(self._lib.hg_profile_read(self.h, kind, C.byref(ms), C.byref(n),
+                                              C.byref(b)), "hg_profile_read")
+        return ms.value, n.value, b.value

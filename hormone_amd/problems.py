@@ -28,6 +28,7 @@ class Params:
     outstyle: int = 1
     endstyle: int = 1
     tnlim: int = 0
+    tn_out: int = 1
     t_end: float = 0.0       # in dt_unit
     dt_out: float = 1.0      # in dt_unit
     dt_unit: str = "s"

@@ -151,6 +151,30 @@ int hg_hydro_step(hg_ctx *ctx, double dt, double ch);
    the caller evaluates on the host. */
 int hg_phidamp(hg_ctx *ctx, double factor);
 
+/* Device-resident main loop; no single counterpart in the reference: it
+   runs the hydro-only iteration of hormone.f90:124-179 -- timestep, hydro
+   step, phidamp (when damp_factor > 0, mag_on and dim > 1), time += dt,
+   tn += 1, then the end check (endstyle 1: time >= t_end; 2: tn >= tnlim)
+   and the output check (outstyle 1: time >= t_out, then t_out += dt_out;
+   2: mod(tn, tn_out) == 0) -- without returning to the host between steps.
+   Returns when the run ends, an output is due (the caller writes it and calls
+   again), max_steps steps have been taken, or an error occurs; *stop_reason
+   is 1, 2, 3 or 4 respectively.  time, tn and t_out are updated in place.
+   Gives the same state, bit for bit, as the per-step entry points. */
+int hg_run(hg_ctx *ctx, double *time, int64_t *tn, double *t_out, double dt_out,
+           double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
+           int64_t tn_out, int64_t max_steps, double damp_factor,
+           int32_t *stop_reason);
+
+/* Kernel timing with HIP events on the context's stream, by category
+   (cf. the reference's profiler, profiler.f90:5-51).  hg_profile(ctx, 1)
+   resets and enables; hg_profile_read returns the summed milliseconds, the
+   number of launches and the algorithmic HBM bytes of one launch. */
+enum { HG_PROF_SWEEP = 0, HG_PROF_RK = 1, HG_PROF_CFL = 2, HG_PROF_N = 3 };
+int hg_profile(hg_ctx *ctx, int32_t enable);
+int hg_profile_read(const hg_ctx *ctx, int32_t kind, double *ms_total,
+                    int64_t *launches, double *bytes_per_launch);
+
 /* Cell (Fortran indices) and direction (1..3, 0 for a cell error) of the
    last HG_ERR_FACE_EOS / HG_ERR_NEG_EINT. */
 int hg_last_error(const hg_ctx *ctx, int32_t *i, int32_t *j, int32_t *k,

@@ -107,6 +107,7 @@ def read_header(path):
 
 
 def reference_runs(meta: dict, long: bool):
+    meta.pop("ot512_t05", None)
     # Sod 1024 cells, courant 0.4, to t = 0.2 (200 ms)
     p = ("&gridcon ie=1024 /\n&out_con output_ascii=.false. /\n"
          "&simucon courant=0.4d0 /\n")
@@ -124,11 +125,31 @@ def reference_runs(meta: dict, long: bool):
     meta["ot512_20steps"] = summary(snap, FIELDS_MAG)
     meta["ot512_20steps"]["source"] = "reference run: orszagtang_xy ie=je=512, tnlim=20"
     if long:
+        # The full run to t=0.5 stops inside the reference: internalenergy
+        # finds e_int <= 0 (eos.f90:833-839).  Record where, and the state
+        # 50 steps before.
         p = ("&gridcon ie=512 je=512 /\n&out_con output_ascii=.false. /\n")
+        try:
+            run_reference("orszagtang_xy", p)
+            fail = None
+        except subprocess.CalledProcessError:
+            fail = True
+        log = open("/tmp/hormone_fixture_run/log.txt").read()
+        import re
+        steps = [int(m) for m in re.findall(r"tn =\s+(\d+)", log)]
+        cell = re.search(r"i=\s*(\d+), j=\s*(\d+), k=\s*(\d+)", log)
+        meta["ot512_t05_failure"] = {
+            "stops": bool(fail or "Negative internal energy" in log),
+            "message": "Error in internalenergy, Negative internal energy",
+            "completed_steps": steps[-1] if steps else None,
+            "cell": [int(cell.group(q)) for q in (1, 2, 3)] if cell else None,
+            "source": "reference run: orszagtang_xy ie=je=512 to t=0.5 (stops early)"}
+        p = ("&gridcon ie=512 je=512 /\n&out_con outstyle=2 endstyle=2 tnlim=2200 "
+             "tn_out=2200 output_ascii=.false. /\n")
         b = run_reference("orszagtang_xy", p)
         snap = read_bin(b, (1, 512, 512), True)
-        meta["ot512_t05"] = summary(snap, FIELDS_MAG)
-        meta["ot512_t05"]["source"] = "reference run: orszagtang_xy ie=je=512 to t=0.5"
+        meta["ot512_2200steps"] = summary(snap, FIELDS_MAG)
+        meta["ot512_2200steps"]["source"] = "reference run: orszagtang_xy ie=je=512, tnlim=2200"
 
 
 def main():

@@ -109,3 +109,39 @@ def test_gpu_matches_oracle_midrun(hiplib):
     assert r1.time == r2.time
     for k in s2:
         assert np.array_equal(s1[k].view(np.uint64), s2[k].view(np.uint64)), k
+
+
+@pytest.mark.parametrize("name", ["sodshock_x", "briowushock_y", "orszagtang_xy", "orszagtang_yz"])
+def test_gpu_device_loop_goldens(hiplib, name):
+    """The device-resident main loop (hg_run) reproduces the goldens too."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print(f"case device-loop {name}", flush=True)
+    prm, g, st = P.make_problem(name)
+    eng = Hydro(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, device_loop=True)
+    fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+    out = D.fetch(eng, g, fields)
+    names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
+    state = {names[i]: out[f] for i, f in enumerate(fields)}
+    if name.startswith("orszagtang"):
+        compare_to_meta(res, state, meta()[name])
+    else:
+        compare_to_npz(res, state, name)
+
+
+def test_gpu_ot512_full_run_device_loop(hiplib):
+    """The benchmark workload to t=0.5 equals the reference program's run."""
+    entry = meta().get("ot512_t05")
+    if entry is None or entry["tn"] == 0:
+        pytest.skip("reference run fixture not generated")
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    prm, g, st = P.make_problem("orszagtang_xy", n=512)
+    eng = Hydro(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, device_loop=True)
+    out = D.fetch(eng, g, D.SNAP_FIELDS_MAG)
+    names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
+    compare_to_meta(res, {names[i]: out[f] for i, f in enumerate(D.SNAP_FIELDS_MAG)}, entry)
