@@ -8,9 +8,11 @@ t = 0.5.  Synthetic initial data (the closed-form vortex); state resident in
 HBM.  A step is one full timestep (CFL reduction + 3 RK stages + phidamp).
 
 Timed region: K steps after W warm-up steps, through the device-resident
-main loop (hg_run).  After it the run continues, untimed, to t = 0.5 and the
-final state is compared bit for bit with the reference program's own run
-(tests/golden/golden_meta.json, entry ot512_t05).
+main loop (hg_run).  After it the run continues, untimed, to step 2200, where
+the state is compared bit for bit with the reference program's own run
+(tests/golden/golden_meta.json, entry ot512_2200steps); then on until the
+reference's own FP64 run stops (negative internal energy during step 2250,
+before t = 0.5), which must happen at the same step and cell.
 
 The roofline object is for the dominant kernel (the fused direction sweep),
 timed with HIP events on the library's stream inside the timed region; its
@@ -95,8 +97,13 @@ def main():
         left = k
         while left > 0:
             n0 = tn
-            tim, tn, t_out, reason = eng.run_loop(tim, tn, t_out, dt_out, t_end, prm.outstyle,
-                                                  prm.endstyle, prm.tnlim, prm.tn_out, left, damp)
+            try:
+                tim, tn, t_out, reason = eng.run_loop(tim, tn, t_out, dt_out, t_end,
+                                                      prm.outstyle, prm.endstyle, prm.tnlim,
+                                                      prm.tn_out, left, damp)
+            except native.HgError as err:
+                tn, tim = err.tn, err.time
+                raise
             left -= tn - n0
             if reason == native.STOP_END:
                 return False
@@ -129,20 +136,31 @@ def main():
                 "rk_avg_launch_us": round(rk_ms / max(rk_n, 1) * 1e3, 2)}
     parity = None
     if not a.no_check and a.n == 512:
-        # finish the run untimed and compare with the reference's own run
-        while advance(1 << 30):
-            pass
+        # Untimed: continue to step 2200 and compare with the reference
+        # program's own run, then run on until the reference's stop (its
+        # FP64 run ends with negative internal energy before t = 0.5).
         import hashlib
         import numpy as np
         from tests.helpers import meta
-        entry = meta().get("ot512_t05")
+        m = meta()
+        advance(2200 - tn)
         out = D.fetch(eng, g, D.SNAP_FIELDS_MAG)
         names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
-        same = entry is not None and tn == entry["tn"] and tim == entry["time"] and all(
+        e = m.get("ot512_2200steps")
+        same = e is not None and tn == e["tn"] and tim == e["time"] and all(
             hashlib.sha256(np.ascontiguousarray(out[f], "<f8").tobytes()).hexdigest()
-            == entry["fields"][nm]["sha256"] for f, nm in zip(D.SNAP_FIELDS_MAG, names))
-        parity = {"final_tn": tn, "final_time": tim,
-                  "bitwise_equal_to_reference_run": bool(same)}
+            == e["fields"][nm]["sha256"] for f, nm in zip(D.SNAP_FIELDS_MAG, names))
+        parity = {"step": tn, "bitwise_equal_to_reference_run_at_step_2200": bool(same)}
+        fail = m.get("ot512_t05_failure")
+        try:
+            advance(1 << 30)
+            parity["reference_stop_reproduced"] = False
+        except native.HgError as err:
+            parity["stop"] = {"code": err.code, "completed_steps": err.tn,
+                              "cell": list(err.cell)}
+            parity["reference_stop_reproduced"] = bool(
+                fail and err.code == native.HG_ERR_NEG_EINT
+                and err.tn == fail["completed_steps"] and list(err.cell) == fail["cell"])
     cpu = None if a.no_cpu else cpu_baseline(a.n, a.cpu_seconds)
     line = {
         "metric": "zone-updates/s (FP64, Orszag-Tang 512^2 MHD)",
@@ -151,7 +169,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (closed-form Orszag-Tang initial state)",
         "config": {"workload": f"orszagtang_xy {a.n}x{a.n} MHD+GLM, MUSCL+HLLD, RK3, "
-                               f"courant 0.9, FP64, to t=0.5", "cells": g.ncells,
+                               f"courant 0.9, FP64 (run toward t=0.5)", "cells": g.ncells,
                    "precision": "fp64"},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
     }

@@ -46,5 +46,36 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "hg_driver")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """The iso_c_binding interface module and the Fortran host program,
+    linked against libhormone_hip.so (rpath $ORIGIN/..).  Needs amdflang."""
+    fc = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        obj = os.path.join(tmp, "hormone_hip_mod.o")
+        cmds = [
+            [fc, "-O2", "-ffp-contract=off", "-c", os.path.join(FORTRAN, "hormone_hip_mod.f90"),
+             "-o", obj, "-J", tmp],
+            [fc, "-O2", "-ffp-contract=off", "-I", tmp, os.path.join(FORTRAN, "hg_driver.f90"),
+             obj, "-L", HERE, "-lhormone_hip", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"],
+        ]
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                sys.stderr.write(r.stdout + r.stderr)
+                raise RuntimeError("amdflang failed")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

@@ -27,6 +27,7 @@ struct hg_ctx {
   double ch = 0.0;
   int err_i = 0, err_j = 0, err_k = 0, err_dir = 0;
   double *h_stage = nullptr;  // pinned staging for field transfers
+  double dtg_unit = 0.0;      // tools.f90:154-184 (gravswitch 3)
   // profiling (the reference's profiler categories, profiler.f90:5-51)
   bool prof = false;
   double prof_ms[HG_PROF_N] = {0};
@@ -65,7 +66,8 @@ static int validate(const hg_desc *d) {
     int a = d->bc[2 * q], b = d->bc[2 * q + 1];
     if (a * b == 0 && a + b != 0) return HG_ERR_ARG;
   }
-  if (d->gravswitch != 0) return HG_ERR_UNSUPPORTED;
+  if (d->gravswitch != 0 && d->gravswitch != 3) return HG_ERR_UNSUPPORTED;
+  if (d->gravswitch == 3 && (d->grktype < 1 || d->grktype > 2)) return HG_ERR_ARG;
   if (d->precision != HG_PREC_FP64) return HG_ERR_UNSUPPORTED;
   const double *const *arr[] = {&d->x1, &d->xi1, &d->dx1, &d->dxi1, &d->idx1, &d->idxi1,
                                 &d->x2, &d->xi2, &d->dx2, &d->dxi2, &d->idx2, &d->idxi2,
@@ -171,6 +173,35 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
   for (int a = 0; a < 3; ++a)
     D.waves[a] = D.solve[a] ? (long)((nn[a] + 59) / 60) * (D.ncell / nn[a]) : 0;
   if ((st = dev_alloc(c, sizeof(double) * 2 * D.nblk_cfl, (void **)&D.part))) return fail(st);
+  if (desc->gravswitch == 3) {
+    D.grav = 1;
+    D.fourpig = 4.0 * std::acos(-1.0) * desc->G;
+    const size_t gb = sizeof(double) * (size_t)(D.n1 + 2) * (D.n2 + 2) * (D.n3 + 2);
+    for (int q = 0; q < 2; ++q) {
+      if ((st = dev_alloc(c, gb, (void **)&D.gphi[q]))) return fail(st);
+      if (hipMemset(D.gphi[q], 0, gb) != hipSuccess) return fail(HG_ERR_DEVICE);
+    }
+    if ((st = dev_alloc(c, fb, (void **)&D.gpsi))) return fail(st);
+    if ((st = dev_alloc(c, fb, (void **)&D.gphi0))) return fail(st);
+    if ((st = dev_alloc(c, fb, (void **)&D.gpsi0))) return fail(st);
+    if (hipMemset(D.gpsi, 0, fb) != hipSuccess) return fail(HG_ERR_DEVICE);
+    D.gcur = 0;
+    // dtg_unit = min over cells of dtgrav_cell with cgrav = 1 (tools.f90:154-184)
+    const double o1 = D.solve[0] ? 1.0 : 0.0, o2 = D.solve[1] ? 1.0 : 0.0,
+                 o3 = D.solve[2] ? 1.0 : 0.0;
+    double m = INFINITY;
+    for (int k = 0; k < D.n3; ++k)
+      for (int j = 0; j < D.n2; ++j)
+        for (int i = 0; i < D.n1; ++i) {
+          const double x1 = desc->dxi1[i + 2], x2 = desc->dxi2[j + 2], x3 = desc->dxi3[k + 2];
+          const double dvol = x1 * x2 * x3;
+          const double sa1 = x2 * x3, sa2 = x1 * x3, sa3 = x1 * x2;
+          const double v =
+              dvol / (1.0 * (o1 * (sa1 + sa1) + o2 * (sa2 + sa2) + o3 * (sa3 + sa3)));
+          m = v < m ? v : m;
+        }
+    c->dtg_unit = m;
+  }
   if ((st = dev_alloc(c, sizeof(Scalars), (void **)&D.sc))) return fail(st);
   if (hipMemset(D.sc, 0, sizeof(Scalars)) != hipSuccess) return fail(HG_ERR_DEVICE);
   if (hipHostMalloc((void **)&c->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess)
@@ -210,14 +241,37 @@ static double *field_dev(hg_ctx *c, int field, bool upload) {
     case HG_F_PHI: return D.mag ? D.W[WPHI] : nullptr;
     case HG_F_EINT: return upload ? nullptr : D.W[WEINT];
     case HG_F_P: return upload ? nullptr : D.W[WEINT];
+    case HG_F_GRVPSI: return D.grav ? D.gpsi : nullptr;
     default:
       if (field >= HG_F_U && field < HG_F_U + D.nv) return D.U[field - HG_F_U];
       return nullptr;
   }
 }
 
+// potential: interior plus one ghost layer, into both stage buffers
+static int upload_grvphi(hg_ctx *c, const double *a) {
+  const Dev &D = c->D;
+  const size_t n = (size_t)(D.n1 + 2) * (D.n2 + 2) * (D.n3 + 2);
+  std::vector<double> h(n);
+  size_t q = 0;
+  for (int k = -1; k <= D.n3; ++k)
+    for (int j = -1; j <= D.n2; ++j)
+      for (int i = -1; i <= D.n1; ++i)
+        h[q++] = a[((size_t)(k + 2) * c->nyg + (j + 2)) * c->nxg + (i + 2)];
+  for (int b = 0; b < 2; ++b)
+    HIPCHK(hipMemcpy(D.gphi[b], h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  c->D.gcur = 0;
+  return HG_OK;
+}
+
 extern "C" int hg_upload(hg_ctx *c, int32_t field, const double *a) {
   if (!c || !a) return HG_ERR_ARG;
+  if (field == HG_F_GRVPHI || field == HG_F_GRVPSI) {
+    if (!c->D.grav) return HG_ERR_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (field == HG_F_GRVPHI) return upload_grvphi(c, a);
+  }
   if (!c->D.mag && field >= HG_F_B1 && field <= HG_F_PHI) return HG_OK;  // B == 0
   double *dst = field_dev(c, field, true);
   if (!dst) return HG_ERR_ARG;
@@ -275,6 +329,18 @@ extern "C" int hg_download(hg_ctx *c, int32_t field, double *a) {
         double *row = a + ((size_t)(k + 2) * c->nyg + (j + 2)) * c->nxg + 2;
         std::memset(row, 0, sizeof(double) * D.n1);
       }
+    return HG_OK;
+  }
+  if (field == HG_F_GRVPHI) {
+    if (!D.grav) return HG_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)(D.n1 + 2) * (D.n2 + 2) * (D.n3 + 2);
+    std::vector<double> h(n);
+    HIPCHK(hipMemcpy(h.data(), D.gphi[D.gcur], n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < D.n3; ++k)
+      for (int j = 0; j < D.n2; ++j)
+        for (int i = 0; i < D.n1; ++i)
+          a[((size_t)(k + 2) * c->nyg + (j + 2)) * c->nxg + (i + 2)] = h[gidx(D, i, j, k)];
     return HG_OK;
   }
   double *src = field_dev(c, field, false);
@@ -361,6 +427,9 @@ extern "C" int hg_profile_read(const hg_ctx *c, int32_t kind, double *ms_total,
     case HG_PROF_CFL:
       bytes = cell * (double)(D.nw - (D.mag ? 2 : 1));
       break;
+    case HG_PROF_GRAV:  // read phi, psi, phi0, psi0, rho; write phi, psi
+      bytes = cell * 7.0;
+      break;
     default:
       bytes = 0.0;
   }
@@ -395,8 +464,54 @@ extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
   c->ch = c->h_sc->ch;
   if (dt_out) *dt_out = c->h_sc->dt;
   if (ch_out) *ch_out = c->h_sc->ch;
-  if (cgrav_out) *cgrav_out = 0.0;
-  if (dtgrav_out) *dtgrav_out = 0.0;
+  double cgrav = 0.0, dtgrav = 0.0;
+  if (c->D.grav) {  // timestep.f90:92-97
+    const double dt = c->h_sc->dt;
+    cgrav = c->desc.HGfac * c->h_sc->cfmax;
+    if (c->desc.maxtngrv > 0) {
+      const double lim = c->desc.hgcfl * c->dtg_unit * (double)c->desc.maxtngrv / dt;
+      cgrav = lim < cgrav ? lim : cgrav;
+    }
+    const double b = c->desc.hgcfl * c->dtg_unit / cgrav;
+    dtgrav = b < dt ? b : dt;
+  }
+  if (cgrav_out) *cgrav_out = cgrav;
+  if (dtgrav_out) *dtgrav_out = dtgrav;
+  return HG_OK;
+}
+
+// gravity_hyperbolic (gravity_hyperbolic.f90:19-50) with gsrc = rho
+// (gravity.f90:114-150): tngrav = ceiling((time+dt-grvtime)/dtgrav) RK
+// sub-steps of the wave equation, then grvtime and cgrav_old advance.
+extern "C" int hg_gravity(hg_ctx *c, double time, double dt, double *grvtime, double cgrav,
+                          double *cgrav_old, double *dtgrav, int32_t *nsub) {
+  if (!c || !grvtime || !cgrav_old || !dtgrav) return HG_ERR_ARG;
+  if (!c->D.grav) return HG_ERR_STATE;
+  HIPCHK(hipSetDevice(c->device));
+  int st = ensure_eint(c);
+  if (st) return st;
+  const int tngrav = (int)std::ceil((time + dt - *grvtime) / *dtgrav);
+  if (tngrav < 1) return HG_ERR_ARG;
+  *dtgrav = (time + dt - *grvtime) / (double)tngrav;
+  const int gk = c->desc.grktype;
+  for (int l = 1; l <= tngrav; ++l) {
+    const double damp =
+        std::exp(-(1.0 - *cgrav_old / cgrav + c->desc.alphagrv * c->desc.hgcfl));
+    for (int g = 1; g <= gk; ++g) {
+      double faco, fact, facn;
+      runge_coeff(g, gk, &faco, &fact, &facn);
+      prof_begin(c, HG_PROF_GRAV);
+      launch_grav_stage(c->D, g, faco, fact, facn, *dtgrav, cgrav, g == gk, damp, c->stream);
+      prof_end(c);
+      c->D.gcur = 1 - c->D.gcur;
+    }
+    *grvtime = *grvtime + *dtgrav;
+    *cgrav_old = cgrav;
+  }
+  HIPCHK(hipGetLastError());
+  if (nsub) *nsub = tngrav;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
   return HG_OK;
 }
 
@@ -446,6 +561,7 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
                       int32_t *stop_reason) {
   if (!c || !time || !tn || !t_out || max_steps < 1) return HG_ERR_ARG;
   if (outstyle == 2 && tn_out < 1) return HG_ERR_ARG;
+  if (c->D.grav) return HG_ERR_UNSUPPORTED;  // gravity runs use the per-step entry points
   HIPCHK(hipSetDevice(c->device));
   int st = ensure_eint(c);
   if (st) return st;

@@ -342,7 +342,36 @@ __device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, doubl
 // CFL: also evaluate the next step's CFL condition from the new primitives
 // (the reference's timestep reads exactly these values at the start of the
 // next step).  DAMP: apply phidamp (source.f90:239-260) to the new psi.
-template <bool MAG, bool CFL, bool DAMP>
+// Gravity source (source.f90:14-85 Cartesian, get_fieldforce :177-228):
+// g_a = -(non-uniform central difference of phi) * rho with the stage-start
+// rho and v; src(imo_a) = g_a, src(iene) = g.v.
+__device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVMAX]) {
+  const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2),
+            k = (int)(m / ((long)D.n1 * D.n2));
+  const double *ph = D.gphi[D.gcur];
+  const long g = gidx(D, i, j, k);
+  const long st[3] = {1, (long)(D.n1 + 2), (long)(D.n1 + 2) * (D.n2 + 2)};
+  const int q3[3] = {i, j, k};
+  const double d = D.W[WD][m];
+  double gr[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int q = q3[a];
+    const double *DX = D.ax[a].dx, *IDX = D.ax[a].idx;
+    gr[a] = -((DX[q] * IDX[q + 1] * ph[g + st[a]] - DX[q + 1] * IDX[q] * ph[g - st[a]]) /
+                  (DX[q] + DX[q + 1]) +
+              (DX[q + 1] - DX[q]) * IDX[q] * IDX[q + 1] * ph[g]) *
+            d;
+  }
+#pragma unroll
+  for (int v = 0; v < NVMAX; ++v) src[v] = 0.0;
+  src[1] = gr[0];
+  src[2] = gr[1];
+  src[3] = gr[2];
+  src[4] = gr[0] * D.W[WV1][m] + gr[1] * D.W[WV2][m] + gr[2] * D.W[WV3][m];
+}
+
+template <bool MAG, bool CFL, bool DAMP, bool GRAV>
 __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
                                             double facn, double damp) {
   if (D.sc->stop) return;
@@ -350,6 +379,8 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
   double dti = __builtin_inf(), cfm = 0.0;
   if (m < D.ncell) {
   const double ft = fact * D.sc->dt;
+  double src[NVMAX];
+  if (GRAV) grav_source(D, m, src);
   double un[NVMAX];
 #pragma unroll
   for (int v = 0; v < NVMAX; ++v) {
@@ -358,7 +389,7 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
     const double a = D.solve[0] ? D.DU[0][v][m] : 0.0;
     const double b = D.solve[1] ? D.DU[1][v][m] : 0.0;
     const double c = D.solve[2] ? D.DU[2][v][m] : 0.0;
-    const double acc = ((a + b) + c) + 0.0;  // + src (zero without gravity)
+    const double acc = ((a + b) + c) + (GRAV ? src[v] : 0.0);  // + src
     const double u = D.U[v][m];
     double u0;
     if (stage == 1) {
@@ -418,6 +449,60 @@ __global__ __launch_bounds__(256) void k_pressure(Dev D) {
     const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
     atomicMin(&D.sc->err, errkey(0, 1, 0, i, j, k, D));
   }
+}
+
+// ---- hyperbolic self-gravity stage (gravity_hyperbolic.f90:61-125) --------
+// Reads the potential from gphi[in] (ghost layer = Dirichlet boundary values)
+// and writes the new potential to gphi[1-in]; psi is updated in place.
+// lap_coeff for crdnt 0 (new; the reference has no Cartesian branch):
+// c0 = -((idx1(i)+idx1(i+1))*dxi2*dxi3 + (idx2(j)+idx2(j+1))*dxi1*dxi3 +
+// (idx3(k)+idx3(k+1))*dxi1*dxi2), c1(i) = dxi2*dxi3*idx1(i+1), c2, c3
+// alike, normalised by the cell volume.  The last stage also applies the
+// psi damping factor.
+template <bool DAMP>
+__global__ __launch_bounds__(256) void k_grav_stage(Dev D, int grungen, int in, double faco,
+                                                    double fact, double facn, double dtg,
+                                                    double cg, double damp) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= D.ncell) return;
+  const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2),
+            k = (int)(m / ((long)D.n1 * D.n2));
+  const double *ph = D.gphi[in];
+  double *out = D.gphi[1 - in];
+  const long g = gidx(D, i, j, k);
+  const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
+  const double *X1 = D.ax[0].dxi, *X2 = D.ax[1].dxi, *X3 = D.ax[2].dxi;
+  const double *I1 = D.ax[0].idx, *I2 = D.ax[1].idx, *I3 = D.ax[2].idx;
+  const double c0 = -((I1[i] + I1[i + 1]) * X2[j] * X3[k] + (I2[j] + I2[j + 1]) * X1[i] * X3[k] +
+                      (I3[k] + I3[k + 1]) * X1[i] * X2[j]);
+  double lap = c0 * ph[g];
+  lap = lap + X2[j] * X3[k] * I1[i] * ph[g - 1];
+  lap = lap + X2[j] * X3[k] * I1[i + 1] * ph[g + 1];
+  if (D.n2 > 1) {
+    lap = lap + X1[i] * X3[k] * I2[j] * ph[g - sy];
+    lap = lap + X1[i] * X3[k] * I2[j + 1] * ph[g + sy];
+  }
+  if (D.n3 > 1) {
+    lap = lap + X1[i] * X2[j] * I3[k] * ph[g - sz];
+    lap = lap + X1[i] * X2[j] * I3[k + 1] * ph[g + sz];
+  }
+  const double lapphi = lap / (X1[i] * X2[j] * X3[k]);
+  const double hgsrc = D.fourpig * D.W[WD][m];
+  const double p = ph[g], q = D.gpsi[m];
+  double p0, q0;
+  if (grungen == 1) {
+    p0 = p;
+    q0 = q;
+    D.gphi0[m] = p;
+    D.gpsi0[m] = q;
+  } else {
+    p0 = D.gphi0[m];
+    q0 = D.gpsi0[m];
+  }
+  out[g] = faco * p0 + facn * p + fact * dtg * cg * q;
+  double qn = faco * q0 + facn * q + fact * dtg * cg * (lapphi - hgsrc);
+  if (DAMP) qn = qn * damp;
+  D.gpsi[m] = qn;
 }
 
 // ---- CFL condition as its own pass (the reference's timestep) ------------
@@ -619,20 +704,36 @@ void launch_sweep(const Dev &D, int stage, hipStream_t s) {
   }
 }
 
-template <bool MAG>
+template <bool MAG, bool GRAV>
 static void rk_mag(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
                    bool damp, double df, hipStream_t s) {
   const unsigned g = nblk(D.ncell, 256);
-  if (cfl && damp) k_rk<MAG, true, true><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
-  else if (cfl) k_rk<MAG, true, false><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
-  else if (damp) k_rk<MAG, false, true><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
-  else k_rk<MAG, false, false><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
+  if (cfl && damp) k_rk<MAG, true, true, GRAV><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
+  else if (cfl) k_rk<MAG, true, false, GRAV><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
+  else if (damp) k_rk<MAG, false, true, GRAV><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
+  else k_rk<MAG, false, false, GRAV><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
 }
 
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
                bool damp, double damp_factor, hipStream_t s) {
-  if (D.mag) rk_mag<true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s);
-  else rk_mag<false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s);
+  if (D.grav) {
+    if (D.mag) rk_mag<true, true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s);
+    else rk_mag<false, true>(D, stage, faco, fact, facn, cfl, false, damp_factor, s);
+  } else {
+    if (D.mag) rk_mag<true, false>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s);
+    else rk_mag<false, false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s);
+  }
+}
+
+void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, double facn,
+                       double dtg, double cg, bool damp, double damp_factor, hipStream_t s) {
+  const unsigned g = nblk(D.ncell, 256);
+  if (damp)
+    k_grav_stage<true><<<g, 256, 0, s>>>(D, grungen, D.gcur, faco, fact, facn, dtg, cg,
+                                         damp_factor);
+  else
+    k_grav_stage<false><<<g, 256, 0, s>>>(D, grungen, D.gcur, faco, fact, facn, dtg, cg,
+                                          damp_factor);
 }
 
 void launch_cfl(const Dev &D, hipStream_t s) {

@@ -59,7 +59,19 @@ struct Dev {
   Scalars *sc;
   int nblk_cfl;
   long waves[3];   // sweep wavefronts per direction (0 if not solved)
+  // hyperbolic self-gravity (gravswitch 3): the potential is stored with a
+  // one-cell ghost layer ((n1+2)(n2+2)(n3+2), Dirichlet values), twice for
+  // the Jacobi-style stage update; psi and the stage-1 copies are interior.
+  int grav;
+  double *gphi[2], *gpsi, *gphi0, *gpsi0;
+  int gcur;          // which gphi buffer holds the current potential
+  double fourpig;    // 4*pi*G
 };
+
+// index of interior cell (i,j,k) in the potential's padded layout
+__host__ __device__ inline long gidx(const Dev &D, int i, int j, int k) {
+  return ((long)(k + 1) * (D.n2 + 2) + (j + 1)) * (long)(D.n1 + 2) + (i + 1);
+}
 
 // Error keys: the smallest key is the first error in the reference's serial
 // loop order (stage, then numflux before the cell update, then direction,
@@ -84,6 +96,8 @@ void launch_set_clock(const Dev &D, double time, double t_out, double t_end, dou
 void launch_sweep(const Dev &D, int stage, hipStream_t s);
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
                bool damp, double damp_factor, hipStream_t s);
+void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, double facn,
+                       double dtg, double cg, bool damp, double damp_factor, hipStream_t s);
 void launch_cfl(const Dev &D, hipStream_t s);
 void launch_cfl_final(const Dev &D, hipStream_t s);
 void launch_step_end(const Dev &D, hipStream_t s);

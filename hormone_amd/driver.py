@@ -46,6 +46,9 @@ def load_initial_state(eng, prm: P.Params, g: Grid, st: P.State):
         eng.upload(f, g.pad(a))
     for n, a in enumerate(u):
         eng.upload(F_U + n, g.pad(a))
+    if prm.gravswitch == 3:
+        eng.upload(F_GRVPHI, st.extra["grvphi_padded"])
+        eng.upload(F_GRVPSI, np.zeros(g.padded_shape))
 
 
 @dataclass
@@ -55,6 +58,7 @@ class RunResult:
     wall_s: float
     steps_timed: int
     dts: list = field(default_factory=list)
+    nsub: list = field(default_factory=list)  # gravity sub-steps per step
 
 
 def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
@@ -65,16 +69,32 @@ def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
     the state is the same bit for bit.
     """
     t_end, dt_out, t_out = prm.times()
-    if device_loop and hasattr(eng, "run_loop") and prm.fixed_dt <= 0 and not record_dt:
+    if (device_loop and hasattr(eng, "run_loop") and prm.fixed_dt <= 0 and not record_dt
+            and prm.gravswitch == 0):
         return _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out)
     tim = 0.0
     tn = 0
     damp = math.exp(-prm.alpha9wave * prm.courant)  # source.f90:253
     dts = []
+    nsubs = []
+    grav = prm.gravswitch == 3
+    grvtime, cgrav_old = 0.0, 0.0  # gravmod module values at start-up
     t0 = _time.perf_counter()
     while True:
-        dt, ch = eng.timestep(tim, t_out, t_end, prm.outstyle, prm.endstyle,
-                              prm.fixed_dt)
+        if grav:
+            dt, ch, cgrav, dtgrav = eng.timestep(tim, t_out, t_end, prm.outstyle,
+                                                 prm.endstyle, prm.fixed_dt, grav=True)
+            # gravity.f90:41-60: at tn == 0 the potential is the initial one
+            # (grav_init_other); afterwards the hyperbolic solver advances it.
+            if tn == 0:
+                grvtime = 0.0
+            else:
+                grvtime, cgrav_old, dtgrav, ns = eng.gravity(tim, dt, grvtime, cgrav,
+                                                             cgrav_old, dtgrav)
+                nsubs.append(ns)
+        else:
+            dt, ch = eng.timestep(tim, t_out, t_end, prm.outstyle, prm.endstyle,
+                                  prm.fixed_dt)
         if record_dt:
             dts.append(dt)
         eng.hydro_step(dt, ch)
@@ -94,7 +114,7 @@ def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
         if max_steps is not None and tn >= max_steps:
             break
     wall = _time.perf_counter() - t0
-    return RunResult(tn, tim, wall, tn, dts)
+    return RunResult(tn, tim, wall, tn, dts, nsubs)
 
 
 def _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out) -> RunResult:

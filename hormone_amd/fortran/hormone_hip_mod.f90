@@ -1,0 +1,131 @@
+! hormone_hip_mod.f90 -- iso_c_binding interface to the C ABI declared in
+! include/hormone_hip.h (libhormone_hip.so).  A Fortran host (the reference's
+! main program, or hg_driver.f90 here) uses this module to hand its
+! Cartesian hydro sweep to the GPU: hg_timestep replaces timestep
+! (src/main/timestep.f90:15-105), hg_hydro_step replaces hydro_step
+! (src/main/hydro.f90:15-52) and hg_phidamp replaces phidamp
+! (src/main/source.f90:239-260).
+module hormone_hip_mod
+ use iso_c_binding
+ implicit none
+
+ integer(c_int),parameter:: HG_OK=0, HG_ERR_ARG=1, HG_ERR_UNSUPPORTED=2, &
+                            HG_ERR_DEVICE=3, HG_ERR_FACE_EOS=4, HG_ERR_NEG_EINT=5, &
+                            HG_ERR_NOMEM=6, HG_ERR_STATE=7
+ integer(c_int),parameter:: HG_LIMITER_FLAT=0, HG_LIMITER_MINMOD=1, &
+                            HG_LIMITER_MODIFIED_MC=2
+ integer(c_int),parameter:: HG_PREC_FP64=0, HG_PREC_FP32MIX=1
+ integer(c_int),parameter:: HG_F_D=0, HG_F_V1=1, HG_F_V2=2, HG_F_V3=3, HG_F_E=4, &
+                            HG_F_B1=5, HG_F_B2=6, HG_F_B3=7, HG_F_PHI=8, &
+                            HG_F_EINT=9, HG_F_P=10, HG_F_GRVPHI=11, HG_F_GRVPSI=12, &
+                            HG_F_U=16
+
+ ! mirrors struct hg_desc field by field
+ type,bind(C):: hg_desc
+  integer(c_int32_t):: is, ie, js, je, ks, ke
+  type(c_ptr):: x1, xi1, dx1, dxi1, idx1, idxi1
+  type(c_ptr):: x2, xi2, dx2, dxi2, idx2, idxi2
+  type(c_ptr):: x3, xi3, dx3, dxi3, idx3, idxi3
+  integer(c_int32_t):: crdnt, eostype, compswitch, mag_on, flux_limiter, rktype
+  integer(c_int32_t):: bc(12)
+  real(c_double):: gamma, courant, alpha9wave
+  integer(c_int32_t):: gravswitch, grktype
+  real(c_double):: HGfac, hgcfl, alphagrv, G
+  integer(c_int32_t):: maxtngrv, precision, device, reserved
+ end type hg_desc
+
+ interface
+  function hg_create(desc, ctx) bind(C, name='hg_create') result(st)
+   import:: c_int, c_ptr, hg_desc
+   type(hg_desc),intent(in):: desc
+   type(c_ptr),intent(out):: ctx
+   integer(c_int):: st
+  end function hg_create
+
+  function hg_destroy(ctx) bind(C, name='hg_destroy') result(st)
+   import:: c_int, c_ptr
+   type(c_ptr),value:: ctx
+   integer(c_int):: st
+  end function hg_destroy
+
+  function hg_ufnmax(ctx) bind(C, name='hg_ufnmax') result(n)
+   import:: c_int, c_ptr
+   type(c_ptr),value:: ctx
+   integer(c_int):: n
+  end function hg_ufnmax
+
+  function hg_upload(ctx, field, padded) bind(C, name='hg_upload') result(st)
+   import:: c_int, c_int32_t, c_ptr
+   type(c_ptr),value:: ctx
+   integer(c_int32_t),value:: field
+   type(c_ptr),value:: padded
+   integer(c_int):: st
+  end function hg_upload
+
+  function hg_download(ctx, field, padded) bind(C, name='hg_download') result(st)
+   import:: c_int, c_int32_t, c_ptr
+   type(c_ptr),value:: ctx
+   integer(c_int32_t),value:: field
+   type(c_ptr),value:: padded
+   integer(c_int):: st
+  end function hg_download
+
+  function hg_timestep(ctx, time, t_out, t_end, outstyle, endstyle, fixed_dt, &
+                       dt, ch, cgrav, dtgrav) bind(C, name='hg_timestep') result(st)
+   import:: c_int, c_int32_t, c_ptr, c_double
+   type(c_ptr),value:: ctx
+   real(c_double),value:: time, t_out, t_end, fixed_dt
+   integer(c_int32_t),value:: outstyle, endstyle
+   real(c_double),intent(out):: dt, ch, cgrav, dtgrav
+   integer(c_int):: st
+  end function hg_timestep
+
+  function hg_hydro_step(ctx, dt, ch) bind(C, name='hg_hydro_step') result(st)
+   import:: c_int, c_ptr, c_double
+   type(c_ptr),value:: ctx
+   real(c_double),value:: dt, ch
+   integer(c_int):: st
+  end function hg_hydro_step
+
+  function hg_phidamp(ctx, factor) bind(C, name='hg_phidamp') result(st)
+   import:: c_int, c_ptr, c_double
+   type(c_ptr),value:: ctx
+   real(c_double),value:: factor
+   integer(c_int):: st
+  end function hg_phidamp
+
+  function hg_last_error(ctx, i, j, k, dir) bind(C, name='hg_last_error') result(st)
+   import:: c_int, c_int32_t, c_ptr
+   type(c_ptr),value:: ctx
+   integer(c_int32_t),intent(out):: i, j, k, dir
+   integer(c_int):: st
+  end function hg_last_error
+ end interface
+
+contains
+
+ ! Print the reference's message for a hot-path error and stop, as the
+ ! reference does at numflux.f90:579-593 and eos.f90:833-839.
+ subroutine hg_check(ctx, st, where)
+  type(c_ptr),intent(in):: ctx
+  integer(c_int),intent(in):: st
+  character(len=*),intent(in):: where
+  integer(c_int32_t):: i, j, k, dir
+  integer(c_int):: s2
+  if(st==HG_OK)return
+  s2 = hg_last_error(ctx, i, j, k, dir)
+  select case(st)
+  case(HG_ERR_FACE_EOS)
+   print*,'Error in calculating pressure'
+   print'("i=",i5,", j=",i5,", k=",i5)',i,j,k
+   print'("In numflux",i2)',dir
+  case(HG_ERR_NEG_EINT)
+   print*,'Error in internalenergy, Negative internal energy'
+   print'("i=",i4,", j=",i4,", k=",i4)',i,j,k
+  case default
+   print*,'hormone_hip error ',st,' in ',where
+  end select
+  stop 1
+ end subroutine hg_check
+
+end module hormone_hip_mod

@@ -40,8 +40,12 @@ class HgDesc(C.Structure):
 
 
 class HgError(RuntimeError):
-    def __init__(self, code, where=""):
+    def __init__(self, code, where="", cell=None, direction=None):
         self.code = code
+        self.cell = cell            # (i, j, k) in the reference's Fortran indices
+        self.direction = direction  # 1..3 for a face error, 0 for a cell error
+        self.tn = None
+        self.time = None
         super().__init__(f"hormone_hip status {code}{(': ' + where) if where else ''}")
 
 
@@ -105,6 +109,8 @@ def _load():
     lib.hg_profile.argtypes = [vp, C.c_int32]
     lib.hg_profile_read.argtypes = [vp, C.c_int32, _dp, C.POINTER(C.c_int64), _dp]
     lib.hg_kat.argtypes = [C.c_int32, C.c_int32, _dp, _dp]
+    lib.hg_gravity.argtypes = [vp, C.c_double, C.c_double, _dp, C.c_double, _dp, _dp,
+                               C.POINTER(C.c_int32)]
     return lib
 
 
@@ -122,11 +128,12 @@ def exported_symbols():
     """Entry points declared in include/hormone_hip.h."""
     return ["hg_version", "hg_create", "hg_destroy", "hg_ufnmax", "hg_upload",
             "hg_download", "hg_timestep", "hg_hydro_step", "hg_phidamp",
-            "hg_last_error", "hg_run", "hg_profile", "hg_profile_read", "hg_kat"]
+            "hg_last_error", "hg_run", "hg_profile", "hg_profile_read", "hg_kat",
+            "hg_gravity"]
 
 
 STOP_END, STOP_OUTPUT, STOP_MAXSTEPS, STOP_ERROR = 1, 2, 3, 4
-PROF_SWEEP, PROF_RK, PROF_CFL = 0, 1, 2
+PROF_SWEEP, PROF_RK, PROF_CFL, PROF_GRAV = 0, 1, 2, 3
 
 
 class Hydro:
@@ -153,11 +160,15 @@ class Hydro:
         except Exception:
             pass
 
-    def _check(self, st, what):
+    def _check(self, st, what, **extra):
         if st != HG_OK:
             i, j, k, d = (C.c_int32() for _ in range(4))
             self._lib.hg_last_error(self.h, C.byref(i), C.byref(j), C.byref(k), C.byref(d))
-            raise HgError(st, f"{what} at i={i.value} j={j.value} k={k.value} dir={d.value}")
+            err = HgError(st, f"{what} at i={i.value} j={j.value} k={k.value} dir={d.value}",
+                          cell=(i.value, j.value, k.value), direction=d.value)
+            for key, val in extra.items():
+                setattr(err, key, val)
+            raise err
 
     def ufnmax(self):
         return self._lib.hg_ufnmax(self.h)
@@ -170,12 +181,22 @@ class Hydro:
         assert padded.flags.c_contiguous and padded.dtype == np.float64
         self._check(self._lib.hg_download(self.h, field, padded.ctypes.data_as(_dp)), "hg_download")
 
-    def timestep(self, time, t_out, t_end, outstyle, endstyle, fixed_dt=-1.0):
+    def timestep(self, time, t_out, t_end, outstyle, endstyle, fixed_dt=-1.0, grav=False):
         dt, ch, cg, dtg = (C.c_double() for _ in range(4))
         self._check(self._lib.hg_timestep(self.h, time, t_out, t_end, outstyle,
                                           endstyle, fixed_dt, C.byref(dt), C.byref(ch),
                                           C.byref(cg), C.byref(dtg)), "hg_timestep")
+        if grav:
+            return dt.value, ch.value, cg.value, dtg.value
         return dt.value, ch.value
+
+    def gravity(self, time, dt, grvtime, cgrav, cgrav_old, dtgrav):
+        """hg_gravity; returns (grvtime, cgrav_old, dtgrav, nsub)."""
+        gt, co, dg, ns = C.c_double(grvtime), C.c_double(cgrav_old), C.c_double(dtgrav), \
+            C.c_int32(0)
+        self._check(self._lib.hg_gravity(self.h, time, dt, C.byref(gt), cgrav, C.byref(co),
+                                         C.byref(dg), C.byref(ns)), "hg_gravity")
+        return gt.value, co.value, dg.value, ns.value
 
     def hydro_step(self, dt, ch):
         self._check(self._lib.hg_hydro_step(self.h, dt, ch), "hg_hydro_step")
@@ -187,9 +208,10 @@ class Hydro:
                  max_steps, damp_factor):
         """Device-resident main loop (hg_run); returns (time, tn, t_out, reason)."""
         t, n, to, r = C.c_double(time), C.c_int64(tn), C.c_double(t_out), C.c_int32(0)
-        self._check(self._lib.hg_run(self.h, C.byref(t), C.byref(n), C.byref(to), dt_out,
-                                     t_end, outstyle, endstyle, tnlim, tn_out, max_steps,
-                                     damp_factor, C.byref(r)), "hg_run")
+        st = self._lib.hg_run(self.h, C.byref(t), C.byref(n), C.byref(to), dt_out, t_end,
+                              outstyle, endstyle, tnlim, tn_out, max_steps, damp_factor,
+                              C.byref(r))
+        self._check(st, "hg_run", tn=n.value, time=t.value)
         return t.value, n.value, to.value, r.value
 
     def profile(self, enable=True):

@@ -223,6 +223,102 @@ def conserve(st: State, e: np.ndarray, mag_on: bool, dim: int):
     return u
 
 
+# --- seeded counter-based noise (same value for a cell whatever the layout) --
+
+_M64 = (1 << 64) - 1
+
+
+def hash_uniform(seed: int, i, j, k) -> np.ndarray:
+    """Uniform numbers in [-1, 1) from a splitmix64-style hash of
+    (seed, i, j, k), with i, j, k the Fortran cell indices."""
+    with np.errstate(over="ignore"):
+        x = (np.uint64(seed & _M64) * np.uint64(0x9E3779B97F4A7C15)
+             + np.asarray(i, dtype=np.uint64) * np.uint64(0xBF58476D1CE4E5B9)
+             + np.asarray(j, dtype=np.uint64) * np.uint64(0x94D049BB133111EB)
+             + np.asarray(k, dtype=np.uint64) * np.uint64(0xD6E8FEB86659FD93))
+        x = x ^ (x >> np.uint64(30))
+        x = x * np.uint64(0xBF58476D1CE4E5B9)
+        x = x ^ (x >> np.uint64(27))
+        x = x * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return (x >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+
+
+def _cell_indices(g: Grid, pad: int = 0):
+    """Fortran indices (i, j, k) broadcast to the (nz, ny, nx) layout."""
+    ii = np.arange(g.ax[0].s - pad, g.ax[0].e + 1 + pad)
+    jj = np.arange(g.ax[1].s - pad, g.ax[1].e + 1 + pad)
+    kk = np.arange(g.ax[2].s - pad, g.ax[2].e + 1 + pad)
+    return np.meshgrid(ii, jj, kk, indexing="ij")
+
+
+# --- n=1 polytrope with hyperbolic self-gravity (new Cartesian set-up) -----
+
+@dataclass
+class Polytrope:
+    M: float = 1.0
+    R: float = 0.3
+    G: float = 1.0
+    rho_bg: float = 1e-6
+    noise: float = 1e-3
+    seed: int = 12345
+
+    @property
+    def rho_c(self):
+        return math.pi * self.M / (4.0 * self.R ** 3)
+
+    @property
+    def K(self):
+        return 2.0 * self.G * self.R ** 2 / math.pi
+
+    def rho(self, r):
+        xi = math.pi * r / self.R
+        with np.errstate(invalid="ignore", divide="ignore"):
+            inside = np.where(xi > 0, np.sin(xi) / np.where(xi > 0, xi, 1.0), 1.0)
+        return np.where(r < self.R, self.rho_c * inside, 0.0)
+
+    def potential(self, r):
+        """Closed-form potential: -GM/R - 2K rho inside, -GM/r outside."""
+        inside = -self.G * self.M / self.R - 2.0 * self.K * self.rho(r)
+        with np.errstate(divide="ignore"):
+            outside = -self.G * self.M / np.where(r > 0, r, 1.0)
+        return np.where(r < self.R, inside, outside)
+
+
+def polytrope_params(n: int = 256, mag_on: bool = False, steps: int = 200, **kw) -> Params:
+    p = Params(n=(n, n, n), lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0), gamma=2.0,
+               mag_on=mag_on, bc=(3,) * 12, gravswitch=3, G=1.0, outstyle=2, endstyle=2,
+               tnlim=steps, tn_out=steps, t_end=1e30, is_test=False)
+    return replace(p, **kw)
+
+
+def _radius(g: Grid, pad: int = 0):
+    def cen(d):
+        a = g.ax[d]
+        return a.x[2 - pad:2 + a.n + pad]
+    x, y, z = np.meshgrid(cen(0), cen(1), cen(2), indexing="ij")
+    return np.sqrt(x * x + y * y + z * z).transpose(2, 1, 0)
+
+
+def polytrope(g: Grid, pol: Polytrope = Polytrope()) -> State:
+    """n=1 polytrope (gamma=2) of mass M, radius R in a uniform background,
+    density perturbed by seeded uniform noise; potential in closed form."""
+    r = _radius(g)
+    rp = pol.rho(r)
+    rho0 = np.maximum(rp, pol.rho_bg)
+    ii, jj, kk = _cell_indices(g)
+    u = hash_uniform(pol.seed, ii, jj, kk).transpose(2, 1, 0)
+    d = rho0 * (1.0 + pol.noise * u)
+    # hydrostatic pressure: K rho^2 in the star plus the background's own
+    # hydrostatic profile rho_bg*G*M/max(r,R) (continuous at r = R)
+    p = pol.K * rp * rp + pol.rho_bg * pol.G * pol.M / np.maximum(r, pol.R)
+    z = np.zeros(g.shape)
+    rp = _radius(g, pad=2)
+    phi_pad = pol.potential(rp)
+    return State(d=d, p=p, v=[z.copy(), z.copy(), z.copy()], b=[z.copy(), z.copy(), z.copy()],
+                 phi=z.copy(), extra={"grvphi_padded": phi_pad})
+
+
 def make_problem(name: str, **kw):
     """(Params, Grid, State) for a reference test problem name."""
     if name.startswith("sodshock_"):
@@ -237,4 +333,9 @@ def make_problem(name: str, **kw):
         prm = orszagtang_params(name.split("_")[1], **kw)
         g = cartesian_grid(prm.n, prm.lo, prm.hi)
         return prm, g, orszagtang(g, prm.gamma)
+    if name == "polytrope3d":
+        pol = kw.pop("polytrope", Polytrope())
+        prm = polytrope_params(**kw)
+        g = cartesian_grid(prm.n, prm.lo, prm.hi, gravity=True)
+        return prm, g, polytrope(g, pol)
     raise ValueError("unknown problem " + name)

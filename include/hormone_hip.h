@@ -33,7 +33,8 @@ extern "C" {
 enum {
   HG_OK = 0,
   HG_ERR_ARG = 1,         /* bad argument / descriptor                       */
-  HG_ERR_UNSUPPORTED = 2, /* crdnt/=0, eostype/=0, compswitch/=0, bc 4/5/9/10 */
+  HG_ERR_UNSUPPORTED = 2, /* crdnt/=0, eostype/=0, compswitch/=0, bc 4/5/9/10,
+                             gravswitch other than 0 and 3                   */
   HG_ERR_DEVICE = 3,      /* a HIP runtime call failed                       */
   HG_ERR_FACE_EOS = 4,    /* numflux.f90:579-593 error_flux: rho<=0 or
                              e_int<=0 at a reconstructed face state          */
@@ -116,7 +117,8 @@ int hg_destroy(hg_ctx *ctx);
 int hg_ufnmax(const hg_ctx *ctx);
 
 /* Copy the interior of one padded host field into the device state
-   (HG_F_D..HG_F_PHI, HG_F_GRVPHI, HG_F_GRVPSI, HG_F_U+n).  After primitive
+   (HG_F_D..HG_F_PHI, HG_F_GRVPHI, HG_F_GRVPSI, HG_F_U+n).  For HG_F_GRVPHI
+   the first ghost layer is copied too: it is the potential's boundary.  After primitive
    fields change, e_int and p are recomputed on the device from e, rho, v, B
    before the next step, exactly as pressure() does (eos.f90:742-842). */
 int hg_upload(hg_ctx *ctx, int32_t field, const double *padded);
@@ -146,6 +148,21 @@ int hg_timestep(hg_ctx *ctx, double time, double t_out, double t_end,
    On HG_ERR_FACE_EOS / HG_ERR_NEG_EINT, hg_last_error reports where. */
 int hg_hydro_step(hg_ctx *ctx, double dt, double ch);
 
+/* Replaces gravity for gravswitch 3 and tn > 0 (gravity.f90:18-64 ->
+   gravity_hyperbolic.f90:19-50): gsrc = rho, tngrav =
+   ceiling((time+dt-grvtime)/dtgrav) sub-steps of the hyperbolic wave
+   equation (RK order grktype, hyperbolic_gravity_step :61-125) with the psi
+   damping exp(-(1-cgrav_old/cgrav+alphagrv*hgcfl)) evaluated on the host,
+   then grvtime += dtgrav and cgrav_old = cgrav per sub-step.  Cartesian
+   lap_coeff (the reference has none, gravity_hyperbolic.f90:413-417):
+   c0 = -((idx1(i)+idx1(i+1))*dxi2*dxi3 + (idx2(j)+idx2(j+1))*dxi1*dxi3 +
+   (idx3(k)+idx3(k+1))*dxi1*dxi2), c1(i) = dxi2*dxi3*idx1(i+1) (c2, c3
+   alike), normalised by dxi1*dxi2*dxi3.  The potential's boundary is the
+   ghost layer uploaded with HG_F_GRVPHI, held fixed (Dirichlet).
+   dtgrav is updated in place; *nsub (may be NULL) receives tngrav. */
+int hg_gravity(hg_ctx *ctx, double time, double dt, double *grvtime, double cgrav,
+               double *cgrav_old, double *dtgrav, int32_t *nsub);
+
 /* Replaces phidamp (source.f90:239-260): multiplies the primitive psi (not
    the conserved u(:,:,:,i9wv)) by factor = exp(-alpha9wave*courant), which
    the caller evaluates on the host. */
@@ -170,7 +187,8 @@ int hg_run(hg_ctx *ctx, double *time, int64_t *tn, double *t_out, double dt_out,
    (cf. the reference's profiler, profiler.f90:5-51).  hg_profile(ctx, 1)
    resets and enables; hg_profile_read returns the summed milliseconds, the
    number of launches and the algorithmic HBM bytes of one launch. */
-enum { HG_PROF_SWEEP = 0, HG_PROF_RK = 1, HG_PROF_CFL = 2, HG_PROF_N = 3 };
+enum { HG_PROF_SWEEP = 0, HG_PROF_RK = 1, HG_PROF_CFL = 2, HG_PROF_GRAV = 3,
+       HG_PROF_N = 4 };
 int hg_profile(hg_ctx *ctx, int32_t enable);
 int hg_profile_read(const hg_ctx *ctx, int32_t kind, double *ms_total,
                     int64_t *launches, double *bytes_per_launch);

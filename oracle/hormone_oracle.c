@@ -49,6 +49,7 @@ struct or_ctx {
   double *sl[3][9];
   /* gravity */
   double *grvphi, *grvpsi, *grvphiorg[2], *lapphi, *hgsrc, *gsrc;
+  double *lapc[5]; /* lap_coeff(-1:3,...) for crdnt 0 (no reference counterpart) */
   double *dti;
   double ch;
   int err_i, err_j, err_k, err_dir;
@@ -155,7 +156,27 @@ or_ctx *or_create(const hg_desc *desc, int *status) {
       c->grvphiorg[0] = alloc_field(N); c->grvphiorg[1] = alloc_field(N);
       c->lapphi = alloc_field(N); c->hgsrc = alloc_field(N);
       c->gsrc = alloc_field(N);
+      for (int q = 0; q < 5; ++q) c->lapc[q] = alloc_field(N);
     }
+  }
+  if (desc->gravswitch == 3) {
+    /* Cartesian lap_coeff: the 3D form of gravity_hyperbolic.f90:382-406
+       with unit metric factors (the reference stops for crdnt 0, :413-417;
+       this is new).  Normalisation lap_coeff(-1) = cell volume. */
+    const double *X1 = c->dxis[0], *X2 = c->dxis[1], *X3 = c->dxis[2];
+    const double *I1 = c->idxs[0], *I2 = c->idxs[1], *I3 = c->idxs[2];
+    for (int k = c->ks - 1; k <= c->ke; ++k)
+      for (int j = c->js - 1; j <= c->je; ++j)
+        for (int i = c->is - 1; i <= c->ie; ++i) {
+          size_t m = ID(c, i, j, k);
+          c->lapc[1][m] = -((I1[i] + I1[i + 1]) * X2[j] * X3[k] +
+                            (I2[j] + I2[j + 1]) * X1[i] * X3[k] +
+                            (I3[k] + I3[k + 1]) * X1[i] * X2[j]);
+          c->lapc[2][m] = X2[j] * X3[k] * I1[i + 1];
+          c->lapc[3][m] = X1[i] * X3[k] * I2[j + 1];
+          c->lapc[4][m] = X1[i] * X2[j] * I3[k + 1];
+          c->lapc[0][m] = X1[i] * X2[j] * X3[k];
+        }
   }
   if (status) *status = HG_OK;
   return c;
@@ -170,7 +191,8 @@ void or_destroy(or_ctx *c) {
   double *f[] = {c->d, c->v1, c->v2, c->v3, c->e, c->p, c->eint, c->b1,
                  c->b2, c->b3, c->phi, c->ptot, c->cs, c->dti, c->grvphi,
                  c->grvpsi, c->grvphiorg[0], c->grvphiorg[1], c->lapphi,
-                 c->hgsrc, c->gsrc};
+                 c->hgsrc, c->gsrc, c->lapc[0], c->lapc[1], c->lapc[2], c->lapc[3],
+                 c->lapc[4]};
   for (size_t a = 0; a < sizeof(f) / sizeof(f[0]); ++a) free(f[a]);
   for (int n = 0; n < NV; ++n) {
     free(c->u[n]); free(c->uorg[n]); free(c->src[n]);
@@ -208,6 +230,11 @@ static double *field_ptr(or_ctx *c, int field) {
 int or_upload(or_ctx *c, int field, const double *a) {
   double *f = field_ptr(c, field);
   if (!f || !a) return HG_ERR_ARG;
+  if (field == HG_F_GRVPHI) {
+    /* the potential's ghost layer carries the gravity boundary values */
+    memcpy(f, a, sizeof(double) * c->ncell);
+    return HG_OK;
+  }
   for (int k = c->ks; k <= c->ke; ++k)
     for (int j = c->js; j <= c->je; ++j)
       for (int i = c->is; i <= c->ie; ++i) f[ID(c, i, j, k)] = a[ID(c, i, j, k)];
@@ -887,6 +914,83 @@ int or_timestep(or_ctx *c, double time, double t_out, double t_end,
     if (cgrav_out) *cgrav_out = cgrav;
     if (dtgrav_out) *dtgrav_out = dtgrav;
   }
+  return HG_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* hyperbolic self-gravity, gravity_hyperbolic.f90:19-125,230-280 and       */
+/* gravity.f90:114-150 (gsrc = rho), for crdnt 0 with the potential's ghost */
+/* layer held at the values uploaded with HG_F_GRVPHI (Dirichlet).          */
+/* ------------------------------------------------------------------------ */
+
+static void hg_stage(or_ctx *c, int grungen, double dtg, double cg) {
+  double faco, fact, facn;
+  get_runge_coeff(grungen, c->prm.grktype, &faco, &fact, &facn);
+  const double fourpig = 4.0 * acos(-1.0) * c->prm.G;
+  size_t st[3] = {1, (size_t)c->nxg, (size_t)c->nxg * c->nyg};
+#pragma omp parallel for collapse(2) num_threads(NT)
+  for (int k = c->ks; k <= c->ke; ++k)
+    for (int j = c->js; j <= c->je; ++j)
+      for (int i = c->is; i <= c->ie; ++i) {
+        size_t m = ID(c, i, j, k);
+        const double *ph = c->grvphi;
+        double lap = c->lapc[1][m] * ph[m];
+        lap = lap + c->lapc[2][m - st[0]] * ph[m - st[0]];
+        lap = lap + c->lapc[2][m] * ph[m + st[0]];
+        if (c->je > c->js) {
+          lap = lap + c->lapc[3][m - st[1]] * ph[m - st[1]];
+          lap = lap + c->lapc[3][m] * ph[m + st[1]];
+        }
+        if (c->ke > c->ks) {
+          lap = lap + c->lapc[4][m - st[2]] * ph[m - st[2]];
+          lap = lap + c->lapc[4][m] * ph[m + st[2]];
+        }
+        c->lapphi[m] = lap / c->lapc[0][m];
+        c->hgsrc[m] = fourpig * c->gsrc[m];
+      }
+#pragma omp parallel for collapse(2) num_threads(NT)
+  for (int k = c->ks; k <= c->ke; ++k)
+    for (int j = c->js; j <= c->je; ++j)
+      for (int i = c->is; i <= c->ie; ++i) {
+        size_t m = ID(c, i, j, k);
+        if (grungen == 1) {
+          c->grvphiorg[0][m] = c->grvphi[m];
+          c->grvphiorg[1][m] = c->grvpsi[m];
+        }
+        c->grvphi[m] = faco * c->grvphiorg[0][m] + facn * c->grvphi[m] +
+                       fact * dtg * cg * c->grvpsi[m];
+        c->grvpsi[m] = faco * c->grvphiorg[1][m] + facn * c->grvpsi[m] +
+                       fact * dtg * cg * (c->lapphi[m] - c->hgsrc[m]);
+      }
+}
+
+int or_gravity(or_ctx *c, double time, double dt, double *grvtime, double cgrav,
+               double *cgrav_old, double *dtgrav, int *nsub) {
+  if (c->prm.gravswitch != 3) return HG_ERR_ARG;
+#pragma omp parallel for collapse(2) num_threads(NT)
+  for (int k = c->ks; k <= c->ke; ++k)
+    for (int j = c->js; j <= c->je; ++j)
+      for (int i = c->is; i <= c->ie; ++i) {
+        size_t m = ID(c, i, j, k);
+        c->gsrc[m] = c->d[m];
+      }
+  int tngrav = (int)ceil((time + dt - *grvtime) / *dtgrav);
+  *dtgrav = (time + dt - *grvtime) / (double)tngrav;
+  for (int l = 1; l <= tngrav; ++l) {
+    for (int g = 1; g <= c->prm.grktype; ++g) hg_stage(c, g, *dtgrav, cgrav);
+    const double damp =
+        exp(-(1.0 - *cgrav_old / cgrav + c->prm.alphagrv * c->prm.hgcfl));
+#pragma omp parallel for collapse(2) num_threads(NT)
+    for (int k = c->ks; k <= c->ke; ++k)
+      for (int j = c->js; j <= c->je; ++j)
+        for (int i = c->is; i <= c->ie; ++i) {
+          size_t m = ID(c, i, j, k);
+          c->grvpsi[m] = c->grvpsi[m] * damp;
+        }
+    *grvtime = *grvtime + *dtgrav;
+    *cgrav_old = cgrav;
+  }
+  if (nsub) *nsub = tngrav;
   return HG_OK;
 }
 

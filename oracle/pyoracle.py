@@ -44,6 +44,8 @@ def lib():
         L.or_phidamp.argtypes = [vp, C.c_double]
         L.or_last_error.argtypes = [vp] + [C.POINTER(C.c_int)] * 4
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_gravity.argtypes = [vp, C.c_double, C.c_double, _dp, C.c_double, _dp, _dp,
+                                 C.POINTER(C.c_int)]
         L.or_kat_hlldflux.argtypes = [_dp, _dp]
         L.or_kat_get_cf.argtypes = [C.c_double] * 5
         L.or_kat_get_cf.restype = C.c_double
@@ -99,7 +101,7 @@ class Oracle:
     def upload(self, field, padded):
         a = np.ascontiguousarray(padded, dtype=np.float64)
         self._check(self._lib.or_upload(self.h, field, a.ctypes.data_as(_dp)), "upload")
-        if field < 9:
+        if field < 9 or field >= 16:
             self._dirty = True
 
     def download(self, field, padded):
@@ -111,13 +113,23 @@ class Oracle:
             self._check(self._lib.or_pressure(self.h), "pressure")
             self._dirty = False
 
-    def timestep(self, time, t_out, t_end, outstyle, endstyle, fixed_dt=-1.0):
+    def timestep(self, time, t_out, t_end, outstyle, endstyle, fixed_dt=-1.0, grav=False):
         self._ensure()
         dt, ch, cg, dtg = (C.c_double() for _ in range(4))
         self._check(self._lib.or_timestep(self.h, time, t_out, t_end, outstyle, endstyle,
                                           fixed_dt, C.byref(dt), C.byref(ch), C.byref(cg),
                                           C.byref(dtg)), "timestep")
+        if grav:
+            return dt.value, ch.value, cg.value, dtg.value
         return dt.value, ch.value
+
+    def gravity(self, time, dt, grvtime, cgrav, cgrav_old, dtgrav):
+        self._ensure()
+        gt, co, dg, ns = C.c_double(grvtime), C.c_double(cgrav_old), C.c_double(dtgrav), \
+            C.c_int(0)
+        self._check(self._lib.or_gravity(self.h, time, dt, C.byref(gt), cgrav, C.byref(co),
+                                         C.byref(dg), C.byref(ns)), "gravity")
+        return gt.value, co.value, dg.value, ns.value
 
     def hydro_step(self, dt, ch):
         self._ensure()

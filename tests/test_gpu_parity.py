@@ -12,7 +12,7 @@ import pytest
 
 from hormone_amd.native import Hydro
 from oracle.pyoracle import Oracle, lib as oracle_lib
-from tests.helpers import compare_to_meta, compare_to_npz, meta, run_problem
+from tests.helpers import compare_to_meta, compare_to_npz, digest, meta, run_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -131,17 +131,68 @@ def test_gpu_device_loop_goldens(hiplib, name):
         compare_to_npz(res, state, name)
 
 
-def test_gpu_ot512_full_run_device_loop(hiplib):
-    """The benchmark workload to t=0.5 equals the reference program's run."""
-    entry = meta().get("ot512_t05")
-    if entry is None or entry["tn"] == 0:
-        pytest.skip("reference run fixture not generated")
+def _ot512_engine():
     from hormone_amd import driver as D
     from hormone_amd import problems as P
     prm, g, st = P.make_problem("orszagtang_xy", n=512)
     eng = Hydro(**D.desc_fields(prm, g))
     D.load_initial_state(eng, prm, g, st)
-    res = D.run(eng, prm, g, device_loop=True)
+    return prm, g, eng
+
+
+def test_gpu_ot512_2200_steps_and_reference_stop(hiplib):
+    """The benchmark workload: state after 2200 steps equals the reference
+    program's, and the run then stops where the reference stops (negative
+    internal energy at the same cell, same step)."""
+    import math
+    from hormone_amd import driver as D
+    from hormone_amd.native import HG_ERR_NEG_EINT, HgError
+    m = meta()
+    if "ot512_2200steps" not in m or "ot512_t05_failure" not in m:
+        pytest.skip("reference-run fixtures not generated")
+    prm, g, eng = _ot512_engine()
+    t_end, dt_out, t_out = prm.times()
+    damp = math.exp(-prm.alpha9wave * prm.courant)
+    tim, tn, t_out, reason = eng.run_loop(0.0, 0, t_out, dt_out, t_end, 1, 1, 0, 1, 2200, damp)
+    assert tn == 2200
     out = D.fetch(eng, g, D.SNAP_FIELDS_MAG)
     names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
-    compare_to_meta(res, {names[i]: out[f] for i, f in enumerate(D.SNAP_FIELDS_MAG)}, entry)
+    state = {names[q]: out[f] for q, f in enumerate(D.SNAP_FIELDS_MAG)}
+    e = m["ot512_2200steps"]
+    assert tim == e["time"]
+    for nm in names:
+        assert digest(state[nm]) == e["fields"][nm]["sha256"], nm
+    fail = m["ot512_t05_failure"]
+    with pytest.raises(HgError) as ei:
+        eng.run_loop(tim, tn, t_out, dt_out, t_end, 1, 1, 0, 1, 1000, damp)
+    assert ei.value.code == HG_ERR_NEG_EINT
+    assert ei.value.tn == fail["completed_steps"]
+    assert list(ei.value.cell) == fail["cell"]
+
+
+def test_gpu_gravity_laplacian_quadratic(hiplib):
+    from tests.test_gravity import lap_from_one_substep, quadratic_setup
+    prm, g, eng = quadratic_setup(Hydro)
+    lap = lap_from_one_substep(eng, g, prm)
+    assert np.max(np.abs(lap - 12.0)) < 1e-9
+
+
+@pytest.mark.parametrize("mag_on", [False, True], ids=["hydro", "mhd"])
+def test_gpu_polytrope_gravity_matches_oracle(hiplib, mag_on):
+    """3D hydro (or MHD with B=0) + hyperbolic gravity: every field and the
+    potential equal the oracle bit for bit after 12 steps at 32^3."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print(f"case polytrope32 mag_on={mag_on}", flush=True)
+    res = []
+    for cls in (Hydro, Oracle):
+        prm, g, st = P.make_problem("polytrope3d", n=32, steps=12, mag_on=mag_on)
+        eng = cls(**D.desc_fields(prm, g))
+        D.load_initial_state(eng, prm, g, st)
+        r = D.run(eng, prm, g)
+        flds = (D.SNAP_FIELDS_MAG if mag_on else D.SNAP_FIELDS) + (D.F_GRVPHI, D.F_GRVPSI)
+        res.append((r, D.fetch(eng, g, flds)))
+    (r1, s1), (r2, s2) = res
+    assert r1.tn == r2.tn == 12 and r1.time == r2.time and r1.nsub == r2.nsub
+    for f in s2:
+        assert np.array_equal(s1[f].view(np.uint64), s2[f].view(np.uint64)), f
