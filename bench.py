@@ -1,46 +1,65 @@
 #!/usr/bin/env python3
 """Zone-updates/s of HORMONE's explicit hydro step on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): 2D
-Orszag-Tang MHD vortex, 512^2 periodic on [0,1]^2, gamma 5/3, MUSCL
-(modified MC) + HLLD with GLM cleaning, SSP-RK3, courant 0.9, FP64, run to
-t = 0.5.  Synthetic initial data (the closed-form vortex); state resident in
-HBM.  A step is one full timestep (CFL reduction + 3 RK stages + phidamp).
+Default workload (the configuration the headline metric is quoted on):
+2D Orszag-Tang MHD vortex, 512^2 periodic on [0,1]^2, gamma 5/3, MUSCL
+(modified MC) + HLLD with GLM cleaning, SSP-RK3, courant 0.9, FP64.  The
+initial data is the closed-form vortex (synthetic); the state stays in HBM.
+A step is one full timestep (CFL reduction + 3 RK stages + phidamp), and a
+zone-update is one interior cell advanced by one step.
 
 Timed region: K steps after W warm-up steps, through the device-resident
-main loop (hg_run).  After it the run continues, untimed, to step 2200, where
-the state is compared bit for bit with the reference program's own run
-(tests/golden/golden_meta.json, entry ot512_2200steps); then on until the
-reference's own FP64 run stops (negative internal energy during step 2250,
-before t = 0.5), which must happen at the same step and cell.
+main loop (hg_run), bracketed by torch.cuda.synchronize().  Afterwards the
+run continues, untimed, to step 2200, where the state is compared bit for bit
+with the reference program's own run (tests/golden/golden_meta.json, entry
+ot512_2200steps).  It then runs on until the reference's own FP64 run stops
+(negative internal energy during step 2250, before t = 0.5); the stop must
+happen at the same step and cell.
 
-The roofline object is for the dominant kernel (the fused direction sweep),
-timed with HIP events on the library's stream inside the timed region; its
-algorithmic bytes per launch are dim*(nw+nv)*8 per cell (primitives read
-once, one flux divergence written per direction).  cpu_baseline is the CPU
-oracle (a C/OpenMP restatement of the reference) on this host's cores over a
-bounded prefix of the same run.
+Other workloads (--workload):
+  polytrope256  3D 256^3 hydro + Cartesian hyperbolic self-gravity, n=1
+                polytrope (M=1, R=0.3, G=1, gamma=2) in [-1,1]^3, 200 steps;
+                reports mass and energy drift
+  kh384         3D 384^3 MHD Kelvin-Helmholtz, periodic, 100 steps
+With --precision fp32mix and --vs-fp64, the same workload is also run in FP64
+and the per-variable relative L1 sum|q-q64|dV / sum|q64|dV is reported.
+
+roofline: the dominant kernel is the fused direction sweep.  Its average
+launch duration comes from HIP events on the library's stream inside the
+timed region.  Its algorithmic bytes per launch are dim*(nw+nv)*8 per cell:
+primitives read once, one flux divergence written per direction.
+cpu_baseline: the CPU oracle (C/OpenMP restatement of the reference) on this
+host's cores over a bounded prefix of the same run.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
+import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+NAMES = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="ot512", choices=["ot512", "polytrope256", "kh384"])
+    ap.add_argument("--n", type=int, default=None, help="cells per side (default per workload)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32mix"])
+    ap.add_argument("--mag", action="store_true", help="polytrope: MHD variant with B=0")
+    ap.add_argument("--vs-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -48,28 +67,100 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(n: int, seconds: float):
+def setup(a, precision):
+    from hormone_amd import problems as P
+    if a.workload == "ot512":
+        return P.make_problem("orszagtang_xy", n=a.n or 512, precision=precision)
+    if a.workload == "polytrope256":
+        return P.make_problem("polytrope3d", n=a.n or 256, steps=a.warmup + a.steps,
+                              mag_on=a.mag, precision=precision)
+    return P.make_problem("kh3d", n=a.n or 384, steps=a.warmup + a.steps, precision=precision)
+
+
+def cpu_baseline(a, seconds: float):
     """Oracle (C/OpenMP restatement of the reference) on the host cores."""
     from hormone_amd import driver as D
-    from hormone_amd import problems as P
     from oracle import pyoracle
 
-    prm, g, st = P.make_problem("orszagtang_xy", n=n)
-    eng = pyoracle.Oracle(**D.desc_fields(prm, g))
+    prm, g, st = setup(a, "fp64")
     pyoracle.set_threads(0)
     cores = pyoracle.lib().or_get_threads()
+    eng = pyoracle.Oracle(**D.desc_fields(prm, g))
+    pyoracle.set_threads(0)
     D.load_initial_state(eng, prm, g, st)
     t0 = time.perf_counter()
     D.run(eng, prm, g, max_steps=2)
     per = (time.perf_counter() - t0) / 2
     steps = max(2, min(200, int(seconds / max(per, 1e-6))))
     eng2 = pyoracle.Oracle(**D.desc_fields(prm, g))
+    pyoracle.set_threads(0)
     D.load_initial_state(eng2, prm, g, st)
     r = D.run(eng2, prm, g, max_steps=steps)
     zus = r.tn * g.ncells / r.wall_s
     return {"value": zus, "unit": "zone-updates/s", "cores": cores, "kind": "port",
-            "sample": f"first {r.tn} steps of the same {n}^2 Orszag-Tang run "
+            "sample": f"first {r.tn} steps of the same {a.workload} run "
                       f"({r.wall_s:.1f} s, {cores} OpenMP threads)"}
+
+
+class Runner:
+    """Drives one engine through the reference's main loop, step by step or
+    with the device-resident loop, keeping the clock on the host."""
+
+    def __init__(self, eng, prm, g):
+        self.eng, self.prm, self.g = eng, prm, g
+        self.t_end, self.dt_out, self.t_out = prm.times()
+        self.damp = math.exp(-prm.alpha9wave * prm.courant)
+        self.tim, self.tn = 0.0, 0
+        self.grvtime, self.cgrav_old = 0.0, 0.0
+        self.nsub = []
+
+    def advance(self, k):
+        """k more steps; False when the run's end is reached."""
+        from hormone_amd import native
+        prm, eng = self.prm, self.eng
+        if prm.gravswitch == 3:
+            for _ in range(k):
+                dt, ch, cg, dtg = eng.timestep(self.tim, self.t_out, self.t_end, prm.outstyle,
+                                               prm.endstyle, prm.fixed_dt, grav=True)
+                if self.tn == 0:
+                    self.grvtime = 0.0
+                else:
+                    self.grvtime, self.cgrav_old, _, ns = eng.gravity(
+                        self.tim, dt, self.grvtime, cg, self.cgrav_old, dtg)
+                    self.nsub.append(ns)
+                eng.hydro_step(dt, ch)
+                if prm.mag_on and self.g.dim > 1:
+                    eng.phidamp(self.damp)
+                self.tim += dt
+                self.tn += 1
+            return True
+        left = k
+        while left > 0:
+            n0 = self.tn
+            try:
+                self.tim, self.tn, self.t_out, reason = eng.run_loop(
+                    self.tim, self.tn, self.t_out, self.dt_out, self.t_end, prm.outstyle,
+                    prm.endstyle, prm.tnlim, prm.tn_out, left, self.damp)
+            except native.HgError as err:
+                self.tn, self.tim = err.tn, err.time
+                raise
+            left -= self.tn - n0
+            if reason == native.STOP_END:
+                return left <= 0
+        return True
+
+
+def sums(D, eng, g, prm):
+    """Mtot, Etot (+ gravitational energy) as output.f90:167-200."""
+    f = [D.F_D, D.F_E] + ([D.F_GRVPHI] if prm.gravswitch == 3 else [])
+    s = D.fetch(eng, g, f)
+    dv = (g.ax[0].dxi[2:-2][None, None, :] * g.ax[1].dxi[2:-2][None, :, None]
+          * g.ax[2].dxi[2:-2][:, None, None])
+    m = float((s[D.F_D] * dv).sum())
+    e = float((s[D.F_E] * dv).sum())
+    if prm.gravswitch == 3:
+        e += float((s[D.F_D] * 0.5 * s[D.F_GRVPHI] * dv).sum())
+    return m, e
 
 
 def main():
@@ -78,101 +169,115 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               f"(the sharded path is a design only).", file=sys.stderr)
         sys.exit(2)
+    defaults = {"ot512": (2000, 20), "polytrope256": (190, 10), "kh384": (90, 10)}
+    if a.steps is None:
+        a.steps = defaults[a.workload][0]
+    if a.warmup is None:
+        a.warmup = defaults[a.workload][1]
     import torch
 
     from hormone_amd import driver as D
     from hormone_amd import native
-    from hormone_amd import problems as P
 
-    prm, g, st = P.make_problem("orszagtang_xy", n=a.n)
+    prm, g, st = setup(a, a.precision)
     eng = native.Hydro(**D.desc_fields(prm, g))
     D.load_initial_state(eng, prm, g, st)
-    t_end, dt_out, t_out = prm.times()
-    import math
-    damp = math.exp(-prm.alpha9wave * prm.courant)
-    tim, tn = 0.0, 0
+    run = Runner(eng, prm, g)
+    m0, e0 = sums(D, eng, g, prm)
 
-    def advance(k):
-        nonlocal tim, tn, t_out
-        left = k
-        while left > 0:
-            n0 = tn
-            try:
-                tim, tn, t_out, reason = eng.run_loop(tim, tn, t_out, dt_out, t_end,
-                                                      prm.outstyle, prm.endstyle, prm.tnlim,
-                                                      prm.tn_out, left, damp)
-            except native.HgError as err:
-                tn, tim = err.tn, err.time
-                raise
-            left -= tn - n0
-            if reason == native.STOP_END:
-                return False
-        return True
-
-    advance(a.warmup)
+    run.advance(a.warmup)
     if not a.no_profile:
         eng.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ok = advance(a.steps)
+    ok = run.advance(a.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    steps_done = a.steps if ok else None
     assert ok, "the run ended inside the timed region; lower --steps"
-    zus = steps_done * g.ncells / wall
+    zus = a.steps * g.ncells / wall
     roof = None
     if not a.no_profile:
         ms, nl, bpl = eng.profile_read(native.PROF_SWEEP)
-        rk_ms, rk_n, _ = eng.profile_read(native.PROF_RK)
+        rk_ms, rk_n, rk_b = eng.profile_read(native.PROF_RK)
+        gr_ms, gr_n, gr_b = eng.profile_read(native.PROF_GRAV)
         eng.profile(False)
         avg_s = ms / 1e3 / nl
         ach = bpl / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_sweep (x+y directions, one launch per RK stage)",
+                "kernel": "k_sweep" + ("_mix" if a.precision == "fp32mix" else "")
+                          + " (all solved directions, one launch per RK stage)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": nl,
                 "algorithmic_bytes_per_launch": bpl,
-                "share_of_step": round(ms / (wall * 1e3), 3),
-                "rk_avg_launch_us": round(rk_ms / max(rk_n, 1) * 1e3, 2)}
+                "share_of_timed_region": round(ms / (wall * 1e3), 3),
+                "rk_avg_launch_us": round(rk_ms / max(rk_n, 1) * 1e3, 2),
+                "rk_achieved_GBs": round(rk_b / (rk_ms / max(rk_n, 1) / 1e3) / 1e9, 1)
+                if rk_n else None}
+        if gr_n:
+            roof["grav_avg_launch_us"] = round(gr_ms / gr_n * 1e3, 2)
+            roof["grav_achieved_GBs"] = round(gr_b / (gr_ms / gr_n / 1e3) / 1e9, 1)
     parity = None
-    if not a.no_check and a.n == 512:
-        # Untimed: continue to step 2200 and compare with the reference
-        # program's own run, then run on until the reference's stop (its
-        # FP64 run ends with negative internal energy before t = 0.5).
-        import hashlib
-        import numpy as np
+    extra = {}
+    if a.workload == "ot512" and a.precision == "fp64" and not a.no_check and (a.n or 512) == 512:
         from tests.helpers import meta
         m = meta()
-        advance(2200 - tn)
+        run.advance(2200 - run.tn)
         out = D.fetch(eng, g, D.SNAP_FIELDS_MAG)
-        names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
         e = m.get("ot512_2200steps")
-        same = e is not None and tn == e["tn"] and tim == e["time"] and all(
+        same = e is not None and run.tn == e["tn"] and run.tim == e["time"] and all(
             hashlib.sha256(np.ascontiguousarray(out[f], "<f8").tobytes()).hexdigest()
-            == e["fields"][nm]["sha256"] for f, nm in zip(D.SNAP_FIELDS_MAG, names))
-        parity = {"step": tn, "bitwise_equal_to_reference_run_at_step_2200": bool(same)}
+            == e["fields"][nm]["sha256"] for f, nm in zip(D.SNAP_FIELDS_MAG, NAMES))
+        parity = {"step": run.tn, "bitwise_equal_to_reference_run_at_step_2200": bool(same)}
         fail = m.get("ot512_t05_failure")
         try:
-            advance(1 << 30)
+            run.advance(1 << 30)
             parity["reference_stop_reproduced"] = False
         except native.HgError as err:
-            parity["stop"] = {"code": err.code, "completed_steps": err.tn,
-                              "cell": list(err.cell)}
+            parity["stop"] = {"code": err.code, "completed_steps": err.tn, "cell": list(err.cell)}
             parity["reference_stop_reproduced"] = bool(
                 fail and err.code == native.HG_ERR_NEG_EINT
                 and err.tn == fail["completed_steps"] and list(err.cell) == fail["cell"])
-    cpu = None if a.no_cpu else cpu_baseline(a.n, a.cpu_seconds)
+    if a.workload != "ot512" or a.precision != "fp64":
+        m1, e1 = sums(D, eng, g, prm)
+        extra["mass_drift"] = (m1 - m0) / m0
+        extra["energy_drift"] = (e1 - e0) / abs(e0)
+        if run.nsub:
+            extra["gravity_substeps_mean"] = float(np.mean(run.nsub))
+    if a.vs_fp64 and a.precision == "fp32mix":
+        fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+        mine = D.fetch(eng, g, fields)
+        n_done = run.tn
+        del eng
+        prm64, g64, st64 = setup(a, "fp64")
+        eng64 = native.Hydro(**D.desc_fields(prm64, g64))
+        D.load_initial_state(eng64, prm64, g64, st64)
+        r64 = Runner(eng64, prm64, g64)
+        r64.advance(n_done)
+        ref = D.fetch(eng64, g64, fields)
+        l1 = {}
+        for q, f in enumerate(fields):
+            den = float(np.abs(ref[f]).sum())
+            num = float(np.abs(mine[f] - ref[f]).sum())
+            l1[NAMES[q]] = num / den if den > 0 else num
+        extra["rel_l1_vs_fp64"] = l1
+        extra["rel_l1_steps"] = n_done
+    cpu = None if a.no_cpu else cpu_baseline(a, a.cpu_seconds)
+    wl = {"ot512": f"orszagtang_xy {g.ax[0].n}x{g.ax[1].n} MHD+GLM, MUSCL+HLLD, RK3, "
+                   f"courant 0.9 (run toward t=0.5)",
+          "polytrope256": f"polytrope {g.ax[0].n}^3 {'MHD B=0' if prm.mag_on else 'hydro'} + "
+                          f"hyperbolic self-gravity, gamma 2, {prm.tnlim} steps",
+          "kh384": f"Kelvin-Helmholtz {g.ax[0].n}^3 MHD, periodic, {prm.tnlim} steps"}
     line = {
-        "metric": "zone-updates/s (FP64, Orszag-Tang 512^2 MHD)",
+        "metric": f"zone-updates/s ({a.precision.upper()}, {a.workload})",
         "value": zus, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": wall / steps_done * 1e3,
+        "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (closed-form Orszag-Tang initial state)",
-        "config": {"workload": f"orszagtang_xy {a.n}x{a.n} MHD+GLM, MUSCL+HLLD, RK3, "
-                               f"courant 0.9, FP64 (run toward t=0.5)", "cells": g.ncells,
-                   "precision": "fp64"},
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic (closed-form initial state)",
+        "config": {"workload": wl[a.workload], "cells": g.ncells, "precision": a.precision},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
     }
+    line.update(extra)
     print(json.dumps(line))
 
 

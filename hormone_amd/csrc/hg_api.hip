@@ -68,7 +68,7 @@ static int validate(const hg_desc *d) {
   }
   if (d->gravswitch != 0 && d->gravswitch != 3) return HG_ERR_UNSUPPORTED;
   if (d->gravswitch == 3 && (d->grktype < 1 || d->grktype > 2)) return HG_ERR_ARG;
-  if (d->precision != HG_PREC_FP64) return HG_ERR_UNSUPPORTED;
+  if (d->precision != HG_PREC_FP64 && d->precision != HG_PREC_FP32MIX) return HG_ERR_ARG;
   const double *const *arr[] = {&d->x1, &d->xi1, &d->dx1, &d->dxi1, &d->idx1, &d->idxi1,
                                 &d->x2, &d->xi2, &d->dx2, &d->dxi2, &d->idx2, &d->idxi2,
                                 &d->x3, &d->xi3, &d->dx3, &d->dxi3, &d->idx3, &d->idxi3};
@@ -120,6 +120,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
     D.i9wv = 8;
   }
   D.nw = D.mag ? 10 : 6;
+  D.mixed = desc->precision == HG_PREC_FP32MIX;
   // coordinates: 6 arrays x 3 axes, each n+4 long
   const int nn[3] = {D.n1, D.n2, D.n3};
   size_t tot = 0;
@@ -156,6 +157,28 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
     if (hipMemcpy(c->coord, h.data(), tot * sizeof(double), hipMemcpyHostToDevice) !=
         hipSuccess)
       return fail(HG_ERR_DEVICE);
+    // FP32-mixed limiter geometry, indices -2..n+1 per axis
+    for (int a = 0; a < 3; ++a) {
+      const int m = nn[a] + 4;
+      std::vector<float> g4(4 * (size_t)m, 0.f), gi(m, 0.f);
+      const double *x = src[a][0], *xi = src[a][1], *idx = src[a][4];
+      for (int q = 1; q < m - 1; ++q) {  // array element q holds cell q-2
+        const double xm = x[q - 1], x0 = x[q], xp = x[q + 1], xim = xi[q - 1], xi0 = xi[q];
+        g4[4 * q + 0] = (float)((xp - x0) / (xi0 - x0));
+        g4[4 * q + 1] = (float)((x0 - xm) / (x0 - xim));
+        g4[4 * q + 2] = (float)((xp - x0) / (x0 - xm));
+        g4[4 * q + 3] = (float)(1.0 / (xp - x0));
+      }
+      for (int q = 0; q < m; ++q) gi[q] = (float)idx[q];
+      float *dg = nullptr, *di = nullptr;
+      if ((st = dev_alloc(c, sizeof(float) * 4 * m, (void **)&dg))) return fail(st);
+      if ((st = dev_alloc(c, sizeof(float) * m, (void **)&di))) return fail(st);
+      if (hipMemcpy(dg, g4.data(), sizeof(float) * 4 * m, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(di, gi.data(), sizeof(float) * m, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(HG_ERR_DEVICE);
+      D.ax[a].limg = reinterpret_cast<const float4 *>(dg) + 2;
+      D.ax[a].idxf = di + 2;
+    }
   }
   const size_t fb = sizeof(double) * (size_t)D.ncell;
   for (int w = 0; w < D.nw; ++w)

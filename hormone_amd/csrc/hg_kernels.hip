@@ -275,6 +275,188 @@ __global__ __launch_bounds__(256) void k_sweep(Dev D, int stage) {
   if (wave < D.waves[2]) sweep_line<2, MAG, LIM>(D, wave, lane, stage);
 }
 
+
+// ---- FP32-mixed sweep -------------------------------------------------------
+// Same mapping and stencil as sweep_line.  Reconstruction (limited slopes),
+// the EOS/fast speeds and the HLLD flux run in FP32 with the hardware
+// reciprocal and square root; the cell state, the face states, the face
+// internal energy (the subtraction of kinetic and magnetic energy from the
+// total energy), the e_int guard decision and the flux-divergence
+// accumulation stay in FP64.
+__device__ __forceinline__ float up1f(float x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ float dn1f(float x) { return __shfl_down(x, 1, 64); }
+
+__device__ __forceinline__ float mc_mixed(float u0, float u1, float u2, float4 g) {
+  const float top = (u1 - u0) * g.z;
+  const float bot = u2 - u1;
+  const float sig = sgn(1.0f, bot);
+  return sig * mx2(0.0f, mn3(sig * 0.5f * (bot + top), sig * g.x * bot, sig * g.y * top)) * g.w;
+}
+
+template <int DIR, bool MAG, int LIM>
+__device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane, int stage) {
+  const Axis &A = D.ax[DIR];
+  const int n = A.n;
+  const long nlines = D.ncell / n;
+  const long line = wave % nlines;
+  const int seg = (int)(wave / nlines);
+  long base;
+  int ci, cj, ck;
+  if (DIR == 0) {
+    base = line * D.n1;
+    ci = 0; cj = (int)(line % D.n2); ck = (int)(line / D.n2);
+  } else if (DIR == 1) {
+    const long i = line % D.n1, k = line / D.n1;
+    base = i + k * (long)D.n1 * D.n2;
+    ci = (int)i; cj = 0; ck = (int)k;
+  } else {
+    base = line;
+    ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
+  }
+  const int c = seg * SEG - 2 + lane;
+  const Cell q = load_cell<DIR, MAG>(D, A, base, c);
+  const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
+  const double qd[8] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2]};
+  const int cc = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
+
+  float S[8];
+  if (LIM == LIM_FLAT) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) S[s] = 0.0f;
+  } else {
+    const int cl = cc > -1 ? cc : -1, cr = cc < n ? cc : n;
+    const float4 geo = A.limg[cl];
+    const float ida = A.idxf[cl], idb = A.idxf[cr + 1];
+    const double hl = A.x[cl] - A.xi[cl - 1], hr = A.xi[cl] - A.x[cl];
+    double fl[8], fr[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      if (!MAG && s >= 5) {
+        S[s] = 0.0f;
+        fl[s] = fr[s] = 0.0;
+        continue;
+      }
+      const float u1 = (float)qd[s];
+      const float u0 = up1f(u1), u2 = dn1f(u1);
+      const float du = (LIM == LIM_MINMOD) ? minmod(u0, u1, u2, ida, idb) : mc_mixed(u0, u1, u2, geo);
+      S[s] = du;
+      fl[s] = qd[s] - hl * (double)du;
+      fr[s] = qd[s] + hr * (double)du;
+    }
+    // e_int guard in FP64 (interpolation.f90:143-163)
+    double eintl = fl[1] - 0.5 * (fl[2] * fl[2] + fl[3] * fl[3] + fl[4] * fl[4]) / fl[0];
+    double eintr = fr[1] - 0.5 * (fr[2] * fr[2] + fr[3] * fr[3] + fr[4] * fr[4]) / fr[0];
+    if (MAG) {
+      eintl = eintl - 0.5 * (fl[5] * fl[5] + fl[6] * fl[6] + fl[7] * fl[7]);
+      eintr = eintr - 0.5 * (fr[5] * fr[5] + fr[6] * fr[6] + fr[7] * fr[7]);
+    }
+    const double el_ = up1(q.eint), em_ = q.eint, er_ = dn1(q.eint);
+    if (eintl >= mx2(el_, em_) || eintl <= mn2(el_, em_) || eintr >= mx2(em_, er_) ||
+        eintr <= mn2(em_, er_)) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) S[s] = 0.0f;
+    }
+  }
+
+  // face c+1/2: right cell from lane+1
+  double rq[8];
+  float RS[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    rq[s] = (!MAG && s >= 5) ? 0.0 : dn1(qd[s]);
+    RS[s] = (!MAG && s >= 5) ? 0.0f : dn1f(S[s]);
+  }
+  const float rphi = MAG ? dn1f((float)q.phi) : 0.0f;
+  const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;
+  const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
+  constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
+  double L[8], R_[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    L[s] = qd[s] + dx1 * (double)S[s];
+    R_[s] = rq[s] - dx2 * (double)RS[s];
+  }
+  // face internal energy in FP64 from the face conserved variables
+  double eil = L[1] - 0.5 * (L[2] * L[2] + L[3] * L[3] + L[4] * L[4]) / L[0];
+  double eir = R_[1] - 0.5 * (R_[2] * R_[2] + R_[3] * R_[3] + R_[4] * R_[4]) / R_[0];
+  if (MAG) {
+    eil = eil - 0.5 * (L[5] * L[5] + L[6] * L[6] + L[7] * L[7]);
+    eir = eir - 0.5 * (R_[5] * R_[5] + R_[6] * R_[6] + R_[7] * R_[7]);
+  }
+  const bool face_valid = (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
+  if (face_valid && (eil <= 0.0 || L[0] <= 0.0 || eir <= 0.0 || R_[0] <= 0.0)) {
+    int ii = ci, jj = cj, kk = ck;
+    if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
+    atomicMin(&D.sc->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
+  }
+  const float gam = (float)D.gamma;
+  const float dl = (float)L[0], dr = (float)R_[0];
+  const float idl = __builtin_amdgcn_rcpf(dl), idr = __builtin_amdgcn_rcpf(dr);
+  const float v1l = (float)L[2 + a_] * idl, v1r = (float)R_[2 + a_] * idr;
+  const float v2l = (float)L[2 + b_] * idl, v2r = (float)R_[2 + b_] * idr;
+  const float v3l = (float)L[2 + c_] * idl, v3r = (float)R_[2 + c_] * idr;
+  float b1l = 0, b1r = 0, b2l = 0, b2r = 0, b3l = 0, b3r = 0, phil = 0, phir = 0;
+  if (MAG) {
+    b1l = (float)L[5 + a_]; b1r = (float)R_[5 + a_];
+    b2l = (float)L[5 + b_]; b2r = (float)R_[5 + b_];
+    b3l = (float)L[5 + c_]; b3r = (float)R_[5 + c_];
+    phil = (float)q.phi;
+    phir = rphi;
+  }
+  float ptl = (gam - 1.0f) * (float)eil;
+  float ptr = (gam - 1.0f) * (float)eir;
+  const float csl = rsqrt_(gam * ptl * idl);
+  const float csr = rsqrt_(gam * ptr * idr);
+  if (MAG) {
+    ptl = ptl + 0.5f * (b1l * b1l + b2l * b2l + b3l * b3l);
+    ptr = ptr + 0.5f * (b1r * b1r + b2r * b2r + b3r * b3r);
+  }
+  const float cfl = get_cf(dl, csl, b1l, b2l, b3l);
+  const float cfr = get_cf(dr, csr, b1r, b2r, b3r);
+  float t[9];
+  hlld(t, cfl, cfr, v1l, v1r, v2l, v2r, v3l, v3r, dl, dr, (float)L[1], (float)R_[1], ptl, ptr,
+       b1l, b1r, b2l, b2r, b3l, b3r, phil, phir, (float)D.sc->ch);
+
+  // flux difference, accumulated in FP64
+  float tm[9];
+#pragma unroll
+  for (int s = 0; s < 9; ++s) tm[s] = up1f(t[s]);
+  if (lane >= 2 && lane <= 61 && c >= 0 && c < n) {
+    const long m = base + (long)c * A.stride;
+    const double ix = A.idxi[c];
+    double *const *DU = D.DU[DIR];
+    DU[0][m] = ix * ((double)tm[0] - (double)t[0]);
+    DU[1 + a_][m] = ix * ((double)tm[1] - (double)t[1]);
+    DU[1 + b_][m] = ix * ((double)tm[2] - (double)t[2]);
+    DU[1 + c_][m] = ix * ((double)tm[3] - (double)t[3]);
+    DU[4][m] = ix * ((double)tm[4] - (double)t[4]);
+    if (MAG) {
+      DU[5 + a_][m] = ix * ((double)tm[5] - (double)t[5]);
+      DU[5 + b_][m] = ix * ((double)tm[6] - (double)t[6]);
+      DU[5 + c_][m] = ix * ((double)tm[7] - (double)t[7]);
+      if (D.i9wv >= 0) DU[8][m] = ix * ((double)tm[8] - (double)t[8]);
+    }
+  }
+}
+
+template <bool MAG, int LIM>
+__global__ __launch_bounds__(256) void k_sweep_mix(Dev D, int stage) {
+  if (D.sc->stop) return;
+  const int lane = threadIdx.x & 63;
+  long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave < D.waves[0]) {
+    sweep_line_mix<0, MAG, LIM>(D, wave, lane, stage);
+    return;
+  }
+  wave -= D.waves[0];
+  if (wave < D.waves[1]) {
+    sweep_line_mix<1, MAG, LIM>(D, wave, lane, stage);
+    return;
+  }
+  wave -= D.waves[1];
+  if (wave < D.waves[2]) sweep_line_mix<2, MAG, LIM>(D, wave, lane, stage);
+}
+
 // ---- CFL condition of one cell (timestep.f90:35-57, dti_cell :125-183) ----
 template <bool MAG>
 __device__ __forceinline__ void cfl_cell(const Dev &D, long m, double d, double eint,
@@ -693,6 +875,18 @@ void launch_set_clock(const Dev &D, double time, double t_out, double t_end, dou
 
 void launch_sweep(const Dev &D, int stage, hipStream_t s) {
   const unsigned g = nblk(D.waves[0] + D.waves[1] + D.waves[2], 4);
+  if (D.mixed) {
+    if (D.mag) {
+      if (D.limiter == LIM_FLAT) k_sweep_mix<true, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
+      else if (D.limiter == LIM_MINMOD) k_sweep_mix<true, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
+      else k_sweep_mix<true, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+    } else {
+      if (D.limiter == LIM_FLAT) k_sweep_mix<false, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
+      else if (D.limiter == LIM_MINMOD) k_sweep_mix<false, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
+      else k_sweep_mix<false, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+    }
+    return;
+  }
   if (D.mag) {
     if (D.limiter == LIM_FLAT) k_sweep<true, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
     else if (D.limiter == LIM_MINMOD) k_sweep<true, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);

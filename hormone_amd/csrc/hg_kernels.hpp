@@ -37,6 +37,10 @@ struct Scalars {
 // geometry of one sweep direction, indices 0-based interior (ghosts -2,-1,n,n+1)
 struct Axis {
   const double *x, *xi, *idx, *idxi, *dx, *dxi;  // base-shifted: valid for c = -2..n+1
+  // FP32-mixed path: per-cell limiter geometry evaluated once in FP64 on the
+  // host and rounded: (cf, cb, (x3-x2)/(x2-x1), 1/(x3-x2)) and idx as float
+  const float4 *limg;
+  const float *idxf;
   int n;
   long stride;       // element stride of this axis in the SoA arrays
   int sc_lo, sc_hi;  // scalar BC codes (bcNis, bcNos)
@@ -62,6 +66,7 @@ struct Dev {
   // hyperbolic self-gravity (gravswitch 3): the potential is stored with a
   // one-cell ghost layer ((n1+2)(n2+2)(n3+2), Dirichlet values), twice for
   // the Jacobi-style stage update; psi and the stage-1 copies are interior.
+  int mixed;       // HG_PREC_FP32MIX
   int grav;
   double *gphi[2], *gpsi, *gphi0, *gpsi0;
   int gcur;          // which gphi buffer holds the current potential

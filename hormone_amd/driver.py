@@ -32,7 +32,8 @@ def desc_fields(prm: P.Params, g: Grid) -> dict:
                 bc=prm.effective_bc(), gamma=prm.gamma, courant=prm.courant,
                 alpha9wave=prm.alpha9wave, gravswitch=prm.gravswitch,
                 grktype=prm.grktype, HGfac=prm.HGfac, hgcfl=prm.hgcfl,
-                alphagrv=prm.alphagrv, G=prm.G, maxtngrv=prm.maxtngrv)
+                alphagrv=prm.alphagrv, G=prm.G, maxtngrv=prm.maxtngrv,
+                precision=1 if prm.precision == "fp32mix" else 0)
 
 
 def load_initial_state(eng, prm: P.Params, g: Grid, st: P.State):

@@ -49,6 +49,7 @@ class Params:
     alphagrv: float = 0.1
     G: float = 6.67430e-8
     is_test: bool = True
+    precision: str = "fp64"   # or "fp32mix"
 
     @property
     def solve(self):
@@ -319,6 +320,32 @@ def polytrope(g: Grid, pol: Polytrope = Polytrope()) -> State:
                  phi=z.copy(), extra={"grvphi_padded": phi_pad})
 
 
+# --- 3D MHD Kelvin-Helmholtz shear layer (new set-up) -----------------------
+
+def kh_params(n: int = 384, steps: int = 100, **kw) -> Params:
+    p = Params(n=(n, n, n), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), gamma=5.0 / 3.0,
+               mag_on=True, bc=(0,) * 12, outstyle=2, endstyle=2, tnlim=steps, tn_out=steps,
+               t_end=1e30, is_test=False)
+    return replace(p, **kw)
+
+
+def kelvin_helmholtz(g: Grid, seed: int = 2024, amp: float = 0.01) -> State:
+    """rho = 1, vx = +0.5 for |y - 1/2| < 1/4, rho = 2, vx = -0.5 outside;
+    p = 2.5, B = (0.1, 0, 0); vy = amp * U(-1,1) * sin(4 pi x), U from the
+    seeded counter hash of the cell indices."""
+    x = g.centres(0)[None, None, :]
+    y = g.centres(1)[None, :, None]
+    inside = np.abs(y - 0.5) < 0.25
+    d = np.broadcast_to(np.where(inside, 1.0, 2.0), g.shape).copy()
+    vx = np.broadcast_to(np.where(inside, 0.5, -0.5), g.shape).copy()
+    ii, jj, kk = _cell_indices(g)
+    u = hash_uniform(seed, ii, jj, kk).transpose(2, 1, 0)
+    vy = amp * u * np.sin(4.0 * math.pi * np.broadcast_to(x, g.shape))
+    z = np.zeros(g.shape)
+    return State(d=d, p=np.full(g.shape, 2.5), v=[vx, vy, z.copy()],
+                 b=[np.full(g.shape, 0.1), z.copy(), z.copy()], phi=z.copy())
+
+
 def make_problem(name: str, **kw):
     """(Params, Grid, State) for a reference test problem name."""
     if name.startswith("sodshock_"):
@@ -333,6 +360,11 @@ def make_problem(name: str, **kw):
         prm = orszagtang_params(name.split("_")[1], **kw)
         g = cartesian_grid(prm.n, prm.lo, prm.hi)
         return prm, g, orszagtang(g, prm.gamma)
+    if name == "kh3d":
+        seed = kw.pop("seed", 2024)
+        prm = kh_params(**kw)
+        g = cartesian_grid(prm.n, prm.lo, prm.hi)
+        return prm, g, kelvin_helmholtz(g, seed)
     if name == "polytrope3d":
         pol = kw.pop("polytrope", Polytrope())
         prm = polytrope_params(**kw)
