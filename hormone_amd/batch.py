@@ -1,0 +1,148 @@
+"""Batches of independent boxes and the launch-shape autotuner.
+
+A :class:`Batch` owns an ``hg_batch``: many device-resident contexts
+(:class:`hormone_amd.native.Hydro`) of uneven sizes that advance together,
+one persistent launch per kernel per RK stage over flattened work-lists,
+replayed from a captured hipGraph.  Each box keeps its own dt and clock, so
+its result is bit-identical to running it alone.
+
+:func:`autotune` picks the launch shapes (sweep wavefronts per block and the
+persistent grid sizes of the sweep and RK launches) by timing real steps of
+the batch itself.  The shapes change only which wavefront computes which
+cell, never the arithmetic, so tuning steps are ordinary steps of the run.
+Results are cached per shape class (rounded total cells and box count).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+from . import native
+
+_dp = C.POINTER(C.c_double)
+_bound = False
+
+
+def _bind():
+    global _bound
+    if _bound:
+        return native.lib()
+    L = native.lib()
+    vp = C.c_void_p
+    L.hg_batch_create.argtypes = [C.c_int32, C.POINTER(vp)]
+    L.hg_batch_destroy.argtypes = [vp]
+    L.hg_batch_add.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_int32, C.c_int32, C.c_int64, C.c_int64]
+    L.hg_batch_remove.argtypes = [vp, vp]
+    L.hg_batch_step.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.hg_batch_member_state.argtypes = [vp, vp, _dp, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int32)]
+    L.hg_batch_config.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    _bound = True
+    return L
+
+
+class Batch:
+    def __init__(self, device: int = 0):
+        self._lib = _bind()
+        h = C.c_void_p()
+        st = self._lib.hg_batch_create(device, C.byref(h))
+        if st:
+            raise native.HgError(st, "hg_batch_create")
+        self.h = h
+        self.members = []
+        self.config = (4, 0, 0, 1)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.hg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, eng: native.Hydro, t_end: float, t_out: float | None = None,
+            dt_out: float = 1.0, outstyle: int = 1, endstyle: int = 1, tnlim: int = 0,
+            tn_out: int = 1, time0: float = 0.0):
+        """Add a context; by default it runs to t_end with t_out = t_end."""
+        if t_out is None:
+            t_out = t_end
+        st = self._lib.hg_batch_add(self.h, eng.h, time0, t_out, dt_out, t_end, outstyle,
+                                    endstyle, tnlim, tn_out)
+        if st:
+            raise native.HgError(st, "hg_batch_add")
+        self.members.append(eng)
+
+    def remove(self, eng: native.Hydro):
+        st = self._lib.hg_batch_remove(self.h, eng.h)
+        if st:
+            raise native.HgError(st, "hg_batch_remove")
+        self.members.remove(eng)
+
+    def step(self, nsteps: int = 1) -> int:
+        n = C.c_int32()
+        st = self._lib.hg_batch_step(self.h, nsteps, C.byref(n))
+        if st:
+            raise native.HgError(st, "hg_batch_step")
+        return n.value
+
+    def state(self, eng: native.Hydro):
+        """(time, tn, stop) of a member after the last step."""
+        t, n, s = C.c_double(), C.c_int64(), C.c_int32()
+        st = self._lib.hg_batch_member_state(self.h, eng.h, C.byref(t), C.byref(n), C.byref(s))
+        if st:
+            raise native.HgError(st, "hg_batch_member_state")
+        return t.value, n.value, s.value
+
+    def configure(self, sweep_block_waves=4, sweep_grid=0, rk_grid=0, use_graph=True):
+        st = self._lib.hg_batch_config(self.h, sweep_block_waves, sweep_grid, rk_grid,
+                                       int(use_graph))
+        if st:
+            raise native.HgError(st, "hg_batch_config")
+        self.config = (sweep_block_waves, sweep_grid, rk_grid, int(use_graph))
+
+
+# --- autotuner ------------------------------------------------------------------
+
+CUS = 256  # MI355X compute units
+_CACHE: dict = {}
+
+
+def shape_class(batch: Batch):
+    cells = sum(m.grid.ncells for m in batch.members)
+    return (len(batch.members), int(round(cells, -4)))
+
+
+def candidates():
+    out = [(4, 0, 0)]
+    for bw in (1, 2, 4):
+        for sg in (0, CUS * 2, CUS * 4, CUS * 8):
+            for rg in (0, CUS * 4):
+                if (bw, sg, rg) not in out:
+                    out.append((bw, sg, rg))
+    return out
+
+
+def autotune(batch: Batch, steps_per_trial: int = 2, use_cache: bool = True):
+    """Time each candidate launch shape on real steps of the batch and keep
+    the fastest.  Returns (best shape, {shape: seconds per step})."""
+    key = shape_class(batch)
+    if use_cache and key in _CACHE:
+        best = _CACHE[key]
+        batch.configure(*best, use_graph=True)
+        return best, {}
+    timings = {}
+    for cand in candidates():
+        batch.configure(*cand, use_graph=True)
+        batch.step(1)  # capture the graph outside the timing
+        t0 = time.perf_counter()
+        if batch.step(steps_per_trial) == 0:
+            break
+        timings[cand] = (time.perf_counter() - t0) / steps_per_trial
+    best = min(timings, key=timings.get) if timings else (4, 0, 0)
+    batch.configure(*best, use_graph=True)
+    _CACHE[key] = best
+    return best, timings

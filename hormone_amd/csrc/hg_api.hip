@@ -121,6 +121,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
   }
   D.nw = D.mag ? 10 : 6;
   D.mixed = desc->precision == HG_PREC_FP32MIX;
+  D.damp = std::exp(-desc->alpha9wave * desc->courant);  // source.f90:253
   // coordinates: 6 arrays x 3 axes, each n+4 long
   const int nn[3] = {D.n1, D.n2, D.n3};
   size_t tot = 0;
@@ -227,6 +228,8 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
   }
   if ((st = dev_alloc(c, sizeof(Scalars), (void **)&D.sc))) return fail(st);
   if (hipMemset(D.sc, 0, sizeof(Scalars)) != hipSuccess) return fail(HG_ERR_DEVICE);
+  launch_clear_error(D, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
   if (hipHostMalloc((void **)&c->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess)
     return fail(HG_ERR_NOMEM);
   if (hipHostMalloc((void **)&c->h_stage, fb, hipHostMallocDefault) != hipSuccess)
@@ -319,7 +322,7 @@ static int check_error(hg_ctx *c) {
   HIPCHK(hipMemcpyAsync(c->h_sc, c->D.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   unsigned long long key = c->h_sc->err;
-  if (!key) return HG_OK;
+  if (key == NOERR) return HG_OK;
   const Dev &D = c->D;
   unsigned long long cell = (key - 1) & ((1ull << 42) - 1);
   unsigned long long cls = (key - 1) >> 42;
@@ -651,4 +654,247 @@ extern "C" int hg_kat(int32_t which, int32_t n, const double *in, double *out) {
   hipFree(din);
   hipFree(dout);
   return st;
+}
+
+// =============================================================================
+// Batches: many independent boxes advanced by one set of launches per step.
+// =============================================================================
+struct hg_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<hg_ctx *> members;
+  Dev *d_boxes = nullptr;
+  int2 *d_sitems = nullptr, *d_citems = nullptr;
+  size_t cap_boxes = 0, cap_s = 0, cap_c = 0;
+  Batch B{};
+  int mag = -1, lim = -1, mixed = -1, rktype = -1;
+  bool dirty = true;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  Scalars *h_sc = nullptr;
+  size_t cap_hsc = 0;
+  // launch shapes: sweep block size (wavefronts per block) and grid sizes of
+  // the persistent sweep / rk launches (0 = one item per wavefront / block)
+  int sweep_block_waves = 4;
+  unsigned sweep_grid = 0, rk_grid = 0;
+};
+
+static void batch_drop_graph(hg_batch *b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+extern "C" int hg_batch_create(int32_t device, hg_batch **out) {
+  if (!out) return HG_ERR_ARG;
+  *out = nullptr;
+  hg_batch *b = new (std::nothrow) hg_batch();
+  if (!b) return HG_ERR_NOMEM;
+  b->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return HG_ERR_DEVICE;
+  }
+  *out = b;
+  return HG_OK;
+}
+
+extern "C" int hg_batch_destroy(hg_batch *b) {
+  if (!b) return HG_OK;
+  hipSetDevice(b->device);
+  hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->d_boxes) hipFree(b->d_boxes);
+  if (b->d_sitems) hipFree(b->d_sitems);
+  if (b->d_citems) hipFree(b->d_citems);
+  if (b->h_sc) hipHostFree(b->h_sc);
+  hipStreamDestroy(b->stream);
+  delete b;
+  return HG_OK;
+}
+
+extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, double dt_out,
+                            double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
+                            int64_t tn_out) {
+  if (!b || !c) return HG_ERR_ARG;
+  if (c->device != b->device || c->D.grav) return HG_ERR_UNSUPPORTED;
+  for (hg_ctx *m : b->members)
+    if (m == c) return HG_ERR_ARG;
+  if (!b->members.empty() &&
+      (c->D.mag != b->mag || c->D.limiter != b->lim || c->D.mixed != b->mixed ||
+       c->desc.rktype != b->rktype))
+    return HG_ERR_UNSUPPORTED;  // one batch runs one kernel variant
+  HIPCHK(hipSetDevice(b->device));
+  int st = ensure_eint(c);
+  if (st) return st;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // the box's clock and its first CFL partials, on the batch stream
+  launch_set_clock(c->D, time, t_out, t_end, dt_out, outstyle, endstyle, 0, tnlim,
+                   tn_out < 1 ? 1 : tn_out, (long long)1 << 62, b->stream);
+  launch_cfl(c->D, b->stream);
+  HIPCHK(hipGetLastError());
+  b->mag = c->D.mag;
+  b->lim = c->D.limiter;
+  b->mixed = c->D.mixed;
+  b->rktype = c->desc.rktype;
+  b->members.push_back(c);
+  b->dirty = true;
+  return HG_OK;
+}
+
+extern "C" int hg_batch_remove(hg_batch *b, hg_ctx *c) {
+  if (!b || !c) return HG_ERR_ARG;
+  for (size_t q = 0; q < b->members.size(); ++q)
+    if (b->members[q] == c) {
+      HIPCHK(hipStreamSynchronize(b->stream));
+      b->members.erase(b->members.begin() + q);
+      b->dirty = true;
+      return HG_OK;
+    }
+  return HG_ERR_ARG;
+}
+
+template <class T>
+static int grow(T **p, size_t *cap, size_t need) {
+  if (need <= *cap) return HG_OK;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  size_t n = need + need / 2 + 16;
+  if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return HG_ERR_NOMEM;
+  *cap = n;
+  return HG_OK;
+}
+
+static void batch_enqueue_step(hg_batch *b) {
+  const Batch &B = b->B;
+  const long sw = (B.nsitems + b->sweep_block_waves - 1) / b->sweep_block_waves;
+  const unsigned gs = b->sweep_grid ? (unsigned)std::min<long>(b->sweep_grid, sw) : (unsigned)sw;
+  const unsigned gr = b->rk_grid ? (unsigned)std::min<long>(b->rk_grid, B.ncitems)
+                                 : (unsigned)B.ncitems;
+  launch_bcfl_final(B, b->stream);
+  for (int stage = 1; stage <= b->rktype; ++stage) {
+    double faco, fact, facn;
+    runge_coeff(stage, b->rktype, &faco, &fact, &facn);
+    launch_bsweep_blocks(B, b->mag, b->lim, b->mixed, stage, gs, b->sweep_block_waves,
+                         b->stream);
+    launch_brk(B, b->mag, stage, faco, fact, facn, stage == b->rktype, gr, b->stream);
+  }
+  launch_bstep_end(B, b->stream);
+}
+
+static int batch_rebuild(hg_batch *b) {
+  batch_drop_graph(b);
+  const size_t nb = b->members.size();
+  std::vector<Dev> boxes(nb);
+  std::vector<int2> si, ci;
+  for (size_t q = 0; q < nb; ++q) {
+    const Dev &D = b->members[q]->D;
+    boxes[q] = D;
+    const long nwv = D.waves[0] + D.waves[1] + D.waves[2];
+    for (long w = 0; w < nwv; ++w) si.push_back(make_int2((int)q, (int)w));
+    for (long k = 0; k < D.nblk_cfl; ++k) ci.push_back(make_int2((int)q, (int)k));
+  }
+  int st;
+  if ((st = grow(&b->d_boxes, &b->cap_boxes, nb ? nb : 1))) return st;
+  if ((st = grow(&b->d_sitems, &b->cap_s, si.size() ? si.size() : 1))) return st;
+  if ((st = grow(&b->d_citems, &b->cap_c, ci.size() ? ci.size() : 1))) return st;
+  if (nb) {
+    HIPCHK(hipMemcpy(b->d_boxes, boxes.data(), nb * sizeof(Dev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_sitems, si.data(), si.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_citems, ci.data(), ci.size() * sizeof(int2), hipMemcpyHostToDevice));
+  }
+  b->B.boxes = b->d_boxes;
+  b->B.nbox = (int)nb;
+  b->B.sitems = b->d_sitems;
+  b->B.nsitems = (long)si.size();
+  b->B.citems = b->d_citems;
+  b->B.ncitems = (long)ci.size();
+  if (b->cap_hsc < (nb ? nb : 1)) {
+    if (b->h_sc) hipHostFree(b->h_sc);
+    b->h_sc = nullptr;
+    b->cap_hsc = (nb ? nb : 1) * 2;
+    if (hipHostMalloc((void **)&b->h_sc, b->cap_hsc * sizeof(Scalars), hipHostMallocDefault) !=
+        hipSuccess) {
+      b->cap_hsc = 0;
+      return HG_ERR_NOMEM;
+    }
+  }
+  if (b->use_graph && nb) {
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    batch_enqueue_step(b);
+    HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  b->dirty = false;
+  return HG_OK;
+}
+
+// Advance every member by nsteps of its own steps (members that reached
+// their end or an output stop simply idle).  *nrunning receives the number
+// of members still running afterwards.
+extern "C" int hg_batch_step(hg_batch *b, int32_t nsteps, int32_t *nrunning) {
+  if (!b || nsteps < 0) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(b->device));
+  if (b->members.empty()) {
+    if (nrunning) *nrunning = 0;
+    return HG_OK;
+  }
+  int st;
+  if (b->dirty && (st = batch_rebuild(b))) return st;
+  for (int q = 0; q < nsteps; ++q) {
+    if (b->exec) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    else batch_enqueue_step(b);
+  }
+  HIPCHK(hipGetLastError());
+  for (size_t q = 0; q < b->members.size(); ++q)
+    HIPCHK(hipMemcpyAsync(&b->h_sc[q], b->members[q]->D.sc, sizeof(Scalars),
+                          hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int run = 0;
+  for (size_t q = 0; q < b->members.size(); ++q) {
+    hg_ctx *c = b->members[q];
+    *c->h_sc = b->h_sc[q];
+    c->ch = b->h_sc[q].ch;
+    if (b->h_sc[q].stop == STOP_NONE) ++run;
+  }
+  if (nrunning) *nrunning = run;
+  return HG_OK;
+}
+
+// Clock of one member after the last hg_batch_step (stop: 0 running,
+// 1 end, 2 output due, 4 error).
+extern "C" int hg_batch_member_state(hg_batch *b, hg_ctx *c, double *time, int64_t *tn,
+                                     int32_t *stop) {
+  if (!b || !c) return HG_ERR_ARG;
+  for (size_t q = 0; q < b->members.size(); ++q)
+    if (b->members[q] == c) {
+      const Scalars &s = b->h_sc[q];
+      if (time) *time = s.time;
+      if (tn) *tn = s.tn;
+      if (stop) *stop = s.stop;
+      if (s.stop == STOP_ERROR) return check_error(c);
+      return HG_OK;
+    }
+  return HG_ERR_ARG;
+}
+
+// Launch shapes (set by the autotuner in hormone_amd/batch.py or by hand):
+// sweep wavefronts per block (1, 2 or 4), persistent grid sizes (0 = one
+// work item per wavefront / block), and whether steps replay a hipGraph.
+extern "C" int hg_batch_config(hg_batch *b, int32_t sweep_block_waves, int32_t sweep_grid,
+                               int32_t rk_grid, int32_t use_graph) {
+  if (!b) return HG_ERR_ARG;
+  if (sweep_block_waves != 1 && sweep_block_waves != 2 && sweep_block_waves != 4)
+    return HG_ERR_ARG;
+  if (sweep_grid < 0 || rk_grid < 0) return HG_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->sweep_block_waves = sweep_block_waves;
+  b->sweep_grid = (unsigned)sweep_grid;
+  b->rk_grid = (unsigned)rk_grid;
+  b->use_graph = use_graph != 0;
+  b->dirty = true;
+  return HG_OK;
 }

@@ -498,8 +498,9 @@ __device__ __forceinline__ double wave_max(double x) {
   return x;
 }
 
-// block (256 threads) reduction of (min dti, max cf) into D.part[blockIdx]
-__device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, double cfm) {
+// block (256 threads) reduction of (min dti, max cf) into D.part[blk]
+__device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, double cfm,
+                                                 long blk) {
   __shared__ double smin[4], smax[4];
   dti = wave_min(dti);
   cfm = wave_max(cfm);
@@ -515,8 +516,8 @@ __device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, doubl
       a = mn2(a, smin[q]);
       b = mx2(b, smax[q]);
     }
-    D.part[2 * blockIdx.x] = a;
-    D.part[2 * blockIdx.x + 1] = b;
+    D.part[2 * blk] = a;
+    D.part[2 * blk + 1] = b;
   }
 }
 
@@ -554,12 +555,9 @@ __device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVM
 }
 
 template <bool MAG, bool CFL, bool DAMP, bool GRAV>
-__global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
-                                            double facn, double damp) {
-  if (D.sc->stop) return;
-  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  double dti = __builtin_inf(), cfm = 0.0;
-  if (m < D.ncell) {
+__device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double faco,
+                                        double fact, double facn, double damp, double &dti,
+                                        double &cfm) {
   const double ft = fact * D.sc->dt;
   double src[NVMAX];
   if (GRAV) grav_source(D, m, src);
@@ -609,8 +607,17 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
     atomicMin(&D.sc->err, errkey(stage, 1, 0, i, j, k, D));
   }
   if (CFL) cfl_cell<MAG>(D, m, d, eint, v1, v2, v3, b1, b2, b3, dti, cfm);
-  }
-  if (CFL) cfl_block_reduce(D, dti, cfm);
+}
+
+template <bool MAG, bool CFL, bool DAMP, bool GRAV>
+__global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
+                                            double facn, double damp) {
+  if (D.sc->stop) return;
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double dti = __builtin_inf(), cfm = 0.0;
+  if (m < D.ncell)
+    rk_cell<MAG, CFL, DAMP, GRAV>(D, m, stage, faco, fact, facn, damp, dti, cfm);
+  if (CFL) cfl_block_reduce(D, dti, cfm, blockIdx.x);
 }
 
 // ---- pressure(): eint from e, d, v, B after an upload ---------------------
@@ -702,12 +709,12 @@ __global__ __launch_bounds__(256) void k_cfl(Dev D) {
     cfl_cell<MAG>(D, m, D.W[WD][m], D.W[WEINT][m], D.W[WV1][m], D.W[WV2][m], D.W[WV3][m],
                   b1, b2, b3, dti, cfm);
   }
-  cfl_block_reduce(D, dti, cfm);
+  cfl_block_reduce(D, dti, cfm, blockIdx.x);
 }
 
 // Final CFL reduction and dt (timestep.f90:74-89): dt = courant*min(dti),
 // clipped to t_out / t_end as the reference does; ch = max fast speed.
-__global__ __launch_bounds__(256) void k_cfl_final(Dev D) {
+__device__ __forceinline__ void cfl_final(const Dev &D) {
   Scalars *sc = D.sc;
   if (sc->stop) return;
   __shared__ double smin[4], smax[4];
@@ -749,12 +756,14 @@ __global__ __launch_bounds__(256) void k_cfl_final(Dev D) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_cfl_final(Dev D) { cfl_final(D); }
+
 // End of one main-loop iteration (hormone.f90:147-177): advance the clock,
 // then the end and output checks in the reference's order.
-__global__ void k_step_end(Dev D) {
+__device__ __forceinline__ void step_end(const Dev &D) {
   Scalars *sc = D.sc;
   if (sc->stop) return;
-  if (sc->err) {
+  if (sc->err != NOERR) {
     sc->stop = STOP_ERROR;
     return;
   }
@@ -786,6 +795,67 @@ __global__ void k_step_end(Dev D) {
   }
   if (sc->steps_left <= 0) sc->stop = STOP_MAXSTEPS;
 }
+
+__global__ void k_step_end(Dev D) { step_end(D); }
+
+// ---- batched boxes: persistent kernels over flattened work-lists ---------
+template <bool MAG, int LIM, bool MIX>
+__global__ __launch_bounds__(256) void k_bsweep(Batch B, int stage) {
+  const int lane = threadIdx.x & 63;
+  const long nw = (long)gridDim.x * (blockDim.x >> 6);
+  for (long it = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < B.nsitems;
+       it += nw) {
+    const int2 x = B.sitems[it];
+    const Dev &D = B.boxes[x.x];
+    if (D.sc->stop) continue;
+    long w = x.y;
+    if (w < D.waves[0]) {
+      if (MIX) sweep_line_mix<0, MAG, LIM>(D, w, lane, stage);
+      else sweep_line<0, MAG, LIM>(D, w, lane, stage);
+      continue;
+    }
+    w -= D.waves[0];
+    if (w < D.waves[1]) {
+      if (MIX) sweep_line_mix<1, MAG, LIM>(D, w, lane, stage);
+      else sweep_line<1, MAG, LIM>(D, w, lane, stage);
+      continue;
+    }
+    w -= D.waves[1];
+    if (MIX) sweep_line_mix<2, MAG, LIM>(D, w, lane, stage);
+    else sweep_line<2, MAG, LIM>(D, w, lane, stage);
+  }
+}
+
+template <bool MAG, bool LAST>
+__global__ __launch_bounds__(256) void k_brk(Batch B, int stage, double faco, double fact,
+                                             double facn) {
+  for (long it = blockIdx.x; it < B.ncitems; it += gridDim.x) {
+    const int2 x = B.citems[it];
+    const Dev &D = B.boxes[x.x];
+    if (D.sc->stop) continue;  // uniform over the block
+    const long m = (long)x.y * 256 + threadIdx.x;
+    double dti = __builtin_inf(), cfm = 0.0;
+    if (m < D.ncell) {
+      if (LAST && MAG && D.i9wv >= 0)
+        rk_cell<MAG, true, true, false>(D, m, stage, faco, fact, facn, D.damp, dti, cfm);
+      else
+        rk_cell<MAG, LAST, false, false>(D, m, stage, faco, fact, facn, 1.0, dti, cfm);
+    }
+    if (LAST) {
+      cfl_block_reduce(D, dti, cfm, x.y);
+      __syncthreads();  // the reduction's shared slots are reused next item
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bcfl_final(Batch B) { cfl_final(B.boxes[blockIdx.x]); }
+
+__global__ void k_bstep_end(Batch B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < B.nbox) step_end(B.boxes[b]);
+}
+
+__global__ void k_clear_error(Scalars *sc) { sc->err = NOERR; }
 
 __global__ void k_set_step(Scalars *sc, double dt, double ch) {
   sc->dt = dt;
@@ -861,6 +931,8 @@ void launch_pressure(const Dev &D, hipStream_t s) {
   if (D.mag) k_pressure<true><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
   else k_pressure<false><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
 }
+
+void launch_clear_error(const Dev &D, hipStream_t s) { k_clear_error<<<1, 1, 0, s>>>(D.sc); }
 
 void launch_set_step(const Dev &D, double dt, double ch, hipStream_t s) {
   k_set_step<<<1, 1, 0, s>>>(D.sc, dt, ch);
@@ -941,6 +1013,46 @@ void launch_step_end(const Dev &D, hipStream_t s) { k_step_end<<<1, 1, 0, s>>>(D
 
 void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
   k_phidamp<<<nblk(D.ncell, 256), 256, 0, s>>>(D, factor);
+}
+
+template <bool MAG, int LIM>
+static void bsweep_mix(const Batch &B, int mixed, int stage, unsigned g, int bs,
+                       hipStream_t s) {
+  if (mixed) k_bsweep<MAG, LIM, true><<<g, bs, 0, s>>>(B, stage);
+  else k_bsweep<MAG, LIM, false><<<g, bs, 0, s>>>(B, stage);
+}
+
+void launch_bsweep_blocks(const Batch &B, int mag, int lim, int mixed, int stage,
+                          unsigned g, int block_waves, hipStream_t s) {
+  const int bs = 64 * block_waves;
+  if (mag) {
+    if (lim == LIM_FLAT) bsweep_mix<true, LIM_FLAT>(B, mixed, stage, g, bs, s);
+    else if (lim == LIM_MINMOD) bsweep_mix<true, LIM_MINMOD>(B, mixed, stage, g, bs, s);
+    else bsweep_mix<true, LIM_MC>(B, mixed, stage, g, bs, s);
+  } else {
+    if (lim == LIM_FLAT) bsweep_mix<false, LIM_FLAT>(B, mixed, stage, g, bs, s);
+    else if (lim == LIM_MINMOD) bsweep_mix<false, LIM_MINMOD>(B, mixed, stage, g, bs, s);
+    else bsweep_mix<false, LIM_MC>(B, mixed, stage, g, bs, s);
+  }
+}
+
+void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, double facn,
+                bool last, unsigned g, hipStream_t s) {
+  if (mag) {
+    if (last) k_brk<true, true><<<g, 256, 0, s>>>(B, stage, faco, fact, facn);
+    else k_brk<true, false><<<g, 256, 0, s>>>(B, stage, faco, fact, facn);
+  } else {
+    if (last) k_brk<false, true><<<g, 256, 0, s>>>(B, stage, faco, fact, facn);
+    else k_brk<false, false><<<g, 256, 0, s>>>(B, stage, faco, fact, facn);
+  }
+}
+
+void launch_bcfl_final(const Batch &B, hipStream_t s) {
+  k_bcfl_final<<<B.nbox, 256, 0, s>>>(B);
+}
+
+void launch_bstep_end(const Batch &B, hipStream_t s) {
+  k_bstep_end<<<(B.nbox + 63) / 64, 64, 0, s>>>(B);
 }
 
 void launch_kat(int which, int n, const double *in, double *out, hipStream_t s) {

@@ -31,7 +31,7 @@ struct Scalars {
   double time, t_out, t_end, dt_out;
   long long tn, tnlim, tn_out, steps_left;
   int outstyle, endstyle, stop, pad0;
-  unsigned long long err;  // first error key (0 = none), see errkey()
+  unsigned long long err;  // first error key (NOERR = none), see errkey()
 };
 
 // geometry of one sweep direction, indices 0-based interior (ghosts -2,-1,n,n+1)
@@ -67,6 +67,7 @@ struct Dev {
   // one-cell ghost layer ((n1+2)(n2+2)(n3+2), Dirichlet values), twice for
   // the Jacobi-style stage update; psi and the stage-1 copies are interior.
   int mixed;       // HG_PREC_FP32MIX
+  double damp;     // phidamp factor exp(-alpha9wave*courant), evaluated on the host
   int grav;
   double *gphi[2], *gpsi, *gphi0, *gpsi0;
   int gcur;          // which gphi buffer holds the current potential
@@ -77,6 +78,10 @@ struct Dev {
 __host__ __device__ inline long gidx(const Dev &D, int i, int j, int k) {
   return ((long)(k + 1) * (D.n2 + 2) + (j + 1)) * (long)(D.n1 + 2) + (i + 1);
 }
+
+// No error recorded: errors lower the key with atomicMin, so "none" must be
+// the largest value.
+constexpr unsigned long long NOERR = ~0ull;
 
 // Error keys: the smallest key is the first error in the reference's serial
 // loop order (stage, then numflux before the cell update, then direction,
@@ -90,10 +95,30 @@ __host__ __device__ inline unsigned long long errkey(int stage, int phase, int d
   return (key << 42) + cell + 1;
 }
 
+// A batch of independent boxes advanced by the same launches: flattened
+// work-lists of (box, sweep wavefront) and (box, 256-cell block) items over
+// a device array of the members' Dev records.
+struct Batch {
+  const Dev *boxes;
+  int nbox;
+  const int2 *sitems;
+  long nsitems;
+  const int2 *citems;
+  long ncitems;
+};
+
+void launch_bsweep_blocks(const Batch &B, int mag, int lim, int mixed, int stage,
+                          unsigned grid, int block_waves, hipStream_t s);
+void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, double facn,
+                bool last, unsigned grid, hipStream_t s);
+void launch_bcfl_final(const Batch &B, hipStream_t s);
+void launch_bstep_end(const Batch &B, hipStream_t s);
+
 // RK coefficients (faco, fact, facn) of get_runge_coeff, rungekutta.f90:146-183
 void runge_coeff(int stage, int rktype, double *faco, double *fact, double *facn);
 
 void launch_pressure(const Dev &D, hipStream_t s);
+void launch_clear_error(const Dev &D, hipStream_t s);
 void launch_set_step(const Dev &D, double dt, double ch, hipStream_t s);
 void launch_set_clock(const Dev &D, double time, double t_out, double t_end, double dt_out,
                       int outstyle, int endstyle, long long tn, long long tnlim,

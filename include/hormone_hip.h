@@ -193,6 +193,35 @@ int hg_profile(hg_ctx *ctx, int32_t enable);
 int hg_profile_read(const hg_ctx *ctx, int32_t kind, double *ms_total,
                     int64_t *launches, double *bytes_per_launch);
 
+/* ---- batches of independent boxes (no counterpart in the reference) ------
+   Many contexts of possibly different sizes and dimensions, sharing one
+   kernel variant (mag_on, flux_limiter, precision, rktype; no gravity),
+   advance together: each step is one launch per kernel over flattened
+   work-lists of (box, sweep wavefront) and (box, cell block) items, replayed
+   from a captured hipGraph.  Each box keeps its own dt, clock and stop
+   condition (the same as hg_run's), so its result is bit-identical to
+   running it alone.  While a context is in a batch, call nothing else on it;
+   after hg_batch_remove it is an ordinary context again. */
+typedef struct hg_batch hg_batch;
+int hg_batch_create(int32_t device, hg_batch **out);
+int hg_batch_destroy(hg_batch *batch);
+/* Add a context with its main-loop clock (as hg_run's arguments). */
+int hg_batch_add(hg_batch *batch, hg_ctx *ctx, double time, double t_out, double dt_out,
+                 double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
+                 int64_t tn_out);
+int hg_batch_remove(hg_batch *batch, hg_ctx *ctx);
+/* Advance every running member by nsteps of its own steps; *nrunning gets
+   the number still running. */
+int hg_batch_step(hg_batch *batch, int32_t nsteps, int32_t *nrunning);
+/* A member's clock after the last hg_batch_step; stop: 0 running, 1 end,
+   2 output due, 4 error (then the member's error status is returned). */
+int hg_batch_member_state(hg_batch *batch, hg_ctx *ctx, double *time, int64_t *tn,
+                          int32_t *stop);
+/* Launch shapes: sweep wavefronts per block (1, 2, 4), persistent grid
+   sizes (0 = one work item per wavefront / block) and hipGraph replay. */
+int hg_batch_config(hg_batch *batch, int32_t sweep_block_waves, int32_t sweep_grid,
+                    int32_t rk_grid, int32_t use_graph);
+
 /* Cell (Fortran indices) and direction (1..3, 0 for a cell error) of the
    last HG_ERR_FACE_EOS / HG_ERR_NEG_EINT. */
 int hg_last_error(const hg_ctx *ctx, int32_t *i, int32_t *j, int32_t *k,

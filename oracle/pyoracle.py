@@ -93,7 +93,8 @@ class Oracle:
         if st:
             i, j, k, d = (C.c_int() for _ in range(4))
             self._lib.or_last_error(self.h, C.byref(i), C.byref(j), C.byref(k), C.byref(d))
-            raise HgError(st, f"{what} at i={i.value} j={j.value} k={k.value} dir={d.value}")
+            raise HgError(st, f"{what} at i={i.value} j={j.value} k={k.value} dir={d.value}",
+                          cell=(i.value, j.value, k.value), direction=d.value)
 
     def ufnmax(self):
         return self._lib.or_ufnmax(self.h)

@@ -196,3 +196,26 @@ def test_gpu_polytrope_gravity_matches_oracle(hiplib, mag_on):
     assert r1.tn == r2.tn == 12 and r1.time == r2.time and r1.nsub == r2.nsub
     for f in s2:
         assert np.array_equal(s1[f].view(np.uint64), s2[f].view(np.uint64)), f
+
+
+@pytest.mark.parametrize("engine", ["hip", "oracle"])
+def test_negative_internal_energy_is_reported_at_its_cell(hiplib, engine):
+    """A state whose total energy is below its kinetic energy in one cell
+    stops like internalenergy (eos.f90:833-839), naming that cell."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    from hormone_amd.native import HG_ERR_NEG_EINT, HgError
+    prm, g, st = P.make_problem("sodshock_x", n=100)
+    st.v[0][0, 0, 36] = 3.0  # Fortran cell i = 37
+    eint, e = P.initial_energy(st, prm.gamma)
+    e[0, 0, 36] = 0.25 * st.d[0, 0, 36] * 9.0  # below the kinetic energy 0.5*d*v^2
+    u = P.conserve(st, e, prm.mag_on, g.dim)
+    eng = (Hydro if engine == "hip" else Oracle)(**D.desc_fields(prm, g))
+    for f, a in {D.F_D: st.d, D.F_V1: st.v[0], D.F_V2: st.v[1], D.F_V3: st.v[2], D.F_E: e}.items():
+        eng.upload(f, g.pad(a))
+    for n, a in enumerate(u):
+        eng.upload(D.F_U + n, g.pad(a))
+    with pytest.raises(HgError) as ei:
+        eng.timestep(0.0, 0.2, 0.2, 1, 1)
+    assert ei.value.code == HG_ERR_NEG_EINT
+    assert tuple(ei.value.cell) == (37, 1, 1)
