@@ -751,6 +751,8 @@ extern "C" int hg_batch_remove(hg_batch *b, hg_ctx *c) {
     if (b->members[q] == c) {
       HIPCHK(hipStreamSynchronize(b->stream));
       b->members.erase(b->members.begin() + q);
+      // keep the cached member states aligned with the member list
+      for (size_t r = q; r < b->members.size(); ++r) b->h_sc[r] = b->h_sc[r + 1];
       b->dirty = true;
       return HG_OK;
     }

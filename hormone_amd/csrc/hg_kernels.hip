@@ -286,11 +286,22 @@ __global__ __launch_bounds__(256) void k_sweep(Dev D, int stage) {
 __device__ __forceinline__ float up1f(float x) { return __shfl_up(x, 1, 64); }
 __device__ __forceinline__ float dn1f(float x) { return __shfl_down(x, 1, 64); }
 
-__device__ __forceinline__ float mc_mixed(float u0, float u1, float u2, float4 g) {
-  const float top = (u1 - u0) * g.z;
-  const float bot = u2 - u1;
+// Limiters on FP32 copies of the FP64 one-sided differences (differencing
+// first keeps the slope accurate to FP32 relative precision even where the
+// neighbouring values agree to many digits).
+__device__ __forceinline__ float mc_mixed(float dm, float dp, float4 g) {
+  const float top = dm * g.z;  // (u2-u1)*(x3-x2)/(x2-x1)
+  const float bot = dp;        // u3-u2
   const float sig = sgn(1.0f, bot);
   return sig * mx2(0.0f, mn3(sig * 0.5f * (bot + top), sig * g.x * bot, sig * g.y * top)) * g.w;
+}
+
+__device__ __forceinline__ float minmod_mixed(float dm, float dp, float ida, float idb) {
+  const float x = 1.4f * dp * idb;
+  const float y = 1.4f * dm * ida;
+  const float z = (dp + dm) * ida * idb * __builtin_amdgcn_rcpf(ida + idb);
+  const float s = sgn(1.0f, x);
+  return s * mx2(0.0f, mn3(fabsf(x), s * y, s * z));
 }
 
 template <int DIR, bool MAG, int LIM>
@@ -336,9 +347,9 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
         fl[s] = fr[s] = 0.0;
         continue;
       }
-      const float u1 = (float)qd[s];
-      const float u0 = up1f(u1), u2 = dn1f(u1);
-      const float du = (LIM == LIM_MINMOD) ? minmod(u0, u1, u2, ida, idb) : mc_mixed(u0, u1, u2, geo);
+      const double u0 = up1(qd[s]), u2 = dn1(qd[s]);
+      const float dm = (float)(qd[s] - u0), dp = (float)(u2 - qd[s]);
+      const float du = (LIM == LIM_MINMOD) ? minmod_mixed(dm, dp, ida, idb) : mc_mixed(dm, dp, geo);
       S[s] = du;
       fl[s] = qd[s] - hl * (double)du;
       fr[s] = qd[s] + hr * (double)du;
