@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--sweep-waves", type=int, default=0,
+                    help="sweep wavefronts per block (0: autotune during warm-up)")
     return ap.parse_args()
 
 
@@ -190,7 +192,21 @@ def main():
     run = Runner(eng, prm, g)
     m0, e0 = sums(D, eng, g, prm)
 
-    run.advance(a.warmup)
+    tuned = None
+    if a.sweep_waves:
+        eng.set_launch(a.sweep_waves)
+        run.advance(a.warmup)
+    else:
+        # autotune the sweep's block size on warm-up steps (launch shapes do
+        # not change results), then finish the warm-up with the winner
+        from hormone_amd.batch import tune_single
+        per = max(1, a.warmup // 5)
+        best, times = tune_single(eng, run.advance, steps=per)
+        tuned = {"sweep_block_waves": best, "us_per_launch": {str(k): round(v, 2)
+                                                              for k, v in times.items()}}
+        left = a.warmup - per * len(times)
+        if left > 0:
+            run.advance(left)
     if not a.no_profile:
         eng.profile(True)
     torch.cuda.synchronize()
@@ -281,6 +297,7 @@ def main():
         "data": "synthetic (closed-form initial state)",
         "config": {"workload": wl[a.workload], "cells": g.ncells, "precision": a.precision},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
+        "launch_tuning": tuned,
     }
     line.update(extra)
     print(json.dumps(line))

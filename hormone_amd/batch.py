@@ -146,3 +146,21 @@ def autotune(batch: Batch, steps_per_trial: int = 2, use_cache: bool = True):
     batch.configure(*best, use_graph=True)
     _CACHE[key] = best
     return best, timings
+
+
+def tune_single(eng: native.Hydro, advance, candidates=(2, 4, 8, 16), steps: int = 4):
+    """Pick the sweep block size of one context by timing its sweep kernel
+    (HIP events) over a few real steps per candidate; ``advance(k)`` must
+    take k steps.  Returns (best, {candidate: microseconds per launch})."""
+    timings = {}
+    for bw in candidates:
+        eng.set_launch(bw)
+        eng.profile(True)
+        advance(steps)
+        ms, n, _ = eng.profile_read(native.PROF_SWEEP)
+        eng.profile(False)
+        if n:
+            timings[bw] = ms / n * 1e3
+    best = min(timings, key=timings.get) if timings else 4
+    eng.set_launch(best)
+    return best, timings

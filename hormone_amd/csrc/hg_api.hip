@@ -194,6 +194,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
   for (int w = 0; w < D.nw; ++w)
     if (hipMemset(D.W[w], 0, fb) != hipSuccess) return fail(HG_ERR_DEVICE);
   D.nblk_cfl = (int)((D.ncell + 255) / 256);
+  D.sweep_bw = 4;
   for (int a = 0; a < 3; ++a)
     D.waves[a] = D.solve[a] ? (long)((nn[a] + 59) / 60) * (D.ncell / nn[a]) : 0;
   if ((st = dev_alloc(c, sizeof(double) * 2 * D.nblk_cfl, (void **)&D.part))) return fail(st);
@@ -619,6 +620,16 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
   c->ch = c->h_sc->ch;
   if (stop_reason) *stop_reason = reason;
   if (reason == STOP_ERROR) return check_error(c);
+  return HG_OK;
+}
+
+// Launch shape of the sweep kernel for this context: wavefronts per block.
+extern "C" int hg_set_launch(hg_ctx *c, int32_t sweep_block_waves) {
+  if (!c) return HG_ERR_ARG;
+  const int w = sweep_block_waves;
+  if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return HG_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->D.sweep_bw = w;
   return HG_OK;
 }
 

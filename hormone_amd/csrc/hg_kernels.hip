@@ -258,7 +258,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
 // x, the next waves[1] sweep y, then z.  Blocks are dispatched in order, so
 // a compute unit mostly runs one direction's code at a time.
 template <bool MAG, int LIM>
-__global__ __launch_bounds__(256) void k_sweep(Dev D, int stage) {
+__global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
   long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -451,7 +451,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
 }
 
 template <bool MAG, int LIM>
-__global__ __launch_bounds__(256) void k_sweep_mix(Dev D, int stage) {
+__global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
   long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -956,28 +956,34 @@ void launch_set_clock(const Dev &D, double time, double t_out, double t_end, dou
                               tnlim, tn_out, steps);
 }
 
+// Sweep launch: D.sweep_bw wavefronts per block.  Consecutive wavefronts of
+// a block own neighbouring lines (consecutive i for the y and z sweeps), so
+// with 16 wavefronts a block uses whole 128-byte cache lines of the strided
+// y/z loads.
 void launch_sweep(const Dev &D, int stage, hipStream_t s) {
-  const unsigned g = nblk(D.waves[0] + D.waves[1] + D.waves[2], 4);
+  const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
+  const unsigned g = nblk(D.waves[0] + D.waves[1] + D.waves[2], bw);
+  const int bs = 64 * bw;
   if (D.mixed) {
     if (D.mag) {
-      if (D.limiter == LIM_FLAT) k_sweep_mix<true, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
-      else if (D.limiter == LIM_MINMOD) k_sweep_mix<true, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
-      else k_sweep_mix<true, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+      if (D.limiter == LIM_FLAT) k_sweep_mix<true, LIM_FLAT><<<g, bs, 0, s>>>(D, stage);
+      else if (D.limiter == LIM_MINMOD) k_sweep_mix<true, LIM_MINMOD><<<g, bs, 0, s>>>(D, stage);
+      else k_sweep_mix<true, LIM_MC><<<g, bs, 0, s>>>(D, stage);
     } else {
-      if (D.limiter == LIM_FLAT) k_sweep_mix<false, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
-      else if (D.limiter == LIM_MINMOD) k_sweep_mix<false, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
-      else k_sweep_mix<false, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+      if (D.limiter == LIM_FLAT) k_sweep_mix<false, LIM_FLAT><<<g, bs, 0, s>>>(D, stage);
+      else if (D.limiter == LIM_MINMOD) k_sweep_mix<false, LIM_MINMOD><<<g, bs, 0, s>>>(D, stage);
+      else k_sweep_mix<false, LIM_MC><<<g, bs, 0, s>>>(D, stage);
     }
     return;
   }
   if (D.mag) {
-    if (D.limiter == LIM_FLAT) k_sweep<true, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
-    else if (D.limiter == LIM_MINMOD) k_sweep<true, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
-    else k_sweep<true, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+    if (D.limiter == LIM_FLAT) k_sweep<true, LIM_FLAT><<<g, bs, 0, s>>>(D, stage);
+    else if (D.limiter == LIM_MINMOD) k_sweep<true, LIM_MINMOD><<<g, bs, 0, s>>>(D, stage);
+    else k_sweep<true, LIM_MC><<<g, bs, 0, s>>>(D, stage);
   } else {
-    if (D.limiter == LIM_FLAT) k_sweep<false, LIM_FLAT><<<g, 256, 0, s>>>(D, stage);
-    else if (D.limiter == LIM_MINMOD) k_sweep<false, LIM_MINMOD><<<g, 256, 0, s>>>(D, stage);
-    else k_sweep<false, LIM_MC><<<g, 256, 0, s>>>(D, stage);
+    if (D.limiter == LIM_FLAT) k_sweep<false, LIM_FLAT><<<g, bs, 0, s>>>(D, stage);
+    else if (D.limiter == LIM_MINMOD) k_sweep<false, LIM_MINMOD><<<g, bs, 0, s>>>(D, stage);
+    else k_sweep<false, LIM_MC><<<g, bs, 0, s>>>(D, stage);
   }
 }
 

@@ -63,6 +63,7 @@ struct Dev {
   Scalars *sc;
   int nblk_cfl;
   long waves[3];   // sweep wavefronts per direction (0 if not solved)
+  int sweep_bw;    // sweep wavefronts per block (1, 2, 4, 8 or 16)
   // hyperbolic self-gravity (gravswitch 3): the potential is stored with a
   // one-cell ghost layer ((n1+2)(n2+2)(n3+2), Dirichlet values), twice for
   // the Jacobi-style stage update; psi and the stage-1 copies are interior.

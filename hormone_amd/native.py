@@ -111,6 +111,7 @@ def _load():
     lib.hg_kat.argtypes = [C.c_int32, C.c_int32, _dp, _dp]
     lib.hg_gravity.argtypes = [vp, C.c_double, C.c_double, _dp, C.c_double, _dp, _dp,
                                C.POINTER(C.c_int32)]
+    lib.hg_set_launch.argtypes = [vp, C.c_int32]
     return lib
 
 
@@ -129,7 +130,7 @@ def exported_symbols():
     return ["hg_version", "hg_create", "hg_destroy", "hg_ufnmax", "hg_upload",
             "hg_download", "hg_timestep", "hg_hydro_step", "hg_phidamp",
             "hg_last_error", "hg_run", "hg_profile", "hg_profile_read", "hg_kat",
-            "hg_gravity"]
+            "hg_gravity", "hg_set_launch"]
 
 
 STOP_END, STOP_OUTPUT, STOP_MAXSTEPS, STOP_ERROR = 1, 2, 3, 4
@@ -213,6 +214,9 @@ class Hydro:
                               C.byref(r))
         self._check(st, "hg_run", tn=n.value, time=t.value)
         return t.value, n.value, to.value, r.value
+
+    def set_launch(self, sweep_block_waves: int):
+        self._check(self._lib.hg_set_launch(self.h, sweep_block_waves), "hg_set_launch")
 
     def profile(self, enable=True):
         self._check(self._lib.hg_profile(self.h, int(enable)), "hg_profile")

@@ -222,6 +222,11 @@ int hg_batch_member_state(hg_batch *batch, hg_ctx *ctx, double *time, int64_t *t
 int hg_batch_config(hg_batch *batch, int32_t sweep_block_waves, int32_t sweep_grid,
                     int32_t rk_grid, int32_t use_graph);
 
+/* Launch shape of a context's sweep kernel: wavefronts per block (1, 2, 4,
+   8 or 16; default 4).  Changes which wavefront computes which line, never
+   the arithmetic.  No counterpart in the reference. */
+int hg_set_launch(hg_ctx *ctx, int32_t sweep_block_waves);
+
 /* Cell (Fortran indices) and direction (1..3, 0 for a cell error) of the
    last HG_ERR_FACE_EOS / HG_ERR_NEG_EINT. */
 int hg_last_error(const hg_ctx *ctx, int32_t *i, int32_t *j, int32_t *k,
