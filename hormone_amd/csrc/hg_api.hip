@@ -1,6 +1,7 @@
 // hg_api.hip -- the C ABI of include/hormone_hip.h over a device context.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -35,6 +36,7 @@ struct hg_ctx {
   std::vector<hipEvent_t> ev;   // event pool
   std::vector<int> ev_kind;     // kind of each recorded pair
   int ev_used = 0;
+  KEv kev{};
 };
 
 #define HIPCHK(x)                                   \
@@ -155,8 +157,9 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
       A.vc_lo = desc->bc[6 + 2 * a];
       A.vc_hi = desc->bc[6 + 2 * a + 1];
     }
-    if (hipMemcpy(c->coord, h.data(), tot * sizeof(double), hipMemcpyHostToDevice) !=
-        hipSuccess)
+    if (hipMemcpyAsync(c->coord, h.data(), tot * sizeof(double), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
       return fail(HG_ERR_DEVICE);
     // FP32-mixed limiter geometry, indices -2..n+1 per axis
     for (int a = 0; a < 3; ++a) {
@@ -174,8 +177,11 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
       float *dg = nullptr, *di = nullptr;
       if ((st = dev_alloc(c, sizeof(float) * 4 * m, (void **)&dg))) return fail(st);
       if ((st = dev_alloc(c, sizeof(float) * m, (void **)&di))) return fail(st);
-      if (hipMemcpy(dg, g4.data(), sizeof(float) * 4 * m, hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(di, gi.data(), sizeof(float) * m, hipMemcpyHostToDevice) != hipSuccess)
+      if (hipMemcpyAsync(dg, g4.data(), sizeof(float) * 4 * m, hipMemcpyHostToDevice,
+                         c->stream) != hipSuccess ||
+          hipMemcpyAsync(di, gi.data(), sizeof(float) * m, hipMemcpyHostToDevice, c->stream) !=
+              hipSuccess ||
+          hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(HG_ERR_DEVICE);
       D.ax[a].limg = reinterpret_cast<const float4 *>(dg) + 2;
       D.ax[a].idxf = di + 2;
@@ -192,7 +198,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
         if ((st = dev_alloc(c, fb, (void **)&D.DU[a][v]))) return fail(st);
   }
   for (int w = 0; w < D.nw; ++w)
-    if (hipMemset(D.W[w], 0, fb) != hipSuccess) return fail(HG_ERR_DEVICE);
+    if (hipMemsetAsync(D.W[w], 0, fb, c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
   D.nblk_cfl = (int)((D.ncell + 255) / 256);
   D.sweep_bw = 4;
   for (int a = 0; a < 3; ++a)
@@ -204,12 +210,12 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
     const size_t gb = sizeof(double) * (size_t)(D.n1 + 2) * (D.n2 + 2) * (D.n3 + 2);
     for (int q = 0; q < 2; ++q) {
       if ((st = dev_alloc(c, gb, (void **)&D.gphi[q]))) return fail(st);
-      if (hipMemset(D.gphi[q], 0, gb) != hipSuccess) return fail(HG_ERR_DEVICE);
+      if (hipMemsetAsync(D.gphi[q], 0, gb, c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
     }
     if ((st = dev_alloc(c, fb, (void **)&D.gpsi))) return fail(st);
     if ((st = dev_alloc(c, fb, (void **)&D.gphi0))) return fail(st);
     if ((st = dev_alloc(c, fb, (void **)&D.gpsi0))) return fail(st);
-    if (hipMemset(D.gpsi, 0, fb) != hipSuccess) return fail(HG_ERR_DEVICE);
+    if (hipMemsetAsync(D.gpsi, 0, fb, c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
     D.gcur = 0;
     // dtg_unit = min over cells of dtgrav_cell with cgrav = 1 (tools.f90:154-184)
     const double o1 = D.solve[0] ? 1.0 : 0.0, o2 = D.solve[1] ? 1.0 : 0.0,
@@ -228,7 +234,11 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
     c->dtg_unit = m;
   }
   if ((st = dev_alloc(c, sizeof(Scalars), (void **)&D.sc))) return fail(st);
-  if (hipMemset(D.sc, 0, sizeof(Scalars)) != hipSuccess) return fail(HG_ERR_DEVICE);
+  // Every initialisation write goes to the context's own (non-blocking)
+  // stream: work on the null stream is not ordered with it, and a zeroing
+  // memset landing after the error-key clear would read as an error.
+  if (hipMemsetAsync(D.sc, 0, sizeof(Scalars), c->stream) != hipSuccess)
+    return fail(HG_ERR_DEVICE);
   launch_clear_error(D, c->stream);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
   if (hipHostMalloc((void **)&c->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess)
@@ -286,7 +296,9 @@ static int upload_grvphi(hg_ctx *c, const double *a) {
       for (int i = -1; i <= D.n1; ++i)
         h[q++] = a[((size_t)(k + 2) * c->nyg + (j + 2)) * c->nxg + (i + 2)];
   for (int b = 0; b < 2; ++b)
-    HIPCHK(hipMemcpy(D.gphi[b], h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(D.gphi[b], h.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   c->D.gcur = 0;
   return HG_OK;
 }
@@ -363,7 +375,9 @@ extern "C" int hg_download(hg_ctx *c, int32_t field, double *a) {
     HIPCHK(hipStreamSynchronize(c->stream));
     const size_t n = (size_t)(D.n1 + 2) * (D.n2 + 2) * (D.n3 + 2);
     std::vector<double> h(n);
-    HIPCHK(hipMemcpy(h.data(), D.gphi[D.gcur], n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(h.data(), D.gphi[D.gcur], n * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     for (int k = 0; k < D.n3; ++k)
       for (int j = 0; j < D.n2; ++j)
         for (int i = 0; i < D.n1; ++i)
@@ -392,24 +406,22 @@ extern "C" int hg_download(hg_ctx *c, int32_t field, double *a) {
 }
 
 // ---- profiling -----------------------------------------------------------------
-static void prof_begin(hg_ctx *c, int kind) {
-  if (!c->prof) return;
+// Events for the next timed launch of the given kind (nullptr when profiling
+// is off); the launch records both on its own dispatch.
+static const KEv *prof_ev(hg_ctx *c, int kind) {
+  if (!c->prof) return nullptr;
   if ((size_t)(2 * c->ev_used + 2) > c->ev.size()) {
     for (int q = 0; q < 64; ++q) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
       c->ev.push_back(e);
     }
   }
   if ((size_t)c->ev_used >= c->ev_kind.size()) c->ev_kind.resize(c->ev_used + 64);
   c->ev_kind[c->ev_used] = kind;
-  hipEventRecord(c->ev[2 * c->ev_used], c->stream);
-}
-
-static void prof_end(hg_ctx *c) {
-  if (!c->prof) return;
-  hipEventRecord(c->ev[2 * c->ev_used + 1], c->stream);
+  c->kev = KEv{c->ev[2 * c->ev_used], c->ev[2 * c->ev_used + 1]};
   ++c->ev_used;
+  return &c->kev;
 }
 
 // after the stream has been synchronised: fold recorded pairs into totals
@@ -481,9 +493,7 @@ extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
   if (st) return st;
   const Dev &D = c->D;
   launch_set_clock(D, time, t_out, t_end, 0.0, outstyle, endstyle, 0, 0, 1, 1, c->stream);
-  prof_begin(c, HG_PROF_CFL);
-  launch_cfl(D, c->stream);
-  prof_end(c);
+  launch_cfl(D, c->stream, prof_ev(c, HG_PROF_CFL));
   launch_cfl_final(D, c->stream);
   HIPCHK(hipGetLastError());
   if ((st = check_error(c))) return st;
@@ -527,9 +537,8 @@ extern "C" int hg_gravity(hg_ctx *c, double time, double dt, double *grvtime, do
     for (int g = 1; g <= gk; ++g) {
       double faco, fact, facn;
       runge_coeff(g, gk, &faco, &fact, &facn);
-      prof_begin(c, HG_PROF_GRAV);
-      launch_grav_stage(c->D, g, faco, fact, facn, *dtgrav, cgrav, g == gk, damp, c->stream);
-      prof_end(c);
+      launch_grav_stage(c->D, g, faco, fact, facn, *dtgrav, cgrav, g == gk, damp, c->stream,
+                        prof_ev(c, HG_PROF_GRAV));
       c->D.gcur = 1 - c->D.gcur;
     }
     *grvtime = *grvtime + *dtgrav;
@@ -548,14 +557,10 @@ static void enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
   for (int stage = 1; stage <= rk; ++stage) {
     double faco, fact, facn;
     runge_coeff(stage, rk, &faco, &fact, &facn);
-    prof_begin(c, HG_PROF_SWEEP);
-    launch_sweep(D, stage, c->stream);
-    prof_end(c);
+    launch_sweep(D, stage, c->stream, prof_ev(c, HG_PROF_SWEEP));
     const bool last = stage == rk;
-    prof_begin(c, HG_PROF_RK);
     launch_rk(D, stage, faco, fact, facn, fuse_tail && last, fuse_tail && last && damp > 0.0,
-              damp, c->stream);
-    prof_end(c);
+              damp, c->stream, prof_ev(c, HG_PROF_RK));
   }
 }
 
@@ -596,20 +601,33 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
   const bool damp = D.mag && D.dim > 1 && damp_factor > 0.0;
   launch_set_clock(D, *time, *t_out, t_end, dt_out, outstyle, endstyle, *tn, tnlim, tn_out,
                    max_steps, c->stream);
-  prof_begin(c, HG_PROF_CFL);
-  launch_cfl(D, c->stream);
-  prof_end(c);
+  launch_cfl(D, c->stream, prof_ev(c, HG_PROF_CFL));
+  // Steps are enqueued 16 at a time (never more than max_steps in all) and
+  // the host reads the stop flag between groups; steps enqueued after a
+  // stop return at once.
   const int batch = 16;
   int reason = STOP_NONE;
+  int64_t left = max_steps, tn_before = *tn;
   for (;;) {
-    for (int s = 0; s < batch; ++s) {
+    const int nb = left < batch ? (int)left : batch;
+    if (nb <= 0) {  // the device stops at max_steps; never enqueue past it
+      reason = STOP_MAXSTEPS;
+      break;
+    }
+    const int ev_base = c->ev_used;
+    for (int s = 0; s < nb; ++s) {
       launch_cfl_final(D, c->stream);
       enqueue_stages(c, true, damp ? damp_factor : 0.0);
       launch_step_end(D, c->stream);
     }
+    left -= nb;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_sc, D.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    // time only the launches of steps that ran (2 timed launches per stage)
+    const int64_t ran = c->h_sc->tn - tn_before;
+    tn_before = c->h_sc->tn;
+    if (c->prof) c->ev_used = ev_base + (int)(std::min<int64_t>(ran, nb) * 2 * c->desc.rktype);
     prof_collect(c);
     reason = c->h_sc->stop;
     if (reason != STOP_NONE) break;
@@ -815,9 +833,13 @@ static int batch_rebuild(hg_batch *b) {
   if ((st = grow(&b->d_sitems, &b->cap_s, si.size() ? si.size() : 1))) return st;
   if ((st = grow(&b->d_citems, &b->cap_c, ci.size() ? ci.size() : 1))) return st;
   if (nb) {
-    HIPCHK(hipMemcpy(b->d_boxes, boxes.data(), nb * sizeof(Dev), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->d_sitems, si.data(), si.size() * sizeof(int2), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->d_citems, ci.data(), ci.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(b->d_boxes, boxes.data(), nb * sizeof(Dev), hipMemcpyHostToDevice,
+                          b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_sitems, si.data(), si.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_citems, ci.data(), ci.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
   }
   b->B.boxes = b->d_boxes;
   b->B.nbox = (int)nb;

@@ -11,6 +11,7 @@
 // the face between l and l+1 needs the right state from l+1, and the flux
 // difference of cell l needs the face flux of l-1.  60 of the 64 lanes
 // produce a result.  Boundary ghosts are evaluated on the fly.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "hg_kernels.hpp"
@@ -33,44 +34,33 @@ __device__ __forceinline__ int ghost_src(int code, bool lo, int c, int n) {
   return s < 0 ? 0 : (s > n - 1 ? n - 1 : s);  // lanes past the ghosts: unused
 }
 
+// One cell of a sweep line, interior or ghost.  Ghost cells
+// (boundarycondition.f90, codes 0..3) take the scalars d, p, phi from the
+// scalar-code source cell and v, B from the vector-code source cell with the
+// normal components transformed, then rebuild eint and e (:221-243).  Both
+// kinds share one set of loads (interior: both sources are the cell
+// itself), so the struct stays in registers.
 template <int DIR, bool MAG>
 __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base, int c) {
   Cell q;
   const int n = A.n;
-  if (c >= 0 && c < n) {
-    long m = base + (long)c * A.stride;
-    q.d = D.W[WD][m];
-    q.v[0] = D.W[WV1][m];
-    q.v[1] = D.W[WV2][m];
-    q.v[2] = D.W[WV3][m];
-    q.e = D.W[WE][m];
-    q.eint = D.W[WEINT][m];
-    if (MAG) {
-      q.b[0] = D.W[WB1][m];
-      q.b[1] = D.W[WB2][m];
-      q.b[2] = D.W[WB3][m];
-      q.phi = D.W[WPHI][m];
-    } else {
-      q.b[0] = q.b[1] = q.b[2] = q.phi = 0.0;
-    }
-    return q;
-  }
-  // ghost cell (boundarycondition.f90, codes 0..3): scalars d, p, phi from
-  // the scalar-code source, v and B from the vector-code source with the
-  // normal components transformed; then ptot/eint/e rebuilt (:221-243).
+  const bool ghost = c < 0 || c >= n;
   const bool lo = c < 0;
-  const int sc = lo ? A.sc_lo : A.sc_hi, vc = lo ? A.vc_lo : A.vc_hi;
-  const long ms = base + (long)ghost_src(sc, lo, c, n) * A.stride;
-  const long mv = base + (long)ghost_src(vc, lo, c, n) * A.stride;
-  const double gm1 = D.gamma - 1.0;
+  const int vc = lo ? A.vc_lo : A.vc_hi;
+  long ms = base + (long)c * A.stride, mv = ms;
+  if (ghost) {
+    const int sc = lo ? A.sc_lo : A.sc_hi;
+    ms = base + (long)ghost_src(sc, lo, c, n) * A.stride;
+    mv = base + (long)ghost_src(vc, lo, c, n) * A.stride;
+  }
   q.d = D.W[WD][ms];
-  const double p = gm1 * D.W[WEINT][ms];
   q.phi = MAG ? D.W[WPHI][ms] : 0.0;
+  const double eint = D.W[WEINT][ms];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     double v = D.W[WV1 + a][mv];
     double b = MAG ? D.W[WB1 + a][mv] : 0.0;
-    if (a == DIR) {
+    if (ghost && a == DIR) {
       if (vc == 1) {
         v = -v;
         b = -b;
@@ -82,14 +72,40 @@ __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base
     q.v[a] = v;
     q.b[a] = b;
   }
-  q.eint = p / gm1;
-  q.e = q.eint + 0.5 * (q.d * (q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]) +
-                        q.b[0] * q.b[0] + q.b[1] * q.b[1] + q.b[2] * q.b[2]);
+  if (!ghost) {
+    q.eint = eint;
+    q.e = D.W[WE][ms];
+  } else {
+    const double gm1 = D.gamma - 1.0;
+    const double p = gm1 * eint;
+    q.eint = p / gm1;
+    q.e = q.eint + 0.5 * (q.d * (q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]) +
+                          q.b[0] * q.b[0] + q.b[1] * q.b[1] + q.b[2] * q.b[2]);
+  }
   return q;
 }
 
+// Neighbour exchange along the line: up1 gives lane l the value of lane l-1,
+// dn1 the value of lane l+1 (the end lanes' results are never used).
+#if HG_SHUFFLE_DPP
+// DPP wavefront shifts (wave_shr:1 = 0x138, wave_shl:1 = 0x130): one VALU
+// move per 32-bit half, no LDS round trip.
+__device__ __forceinline__ int wshr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int wshl1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ double up1(double x) {
+  return __hiloint2double(wshr1(__double2hiint(x)), wshr1(__double2loint(x)));
+}
+__device__ __forceinline__ double dn1(double x) {
+  return __hiloint2double(wshl1(__double2hiint(x)), wshl1(__double2loint(x)));
+}
+__device__ __forceinline__ float up1f(float x) { return __int_as_float(wshr1(__float_as_int(x))); }
+__device__ __forceinline__ float dn1f(float x) { return __int_as_float(wshl1(__float_as_int(x))); }
+#else
 __device__ __forceinline__ double up1(double x) { return __shfl_up(x, 1, 64); }
 __device__ __forceinline__ double dn1(double x) { return __shfl_down(x, 1, 64); }
+__device__ __forceinline__ float up1f(float x) { return __shfl_up(x, 1, 64); }
+__device__ __forceinline__ float dn1f(float x) { return __shfl_down(x, 1, 64); }
+#endif
 
 template <int DIR, bool MAG, int LIM>
 __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage) {
@@ -283,8 +299,6 @@ __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
 // internal energy (the subtraction of kinetic and magnetic energy from the
 // total energy), the e_int guard decision and the flux-divergence
 // accumulation stay in FP64.
-__device__ __forceinline__ float up1f(float x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ float dn1f(float x) { return __shfl_down(x, 1, 64); }
 
 // Limiters on FP32 copies of the FP64 one-sided differences (differencing
 // first keeps the slope accurate to FP32 relative precision even where the
@@ -938,6 +952,17 @@ void runge_coeff(int stage, int rk, double *faco, double *fact, double *facn) {
   }
 }
 
+// Launch k<<<g, b, 0, s>>>(args...); with ev the dispatch itself carries the
+// start/stop events.
+template <typename... KA, typename... A>
+static inline void klaunch(void (*k)(KA...), unsigned g, unsigned b, hipStream_t s,
+                           const KEv *ev, A... args) {
+  if (ev)
+    hipExtLaunchKernelGGL(k, dim3(g), dim3(b), 0, s, ev->start, ev->stop, 0, KA(args)...);
+  else
+    k<<<g, b, 0, s>>>(KA(args)...);
+}
+
 void launch_pressure(const Dev &D, hipStream_t s) {
   if (D.mag) k_pressure<true><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
   else k_pressure<false><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
@@ -960,68 +985,69 @@ void launch_set_clock(const Dev &D, double time, double t_out, double t_end, dou
 // a block own neighbouring lines (consecutive i for the y and z sweeps), so
 // with 16 wavefronts a block uses whole 128-byte cache lines of the strided
 // y/z loads.
-void launch_sweep(const Dev &D, int stage, hipStream_t s) {
+void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev) {
   const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
   const unsigned g = nblk(D.waves[0] + D.waves[1] + D.waves[2], bw);
   const int bs = 64 * bw;
   if (D.mixed) {
     if (D.mag) {
-      if (D.limiter == LIM_FLAT) k_sweep_mix<true, LIM_FLAT><<<g, bs, 0, s>>>(D, stage);
-      else if (D.limiter == LIM_MINMOD) k_sweep_mix<true, LIM_MINMOD><<<g, bs, 0, s>>>(D, stage);
-      else k_sweep_mix<true, LIM_MC><<<g, bs, 0, s>>>(D, stage);
+      if (D.limiter == LIM_FLAT) klaunch(k_sweep_mix<true, LIM_FLAT>, g, bs, s, ev, D, stage);
+      else if (D.limiter == LIM_MINMOD) klaunch(k_sweep_mix<true, LIM_MINMOD>, g, bs, s, ev, D, stage);
+      else klaunch(k_sweep_mix<true, LIM_MC>, g, bs, s, ev, D, stage);
     } else {
-      if (D.limiter == LIM_FLAT) k_sweep_mix<false, LIM_FLAT><<<g, bs, 0, s>>>(D, stage);
-      else if (D.limiter == LIM_MINMOD) k_sweep_mix<false, LIM_MINMOD><<<g, bs, 0, s>>>(D, stage);
-      else k_sweep_mix<false, LIM_MC><<<g, bs, 0, s>>>(D, stage);
+      if (D.limiter == LIM_FLAT) klaunch(k_sweep_mix<false, LIM_FLAT>, g, bs, s, ev, D, stage);
+      else if (D.limiter == LIM_MINMOD) klaunch(k_sweep_mix<false, LIM_MINMOD>, g, bs, s, ev, D, stage);
+      else klaunch(k_sweep_mix<false, LIM_MC>, g, bs, s, ev, D, stage);
     }
     return;
   }
   if (D.mag) {
-    if (D.limiter == LIM_FLAT) k_sweep<true, LIM_FLAT><<<g, bs, 0, s>>>(D, stage);
-    else if (D.limiter == LIM_MINMOD) k_sweep<true, LIM_MINMOD><<<g, bs, 0, s>>>(D, stage);
-    else k_sweep<true, LIM_MC><<<g, bs, 0, s>>>(D, stage);
+    if (D.limiter == LIM_FLAT) klaunch(k_sweep<true, LIM_FLAT>, g, bs, s, ev, D, stage);
+    else if (D.limiter == LIM_MINMOD) klaunch(k_sweep<true, LIM_MINMOD>, g, bs, s, ev, D, stage);
+    else klaunch(k_sweep<true, LIM_MC>, g, bs, s, ev, D, stage);
   } else {
-    if (D.limiter == LIM_FLAT) k_sweep<false, LIM_FLAT><<<g, bs, 0, s>>>(D, stage);
-    else if (D.limiter == LIM_MINMOD) k_sweep<false, LIM_MINMOD><<<g, bs, 0, s>>>(D, stage);
-    else k_sweep<false, LIM_MC><<<g, bs, 0, s>>>(D, stage);
+    if (D.limiter == LIM_FLAT) klaunch(k_sweep<false, LIM_FLAT>, g, bs, s, ev, D, stage);
+    else if (D.limiter == LIM_MINMOD) klaunch(k_sweep<false, LIM_MINMOD>, g, bs, s, ev, D, stage);
+    else klaunch(k_sweep<false, LIM_MC>, g, bs, s, ev, D, stage);
   }
 }
 
 template <bool MAG, bool GRAV>
 static void rk_mag(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
-                   bool damp, double df, hipStream_t s) {
+                   bool damp, double df, hipStream_t s, const KEv *ev) {
   const unsigned g = nblk(D.ncell, 256);
-  if (cfl && damp) k_rk<MAG, true, true, GRAV><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
-  else if (cfl) k_rk<MAG, true, false, GRAV><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
-  else if (damp) k_rk<MAG, false, true, GRAV><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
-  else k_rk<MAG, false, false, GRAV><<<g, 256, 0, s>>>(D, stage, faco, fact, facn, df);
+  if (cfl && damp) klaunch(k_rk<MAG, true, true, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
+  else if (cfl) klaunch(k_rk<MAG, true, false, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
+  else if (damp) klaunch(k_rk<MAG, false, true, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
+  else klaunch(k_rk<MAG, false, false, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
 }
 
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
-               bool damp, double damp_factor, hipStream_t s) {
+               bool damp, double damp_factor, hipStream_t s, const KEv *ev) {
   if (D.grav) {
-    if (D.mag) rk_mag<true, true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s);
-    else rk_mag<false, true>(D, stage, faco, fact, facn, cfl, false, damp_factor, s);
+    if (D.mag) rk_mag<true, true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev);
+    else rk_mag<false, true>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev);
   } else {
-    if (D.mag) rk_mag<true, false>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s);
-    else rk_mag<false, false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s);
+    if (D.mag) rk_mag<true, false>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev);
+    else rk_mag<false, false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev);
   }
 }
 
 void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, double facn,
-                       double dtg, double cg, bool damp, double damp_factor, hipStream_t s) {
+                       double dtg, double cg, bool damp, double damp_factor, hipStream_t s,
+                       const KEv *ev) {
   const unsigned g = nblk(D.ncell, 256);
   if (damp)
-    k_grav_stage<true><<<g, 256, 0, s>>>(D, grungen, D.gcur, faco, fact, facn, dtg, cg,
-                                         damp_factor);
+    klaunch(k_grav_stage<true>, g, 256, s, ev, D, grungen, D.gcur, faco, fact, facn, dtg, cg,
+            damp_factor);
   else
-    k_grav_stage<false><<<g, 256, 0, s>>>(D, grungen, D.gcur, faco, fact, facn, dtg, cg,
-                                          damp_factor);
+    klaunch(k_grav_stage<false>, g, 256, s, ev, D, grungen, D.gcur, faco, fact, facn, dtg, cg,
+            damp_factor);
 }
 
-void launch_cfl(const Dev &D, hipStream_t s) {
-  if (D.mag) k_cfl<true><<<D.nblk_cfl, 256, 0, s>>>(D);
-  else k_cfl<false><<<D.nblk_cfl, 256, 0, s>>>(D);
+void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev) {
+  if (D.mag) klaunch(k_cfl<true>, (unsigned)D.nblk_cfl, 256, s, ev, D);
+  else klaunch(k_cfl<false>, (unsigned)D.nblk_cfl, 256, s, ev, D);
 }
 
 void launch_cfl_final(const Dev &D, hipStream_t s) { k_cfl_final<<<1, 256, 0, s>>>(D); }

@@ -124,12 +124,20 @@ void launch_set_step(const Dev &D, double dt, double ch, hipStream_t s);
 void launch_set_clock(const Dev &D, double time, double t_out, double t_end, double dt_out,
                       int outstyle, int endstyle, long long tn, long long tnlim,
                       long long tn_out, long long steps, hipStream_t s);
-void launch_sweep(const Dev &D, int stage, hipStream_t s);
+// Optional kernel timing: a launch given a KEv carries the two events on the
+// dispatch itself (hipExtLaunchKernel), so their elapsed time is the kernel's
+// own duration, with no separate marker packets around it.
+struct KEv {
+  hipEvent_t start, stop;
+};
+
+void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev = nullptr);
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
-               bool damp, double damp_factor, hipStream_t s);
+               bool damp, double damp_factor, hipStream_t s, const KEv *ev = nullptr);
 void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, double facn,
-                       double dtg, double cg, bool damp, double damp_factor, hipStream_t s);
-void launch_cfl(const Dev &D, hipStream_t s);
+                       double dtg, double cg, bool damp, double damp_factor, hipStream_t s,
+                       const KEv *ev = nullptr);
+void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev = nullptr);
 void launch_cfl_final(const Dev &D, hipStream_t s);
 void launch_step_end(const Dev &D, hipStream_t s);
 void launch_phidamp(const Dev &D, double factor, hipStream_t s);

@@ -52,6 +52,36 @@ def load_initial_state(eng, prm: P.Params, g: Grid, st: P.State):
         eng.upload(F_GRVPSI, np.zeros(g.padded_shape))
 
 
+def rel_l1_errors(mine: dict, ref: dict) -> dict:
+    """Relative L1 error of each snapshot field against a reference run.
+
+    Scalars (d, e, phi): sum|q - q_ref| / sum|q_ref|.  Vector components are
+    normalised by the reference's vector magnitude, as the reference's
+    regression norm does (tests.f90:141-153: |v| for v1..v3, |B| for
+    b1..b3), so a component that happens to be near zero everywhere does not
+    read as a large relative error: sum|v_i - v_i,ref| / sum|v_ref|.
+    """
+    def mag(fs):
+        return np.sqrt(sum(ref[f] * ref[f] for f in fs))
+
+    scale = {}
+    vmag = mag((F_V1, F_V2, F_V3))
+    for f in (F_V1, F_V2, F_V3):
+        scale[f] = vmag
+    if F_B1 in ref:
+        bmag = mag((F_B1, F_B2, F_B3))
+        for f in (F_B1, F_B2, F_B3):
+            scale[f] = bmag
+    names = {F_D: "d", F_V1: "v1", F_V2: "v2", F_V3: "v3", F_E: "e", F_B1: "b1",
+             F_B2: "b2", F_B3: "b3", F_PHI: "phi"}
+    out = {}
+    for f, a in mine.items():
+        den = float(np.abs(scale.get(f, ref[f])).sum())
+        num = float(np.abs(a - ref[f]).sum())
+        out[names.get(f, str(f))] = num / den if den > 0 else num
+    return out
+
+
 @dataclass
 class RunResult:
     tn: int

@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhormone_hip.so")
+# HORMONE_HIP_LIB names another build of the same library (a variant built
+# with different compile-time options); the default is the in-tree build.
+LIB_PATH = os.environ.get("HORMONE_HIP_LIB") or os.path.join(HERE, "libhormone_hip.so")
 
 HG_OK, HG_ERR_ARG, HG_ERR_UNSUPPORTED, HG_ERR_DEVICE = 0, 1, 2, 3
 HG_ERR_FACE_EOS, HG_ERR_NEG_EINT, HG_ERR_NOMEM, HG_ERR_STATE = 4, 5, 6, 7

@@ -219,3 +219,21 @@ def test_negative_internal_energy_is_reported_at_its_cell(hiplib, engine):
         eng.timestep(0.0, 0.2, 0.2, 1, 1)
     assert ei.value.code == HG_ERR_NEG_EINT
     assert tuple(ei.value.cell) == (37, 1, 1)
+
+
+def test_large_contexts_created_back_to_back_start_clean(hiplib):
+    """Regression: context set-up zeroes its buffers on its own stream before
+    clearing the error key.  Zeroing on the null stream was not ordered with
+    the context's non-blocking stream, so a second large context (kernels
+    already loaded, clear issued at once) could read a spurious error."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    engs = []
+    for q in range(2):
+        print(f"case back-to-back context {q}", flush=True)
+        prm, g, st = P.make_problem("kh3d", n=160, steps=2)
+        eng = Hydro(**D.desc_fields(prm, g))
+        D.load_initial_state(eng, prm, g, st)
+        r = D.run(eng, prm, g, device_loop=True)
+        assert r.tn == 2
+        engs.append(eng)
