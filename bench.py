@@ -152,6 +152,34 @@ class Runner:
         return True
 
 
+def advance_to(runner, t_target):
+    """Step a Runner until its clock reaches t_target exactly: the end
+    condition becomes endstyle 1 (time >= t_end), whose dt clipping
+    (timestep.f90:85) lands the last step on t_target; output stops are off."""
+    from hormone_amd import native
+    prm, eng = runner.prm, runner.eng
+    if prm.gravswitch == 3:
+        while runner.tim < t_target:
+            dt, ch, cg, dtg = eng.timestep(runner.tim, t_target, t_target, 0, 1, 0.0, grav=True)
+            if runner.tn == 0:
+                runner.grvtime = 0.0
+            else:
+                runner.grvtime, runner.cgrav_old, _, ns = eng.gravity(
+                    runner.tim, dt, runner.grvtime, cg, runner.cgrav_old, dtg)
+            eng.hydro_step(dt, ch)
+            if prm.mag_on and runner.g.dim > 1:
+                eng.phidamp(runner.damp)
+            runner.tim += dt
+            runner.tn += 1
+        return
+    while runner.tim < t_target:
+        runner.tim, runner.tn, _, reason = eng.run_loop(
+            runner.tim, runner.tn, t_target, 0.0, t_target, 0, 1, 1 << 40, 1, 1 << 40,
+            runner.damp)
+        if reason == native.STOP_END:
+            return
+
+
 def sums(D, eng, g, prm):
     """Mtot, Etot (+ gravitational energy) as output.f90:167-200."""
     f = [D.F_D, D.F_E] + ([D.F_GRVPHI] if prm.gravswitch == 3 else [])
@@ -272,8 +300,11 @@ def main():
         prm64, g64, st64 = setup(a, "fp64")
         eng64 = native.Hydro(**D.desc_fields(prm64, g64))
         D.load_initial_state(eng64, prm64, g64, st64)
+        # compare at the same physical time: the mixed run's dt sequence
+        # differs slightly (its CFL speeds carry FP32 rounding), so the FP64
+        # run is stepped to the mixed run's clock (last dt clipped onto it)
         r64 = Runner(eng64, prm64, g64)
-        r64.advance(n_done)
+        advance_to(r64, run.tim)
         ref = D.fetch(eng64, g64, fields + (D.F_P,))
         extra["rel_l1_vs_fp64"] = D.rel_l1_errors(mine, ref)
         # velocity error in units of the local sound speed (the relevant
@@ -282,6 +313,8 @@ def main():
         extra["vel_l1_over_cs"] = float(sum(np.abs(mine[f] - ref[f]).sum() for f in
                                             (D.F_V1, D.F_V2, D.F_V3)) / (3.0 * cs.sum()))
         extra["rel_l1_steps"] = n_done
+        extra["rel_l1_time"] = run.tim
+        extra["fp64_steps_to_same_time"] = r64.tn
     cpu = None if a.no_cpu else cpu_baseline(a, a.cpu_seconds)
     wl = {"ot512": f"orszagtang_xy {g.ax[0].n}x{g.ax[1].n} MHD+GLM, MUSCL+HLLD, RK3, "
                    f"courant 0.9 (run toward t=0.5)",

@@ -85,11 +85,21 @@ __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base
   return q;
 }
 
+// FP64 reciprocal for the mixed-precision path: hardware estimate refined
+// by two Newton steps (full FP64 accuracy, not correctly rounded).
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 // Neighbour exchange along the line: up1 gives lane l the value of lane l-1,
-// dn1 the value of lane l+1 (the end lanes' results are never used).
-#if HG_SHUFFLE_DPP
-// DPP wavefront shifts (wave_shr:1 = 0x138, wave_shl:1 = 0x130): one VALU
-// move per 32-bit half, no LDS round trip.
+// dn1 the value of lane l+1 (the end lanes' results are never used).  DPP
+// wavefront shifts (wave_shr:1 = 0x138, wave_shl:1 = 0x130): one VALU move
+// per 32-bit half and no LDS round trip (measured 1-3% faster per sweep
+// than ds_bpermute shuffles).
 __device__ __forceinline__ int wshr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
 __device__ __forceinline__ int wshl1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
 __device__ __forceinline__ double up1(double x) {
@@ -100,12 +110,6 @@ __device__ __forceinline__ double dn1(double x) {
 }
 __device__ __forceinline__ float up1f(float x) { return __int_as_float(wshr1(__float_as_int(x))); }
 __device__ __forceinline__ float dn1f(float x) { return __int_as_float(wshl1(__float_as_int(x))); }
-#else
-__device__ __forceinline__ double up1(double x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ double dn1(double x) { return __shfl_down(x, 1, 64); }
-__device__ __forceinline__ float up1f(float x) { return __shfl_up(x, 1, 64); }
-__device__ __forceinline__ float dn1f(float x) { return __shfl_down(x, 1, 64); }
-#endif
 
 template <int DIR, bool MAG, int LIM>
 __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage) {
@@ -391,7 +395,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     rq[s] = (!MAG && s >= 5) ? 0.0 : dn1(qd[s]);
     RS[s] = (!MAG && s >= 5) ? 0.0f : dn1f(S[s]);
   }
-  const float rphi = MAG ? dn1f((float)q.phi) : 0.0f;
+  const double rphi = MAG ? dn1(q.phi) : 0.0;
   const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;
   const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
   constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
@@ -401,12 +405,19 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     L[s] = qd[s] + dx1 * (double)S[s];
     R_[s] = rq[s] - dx2 * (double)RS[s];
   }
-  // face internal energy in FP64 from the face conserved variables
-  double eil = L[1] - 0.5 * (L[2] * L[2] + L[3] * L[3] + L[4] * L[4]) / L[0];
-  double eir = R_[1] - 0.5 * (R_[2] * R_[2] + R_[3] * R_[3] + R_[4] * R_[4]) / R_[0];
+  // face primitives in FP64: velocities, internal energy (the subtraction
+  // of kinetic and magnetic energy from the total energy), total pressure
+  const double idl = rcp64(L[0]), idr = rcp64(R_[0]);
+  const double vl[3] = {L[2 + a_] * idl, L[2 + b_] * idl, L[2 + c_] * idl};
+  const double vr[3] = {R_[2 + a_] * idr, R_[2 + b_] * idr, R_[2 + c_] * idr};
+  double eil = L[1] - 0.5 * L[0] * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]);
+  double eir = R_[1] - 0.5 * R_[0] * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+  double bbl = 0.0, bbr = 0.0;
   if (MAG) {
-    eil = eil - 0.5 * (L[5] * L[5] + L[6] * L[6] + L[7] * L[7]);
-    eir = eir - 0.5 * (R_[5] * R_[5] + R_[6] * R_[6] + R_[7] * R_[7]);
+    bbl = 0.5 * (L[5] * L[5] + L[6] * L[6] + L[7] * L[7]);
+    bbr = 0.5 * (R_[5] * R_[5] + R_[6] * R_[6] + R_[7] * R_[7]);
+    eil = eil - bbl;
+    eir = eir - bbr;
   }
   const bool face_valid = (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
   if (face_valid && (eil <= 0.0 || L[0] <= 0.0 || eir <= 0.0 || R_[0] <= 0.0)) {
@@ -414,52 +425,55 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
     atomicMin(&D.sc->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
   }
+  const double gm1 = D.gamma - 1.0;
+  const double pl = gm1 * eil, pr = gm1 * eir;
+  // fast speeds in FP32 (eos_p_cs, get_cf)
   const float gam = (float)D.gamma;
-  const float dl = (float)L[0], dr = (float)R_[0];
-  const float idl = __builtin_amdgcn_rcpf(dl), idr = __builtin_amdgcn_rcpf(dr);
-  const float v1l = (float)L[2 + a_] * idl, v1r = (float)R_[2 + a_] * idr;
-  const float v2l = (float)L[2 + b_] * idl, v2r = (float)R_[2 + b_] * idr;
-  const float v3l = (float)L[2 + c_] * idl, v3r = (float)R_[2 + c_] * idr;
-  float b1l = 0, b1r = 0, b2l = 0, b2r = 0, b3l = 0, b3r = 0, phil = 0, phir = 0;
+  const float dlf = (float)L[0], drf = (float)R_[0];
+  const float csl = rsqrt_(gam * (float)pl * __builtin_amdgcn_rcpf(dlf));
+  const float csr = rsqrt_(gam * (float)pr * __builtin_amdgcn_rcpf(drf));
+  float cfl, cfr;
   if (MAG) {
-    b1l = (float)L[5 + a_]; b1r = (float)R_[5 + a_];
-    b2l = (float)L[5 + b_]; b2r = (float)R_[5 + b_];
-    b3l = (float)L[5 + c_]; b3r = (float)R_[5 + c_];
-    phil = (float)q.phi;
-    phir = rphi;
+    cfl = get_cf(dlf, csl, (float)L[5 + a_], (float)L[5 + b_], (float)L[5 + c_]);
+    cfr = get_cf(drf, csr, (float)R_[5 + a_], (float)R_[5 + b_], (float)R_[5 + c_]);
+  } else {
+    cfl = csl;  // get_cf with B = 0 (exactly sqrt(0.5*(cs^2 + |cs^2|)))
+    cfr = csr;
   }
-  float ptl = (gam - 1.0f) * (float)eil;
-  float ptr = (gam - 1.0f) * (float)eir;
-  const float csl = rsqrt_(gam * ptl * idl);
-  const float csr = rsqrt_(gam * ptr * idr);
+  // GLM face values (hlldflux.f90:37-38) in FP64
+  const double ch = D.sc->ch;
+  double b1 = 0.0, phi = 0.0;
   if (MAG) {
-    ptl = ptl + 0.5f * (b1l * b1l + b2l * b2l + b3l * b3l);
-    ptr = ptr + 0.5f * (b1r * b1r + b2r * b2r + b3r * b3r);
+    b1 = 0.5 * ((L[5 + a_] + R_[5 + a_]) - (rphi - q.phi) * rcp64(ch));
+    phi = 0.5 * ((q.phi + rphi) - (R_[5 + a_] - L[5 + a_]) * ch);
   }
-  const float cfl = get_cf(dl, csl, b1l, b2l, b3l);
-  const float cfr = get_cf(dr, csr, b1r, b2r, b3r);
-  float t[9];
-  hlld(t, cfl, cfr, v1l, v1r, v2l, v2r, v3l, v3r, dl, dr, (float)L[1], (float)R_[1], ptl, ptr,
-       b1l, b1r, b2l, b2r, b3l, b3r, phil, phir, (float)D.sc->ch);
+  const double fl[8] = {L[0], vl[0], vl[1], vl[2], MAG ? L[5 + b_] : 0.0,
+                        MAG ? L[5 + c_] : 0.0, L[1], pl + bbl};
+  const double fr[8] = {R_[0], vr[0], vr[1], vr[2], MAG ? R_[5 + b_] : 0.0,
+                        MAG ? R_[5 + c_] : 0.0, R_[1], pr + bbr};
+  double t[9];
+  hlld_delta(t, fl, fr, b1, cfl, cfr);
+  t[5] = phi;
+  t[8] = ch * ch * b1;
 
-  // flux difference, accumulated in FP64
-  float tm[9];
+  // flux difference in FP64
+  double tm[9];
 #pragma unroll
-  for (int s = 0; s < 9; ++s) tm[s] = up1f(t[s]);
+  for (int s = 0; s < 9; ++s) tm[s] = (!MAG && s >= 5) ? 0.0 : up1(t[s]);
   if (lane >= 2 && lane <= 61 && c >= 0 && c < n) {
     const long m = base + (long)c * A.stride;
     const double ix = A.idxi[c];
     double *const *DU = D.DU[DIR];
-    DU[0][m] = ix * ((double)tm[0] - (double)t[0]);
-    DU[1 + a_][m] = ix * ((double)tm[1] - (double)t[1]);
-    DU[1 + b_][m] = ix * ((double)tm[2] - (double)t[2]);
-    DU[1 + c_][m] = ix * ((double)tm[3] - (double)t[3]);
-    DU[4][m] = ix * ((double)tm[4] - (double)t[4]);
+    DU[0][m] = ix * (tm[0] - t[0]);
+    DU[1 + a_][m] = ix * (tm[1] - t[1]);
+    DU[1 + b_][m] = ix * (tm[2] - t[2]);
+    DU[1 + c_][m] = ix * (tm[3] - t[3]);
+    DU[4][m] = ix * (tm[4] - t[4]);
     if (MAG) {
-      DU[5 + a_][m] = ix * ((double)tm[5] - (double)t[5]);
-      DU[5 + b_][m] = ix * ((double)tm[6] - (double)t[6]);
-      DU[5 + c_][m] = ix * ((double)tm[7] - (double)t[7]);
-      if (D.i9wv >= 0) DU[8][m] = ix * ((double)tm[8] - (double)t[8]);
+      DU[5 + a_][m] = ix * (tm[5] - t[5]);
+      DU[5 + b_][m] = ix * (tm[6] - t[6]);
+      DU[5 + c_][m] = ix * (tm[7] - t[7]);
+      if (D.i9wv >= 0) DU[8][m] = ix * (tm[8] - t[8]);
     }
   }
 }

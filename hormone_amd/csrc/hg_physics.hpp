@@ -169,4 +169,120 @@ __device__ __forceinline__ void hlld(R f[9], R cfl, R cfr, R v1l, R v1r, R v2l, 
   f[8] = ch * ch * b1;
 }
 
+// ---- FP32-mixed HLLD in perturbation form ----------------------------------
+// The Riemann solver of hlld() above (hlldflux.f90:19-179, same star states,
+// same state selection), rearranged so that FP32 only ever carries
+// differences.  Every star state is written as the left face state plus a
+// correction, and the returned flux is the FP64 flux function of the left
+// state (fstar with the GLM normal field b1 and the left total pressure)
+// plus an FP32 correction.  In exact arithmetic this equals hlld(); in
+// rounding, the FP32 error scales with the jump across the face instead of
+// with the flux itself, so smooth and near-static flows (a hydrostatic star,
+// a weak shear layer) stay close to the FP64 result while the wave speeds,
+// divisions and square roots run at FP32 cost.
+//
+// l, r: FP64 face primitives d, v1 (normal), v2, v3, b2, b3, e (total
+// energy), pt (total pressure).  b1: GLM normal field (FP64).  cfl, cfr:
+// fast speeds.  Writes f[0..4], f[6], f[7]; the GLM components f[5], f[8]
+// are the caller's.
+__device__ __forceinline__ void hlld_delta(double f[9], const double l[8], const double r[8],
+                                           double b1, float cfl, float cfr) {
+  const float h = 0.5f;
+  const float dl = (float)l[0], dr = (float)r[0];
+  const float v1l = (float)l[1], v1r = (float)r[1];
+  const float v2l = (float)l[2], v2r = (float)r[2], v3l = (float)l[3], v3r = (float)r[3];
+  const float b2l = (float)l[4], b2r = (float)r[4], b3l = (float)l[5], b3r = (float)r[5];
+  const float b1f = (float)b1;
+  // jumps across the face, differenced in FP64
+  const float Dd = (float)(r[0] - l[0]), Dv1 = (float)(r[1] - l[1]);
+  const float Dv2 = (float)(r[2] - l[2]), Dv3 = (float)(r[3] - l[3]);
+  const float Db2 = (float)(r[4] - l[4]), Db3 = (float)(r[5] - l[5]);
+  const float De = (float)(r[6] - l[6]), Dpt = (float)(r[7] - l[7]);
+  const float ehl = (float)(l[6] + l[7]), ehr = (float)(r[6] + r[7]);
+
+  const float sl = mn3(v1r - cfr, v1l - cfl, 0.0f);
+  const float sr = mx3(v1r + cfr, v1l + cfl, 0.0f);
+  const float slv = sl - v1l, srv = sr - v1r;
+  const float den = srv * dr - slv * dl;
+  const float dv = fdiv(srv * dr * Dv1 - Dpt, den);  // sm - v1l
+  const float dvr = dv - Dv1;                         // sm - v1r
+  const float sm = v1l + dv;
+  const float slm = slv - dv, srm = srv - dvr;        // sl - sm, sr - sm
+  const float dp = dl * slv * dv;                     // pta - ptl
+  const float dpr = dr * srv * dvr;                   // pta - ptr
+  const float ddla = fdiv(dl * dv, slm);              // dla - dl
+  const float ddra = fdiv(dr * dvr, srm);             // dra - dr
+  const float dla = dl + ddla, dra = dr + ddra;
+  const float sqla = rsqrt_(dla), sqra = rsqrt_(dra);
+  const float ab1 = abs_(b1f);
+  const float sla = sm - fdiv(ab1, sqla), sra = sm + fdiv(ab1, sqra);
+
+  const float b1b1 = b1f * b1f;
+  const float idenl = fdiv(1.0f, dl * slv * slm - b1b1);
+  const float idenr = fdiv(1.0f, dr * srv * srm - b1b1);
+  // single-star transverse states relative to their own side
+  const float dv2l = -b1f * b2l * dv * idenl, dv3l = -b1f * b3l * dv * idenl;
+  const float dv2r = -b1f * b2r * dvr * idenr, dv3r = -b1f * b3r * dvr * idenr;
+  const float gl = dl * slv * dv * idenl, gr = dr * srv * dvr * idenr;
+  const float db2l = b2l * gl, db3l = b3l * gl, db2r = b2r * gr, db3r = b3r * gr;
+  const float b2la = b2l + db2l, b3la = b3l + db3l, b2ra = b2r + db2r, b3ra = b3r + db3r;
+  // (v.B) of the side state minus that of its star state
+  const float xl = -b1f * dv - (v2l * db2l + dv2l * b2la) - (v3l * db3l + dv3l * b3la);
+  const float xr = -b1f * dvr - (v2r * db2r + dv2r * b2ra) - (v3r * db3r + dv3r * b3ra);
+  const float dela = fdiv(ehl * dv + dp * sm + b1f * xl, slm);   // ela - el
+  const float dera = fdiv(ehr * dvr + dpr * sm + b1f * xr, srm);  // era - er
+
+  // double-star (Alfven) states, relative to the left face state
+  const float sb1 = sgn(1.0f, b1f);
+  const float isqs = fdiv(1.0f, sqla + sqra);
+  const float sqp = rsqrt_(dla * dra);
+  const float dv2ra = Dv2 + dv2r, dv3ra = Dv3 + dv3r;  // v2ra - v2l
+  const float db2ra = Db2 + db2r, db3ra = Db3 + db3r;  // b2ra - b2l
+  const float dv2aa = (sqla * dv2l + sqra * dv2ra + (db2ra - db2l) * sb1) * isqs;
+  const float dv3aa = (sqla * dv3l + sqra * dv3ra + (db3ra - db3l) * sb1) * isqs;
+  const float db2aa = (sqla * db2ra + sqra * db2l + sqp * (dv2ra - dv2l) * sb1) * isqs;
+  const float db3aa = (sqla * db3ra + sqra * db3l + sqp * (dv3ra - dv3l) * sb1) * isqs;
+  const float b2aa = b2l + db2aa, b3aa = b3l + db3aa;
+  // (v2 b2 + v3 b3) of a state minus that of the Alfven state
+  const float pla = v2l * (db2l - db2aa) + dv2l * b2la - dv2aa * b2aa +
+                    v3l * (db3l - db3aa) + dv3l * b3la - dv3aa * b3aa;
+  const float pra = v2l * (db2ra - db2aa) + dv2ra * b2ra - dv2aa * b2aa +
+                    v3l * (db3ra - db3aa) + dv3ra * b3ra - dv3aa * b3aa;
+  const float delaa = dela - sb1 * sqla * pla;              // elaa - el
+  const float deraa = (De + dera) + sb1 * sqra * pra;       // eraa - el
+
+  // state selection (hlldflux.f90:140-154), as corrections to the left state
+  const float signsm = sgn(h, sm);
+  const float wla = h + sgn(h, sla), wra = h + sgn(h, -sra), waa = h - sgn(h, sla * sra);
+  const float dd_ = (h + signsm) * ddla + (h - signsm) * (Dd + ddra);
+  const float dv2_ = wla * dv2l + waa * dv2aa + wra * dv2ra;
+  const float dv3_ = wla * dv3l + waa * dv3aa + wra * dv3ra;
+  const float db2_ = wla * db2l + waa * db2aa + wra * db2ra;
+  const float db3_ = wla * db3l + waa * db3aa + wra * db3ra;
+  const float de_ = wla * dela + (h - sgn(h, sla * sm)) * delaa +
+                    (h + sgn(h, -sm * sra)) * deraa + (h - sgn(h, sra)) * (De + dera);
+  const float v2_ = v2l + dv2_, v3_ = v3l + dv3_, b2_ = b2l + db2_, b3_ = b3l + db3_;
+
+  // fstar(selected) - fstar(left), hlldflux.f90:156-179
+  const float dlv1l = (float)(l[0] * l[1]);
+  const float df0 = dd_ * sm + dl * dv;
+  const float df1 = df0 * sm + dlv1l * dv + dp;
+  const float df2 = df0 * v2_ + dlv1l * dv2_ - b1f * db2_;
+  const float df3 = df0 * v3_ + dlv1l * dv3_ - b1f * db3_;
+  const float dvb = b1f * dv + (dv2_ * b2_ + v2l * db2_) + (dv3_ * b3_ + v3l * db3_);
+  const float df4 = (de_ + dp) * sm + ehl * dv - b1f * dvb;
+  const float df6 = db2_ * sm + b2l * dv - b1f * dv2_;
+  const float df7 = db3_ * sm + b3l * dv - b1f * dv3_;
+
+  // FP64 flux function of the left state
+  const double u = l[1], du = l[0] * l[1];
+  f[0] = du + (double)df0;
+  f[1] = du * u + l[7] - b1 * b1 + (double)df1;
+  f[2] = du * l[2] - b1 * l[4] + (double)df2;
+  f[3] = du * l[3] - b1 * l[5] + (double)df3;
+  f[4] = (l[6] + l[7]) * u - b1 * (u * b1 + l[2] * l[4] + l[3] * l[5]) + (double)df4;
+  f[6] = l[4] * u - b1 * l[2] + (double)df6;
+  f[7] = l[5] * u - b1 * l[3] + (double)df7;
+}
+
 }  // namespace hg
