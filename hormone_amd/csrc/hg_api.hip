@@ -161,30 +161,30 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
                        c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
       return fail(HG_ERR_DEVICE);
-    // FP32-mixed limiter geometry, indices -2..n+1 per axis
+    // mixed-precision limiter geometry, indices -2..n+1 per axis
     for (int a = 0; a < 3; ++a) {
       const int m = nn[a] + 4;
-      std::vector<float> g4(4 * (size_t)m, 0.f), gi(m, 0.f);
+      std::vector<double> g4(4 * (size_t)m, 0.0), gw(m, 0.0);
       const double *x = src[a][0], *xi = src[a][1], *idx = src[a][4];
       for (int q = 1; q < m - 1; ++q) {  // array element q holds cell q-2
         const double xm = x[q - 1], x0 = x[q], xp = x[q + 1], xim = xi[q - 1], xi0 = xi[q];
-        g4[4 * q + 0] = (float)((xp - x0) / (xi0 - x0));
-        g4[4 * q + 1] = (float)((x0 - xm) / (x0 - xim));
-        g4[4 * q + 2] = (float)((xp - x0) / (x0 - xm));
-        g4[4 * q + 3] = (float)(1.0 / (xp - x0));
+        g4[4 * q + 0] = (xp - x0) / (xi0 - x0);
+        g4[4 * q + 1] = (x0 - xm) / (x0 - xim);
+        g4[4 * q + 2] = (xp - x0) / (x0 - xm);
+        g4[4 * q + 3] = 1.0 / (xp - x0);
+        gw[q] = idx[q] * idx[q + 1] / (idx[q] + idx[q + 1]);
       }
-      for (int q = 0; q < m; ++q) gi[q] = (float)idx[q];
-      float *dg = nullptr, *di = nullptr;
-      if ((st = dev_alloc(c, sizeof(float) * 4 * m, (void **)&dg))) return fail(st);
-      if ((st = dev_alloc(c, sizeof(float) * m, (void **)&di))) return fail(st);
-      if (hipMemcpyAsync(dg, g4.data(), sizeof(float) * 4 * m, hipMemcpyHostToDevice,
+      double *dg = nullptr, *dw = nullptr;
+      if ((st = dev_alloc(c, sizeof(double) * 4 * m, (void **)&dg))) return fail(st);
+      if ((st = dev_alloc(c, sizeof(double) * m, (void **)&dw))) return fail(st);
+      if (hipMemcpyAsync(dg, g4.data(), sizeof(double) * 4 * m, hipMemcpyHostToDevice,
                          c->stream) != hipSuccess ||
-          hipMemcpyAsync(di, gi.data(), sizeof(float) * m, hipMemcpyHostToDevice, c->stream) !=
+          hipMemcpyAsync(dw, gw.data(), sizeof(double) * m, hipMemcpyHostToDevice, c->stream) !=
               hipSuccess ||
           hipStreamSynchronize(c->stream) != hipSuccess)
         return fail(HG_ERR_DEVICE);
-      D.ax[a].limg = reinterpret_cast<const float4 *>(dg) + 2;
-      D.ax[a].idxf = di + 2;
+      D.ax[a].limg = reinterpret_cast<const double4 *>(dg) + 2;
+      D.ax[a].mmw = dw + 2;
     }
   }
   const size_t fb = sizeof(double) * (size_t)D.ncell;
@@ -604,10 +604,12 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
   launch_cfl(D, c->stream, prof_ev(c, HG_PROF_CFL));
   // Steps are enqueued 16 at a time (never more than max_steps in all) and
   // the host reads the stop flag between groups; steps enqueued after a
-  // stop return at once.
+  // stop return at once.  Each step ends with one launch that closes it
+  // (step_end) and computes the next step's dt and ch (cfl_final).
   const int batch = 16;
   int reason = STOP_NONE;
   int64_t left = max_steps, tn_before = *tn;
+  launch_cfl_final(D, c->stream);
   for (;;) {
     const int nb = left < batch ? (int)left : batch;
     if (nb <= 0) {  // the device stops at max_steps; never enqueue past it
@@ -616,9 +618,8 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
     }
     const int ev_base = c->ev_used;
     for (int s = 0; s < nb; ++s) {
-      launch_cfl_final(D, c->stream);
       enqueue_stages(c, true, damp ? damp_factor : 0.0);
-      launch_step_end(D, c->stream);
+      launch_step_end_cfl(D, c->stream);
     }
     left -= nb;
     HIPCHK(hipGetLastError());
@@ -764,6 +765,7 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
   launch_set_clock(c->D, time, t_out, t_end, dt_out, outstyle, endstyle, 0, tnlim,
                    tn_out < 1 ? 1 : tn_out, (long long)1 << 62, b->stream);
   launch_cfl(c->D, b->stream);
+  launch_cfl_final(c->D, b->stream);  // its first dt; later ones close each step
   HIPCHK(hipGetLastError());
   b->mag = c->D.mag;
   b->lim = c->D.limiter;
@@ -805,7 +807,6 @@ static void batch_enqueue_step(hg_batch *b) {
   const unsigned gs = b->sweep_grid ? (unsigned)std::min<long>(b->sweep_grid, sw) : (unsigned)sw;
   const unsigned gr = b->rk_grid ? (unsigned)std::min<long>(b->rk_grid, B.ncitems)
                                  : (unsigned)B.ncitems;
-  launch_bcfl_final(B, b->stream);
   for (int stage = 1; stage <= b->rktype; ++stage) {
     double faco, fact, facn;
     runge_coeff(stage, b->rktype, &faco, &fact, &facn);
@@ -813,7 +814,7 @@ static void batch_enqueue_step(hg_batch *b) {
                          b->stream);
     launch_brk(B, b->mag, stage, faco, fact, facn, stage == b->rktype, gr, b->stream);
   }
-  launch_bstep_end(B, b->stream);
+  launch_bstep_end_cfl(B, b->stream);
 }
 
 static int batch_rebuild(hg_batch *b) {

@@ -108,8 +108,6 @@ __device__ __forceinline__ double up1(double x) {
 __device__ __forceinline__ double dn1(double x) {
   return __hiloint2double(wshl1(__double2hiint(x)), wshl1(__double2loint(x)));
 }
-__device__ __forceinline__ float up1f(float x) { return __int_as_float(wshr1(__float_as_int(x))); }
-__device__ __forceinline__ float dn1f(float x) { return __int_as_float(wshl1(__float_as_int(x))); }
 
 template <int DIR, bool MAG, int LIM>
 __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage) {
@@ -297,29 +295,36 @@ __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
 
 
 // ---- FP32-mixed sweep -------------------------------------------------------
-// Same mapping and stencil as sweep_line.  Reconstruction (limited slopes),
-// the EOS/fast speeds and the HLLD flux run in FP32 with the hardware
-// reciprocal and square root; the cell state, the face states, the face
-// internal energy (the subtraction of kinetic and magnetic energy from the
-// total energy), the e_int guard decision and the flux-divergence
-// accumulation stay in FP64.
+// Same mapping and stencil as sweep_line.  The Riemann solver (fast speeds,
+// star states, state selection) runs in FP32 in perturbation form
+// (hlld_delta), with the hardware reciprocal and square root.  Everything
+// whose FP64 form costs no more than FP32 on this hardware (additions and
+// multiplications run at the same rate in both when not packed) stays FP64:
+// the cell state, the limited slopes (with the limiter divisions replaced by
+// per-cell geometry factors), the face states, the face internal energy
+// (the subtraction of kinetic and magnetic energy from the total energy), the
+// e_int guard, the base flux and the flux-divergence accumulation.  Keeping
+// the limiter and guard decisions in FP64 matters: a decision that flips
+// changes a face state by a whole slope, and such flips dominated the error
+// of an all-FP32 reconstruction.
 
-// Limiters on FP32 copies of the FP64 one-sided differences (differencing
-// first keeps the slope accurate to FP32 relative precision even where the
-// neighbouring values agree to many digits).
-__device__ __forceinline__ float mc_mixed(float dm, float dp, float4 g) {
-  const float top = dm * g.z;  // (u2-u1)*(x3-x2)/(x2-x1)
-  const float bot = dp;        // u3-u2
-  const float sig = sgn(1.0f, bot);
-  return sig * mx2(0.0f, mn3(sig * 0.5f * (bot + top), sig * g.x * bot, sig * g.y * top)) * g.w;
+// Modified MC limiter (fluxlimiter.f90:58-75) on the one-sided differences
+// dm = u2-u1, dp = u3-u2 with g = (cf, cb, (x3-x2)/(x2-x1), 1/(x3-x2)).
+__device__ __forceinline__ double mc_geo(double dm, double dp, double4 g) {
+  const double top = dm * g.z;
+  const double bot = dp;
+  const double sig = sgn(1.0, bot);
+  return sig * mx2(0.0, mn3(sig * 0.5 * (bot + top), sig * g.x * bot, sig * g.y * top)) * g.w;
 }
 
-__device__ __forceinline__ float minmod_mixed(float dm, float dp, float ida, float idb) {
-  const float x = 1.4f * dp * idb;
-  const float y = 1.4f * dm * ida;
-  const float z = (dp + dm) * ida * idb * __builtin_amdgcn_rcpf(ida + idb);
-  const float s = sgn(1.0f, x);
-  return s * mx2(0.0f, mn3(fabsf(x), s * y, s * z));
+// Generalised minmod (fluxlimiter.f90:40-56); w = ida*idb/(ida+idb).
+__device__ __forceinline__ double minmod_geo(double dm, double dp, double ida, double idb,
+                                             double w) {
+  const double x = 1.4 * dp * idb;
+  const double y = 1.4 * dm * ida;
+  const double z = (dp + dm) * w;
+  const double s = sgn(1.0, x);
+  return s * mx2(0.0, mn3(fabs(x), s * y, s * z));
 }
 
 template <int DIR, bool MAG, int LIM>
@@ -348,33 +353,33 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   const double qd[8] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2]};
   const int cc = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
 
-  float S[8];
+  double S[8];
   if (LIM == LIM_FLAT) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) S[s] = 0.0f;
+    for (int s = 0; s < 8; ++s) S[s] = 0.0;
   } else {
     const int cl = cc > -1 ? cc : -1, cr = cc < n ? cc : n;
-    const float4 geo = A.limg[cl];
-    const float ida = A.idxf[cl], idb = A.idxf[cr + 1];
+    const double4 geo = A.limg[cl];
+    const double ida = A.idx[cl], idb = A.idx[cr + 1], mw = A.mmw[cl];
     const double hl = A.x[cl] - A.xi[cl - 1], hr = A.xi[cl] - A.x[cl];
     double fl[8], fr[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       if (!MAG && s >= 5) {
-        S[s] = 0.0f;
+        S[s] = 0.0;
         fl[s] = fr[s] = 0.0;
         continue;
       }
       const double u0 = up1(qd[s]), u2 = dn1(qd[s]);
-      const float dm = (float)(qd[s] - u0), dp = (float)(u2 - qd[s]);
-      const float du = (LIM == LIM_MINMOD) ? minmod_mixed(dm, dp, ida, idb) : mc_mixed(dm, dp, geo);
+      const double dm = qd[s] - u0, dp = u2 - qd[s];
+      const double du = (LIM == LIM_MINMOD) ? minmod_geo(dm, dp, ida, idb, mw) : mc_geo(dm, dp, geo);
       S[s] = du;
-      fl[s] = qd[s] - hl * (double)du;
-      fr[s] = qd[s] + hr * (double)du;
+      fl[s] = qd[s] - hl * du;
+      fr[s] = qd[s] + hr * du;
     }
-    // e_int guard in FP64 (interpolation.f90:143-163)
-    double eintl = fl[1] - 0.5 * (fl[2] * fl[2] + fl[3] * fl[3] + fl[4] * fl[4]) / fl[0];
-    double eintr = fr[1] - 0.5 * (fr[2] * fr[2] + fr[3] * fr[3] + fr[4] * fr[4]) / fr[0];
+    // e_int guard (interpolation.f90:143-163)
+    double eintl = fl[1] - 0.5 * (fl[2] * fl[2] + fl[3] * fl[3] + fl[4] * fl[4]) * rcp64(fl[0]);
+    double eintr = fr[1] - 0.5 * (fr[2] * fr[2] + fr[3] * fr[3] + fr[4] * fr[4]) * rcp64(fr[0]);
     if (MAG) {
       eintl = eintl - 0.5 * (fl[5] * fl[5] + fl[6] * fl[6] + fl[7] * fl[7]);
       eintr = eintr - 0.5 * (fr[5] * fr[5] + fr[6] * fr[6] + fr[7] * fr[7]);
@@ -383,17 +388,16 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     if (eintl >= mx2(el_, em_) || eintl <= mn2(el_, em_) || eintr >= mx2(em_, er_) ||
         eintr <= mn2(em_, er_)) {
 #pragma unroll
-      for (int s = 0; s < 8; ++s) S[s] = 0.0f;
+      for (int s = 0; s < 8; ++s) S[s] = 0.0;
     }
   }
 
   // face c+1/2: right cell from lane+1
-  double rq[8];
-  float RS[8];
+  double rq[8], RS[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     rq[s] = (!MAG && s >= 5) ? 0.0 : dn1(qd[s]);
-    RS[s] = (!MAG && s >= 5) ? 0.0f : dn1f(S[s]);
+    RS[s] = (!MAG && s >= 5) ? 0.0 : dn1(S[s]);
   }
   const double rphi = MAG ? dn1(q.phi) : 0.0;
   const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;
@@ -402,8 +406,8 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   double L[8], R_[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
-    L[s] = qd[s] + dx1 * (double)S[s];
-    R_[s] = rq[s] - dx2 * (double)RS[s];
+    L[s] = qd[s] + dx1 * S[s];
+    R_[s] = rq[s] - dx2 * RS[s];
   }
   // face primitives in FP64: velocities, internal energy (the subtraction
   // of kinetic and magnetic energy from the total energy), total pressure
@@ -835,7 +839,13 @@ __device__ __forceinline__ void step_end(const Dev &D) {
   if (sc->steps_left <= 0) sc->stop = STOP_MAXSTEPS;
 }
 
-__global__ void k_step_end(Dev D) { step_end(D); }
+// The end of one step and the dt of the next in one launch: step_end on one
+// thread, then (if the run goes on) the CFL reduction of cfl_final.
+__global__ __launch_bounds__(256) void k_step_end_cfl(Dev D) {
+  if (threadIdx.x == 0) step_end(D);
+  __syncthreads();
+  cfl_final(D);
+}
 
 // ---- batched boxes: persistent kernels over flattened work-lists ---------
 template <bool MAG, int LIM, bool MIX>
@@ -887,11 +897,12 @@ __global__ __launch_bounds__(256) void k_brk(Batch B, int stage, double faco, do
   }
 }
 
-__global__ __launch_bounds__(256) void k_bcfl_final(Batch B) { cfl_final(B.boxes[blockIdx.x]); }
-
-__global__ void k_bstep_end(Batch B) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < B.nbox) step_end(B.boxes[b]);
+// One block per box: close its step, then compute its next dt and ch.
+__global__ __launch_bounds__(256) void k_bstep_end_cfl(Batch B) {
+  const Dev &D = B.boxes[blockIdx.x];
+  if (threadIdx.x == 0) step_end(D);
+  __syncthreads();
+  cfl_final(D);
 }
 
 __global__ void k_clear_error(Scalars *sc) { sc->err = NOERR; }
@@ -1066,7 +1077,7 @@ void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev) {
 
 void launch_cfl_final(const Dev &D, hipStream_t s) { k_cfl_final<<<1, 256, 0, s>>>(D); }
 
-void launch_step_end(const Dev &D, hipStream_t s) { k_step_end<<<1, 1, 0, s>>>(D); }
+void launch_step_end_cfl(const Dev &D, hipStream_t s) { k_step_end_cfl<<<1, 256, 0, s>>>(D); }
 
 void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
   k_phidamp<<<nblk(D.ncell, 256), 256, 0, s>>>(D, factor);
@@ -1104,12 +1115,8 @@ void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, do
   }
 }
 
-void launch_bcfl_final(const Batch &B, hipStream_t s) {
-  k_bcfl_final<<<B.nbox, 256, 0, s>>>(B);
-}
-
-void launch_bstep_end(const Batch &B, hipStream_t s) {
-  k_bstep_end<<<(B.nbox + 63) / 64, 64, 0, s>>>(B);
+void launch_bstep_end_cfl(const Batch &B, hipStream_t s) {
+  k_bstep_end_cfl<<<B.nbox, 256, 0, s>>>(B);
 }
 
 void launch_kat(int which, int n, const double *in, double *out, hipStream_t s) {

@@ -37,10 +37,11 @@ struct Scalars {
 // geometry of one sweep direction, indices 0-based interior (ghosts -2,-1,n,n+1)
 struct Axis {
   const double *x, *xi, *idx, *idxi, *dx, *dxi;  // base-shifted: valid for c = -2..n+1
-  // FP32-mixed path: per-cell limiter geometry evaluated once in FP64 on the
-  // host and rounded: (cf, cb, (x3-x2)/(x2-x1), 1/(x3-x2)) and idx as float
-  const float4 *limg;
-  const float *idxf;
+  // mixed-precision path: per-cell limiter geometry evaluated once on the
+  // host, (cf, cb, (x3-x2)/(x2-x1), 1/(x3-x2)), and the minmod weight
+  // idx(i)*idx(i+1)/(idx(i)+idx(i+1))
+  const double4 *limg;
+  const double *mmw;
   int n;
   long stride;       // element stride of this axis in the SoA arrays
   int sc_lo, sc_hi;  // scalar BC codes (bcNis, bcNos)
@@ -112,8 +113,7 @@ void launch_bsweep_blocks(const Batch &B, int mag, int lim, int mixed, int stage
                           unsigned grid, int block_waves, hipStream_t s);
 void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, double facn,
                 bool last, unsigned grid, hipStream_t s);
-void launch_bcfl_final(const Batch &B, hipStream_t s);
-void launch_bstep_end(const Batch &B, hipStream_t s);
+void launch_bstep_end_cfl(const Batch &B, hipStream_t s);  // per box: step_end, cfl_final
 
 // RK coefficients (faco, fact, facn) of get_runge_coeff, rungekutta.f90:146-183
 void runge_coeff(int stage, int rktype, double *faco, double *fact, double *facn);
@@ -139,7 +139,7 @@ void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, doub
                        const KEv *ev = nullptr);
 void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev = nullptr);
 void launch_cfl_final(const Dev &D, hipStream_t s);
-void launch_step_end(const Dev &D, hipStream_t s);
+void launch_step_end_cfl(const Dev &D, hipStream_t s);  // step_end, then cfl_final
 void launch_phidamp(const Dev &D, double factor, hipStream_t s);
 void launch_kat(int which, int n, const double *in, double *out, hipStream_t s);
 

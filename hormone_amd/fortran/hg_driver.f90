@@ -4,9 +4,11 @@
 ! &simucon &bouncon, same variable names and meanings as src/main/setup.f90,
 ! plus its own &gpucon group), builds the uniform Cartesian grid the way
 ! gridset does (src/main/gridset.f90:43-158), sets the shock-tube or
-! Orszag-Tang initial state (src/init/shocktube.f90, src/init/orszagtang.f90),
-! and runs the hydro-only main loop of hormone.f90:124-179 through the C ABI
-! (hormone_hip_mod).  The final state is written in the reference's binary
+! Orszag-Tang initial state (src/init/shocktube.f90, src/init/orszagtang.f90)
+! or reads an initial state from a file (&gpucon initfile), and runs the main
+! loop of hormone.f90:124-179 through the C ABI (hormone_hip_mod): timestep,
+! the hyperbolic self-gravity solver when &gravcon gravswitch=3, the hydro
+! step and phidamp.  The final state is written in the reference's binary
 ! snapshot layout (output.f90:697-795), so it can be compared with the
 ! reference's golden files directly.
 !
@@ -42,11 +44,17 @@ program hg_driver
  integer:: bc1is=0, bc1os=0, bc2is=0, bc2os=0, bc3is=0, bc3os=0
  integer:: bc1iv=0, bc1ov=0, bc2iv=0, bc2ov=0, bc3iv=0, bc3ov=0
  logical:: eq_sym=.false.
+ ! &gravcon (the variables this path reads; the rest are accepted and ignored)
+ integer:: gravswitch=0, grvsrctype=0, maxtngrv=-1, gbtype=0, grktype=2
+ integer:: gis=1, gie=1, gjs=1, gje=1, gks=1, gke=1
+ real(8):: grverr=1d-15, cgerr=1d-15, HGfac=5d0, hgcfl=0.9d0, alphagrv=0.1d0
+ logical:: grav_init_other=.false., grav_init_relax=.false., include_extgrv=.false.
  ! &gpucon (this driver only)
  character(len=40):: simtype='orszagtang_xy'
- character(len=256):: outfile='final.bin'
+ character(len=256):: outfile='final.bin', initfile=''
  logical:: is_test=.true.
  integer:: device=0
+ real(8):: gconst=6.67430d-8   ! G (a parameter of the reference, G=1 for test set-ups)
 
  namelist /gridcon/ xi1s,xi1e,xi2s,xi2e,xi3s,xi3e,is,ie,js,je,ks,ke, &
                     imesh,jmesh,kmesh,x1min,x2min,x3min,fmr_max,fmr_lvl
@@ -60,20 +68,24 @@ program hg_driver
                     solve_hydro,solve_grav,fixed_dt,smeartype
  namelist /bouncon/ bc1is,bc1os,bc2is,bc2os,bc3is,bc3os, &
                     bc1iv,bc1ov,bc2iv,bc2ov,bc3iv,bc3ov,eq_sym
- namelist /gpucon/ simtype,outfile,is_test,device
+ namelist /gravcon/ gravswitch,grvsrctype,grverr,cgerr,HGfac,hgcfl,maxtngrv,gbtype, &
+                    grktype,alphagrv,grav_init_other,grav_init_relax,include_extgrv, &
+                    gis,gie,gjs,gje,gks,gke
+ namelist /gpucon/ simtype,outfile,initfile,is_test,device,gconst
 
  real(8),allocatable,target:: x1(:),xi1(:),dx1(:),dxi1(:),idx1(:),idxi1(:)
  real(8),allocatable,target:: x2(:),xi2(:),dx2(:),dxi2(:),idx2(:),idxi2(:)
  real(8),allocatable,target:: x3(:),xi3(:),dx3(:),dxi3(:),idx3(:),idxi3(:)
  real(8),allocatable,target:: d(:,:,:),p(:,:,:),v1(:,:,:),v2(:,:,:),v3(:,:,:), &
                               e(:,:,:),b1(:,:,:),b2(:,:,:),b3(:,:,:),phi(:,:,:), &
-                              eint(:,:,:),u(:,:,:,:)
+                              eint(:,:,:),u(:,:,:,:),grvphi(:,:,:),grvpsi(:,:,:)
  type(hg_desc):: desc
  type(c_ptr):: ctx
  integer(c_int):: st
  integer:: i,j,k,n,ufnmax,dim,tn,ui
  logical:: solve_i,solve_j,solve_k
- real(8):: time,dt,ch,cgrav,dtgrav,t_out,unit_sec,damp,pi
+ real(8):: time,dt,ch,cgrav,dtgrav,t_out,unit_sec,damp,pi,grvtime,cgrav_old
+ integer(c_int32_t):: nsub
  character(len=256):: parafile
 
  call get_command_argument(1, parafile)
@@ -119,9 +131,15 @@ program hg_driver
  allocate(u(is-2:ie+2,js-2:je+2,ks-2:ke+2,1:ufnmax))
  d=0d0; p=0d0; v1=0d0; v2=0d0; v3=0d0; e=0d0; b1=0d0; b2=0d0; b3=0d0; phi=0d0
  u=0d0
+ allocate(grvphi,grvpsi,mold=d)
+ grvphi=0d0; grvpsi=0d0
 
  pi = acos(-1d0)
- call initial_state
+ if(len_trim(initfile)>0)then
+  call read_state(trim(initfile))
+ else
+  call initial_state
+ end if
  ! initialcondition.f90:133-150, eostype 0
  do k=ks,ke ; do j=js,je ; do i=is,ie
   eint(i,j,k) = p(i,j,k)/(gamma-1d0)
@@ -153,9 +171,9 @@ program hg_driver
  end select
  desc%bc = [bc1is,bc1os,bc2is,bc2os,bc3is,bc3os,bc1iv,bc1ov,bc2iv,bc2ov,bc3iv,bc3ov]
  desc%gamma = gamma ; desc%courant = courant ; desc%alpha9wave = alpha9wave
- desc%gravswitch = 0 ; desc%grktype = 2
- desc%HGfac = 5d0 ; desc%hgcfl = 0.9d0 ; desc%alphagrv = 0.1d0 ; desc%G = 6.67430d-8
- desc%maxtngrv = -1 ; desc%precision = HG_PREC_FP64 ; desc%device = device
+ desc%gravswitch = gravswitch ; desc%grktype = grktype
+ desc%HGfac = HGfac ; desc%hgcfl = hgcfl ; desc%alphagrv = alphagrv ; desc%G = gconst
+ desc%maxtngrv = maxtngrv ; desc%precision = HG_PREC_FP64 ; desc%device = device
  desc%reserved = 0
 
  st = hg_create(desc, ctx)
@@ -175,14 +193,25 @@ program hg_driver
   st = hg_upload(ctx, HG_F_U+n-1, c_loc(u(is-2,js-2,ks-2,n)))
   call hg_check(ctx, st, 'upload')
  end do
+ if(gravswitch==3)then
+  st = hg_upload(ctx, HG_F_GRVPHI, c_loc(grvphi)) ; call hg_check(ctx, st, 'upload')
+  st = hg_upload(ctx, HG_F_GRVPSI, c_loc(grvpsi)) ; call hg_check(ctx, st, 'upload')
+ end if
 
  ! main loop, hormone.f90:124-179 (hydro only)
  damp = exp(-alpha9wave*courant)
  time = 0d0 ; tn = 0
+ grvtime = 0d0 ; cgrav_old = 0d0   ! gravmod values at start-up
  main_loop: do
   st = hg_timestep(ctx, time, t_out, t_end, outstyle, endstyle, fixed_dt, &
                    dt, ch, cgrav, dtgrav)
   call hg_check(ctx, st, 'hg_timestep')
+  ! gravity.f90:41-60: the initial potential at tn = 0, the hyperbolic
+  ! solver afterwards
+  if(gravswitch==3.and.tn>0)then
+   st = hg_gravity(ctx, time, dt, grvtime, cgrav, cgrav_old, dtgrav, nsub)
+   call hg_check(ctx, st, 'hg_gravity')
+  end if
   if(solve_hydro)then
    st = hg_hydro_step(ctx, dt, ch)
    call hg_check(ctx, st, 'hg_hydro_step')
@@ -238,9 +267,25 @@ contains
   read(u_, nml=eos_con, iostat=ios) ; rewind(u_)
   read(u_, nml=simucon, iostat=ios) ; rewind(u_)
   read(u_, nml=bouncon, iostat=ios) ; rewind(u_)
+  read(u_, nml=gravcon, iostat=ios) ; rewind(u_)
   read(u_, nml=gpucon, iostat=ios)
   close(u_)
  end subroutine read_groups
+
+ ! Initial state from a stream file: d, p, v1, v2, v3, b1, b2, b3 over the
+ ! interior (x fastest), then, for gravswitch 3, the potential over the
+ ! padded range is-2:ie+2 (its outer layer is the boundary value).
+ subroutine read_state(fname)
+  character(len=*),intent(in):: fname
+  integer:: u_
+  open(newunit=u_, file=fname, form='unformatted', access='stream', status='old', &
+       action='read')
+  read(u_) d(is:ie,js:je,ks:ke), p(is:ie,js:je,ks:ke), v1(is:ie,js:je,ks:ke), &
+           v2(is:ie,js:je,ks:ke), v3(is:ie,js:je,ks:ke), b1(is:ie,js:je,ks:ke), &
+           b2(is:ie,js:je,ks:ke), b3(is:ie,js:je,ks:ke)
+  if(gravswitch==3) read(u_) grvphi
+  close(u_)
+ end subroutine read_state
 
  ! uniform Cartesian axis with running sums, gridset.f90:50-66
  subroutine make_axis(xs,xe,s,e_,x,xi,dx,dxi,idx,idxi)

@@ -2,9 +2,11 @@
 ! include/hormone_hip.h (libhormone_hip.so).  A Fortran host (the reference's
 ! main program, or hg_driver.f90 here) uses this module to hand its
 ! Cartesian hydro sweep to the GPU: hg_timestep replaces timestep
-! (src/main/timestep.f90:15-105), hg_hydro_step replaces hydro_step
-! (src/main/hydro.f90:15-52) and hg_phidamp replaces phidamp
-! (src/main/source.f90:239-260).
+! (src/main/timestep.f90:15-105), hg_gravity replaces gravity for the
+! hyperbolic solver (src/main/gravity.f90:18-64), hg_hydro_step replaces
+! hydro_step (src/main/hydro.f90:15-52) and hg_phidamp replaces phidamp
+! (src/main/source.f90:239-260); hg_run runs the hydro-only loop on the
+! device.
 module hormone_hip_mod
  use iso_c_binding
  implicit none
@@ -93,6 +95,41 @@ module hormone_hip_mod
    real(c_double),value:: factor
    integer(c_int):: st
   end function hg_phidamp
+
+  ! replaces gravity (gravity.f90:18-64 -> gravity_hyperbolic.f90:19-50) for
+  ! gravswitch 3 at tn > 0; cgrav and dtgrav come from hg_timestep
+  function hg_gravity(ctx, time, dt, grvtime, cgrav, cgrav_old, dtgrav, nsub) &
+                      bind(C, name='hg_gravity') result(st)
+   import:: c_int, c_int32_t, c_ptr, c_double
+   type(c_ptr),value:: ctx
+   real(c_double),value:: time, dt, cgrav
+   real(c_double),intent(inout):: grvtime, cgrav_old, dtgrav
+   integer(c_int32_t),intent(out):: nsub
+   integer(c_int):: st
+  end function hg_gravity
+
+  ! hydro-only main loop on the device (hormone.f90:124-179 without gravity)
+  function hg_run(ctx, time, tn, t_out, dt_out, t_end, outstyle, endstyle, tnlim, &
+                  tn_out, max_steps, damp_factor, stop_reason) &
+                  bind(C, name='hg_run') result(st)
+   import:: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+   type(c_ptr),value:: ctx
+   real(c_double),intent(inout):: time, t_out
+   integer(c_int64_t),intent(inout):: tn
+   real(c_double),value:: dt_out, t_end, damp_factor
+   integer(c_int32_t),value:: outstyle, endstyle
+   integer(c_int64_t),value:: tnlim, tn_out, max_steps
+   integer(c_int32_t),intent(out):: stop_reason
+   integer(c_int):: st
+  end function hg_run
+
+  ! sweep launch shape (wavefronts per block: 1, 2, 4, 8 or 16)
+  function hg_set_launch(ctx, sweep_block_waves) bind(C, name='hg_set_launch') result(st)
+   import:: c_int, c_int32_t, c_ptr
+   type(c_ptr),value:: ctx
+   integer(c_int32_t),value:: sweep_block_waves
+   integer(c_int):: st
+  end function hg_set_launch
 
   function hg_last_error(ctx, i, j, k, dir) bind(C, name='hg_last_error') result(st)
    import:: c_int, c_int32_t, c_ptr

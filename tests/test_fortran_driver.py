@@ -64,3 +64,49 @@ def test_fortran_driver_matches_golden(simtype, tmp_path, hiplib):
         assert snap["tn"] == int(z["tn"]) and snap["time"] == float(z["time"])
         for nm in names:
             assert np.array_equal(snap[nm].reshape(-1).view(np.uint64), z[nm].view(np.uint64)), nm
+
+
+POLY_PARAMS = """&gridcon is=1 ie={n} js=1 je={n} ks=1 ke={n}
+         xi1s=-1d0 xi1e=1d0 xi2s=-1d0 xi2e=1d0 xi3s=-1d0 xi3e=1d0 /
+&out_con outstyle=2 endstyle=2 tnlim={steps} tn_out={steps} t_end=1d30 dt_out=1d0 dt_unit='s' /
+&eos_con eostype=0 gamma=2d0 /
+&simucon crdnt=0 courant=0.9d0 rktype=3 mag_on=.false. flux_limiter='modified_mc' /
+&bouncon bc1is=3 bc1os=3 bc2is=3 bc2os=3 bc3is=3 bc3os=3
+         bc1iv=3 bc1ov=3 bc2iv=3 bc2ov=3 bc3iv=3 bc3ov=3 /
+&gravcon gravswitch=3 HGfac=5d0 hgcfl=0.9d0 grktype=2 alphagrv=0.1d0 maxtngrv=-1 /
+"""
+
+
+def test_fortran_driver_gravity_matches_python_host(tmp_path, hiplib):
+    """The Fortran host drives the hyperbolic self-gravity path (hg_timestep ->
+    hg_gravity -> hg_hydro_step) through iso_c_binding and ends bit for bit
+    where the Python host does on the same initial state (an n=1 polytrope,
+    24^3, 6 steps)."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    from hormone_amd.native import Hydro
+    assert os.path.exists(DRIVER), "hg_driver not built (needs amdflang at build time)"
+    n, steps = 24, 6
+    prm, g, st = P.make_problem("polytrope3d", n=n, steps=steps)
+    init = tmp_path / "init.bin"
+    with open(init, "wb") as f:
+        for a in (st.d, st.p, st.v[0], st.v[1], st.v[2], st.b[0], st.b[1], st.b[2]):
+            f.write(np.ascontiguousarray(a, "<f8").tobytes())
+        f.write(np.ascontiguousarray(st.extra["grvphi_padded"], "<f8").tobytes())
+    par = tmp_path / "parameters"
+    out = tmp_path / "final.bin"
+    par.write_text(POLY_PARAMS.format(n=n, steps=steps)
+                   + f"&gpucon simtype='polytrope' initfile='{init}' outfile='{out}' "
+                     f"is_test=.false. gconst=1d0 /\n")
+    print("case fortran driver polytrope gravity", flush=True)
+    r = subprocess.run([DRIVER, str(par)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    snap = read_bin(str(out), (n, n, n), False)
+
+    eng = Hydro(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g)
+    mine = D.fetch(eng, g, D.SNAP_FIELDS)
+    assert snap["tn"] == res.tn == steps and snap["time"] == res.time
+    for f, nm in zip(D.SNAP_FIELDS, ("d", "v1", "v2", "v3", "e")):
+        assert np.array_equal(snap[nm].view(np.uint64), mine[f].view(np.uint64)), nm
