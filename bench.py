@@ -128,7 +128,7 @@ def cpu_baseline(a, seconds: float):
     t0 = time.perf_counter()
     D.run(eng, prm, g, max_steps=2)
     per = (time.perf_counter() - t0) / 2
-    steps = max(2, min(200, int(seconds / max(per, 1e-6))))
+    steps = max(2, min(3000, int(seconds / max(per, 1e-6))))
     eng2 = pyoracle.Oracle(**D.desc_fields(prm, g))
     pyoracle.set_threads(0)
     D.load_initial_state(eng2, prm, g, st)

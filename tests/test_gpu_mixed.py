@@ -1,15 +1,23 @@
-"""FP32-mixed precision path: reconstruction, EOS and HLLD in FP32; state,
-face internal energy, e_int guard and flux-divergence accumulation in FP64.
+"""FP32-mixed precision path: the Riemann solver (fast speeds, star states,
+state selection) in FP32, in perturbation form about the left face state;
+state, division-free reconstruction, limiter and e_int-guard decisions,
+face internal energy, base flux and flux-divergence accumulation in FP64.
 
-Not bit-exact by design.  The tolerances below are the measured relative L1
-(sum|q - q64| / sum|q64|) against the FP64 path with margin; strongly
-magnetised flows (Orszag-Tang) lose more than weakly magnetised ones because
-the HLLD star-state denominators cancel in FP32.  Mass is conserved to FP64
-rounding because the flux differences are accumulated in FP64.
+Not bit-exact by design.  Two kinds of checks:
+
+* The mixed result is compared with the FP64 result at the same physical
+  time, and with the *rounding floor* of the problem: the FP64 run started
+  from a density perturbed by 1e-14.  The scheme's limiters and e_int guard
+  are switches, so on noisy or turbulent data any rounding-level difference
+  grows to a problem-dependent floor; a mixed path whose error sits at that
+  floor is as close to FP64 as FP64 is to itself under a change of rounding.
+* Tolerances measured on smooth early-time flows, and mass conservation to
+  FP64 rounding (the flux differences are accumulated in FP64).
 """
 import numpy as np
 import pytest
 
+import bench as B
 from hormone_amd import driver as D
 from hormone_amd import problems as P
 from hormone_amd.native import Hydro
@@ -17,36 +25,63 @@ from hormone_amd.native import Hydro
 pytestmark = pytest.mark.gpu
 
 
-def _run(name, prec, max_steps=None, **kw):
+def _start(name, prec, kw, perturb=0.0):
     prm, g, st = P.make_problem(name, precision=prec, **kw)
+    if perturb:
+        st.d = st.d * (1.0 + perturb)
     eng = Hydro(**D.desc_fields(prm, g))
     D.load_initial_state(eng, prm, g, st)
-    r = D.run(eng, prm, g, max_steps=max_steps, device_loop=True)
-    f = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
-    return g, r, D.fetch(eng, g, f)
+    return prm, g, eng, B.Runner(eng, prm, g)
 
 
-def _l1(a, b):
-    return float(np.abs(a - b).sum() / np.abs(b).sum())
+def _fields(prm):
+    return D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+
+
+def _mixed_vs_fp64(name, steps, kw, floor=False):
+    prm, g, eng, r = _start(name, "fp32mix", kw)
+    r.advance(steps)
+    mine = D.fetch(eng, g, _fields(prm))
+    prm6, g6, e6, r6 = _start(name, "fp64", kw)
+    B.advance_to(r6, r.tim)
+    ref = D.fetch(e6, g6, _fields(prm))
+    out = {"mixed": D.rel_l1_errors(mine, ref)}
+    if floor:
+        prm7, g7, e7, r7 = _start(name, "fp64", kw, perturb=1e-14)
+        B.advance_to(r7, r.tim)
+        out["floor"] = D.rel_l1_errors(D.fetch(e7, g7, _fields(prm)), ref)
+    return out
+
+
+@pytest.mark.parametrize("name,steps,kw", [
+    ("kh3d", 60, dict(n=48, steps=60)),
+    ("polytrope3d", 40, dict(n=48, steps=40)),
+])
+def test_mixed_error_at_rounding_floor(hiplib, name, steps, kw):
+    print(f"case mixed floor {name}", flush=True)
+    res = _mixed_vs_fp64(name, steps, kw, floor=True)
+    print(res, flush=True)
+    for q in ("d", "e"):
+        assert res["mixed"][q] <= 3.0 * res["floor"][q] + 1e-9, (q, res)
 
 
 @pytest.mark.parametrize("name,steps,kw,tol", [
-    ("kh3d", None, dict(n=32, steps=30), 1e-4),
-    ("orszagtang_xy", 20, dict(n=128), 2e-3),
-    ("sodshock_x", 300, dict(n=1024, courant=0.4), 2e-3),
+    ("orszagtang_xy", 20, dict(n=128), 2e-4),
+    ("sodshock_x", 300, dict(n=1024, courant=0.4), 1e-3),
 ])
 def test_mixed_close_to_fp64(hiplib, name, steps, kw, tol):
     print(f"case mixed {name}", flush=True)
-    g, r64, s64 = _run(name, "fp64", steps, **kw)
-    _, r32, s32 = _run(name, "fp32mix", steps, **kw)
-    assert r64.tn == r32.tn
-    for f in (D.F_D, D.F_E):
-        assert _l1(s32[f], s64[f]) < tol, (f, _l1(s32[f], s64[f]))
+    res = _mixed_vs_fp64(name, steps, kw)
+    print(res, flush=True)
+    for q in ("d", "e"):
+        assert res["mixed"][q] < tol, (q, res)
 
 
 def test_mixed_conserves_mass_to_fp64_rounding(hiplib):
-    g, r, s = _run("orszagtang_xy", "fp32mix", 40, n=128)
-    prm, g0, st = P.make_problem("orszagtang_xy", n=128)
+    prm, g, eng, r = _start("orszagtang_xy", "fp32mix", dict(n=128))
+    r.advance(40)
+    s = D.fetch(eng, g, [D.F_D])
+    prm0, g0, st = P.make_problem("orszagtang_xy", n=128)
     m0 = float(st.d.sum())
     m1 = float(s[D.F_D].sum())
     assert abs(m1 - m0) / m0 < 1e-12
