@@ -21,6 +21,11 @@ Other workloads (--workload):
                 polytrope (M=1, R=0.3, G=1, gamma=2) in [-1,1]^3, 200 steps;
                 reports mass and energy drift
   kh384         3D 384^3 MHD Kelvin-Helmholtz, periodic, 100 steps
+  serving       continuous-batching ensemble: seeded Poisson arrivals (50/s)
+                of independent 2D/3D MHD Riemann or blast problems, grids
+                log-uniform from 64^2 to 192^3 cells, each to t = 0.1, admitted
+                against an HBM budget into one batch of persistent kernels;
+                value = aggregate zone-updates/s, with p50/p99 time-to-solution
 With --precision fp32mix and --vs-fp64, the same workload is also run in FP64
 and the per-variable relative L1 sum|q-q64|dV / sum|q64|dV is reported.
 
@@ -43,6 +48,11 @@ import time
 
 import numpy as np
 
+# CPU baseline threads: one per physical core, packed (read by the OpenMP
+# runtime when it starts, so set before anything loads it)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -55,7 +65,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="ot512", choices=["ot512", "polytrope256", "kh384"])
+    ap.add_argument("--workload", default="ot512",
+                    choices=["ot512", "polytrope256", "kh384", "serving"])
+    ap.add_argument("--requests", type=int, default=64, help="serving: number of requests")
+    ap.add_argument("--hbm-budget-gib", type=float, default=32.0,
+                    help="serving: HBM admission budget")
     ap.add_argument("--n", type=int, default=None, help="cells per side (default per workload)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32mix"])
     ap.add_argument("--mag", action="store_true", help="polytrope: MHD variant with B=0")
@@ -134,9 +148,19 @@ def cpu_baseline(a, seconds: float):
     D.load_initial_state(eng2, prm, g, st)
     r = D.run(eng2, prm, g, max_steps=steps)
     zus = r.tn * g.ncells / r.wall_s
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": zus, "unit": "zone-updates/s", "cores": cores, "kind": "port",
             "sample": f"first {r.tn} steps of the same {a.workload} run "
-                      f"({r.wall_s:.1f} s, {cores} OpenMP threads)"}
+                      f"({r.wall_s:.1f} s, {cores} OpenMP threads, OMP_PROC_BIND=close, "
+                      f"OMP_PLACES=cores, {model})"}
 
 
 class Runner:
@@ -228,10 +252,60 @@ def sums(D, eng, g, prm):
     return m, e
 
 
+def serving_bench(a):
+    """The serving ensemble: arrivals honoured in wall-clock time from the
+    start, one request = one independent simulation to t = 0.1."""
+    import torch
+
+    from hormone_amd import serving as S
+    reqs = S.make_requests(a.requests, rate=50.0, seed=7)
+    # spot check after the run: the smallest and the largest request, rerun
+    # alone, must end in the same state bit for bit as in the mix
+    by_size = sorted(reqs, key=lambda r: r.cells)
+    check = {by_size[0].rid, by_size[-1].rid}
+    torch.cuda.synchronize()
+    rep = S.serve(reqs, hbm_budget=a.hbm_budget_gib * 2 ** 30, steps_per_round=4,
+                  digest_ids=check)
+    torch.cuda.synchronize()
+    sm = rep.summary()
+    done = [r for r in reqs if r.finished is not None and r.error is None]
+    same = {}
+    for r in reqs:
+        if r.rid in check and r.digest is not None:
+            tn, tm, dg = S.run_alone(r)
+            same[f"request {r.rid} ({r.dim}D, {r.n}^{r.dim})"] = bool(
+                dg == r.digest and tn == r.steps)
+    line = {
+        "metric": "zone-updates/s (serving ensemble, aggregate)", "value": sm["aggregate_zone_updates_per_s"],
+        "unit": "zone-updates/s", "n_gpus": 1, "steps": int(sum(r.steps for r in done)),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded request generator)",
+        "config": {"workload": f"serving: {a.requests} Poisson(50/s) MHD Riemann/blast requests, "
+                               f"64^2..192^3 cells, t_end 0.1, HBM budget {a.hbm_budget_gib:g} GiB",
+                   "requests": a.requests},
+        "roofline": None, "cpu_baseline": None,
+        "p50_time_to_solution_s": sm["p50_s"], "p99_time_to_solution_s": sm["p99_s"],
+        "completed": sm["completed"], "failed": sm["failed"], "makespan_s": sm["makespan_s"],
+        "zone_updates": rep.zone_updates, "tuned_shape": sm["tuned_shape"],
+        "max_in_flight": max((e["active"] for e in rep.events), default=None),
+        "alone_equals_in_mix": same,
+        "failures": [{"request": r.rid, "cells": r.cells, "kind": r.kind, "error": r.error}
+                     for r in reqs if r.error is not None],
+    }
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.workload == "serving":
+        if a.gpus != 1:
+            print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+                  f"(the sharded path is a design only).", file=sys.stderr)
+            sys.exit(2)
+        serving_bench(a)
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "

@@ -112,36 +112,57 @@ _CACHE: dict = {}
 
 
 def shape_class(batch: Batch):
+    """Coarse class of a batch for the tuning cache: member count and total
+    cells, each rounded to a power of two."""
+    import math
     cells = sum(m.grid.ncells for m in batch.members)
-    return (len(batch.members), int(round(cells, -4)))
+    n = len(batch.members)
+    return (round(math.log2(max(n, 1))), round(math.log2(max(cells, 1))))
 
 
-def candidates():
-    out = [(4, 0, 0)]
-    for bw in (1, 2, 4):
-        for sg in (0, CUS * 2, CUS * 4, CUS * 8):
-            for rg in (0, CUS * 4):
-                if (bw, sg, rg) not in out:
-                    out.append((bw, sg, rg))
-    return out
+def _time_steps(batch: Batch, cand, steps: int):
+    batch.configure(*cand, use_graph=True)
+    batch.step(1)  # capture the graph outside the timing
+    t0 = time.perf_counter()
+    if batch.step(steps) == 0:
+        return None
+    return (time.perf_counter() - t0) / steps
 
 
 def autotune(batch: Batch, steps_per_trial: int = 2, use_cache: bool = True):
-    """Time each candidate launch shape on real steps of the batch and keep
-    the fastest.  Returns (best shape, {shape: seconds per step})."""
+    """Pick the launch shape of a batch by timing real steps of it, in three
+    stages: sweep wavefronts per block (4, 8, 16) with one work item per
+    wavefront; then the persistent sweep grid; then the RK grid.  Every
+    trial step is an ordinary step of the run (the shape never changes the
+    arithmetic).  Returns (best (block waves, sweep grid, rk grid),
+    {shape: seconds per step})."""
     key = shape_class(batch)
     if use_cache and key in _CACHE:
         best = _CACHE[key]
         batch.configure(*best, use_graph=True)
         return best, {}
     timings = {}
-    for cand in candidates():
-        batch.configure(*cand, use_graph=True)
-        batch.step(1)  # capture the graph outside the timing
-        t0 = time.perf_counter()
-        if batch.step(steps_per_trial) == 0:
+
+    def trial(c):
+        if c not in timings:
+            t = _time_steps(batch, c, steps_per_trial)
+            if t is None:
+                return False
+            timings[c] = t
+        return True
+
+    for bw in (4, 8, 16):
+        if not trial((bw, 0, 0)):
             break
-        timings[cand] = (time.perf_counter() - t0) / steps_per_trial
+    if timings:
+        bw = min(timings, key=timings.get)[0]
+        for sg in (CUS * 4, CUS * 8, CUS * 16):
+            if not trial((bw, sg, 0)):
+                break
+        best = min(timings, key=timings.get)
+        for rg in (CUS * 4, CUS * 8):
+            if not trial((best[0], best[1], rg)):
+                break
     best = min(timings, key=timings.get) if timings else (4, 0, 0)
     batch.configure(*best, use_graph=True)
     _CACHE[key] = best

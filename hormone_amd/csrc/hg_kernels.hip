@@ -652,7 +652,10 @@ __device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double 
   if (CFL) cfl_cell<MAG>(D, m, d, eint, v1, v2, v3, b1, b2, b3, dti, cfm);
 }
 
-template <bool MAG, bool CFL, bool DAMP, bool GRAV>
+__device__ __forceinline__ void cfl_final(const Dev &D);
+__device__ __forceinline__ void step_end(const Dev &D);
+
+template <bool MAG, bool CFL, bool DAMP, bool GRAV, bool END = false>
 __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
                                             double facn, double damp) {
   if (D.sc->stop) return;
@@ -661,6 +664,26 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
   if (m < D.ncell)
     rk_cell<MAG, CFL, DAMP, GRAV>(D, m, stage, faco, fact, facn, damp, dti, cfm);
   if (CFL) cfl_block_reduce(D, dti, cfm, blockIdx.x);
+  if (END) {
+    // Last block done: every partial has been written (each block fences
+    // its partial before counting itself), so it closes the step and
+    // reduces the partials into the next dt and ch.
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+      __threadfence();
+      last = atomicAdd(&D.sc->done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+      __threadfence();
+      if (threadIdx.x == 0) {
+        D.sc->done = 0;
+        step_end(D);
+      }
+      __syncthreads();
+      cfl_final(D);
+    }
+  }
 }
 
 // ---- pressure(): eint from e, d, v, B after an upload ---------------------
@@ -839,25 +862,20 @@ __device__ __forceinline__ void step_end(const Dev &D) {
   if (sc->steps_left <= 0) sc->stop = STOP_MAXSTEPS;
 }
 
-// The end of one step and the dt of the next in one launch: step_end on one
-// thread, then (if the run goes on) the CFL reduction of cfl_final.
-__global__ __launch_bounds__(256) void k_step_end_cfl(Dev D) {
-  if (threadIdx.x == 0) step_end(D);
-  __syncthreads();
-  cfl_final(D);
-}
 
 // ---- batched boxes: persistent kernels over flattened work-lists ---------
 template <bool MAG, int LIM, bool MIX>
-__global__ __launch_bounds__(256) void k_bsweep(Batch B, int stage) {
+__global__ __launch_bounds__(1024) void k_bsweep(Batch B, int stage) {
   const int lane = threadIdx.x & 63;
   const long nw = (long)gridDim.x * (blockDim.x >> 6);
   for (long it = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < B.nsitems;
        it += nw) {
+    // the work item is the same for the whole wavefront: make that visible
+    // so the box record is read with scalar loads
     const int2 x = B.sitems[it];
-    const Dev &D = B.boxes[x.x];
+    const Dev &D = B.boxes[__builtin_amdgcn_readfirstlane(x.x)];
     if (D.sc->stop) continue;
-    long w = x.y;
+    long w = __builtin_amdgcn_readfirstlane(x.y);
     if (w < D.waves[0]) {
       if (MIX) sweep_line_mix<0, MAG, LIM>(D, w, lane, stage);
       else sweep_line<0, MAG, LIM>(D, w, lane, stage);
@@ -880,7 +898,7 @@ __global__ __launch_bounds__(256) void k_brk(Batch B, int stage, double faco, do
                                              double facn) {
   for (long it = blockIdx.x; it < B.ncitems; it += gridDim.x) {
     const int2 x = B.citems[it];
-    const Dev &D = B.boxes[x.x];
+    const Dev &D = B.boxes[__builtin_amdgcn_readfirstlane(x.x)];
     if (D.sc->stop) continue;  // uniform over the block
     const long m = (long)x.y * 256 + threadIdx.x;
     double dti = __builtin_inf(), cfm = 0.0;
@@ -1039,8 +1057,13 @@ void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev) {
 
 template <bool MAG, bool GRAV>
 static void rk_mag(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
-                   bool damp, double df, hipStream_t s, const KEv *ev) {
+                   bool damp, double df, hipStream_t s, const KEv *ev, bool end) {
   const unsigned g = nblk(D.ncell, 256);
+  if (!GRAV && cfl && end) {
+    if (damp) klaunch(k_rk<MAG, true, true, false, true>, g, 256, s, ev, D, stage, faco, fact, facn, df);
+    else klaunch(k_rk<MAG, true, false, false, true>, g, 256, s, ev, D, stage, faco, fact, facn, df);
+    return;
+  }
   if (cfl && damp) klaunch(k_rk<MAG, true, true, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
   else if (cfl) klaunch(k_rk<MAG, true, false, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
   else if (damp) klaunch(k_rk<MAG, false, true, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
@@ -1048,13 +1071,13 @@ static void rk_mag(const Dev &D, int stage, double faco, double fact, double fac
 }
 
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
-               bool damp, double damp_factor, hipStream_t s, const KEv *ev) {
+               bool damp, double damp_factor, hipStream_t s, const KEv *ev, bool end_step) {
   if (D.grav) {
-    if (D.mag) rk_mag<true, true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev);
-    else rk_mag<false, true>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev);
+    if (D.mag) rk_mag<true, true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev, false);
+    else rk_mag<false, true>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev, false);
   } else {
-    if (D.mag) rk_mag<true, false>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev);
-    else rk_mag<false, false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev);
+    if (D.mag) rk_mag<true, false>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev, end_step);
+    else rk_mag<false, false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev, end_step);
   }
 }
 
@@ -1077,7 +1100,6 @@ void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev) {
 
 void launch_cfl_final(const Dev &D, hipStream_t s) { k_cfl_final<<<1, 256, 0, s>>>(D); }
 
-void launch_step_end_cfl(const Dev &D, hipStream_t s) { k_step_end_cfl<<<1, 256, 0, s>>>(D); }
 
 void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
   k_phidamp<<<nblk(D.ncell, 256), 256, 0, s>>>(D, factor);

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import driver as D
 from . import problems as P
-from .batch import Batch, autotune
+from .batch import Batch, autotune, shape_class
 from .grid import cartesian_grid
 from .native import Hydro, HgError
 
@@ -165,9 +165,10 @@ class ServeReport:
 
 
 def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int = 4,
-          tune: bool = True, keep_digest: bool = False) -> ServeReport:
+          tune: bool = True, keep_digest: bool = False, digest_ids=None) -> ServeReport:
     """Continuous batching over arriving requests (arrival times are honoured
-    in wall-clock time from the start of the call)."""
+    in wall-clock time from the start of the call).  A final-state digest is
+    kept for every request (keep_digest) or for the ids in digest_ids."""
     queue = sorted(requests, key=lambda r: r.arrival)
     qi = 0
     batch = Batch()
@@ -175,6 +176,9 @@ def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int
     used = 0
     zu = 0
     tuned = None
+    tuned_class = None
+    tunings = 0
+    events = []
     t0 = time.perf_counter()
     while qi < len(queue) or active:
         now = time.perf_counter() - t0
@@ -199,14 +203,21 @@ def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int
             if qi < len(queue):
                 time.sleep(max(0.0, min(0.01, queue[qi].arrival - now)))
             continue
-        if tune and tuned is None and len(active) >= 2:
-            tuned, _ = autotune(batch)
-        elif admitted and tuned is not None:
-            pass
+        # (re)tune the launch shapes whenever the mix has moved to another
+        # size class (tuned shapes are cached per class)
+        if tune and len(active) >= 2:
+            cls = shape_class(batch)
+            if cls != tuned_class:
+                tuned, _ = autotune(batch)
+                tuned_class = cls
+                tunings += 1
         try:
             batch.step(steps_per_round)
         except HgError:
             pass
+        events.append({"t": round(time.perf_counter() - t0, 4), "active": len(active),
+                       "queued": sum(1 for r in queue[qi:] if r.arrival <= now),
+                       "hbm_bytes": used})
         for key, (req, eng, g) in list(active.items()):
             try:
                 tm, tn, stop = batch.state(eng)
@@ -218,7 +229,7 @@ def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int
                 req.steps = tn
                 if stop == 1 and req.error is None:
                     zu += req.cells * tn
-                    if keep_digest:
+                    if keep_digest or (digest_ids is not None and req.rid in digest_ids):
                         batch.remove(eng)
                         req.digest = state_digest(eng, g)
                     else:
@@ -230,4 +241,4 @@ def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int
                 del active[key]
     makespan = time.perf_counter() - t0
     batch.close()
-    return ServeReport(requests, makespan, zu, tuned)
+    return ServeReport(requests, makespan, zu, tuned, events)

@@ -217,7 +217,7 @@ int hg_batch_step(hg_batch *batch, int32_t nsteps, int32_t *nrunning);
    2 output due, 4 error (then the member's error status is returned). */
 int hg_batch_member_state(hg_batch *batch, hg_ctx *ctx, double *time, int64_t *tn,
                           int32_t *stop);
-/* Launch shapes: sweep wavefronts per block (1, 2, 4), persistent grid
+/* Launch shapes: sweep wavefronts per block (1, 2, 4, 8, 16), persistent grid
    sizes (0 = one work item per wavefront / block) and hipGraph replay. */
 int hg_batch_config(hg_batch *batch, int32_t sweep_block_waves, int32_t sweep_grid,
                     int32_t rk_grid, int32_t use_graph);
