@@ -560,7 +560,7 @@ static void enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
     launch_sweep(D, stage, c->stream, prof_ev(c, HG_PROF_SWEEP));
     const bool last = stage == rk;
     launch_rk(D, stage, faco, fact, facn, fuse_tail && last, fuse_tail && last && damp > 0.0,
-              damp, c->stream, prof_ev(c, HG_PROF_RK), fuse_tail && last);
+              damp, c->stream, prof_ev(c, HG_PROF_RK));
   }
 }
 
@@ -604,9 +604,8 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
   launch_cfl(D, c->stream, prof_ev(c, HG_PROF_CFL));
   // Steps are enqueued 16 at a time (never more than max_steps in all) and
   // the host reads the stop flag between groups; steps enqueued after a
-  // stop return at once.  The last RK launch of each step also closes the
-  // step (step_end) and computes the next step's dt and ch (cfl_final), in
-  // the block that finishes last.
+  // stop return at once.  Each step ends with one launch that closes it
+  // (step_end) and computes the next step's dt and ch (cfl_final).
   const int batch = 16;
   int reason = STOP_NONE;
   int64_t left = max_steps, tn_before = *tn;
@@ -618,7 +617,10 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
       break;
     }
     const int ev_base = c->ev_used;
-    for (int s = 0; s < nb; ++s) enqueue_stages(c, true, damp ? damp_factor : 0.0);
+    for (int s = 0; s < nb; ++s) {
+      enqueue_stages(c, true, damp ? damp_factor : 0.0);
+      launch_step_end_cfl(D, c->stream);
+    }
     left -= nb;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_sc, D.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));

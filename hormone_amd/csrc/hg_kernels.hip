@@ -652,10 +652,7 @@ __device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double 
   if (CFL) cfl_cell<MAG>(D, m, d, eint, v1, v2, v3, b1, b2, b3, dti, cfm);
 }
 
-__device__ __forceinline__ void cfl_final(const Dev &D);
-__device__ __forceinline__ void step_end(const Dev &D);
-
-template <bool MAG, bool CFL, bool DAMP, bool GRAV, bool END = false>
+template <bool MAG, bool CFL, bool DAMP, bool GRAV>
 __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
                                             double facn, double damp) {
   if (D.sc->stop) return;
@@ -664,26 +661,6 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
   if (m < D.ncell)
     rk_cell<MAG, CFL, DAMP, GRAV>(D, m, stage, faco, fact, facn, damp, dti, cfm);
   if (CFL) cfl_block_reduce(D, dti, cfm, blockIdx.x);
-  if (END) {
-    // Last block done: every partial has been written (each block fences
-    // its partial before counting itself), so it closes the step and
-    // reduces the partials into the next dt and ch.
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-      __threadfence();
-      last = atomicAdd(&D.sc->done, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last) {
-      __threadfence();
-      if (threadIdx.x == 0) {
-        D.sc->done = 0;
-        step_end(D);
-      }
-      __syncthreads();
-      cfl_final(D);
-    }
-  }
 }
 
 // ---- pressure(): eint from e, d, v, B after an upload ---------------------
@@ -915,6 +892,14 @@ __global__ __launch_bounds__(256) void k_brk(Batch B, int stage, double faco, do
   }
 }
 
+// The end of one step and the dt of the next in one launch: step_end on one
+// thread, then (if the run goes on) the CFL reduction of cfl_final.
+__global__ __launch_bounds__(256) void k_step_end_cfl(Dev D) {
+  if (threadIdx.x == 0) step_end(D);
+  __syncthreads();
+  cfl_final(D);
+}
+
 // One block per box: close its step, then compute its next dt and ch.
 __global__ __launch_bounds__(256) void k_bstep_end_cfl(Batch B) {
   const Dev &D = B.boxes[blockIdx.x];
@@ -1057,13 +1042,8 @@ void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev) {
 
 template <bool MAG, bool GRAV>
 static void rk_mag(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
-                   bool damp, double df, hipStream_t s, const KEv *ev, bool end) {
+                   bool damp, double df, hipStream_t s, const KEv *ev) {
   const unsigned g = nblk(D.ncell, 256);
-  if (!GRAV && cfl && end) {
-    if (damp) klaunch(k_rk<MAG, true, true, false, true>, g, 256, s, ev, D, stage, faco, fact, facn, df);
-    else klaunch(k_rk<MAG, true, false, false, true>, g, 256, s, ev, D, stage, faco, fact, facn, df);
-    return;
-  }
   if (cfl && damp) klaunch(k_rk<MAG, true, true, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
   else if (cfl) klaunch(k_rk<MAG, true, false, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
   else if (damp) klaunch(k_rk<MAG, false, true, GRAV>, g, 256, s, ev, D, stage, faco, fact, facn, df);
@@ -1071,13 +1051,13 @@ static void rk_mag(const Dev &D, int stage, double faco, double fact, double fac
 }
 
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
-               bool damp, double damp_factor, hipStream_t s, const KEv *ev, bool end_step) {
+               bool damp, double damp_factor, hipStream_t s, const KEv *ev) {
   if (D.grav) {
-    if (D.mag) rk_mag<true, true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev, false);
-    else rk_mag<false, true>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev, false);
+    if (D.mag) rk_mag<true, true>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev);
+    else rk_mag<false, true>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev);
   } else {
-    if (D.mag) rk_mag<true, false>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev, end_step);
-    else rk_mag<false, false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev, end_step);
+    if (D.mag) rk_mag<true, false>(D, stage, faco, fact, facn, cfl, damp, damp_factor, s, ev);
+    else rk_mag<false, false>(D, stage, faco, fact, facn, cfl, false, damp_factor, s, ev);
   }
 }
 
@@ -1136,6 +1116,8 @@ void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, do
     else k_brk<false, false><<<g, 256, 0, s>>>(B, stage, faco, fact, facn);
   }
 }
+
+void launch_step_end_cfl(const Dev &D, hipStream_t s) { k_step_end_cfl<<<1, 256, 0, s>>>(D); }
 
 void launch_bstep_end_cfl(const Batch &B, hipStream_t s) {
   k_bstep_end_cfl<<<B.nbox, 256, 0, s>>>(B);

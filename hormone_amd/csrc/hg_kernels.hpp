@@ -30,8 +30,7 @@ struct Scalars {
   double dtimin, cfmax;
   double time, t_out, t_end, dt_out;
   long long tn, tnlim, tn_out, steps_left;
-  int outstyle, endstyle, stop;
-  unsigned done;  // blocks of the current fused last-stage launch that have finished
+  int outstyle, endstyle, stop, pad0;
   unsigned long long err;  // first error key (NOERR = none), see errkey()
 };
 
@@ -133,17 +132,15 @@ struct KEv {
 };
 
 void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev = nullptr);
-// cfl: also the next step's CFL partials; damp: phidamp; end_step (with cfl,
-// no gravity): the block that finishes last also closes the step and
-// computes the next dt and ch (step_end, then cfl_final).
+// cfl: also the next step's CFL partials; damp: phidamp
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
-               bool damp, double damp_factor, hipStream_t s, const KEv *ev = nullptr,
-               bool end_step = false);
+               bool damp, double damp_factor, hipStream_t s, const KEv *ev = nullptr);
 void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, double facn,
                        double dtg, double cg, bool damp, double damp_factor, hipStream_t s,
                        const KEv *ev = nullptr);
 void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev = nullptr);
 void launch_cfl_final(const Dev &D, hipStream_t s);
+void launch_step_end_cfl(const Dev &D, hipStream_t s);  // step_end, then cfl_final
 void launch_phidamp(const Dev &D, double factor, hipStream_t s);
 void launch_kat(int which, int n, const double *in, double *out, hipStream_t s);
 
