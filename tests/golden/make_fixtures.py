@@ -14,7 +14,7 @@ Two sources, both data produced by the reference itself:
    full run to t=0.5).  The binary runs in a scratch directory under /tmp
    whose ../para points at the reference's parameter files.
 
-Usage: python tests/golden/make_fixtures.py [--skip-long]
+Usage: python tests/golden/make_fixtures.py [--skip-long | --extra-only]
 """
 from __future__ import annotations
 
@@ -152,12 +152,88 @@ def reference_runs(meta: dict, long: bool):
         meta["ot512_2200steps"]["source"] = "reference run: orszagtang_xy ie=je=512, tnlim=2200"
 
 
+# Reference runs that exercise what the committed goldens do not: boundary
+# codes 1 (reflective) and 2 (outgoing) on each axis, the minmod limiter,
+# and RK orders 2 and 1.  Each entry: simtype, the Python problem keywords
+# (tests/helpers.EXTRA_CASES restates them), and the namelist overrides the
+# reference reads on top of para/parameters_base and para/parameters_SIMTYPE.
+def _bc_text(axis, code):
+    return (f"&bouncon bc{axis}is={code} bc{axis}os={code} bc{axis}iv={code} "
+            f"bc{axis}ov={code} /\n")
+
+
+EXTRA = {
+    "sod_x_reflective": ("sodshock_x", "&gridcon ie=256 /\n" + _bc_text(1, 1)
+                         + "&out_con t_end=400d0 dt_out=400d0 output_ascii=.false. /\n"),
+    "sod_x_outgoing": ("sodshock_x", "&gridcon ie=256 /\n" + _bc_text(1, 2)
+                       + "&out_con t_end=300d0 dt_out=300d0 output_ascii=.false. /\n"),
+    "sod_y_outgoing": ("sodshock_y", "&gridcon je=256 /\n" + _bc_text(2, 2)
+                       + "&out_con t_end=300d0 dt_out=300d0 output_ascii=.false. /\n"),
+    # reflective walls reverse the normal B of the uniform B_x = 0.75: the
+    # reference stops during step 6 (negative e_int at the wall); the
+    # 5-step state and the stop are both pinned
+    "briowu_x_reflective_5steps": ("briowushock_x", "&gridcon ie=256 /\n" + _bc_text(1, 1)
+                                   + "&out_con outstyle=2 endstyle=2 tnlim=5 tn_out=5 "
+                                     "output_ascii=.false. /\n"),
+    "briowu_x_reflective_failure": ("briowushock_x", "&gridcon ie=256 /\n" + _bc_text(1, 1)
+                                    + "&out_con t_end=300d0 dt_out=300d0 "
+                                      "output_ascii=.false. /\n"),
+    "briowu_z_outgoing": ("briowushock_z", "&gridcon ke=256 /\n" + _bc_text(3, 2)
+                          + "&out_con t_end=300d0 dt_out=300d0 output_ascii=.false. /\n"),
+    "sod_x_minmod_rk2": ("sodshock_x", "&gridcon ie=256 /\n&simucon flux_limiter='minmod' "
+                         "rktype=2 /\n&out_con output_ascii=.false. /\n"),
+    "briowu_y_minmod": ("briowushock_y", "&gridcon je=256 /\n&simucon flux_limiter='minmod' /\n"
+                        "&out_con output_ascii=.false. /\n"),
+    "sod_x_flat_rk1": ("sodshock_x", "&gridcon ie=256 /\n&simucon flux_limiter='flat' "
+                       "rktype=1 /\n&out_con output_ascii=.false. /\n"),
+    "ot64_minmod_rk2": ("orszagtang_xy", "&gridcon ie=64 je=64 /\n&simucon flux_limiter='minmod' "
+                        "rktype=2 /\n&out_con outstyle=2 endstyle=2 tnlim=100 tn_out=100 "
+                        "output_ascii=.false. /\n"),
+}
+
+
+def reference_runs_extra(meta: dict):
+    import re
+    meta.pop("briowu_x_reflective", None)
+    for name, (simtype, params) in EXTRA.items():
+        b = run_reference(simtype, params)
+        if name.endswith("_failure"):
+            log = open("/tmp/hormone_fixture_run/log.txt").read()
+            steps = [int(m) for m in re.findall(r"tn =\s+(\d+)", log)]
+            cell = re.search(r"i=\s*(\d+), j=\s*(\d+), k=\s*(\d+)", log)
+            meta[name] = {
+                "stops": "Negative internal energy" in log,
+                "message": "Error in internalenergy, Negative internal energy",
+                "completed_steps": steps[-1] if steps else None,
+                "cell": [int(cell.group(q)) for q in (1, 2, 3)] if cell else None,
+                "source": f"reference run: {simtype} with " + " ".join(params.split())}
+            continue
+        mag = simtype.startswith(("briowushock", "orszagtang"))
+        if simtype.startswith("orszagtang"):
+            shape = (1, 64, 64)
+        else:
+            ax = simtype[-1]
+            shape = {"x": (1, 1, 256), "y": (1, 256, 1), "z": (256, 1, 1)}[ax]
+        snap = read_bin(b, shape, mag)
+        meta[name] = summary(snap, FIELDS_MAG if mag else FIELDS)
+        meta[name]["source"] = f"reference run: {simtype} with " + " ".join(params.split())
+
+
 def main():
     long = "--skip-long" not in sys.argv
+    if "--extra-only" in sys.argv:
+        path = os.path.join(HERE, "golden_meta.json")
+        meta = json.load(open(path))
+        reference_runs_extra(meta)
+        with open(path, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+        print("wrote", path)
+        return
     path = os.path.join(HERE, "golden_meta.json")
     meta = json.load(open(path)) if os.path.exists(path) else {}
     convert_reference_goldens(meta)
     reference_runs(meta, long)
+    reference_runs_extra(meta)
     with open(path, "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print("wrote", path)

@@ -56,3 +56,39 @@ def compare_to_npz(res, state, name):
 def rel_l1(a, b):
     den = np.abs(b).sum()
     return float(np.abs(a - b).sum() / den) if den > 0 else float(np.abs(a - b).sum())
+
+
+# Reference runs of features the committed goldens do not exercise
+# (tests/golden/make_fixtures.py, EXTRA): name -> (problem, keywords).
+# bc tuples are (bc1is bc1os bc2is bc2os bc3is bc3os bc1iv bc1ov bc2iv bc2ov bc3iv bc3ov).
+def _bc(axis, code, default=3):
+    bc = [default] * 12
+    for q in (2 * (axis - 1), 2 * (axis - 1) + 1, 6 + 2 * (axis - 1), 6 + 2 * (axis - 1) + 1):
+        bc[q] = code
+    return tuple(bc)
+
+
+EXTRA_CASES = {
+    "sod_x_reflective": ("sodshock_x", dict(n=256, bc=_bc(1, 1), t_end=400.0, dt_out=400.0)),
+    "sod_x_outgoing": ("sodshock_x", dict(n=256, bc=_bc(1, 2), t_end=300.0, dt_out=300.0)),
+    "sod_y_outgoing": ("sodshock_y", dict(n=256, bc=_bc(2, 2), t_end=300.0, dt_out=300.0)),
+    "briowu_x_reflective_5steps": ("briowushock_x", dict(n=256, bc=_bc(1, 1), outstyle=2,
+                                                         endstyle=2, tnlim=5, tn_out=5)),
+    "briowu_z_outgoing": ("briowushock_z", dict(n=256, bc=_bc(3, 2), t_end=300.0, dt_out=300.0)),
+    "sod_x_minmod_rk2": ("sodshock_x", dict(n=256, flux_limiter="minmod", rktype=2)),
+    "briowu_y_minmod": ("briowushock_y", dict(n=256, flux_limiter="minmod")),
+    "sod_x_flat_rk1": ("sodshock_x", dict(n=256, flux_limiter="flat", rktype=1)),
+    "ot64_minmod_rk2": ("orszagtang_xy", dict(n=64, flux_limiter="minmod", rktype=2, outstyle=2,
+                                              endstyle=2, tnlim=100, tn_out=100)),
+}
+
+
+def run_extra(engine_cls, case, max_steps=None):
+    name, kw = EXTRA_CASES[case]
+    prm, g, st = P.make_problem(name, **kw)
+    eng = engine_cls(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, max_steps=max_steps)
+    fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+    out = D.fetch(eng, g, fields)
+    return res, {NAMES[i]: out[f] for i, f in enumerate(fields)}, eng

@@ -12,7 +12,8 @@ import pytest
 
 from hormone_amd.native import Hydro
 from oracle.pyoracle import Oracle, lib as oracle_lib
-from tests.helpers import compare_to_meta, compare_to_npz, digest, meta, run_problem
+from tests.helpers import (EXTRA_CASES, compare_to_meta, compare_to_npz, digest, meta,
+                           run_extra, run_problem)
 
 pytestmark = pytest.mark.gpu
 
@@ -237,3 +238,30 @@ def test_large_contexts_created_back_to_back_start_clean(hiplib):
         r = D.run(eng, prm, g, device_loop=True)
         assert r.tn == 2
         engs.append(eng)
+
+
+@pytest.mark.parametrize("case", list(EXTRA_CASES))
+def test_gpu_extra_reference_runs(hiplib, case):
+    """Reflective and outgoing boundaries, minmod, RK2 and RK1: bit-identical
+    to runs of the reference program (tests/golden/make_fixtures.py)."""
+    print(f"case extra {case}", flush=True)
+    res, state, _ = run_extra(Hydro, case)
+    compare_to_meta(res, state, meta()[case])
+
+
+def test_gpu_reflective_briowu_stops_like_the_reference(hiplib):
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    from hormone_amd.native import HG_ERR_NEG_EINT, HgError
+    fail = meta()["briowu_x_reflective_failure"]
+    name, kw = EXTRA_CASES["briowu_x_reflective_5steps"]
+    kw = dict(kw, outstyle=1, endstyle=1, t_end=300.0, dt_out=300.0)
+    prm, g, st = P.make_problem(name, **kw)
+    eng = Hydro(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, max_steps=fail["completed_steps"])
+    assert res.tn == fail["completed_steps"]
+    with pytest.raises(HgError) as ei:
+        D.run(eng, prm, g, max_steps=1)
+    assert ei.value.code == HG_ERR_NEG_EINT
+    assert list(ei.value.cell) == fail["cell"]

@@ -8,7 +8,8 @@ sizes.  Runs on the CPU.
 import pytest
 
 from oracle.pyoracle import Oracle
-from tests.helpers import compare_to_meta, compare_to_npz, meta, run_problem
+from tests.helpers import (EXTRA_CASES, compare_to_meta, compare_to_npz, meta, run_extra,
+                           run_problem)
 
 ONE_D = [f"{k}_{d}" for k in ("sodshock", "briowushock") for d in "xyz"]
 
@@ -39,3 +40,31 @@ def test_oracle_ot512_prefix():
     res, state, _ = run_problem(Oracle, "orszagtang_xy", n=512, outstyle=2, endstyle=2,
                                 tnlim=20)
     compare_to_meta(res, state, meta()["ot512_20steps"])
+
+
+@pytest.mark.parametrize("case", list(EXTRA_CASES))
+def test_oracle_extra_reference_runs(case):
+    """Reflective and outgoing boundaries, minmod, RK2 and RK1 against runs of
+    the reference program built here (tests/golden/make_fixtures.py)."""
+    res, state, _ = run_extra(Oracle, case)
+    compare_to_meta(res, state, meta()[case])
+
+
+def test_oracle_reflective_briowu_stops_like_the_reference():
+    """Reflective walls flip the uniform normal field of Brio-Wu; the
+    reference stops during step 6 with a negative internal energy at the
+    wall cell.  The oracle stops at the same step and cell."""
+    from oracle.pyoracle import HgError
+    fail = meta()["briowu_x_reflective_failure"]
+    name, kw = EXTRA_CASES["briowu_x_reflective_5steps"]
+    kw = dict(kw, outstyle=1, endstyle=1, t_end=300.0, dt_out=300.0)
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    prm, g, st = P.make_problem(name, **kw)
+    eng = Oracle(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, max_steps=fail["completed_steps"])
+    assert res.tn == fail["completed_steps"]
+    with pytest.raises(HgError) as ei:
+        D.run(eng, prm, g, max_steps=1)
+    assert list(ei.value.cell) == fail["cell"]
