@@ -203,6 +203,18 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
   D.sweep_bw = 4;
   for (int a = 0; a < 3; ++a)
     D.waves[a] = D.solve[a] ? (long)((nn[a] + 59) / 60) * (D.ncell / nn[a]) : 0;
+  // y and z wavefront ranges start on a multiple of 16 (the largest block),
+  // so a sweep block never holds two directions
+  D.woff[0] = 0;
+  D.woff[1] = (D.waves[0] + 15) / 16 * 16;
+  D.woff[2] = D.woff[1] + (D.waves[1] + 15) / 16 * 16;
+  {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+        hipSuccess)
+      lds = 0;
+    D.lds_max = lds;
+  }
   if ((st = dev_alloc(c, sizeof(double) * 2 * D.nblk_cfl, (void **)&D.part))) return fail(st);
   if (desc->gravswitch == 3) {
     D.grav = 1;

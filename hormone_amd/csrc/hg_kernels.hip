@@ -34,32 +34,25 @@ __device__ __forceinline__ int ghost_src(int code, bool lo, int c, int n) {
   return s < 0 ? 0 : (s > n - 1 ? n - 1 : s);  // lanes past the ghosts: unused
 }
 
-// One cell of a sweep line, interior or ghost.  Ghost cells
-// (boundarycondition.f90, codes 0..3) take the scalars d, p, phi from the
-// scalar-code source cell and v, B from the vector-code source cell with the
-// normal components transformed, then rebuild eint and e (:221-243).  Both
-// kinds share one set of loads (interior: both sources are the cell
-// itself), so the struct stays in registers.
+// One cell of a sweep line, interior or ghost, from the values stored at its
+// source cells: f[] holds W[d v1 v2 v3 e eint b1 b2 b3 phi], the scalars
+// (d, e, eint, phi) read at the scalar-code source and v, B at the
+// vector-code source.  Ghost cells (boundarycondition.f90, codes 0..3)
+// transform the normal components, then rebuild eint and e (:221-243).
 template <int DIR, bool MAG>
-__device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base, int c) {
+__device__ __forceinline__ Cell finish_cell(const Dev &D, const Axis &A, int c,
+                                            const double f[NWMAX]) {
   Cell q;
   const int n = A.n;
   const bool ghost = c < 0 || c >= n;
   const bool lo = c < 0;
   const int vc = lo ? A.vc_lo : A.vc_hi;
-  long ms = base + (long)c * A.stride, mv = ms;
-  if (ghost) {
-    const int sc = lo ? A.sc_lo : A.sc_hi;
-    ms = base + (long)ghost_src(sc, lo, c, n) * A.stride;
-    mv = base + (long)ghost_src(vc, lo, c, n) * A.stride;
-  }
-  q.d = D.W[WD][ms];
-  q.phi = MAG ? D.W[WPHI][ms] : 0.0;
-  const double eint = D.W[WEINT][ms];
+  q.d = f[WD];
+  q.phi = MAG ? f[WPHI] : 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    double v = D.W[WV1 + a][mv];
-    double b = MAG ? D.W[WB1 + a][mv] : 0.0;
+    double v = f[WV1 + a];
+    double b = MAG ? f[WB1 + a] : 0.0;
     if (ghost && a == DIR) {
       if (vc == 1) {
         v = -v;
@@ -73,16 +66,127 @@ __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base
     q.b[a] = b;
   }
   if (!ghost) {
-    q.eint = eint;
-    q.e = D.W[WE][ms];
+    q.eint = f[WEINT];
+    q.e = f[WE];
   } else {
     const double gm1 = D.gamma - 1.0;
-    const double p = gm1 * eint;
+    const double p = gm1 * f[WEINT];
     q.eint = p / gm1;
     q.e = q.eint + 0.5 * (q.d * (q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]) +
                           q.b[0] * q.b[0] + q.b[1] * q.b[1] + q.b[2] * q.b[2]);
   }
   return q;
+}
+
+// Source rows of cell c of a line (both the cell itself when interior).
+__device__ __forceinline__ void cell_sources(const Axis &A, long base, int c, long &ms,
+                                             long &mv) {
+  const int n = A.n;
+  ms = base + (long)c * A.stride;
+  mv = ms;
+  if (c < 0 || c >= n) {
+    const bool lo = c < 0;
+    ms = base + (long)ghost_src(lo ? A.sc_lo : A.sc_hi, lo, c, n) * A.stride;
+    mv = base + (long)ghost_src(lo ? A.vc_lo : A.vc_hi, lo, c, n) * A.stride;
+  }
+}
+
+// One cell read straight from HBM by its own lane.
+template <int DIR, bool MAG>
+__device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base, int c) {
+  long ms, mv;
+  cell_sources(A, base, c, ms, mv);
+  double f[NWMAX];
+  f[WD] = D.W[WD][ms];
+  f[WE] = D.W[WE][ms];
+  f[WEINT] = D.W[WEINT][ms];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) f[WV1 + a] = D.W[WV1 + a][mv];
+  if (MAG) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f[WB1 + a] = D.W[WB1 + a][mv];
+    f[WPHI] = D.W[WPHI][ms];
+  }
+  return finish_cell<DIR, MAG>(D, A, c, f);
+}
+
+// ---- LDS-staged lines for the strided (y, z) sweeps ------------------------
+// A block's wavefronts own lines of consecutive x index over the same
+// 64-cell segment, so the block's cells form a tile [64 cells][bw lines]
+// whose rows are contiguous in HBM (bw doubles each).  Loading the tile row
+// by row (one element per thread and field) replaces 64 lane-strided
+// accesses per instruction -- one cache line and, along z, one page per
+// lane -- by bw/64-row coalesced ones; the flux divergences go back out the
+// same way.  LDS layout [field][cell row][line], rows padded to bw+1.
+__device__ __forceinline__ int tile_ld(int bw) { return bw + 1; }
+
+template <int DIR, bool MAG>
+__device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long base, int seg,
+                                          int c, double *lds) {
+  const int bw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ld = tile_ld(bw), fs = 64 * ld;
+  {  // cooperative load: thread t -> (row r, line col)
+    const int t = threadIdx.x, r = t / bw, col = t - r * bw;
+    long ms, mv;
+    cell_sources(A, base - w + col, seg * SEG - 2 + r, ms, mv);
+    double *p = lds + r * ld + col;
+    p[WD * fs] = D.W[WD][ms];
+    p[WE * fs] = D.W[WE][ms];
+    p[WEINT * fs] = D.W[WEINT][ms];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[(WV1 + a) * fs] = D.W[WV1 + a][mv];
+    if (MAG) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) p[(WB1 + a) * fs] = D.W[WB1 + a][mv];
+      p[WPHI * fs] = D.W[WPHI][ms];
+    }
+  }
+  __syncthreads();
+  double f[NWMAX];
+  const double *p = lds + lane * ld + w;
+  const int nf = MAG ? NWMAX : WEINT + 1;
+#pragma unroll
+  for (int k = 0; k < NWMAX; ++k) f[k] = k < nf ? p[k * fs] : 0.0;
+  return finish_cell<DIR, MAG>(D, A, c, f);
+}
+
+// Flux-divergence stores of a line: du[v] is the value of conserved
+// variable v for this lane's cell c (lanes 2..61 with 0 <= c < n own one).
+template <int DIR, bool MAG, bool TILE>
+__device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base, int seg,
+                                         int c, int lane, const double du[NVMAX],
+                                         double *lds) {
+  const int n = A.n;
+  double *const *DU = D.DU[DIR];
+  const int nv = MAG ? (D.i9wv >= 0 ? 9 : 8) : 5;
+  if (!TILE) {
+    if (lane >= 2 && lane <= 61 && c >= 0 && c < n) {
+      const long m = base + (long)c * A.stride;
+#pragma unroll
+      for (int v = 0; v < NVMAX; ++v)
+        if (v < nv) DU[v][m] = du[v];
+    }
+    return;
+  }
+  const int bw = blockDim.x >> 6, w = threadIdx.x >> 6;
+  const int ld = tile_ld(bw), fs = 64 * ld;
+  __syncthreads();  // every wavefront has read its cells out of the tile
+  {
+    double *p = lds + lane * ld + w;
+#pragma unroll
+    for (int v = 0; v < NVMAX; ++v)
+      if (v < nv) p[v * fs] = du[v];
+  }
+  __syncthreads();
+  const int t = threadIdx.x, r = t / bw, col = t - r * bw;
+  const int cr = seg * SEG - 2 + r;
+  if (r >= 2 && r <= 61 && cr >= 0 && cr < n) {
+    const long m = base - w + col + (long)cr * A.stride;
+    const double *p = lds + r * ld + col;
+#pragma unroll
+    for (int v = 0; v < NVMAX; ++v)
+      if (v < nv) DU[v][m] = p[v * fs];
+  }
 }
 
 // FP64 reciprocal for the mixed-precision path: hardware estimate refined
@@ -109,8 +213,9 @@ __device__ __forceinline__ double dn1(double x) {
   return __hiloint2double(wshl1(__double2hiint(x)), wshl1(__double2loint(x)));
 }
 
-template <int DIR, bool MAG, int LIM>
-__device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage) {
+template <int DIR, bool MAG, int LIM, bool TILE = false>
+__device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage,
+                                           double *lds = nullptr) {
   const Axis &A = D.ax[DIR];
   const int n = A.n;
   const long nlines = D.ncell / n;
@@ -130,7 +235,8 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
     ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
   }
   const int c = seg * SEG - 2 + lane;
-  const Cell q = load_cell<DIR, MAG>(D, A, base, c);
+  const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, base, seg, c, lds)
+                      : load_cell<DIR, MAG>(D, A, base, c);
   const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
 
   // ---- slopes of cell c (interpolation.f90, loop c = -1 .. n) -----------
@@ -253,46 +359,48 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   double tm[9];
 #pragma unroll
   for (int s = 0; s < 9; ++s) tm[s] = up1(t[s]);
-  if (lane >= 2 && lane <= 61 && c >= 0 && c < n) {
-    const long m = base + (long)c * A.stride;
-    const double ix = A.idxi[c];
-    // equation order: icnt imo1 imo2 imo3 iene img1 img2 img3 i9wv
-    double *const *DU = D.DU[DIR];
-    DU[0][m] = ix * (tm[0] - t[0]);
-    DU[1 + a_][m] = ix * (tm[1] - t[1]);
-    DU[1 + b_][m] = ix * (tm[2] - t[2]);
-    DU[1 + c_][m] = ix * (tm[3] - t[3]);
-    DU[4][m] = ix * (tm[4] - t[4]);
-    if (MAG) {
-      DU[5 + a_][m] = ix * (tm[5] - t[5]);
-      DU[5 + b_][m] = ix * (tm[6] - t[6]);
-      DU[5 + c_][m] = ix * (tm[7] - t[7]);
-      if (D.i9wv >= 0) DU[8][m] = ix * (tm[8] - t[8]);
-    }
-  }
+  // equation order: icnt imo1 imo2 imo3 iene img1 img2 img3 i9wv
+  const double ix = A.idxi[cc];
+  double du[NVMAX];
+  du[0] = ix * (tm[0] - t[0]);
+  du[1 + a_] = ix * (tm[1] - t[1]);
+  du[1 + b_] = ix * (tm[2] - t[2]);
+  du[1 + c_] = ix * (tm[3] - t[3]);
+  du[4] = ix * (tm[4] - t[4]);
+  du[5 + a_] = ix * (tm[5] - t[5]);
+  du[5 + b_] = ix * (tm[6] - t[6]);
+  du[5 + c_] = ix * (tm[7] - t[7]);
+  du[8] = ix * (tm[8] - t[8]);
+  store_du<DIR, MAG, TILE>(D, A, base, seg, c, lane, du, lds);
 }
 
 // One launch covers every solved direction: wavefronts [0, waves[0]) sweep
-// x, the next waves[1] sweep y, then z.  Blocks are dispatched in order, so
-// a compute unit mostly runs one direction's code at a time.
+// x, then y from woff[1], then z from woff[2] (the y and z ranges start on a
+// multiple of 16 wavefronts, so a block never mixes directions).  Blocks are
+// dispatched in order, so a compute unit mostly runs one direction's code at
+// a time.  tiled: the y/z lines of a block go through the LDS tile (the
+// caller checks that a block's lines have consecutive x index).
 template <bool MAG, int LIM>
-__global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
+__global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage, int tiled) {
+  extern __shared__ double lds[];
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
-  long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wave < D.waves[0]) {
-    sweep_line<0, MAG, LIM>(D, wave, lane, stage);
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave < D.woff[1]) {
+    if (wave < D.waves[0]) sweep_line<0, MAG, LIM>(D, wave, lane, stage);
     return;
   }
-  wave -= D.waves[0];
-  if (wave < D.waves[1]) {
-    sweep_line<1, MAG, LIM>(D, wave, lane, stage);
+  if (wave < D.woff[2]) {
+    if (wave - D.woff[1] >= D.waves[1]) return;  // padding: whole blocks when tiled
+    if (tiled) sweep_line<1, MAG, LIM, true>(D, wave - D.woff[1], lane, stage, lds);
+    else sweep_line<1, MAG, LIM>(D, wave - D.woff[1], lane, stage);
     return;
   }
-  wave -= D.waves[1];
-  if (wave < D.waves[2]) sweep_line<2, MAG, LIM>(D, wave, lane, stage);
+  if (wave - D.woff[2] < D.waves[2]) {
+    if (tiled) sweep_line<2, MAG, LIM, true>(D, wave - D.woff[2], lane, stage, lds);
+    else sweep_line<2, MAG, LIM>(D, wave - D.woff[2], lane, stage);
+  }
 }
-
 
 // ---- FP32-mixed sweep -------------------------------------------------------
 // Same mapping and stencil as sweep_line.  The Riemann solver (fast speeds,
@@ -327,8 +435,9 @@ __device__ __forceinline__ double minmod_geo(double dm, double dp, double ida, d
   return s * mx2(0.0, mn3(fabs(x), s * y, s * z));
 }
 
-template <int DIR, bool MAG, int LIM>
-__device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane, int stage) {
+template <int DIR, bool MAG, int LIM, bool TILE = false>
+__device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane, int stage,
+                                               double *lds = nullptr) {
   const Axis &A = D.ax[DIR];
   const int n = A.n;
   const long nlines = D.ncell / n;
@@ -348,7 +457,8 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
   }
   const int c = seg * SEG - 2 + lane;
-  const Cell q = load_cell<DIR, MAG>(D, A, base, c);
+  const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, base, seg, c, lds)
+                      : load_cell<DIR, MAG>(D, A, base, c);
   const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
   const double qd[8] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2]};
   const int cc = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
@@ -464,40 +574,41 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   double tm[9];
 #pragma unroll
   for (int s = 0; s < 9; ++s) tm[s] = (!MAG && s >= 5) ? 0.0 : up1(t[s]);
-  if (lane >= 2 && lane <= 61 && c >= 0 && c < n) {
-    const long m = base + (long)c * A.stride;
-    const double ix = A.idxi[c];
-    double *const *DU = D.DU[DIR];
-    DU[0][m] = ix * (tm[0] - t[0]);
-    DU[1 + a_][m] = ix * (tm[1] - t[1]);
-    DU[1 + b_][m] = ix * (tm[2] - t[2]);
-    DU[1 + c_][m] = ix * (tm[3] - t[3]);
-    DU[4][m] = ix * (tm[4] - t[4]);
-    if (MAG) {
-      DU[5 + a_][m] = ix * (tm[5] - t[5]);
-      DU[5 + b_][m] = ix * (tm[6] - t[6]);
-      DU[5 + c_][m] = ix * (tm[7] - t[7]);
-      if (D.i9wv >= 0) DU[8][m] = ix * (tm[8] - t[8]);
-    }
-  }
+  // equation order: icnt imo1 imo2 imo3 iene img1 img2 img3 i9wv
+  const double ix = A.idxi[cc];
+  double du[NVMAX];
+  du[0] = ix * (tm[0] - t[0]);
+  du[1 + a_] = ix * (tm[1] - t[1]);
+  du[1 + b_] = ix * (tm[2] - t[2]);
+  du[1 + c_] = ix * (tm[3] - t[3]);
+  du[4] = ix * (tm[4] - t[4]);
+  du[5 + a_] = ix * (tm[5] - t[5]);
+  du[5 + b_] = ix * (tm[6] - t[6]);
+  du[5 + c_] = ix * (tm[7] - t[7]);
+  du[8] = ix * (tm[8] - t[8]);
+  store_du<DIR, MAG, TILE>(D, A, base, seg, c, lane, du, lds);
 }
 
 template <bool MAG, int LIM>
-__global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
+__global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage, int tiled) {
+  extern __shared__ double lds[];
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
-  long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wave < D.waves[0]) {
-    sweep_line_mix<0, MAG, LIM>(D, wave, lane, stage);
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave < D.woff[1]) {
+    if (wave < D.waves[0]) sweep_line_mix<0, MAG, LIM>(D, wave, lane, stage);
     return;
   }
-  wave -= D.waves[0];
-  if (wave < D.waves[1]) {
-    sweep_line_mix<1, MAG, LIM>(D, wave, lane, stage);
+  if (wave < D.woff[2]) {
+    if (wave - D.woff[1] >= D.waves[1]) return;  // padding: whole blocks when tiled
+    if (tiled) sweep_line_mix<1, MAG, LIM, true>(D, wave - D.woff[1], lane, stage, lds);
+    else sweep_line_mix<1, MAG, LIM>(D, wave - D.woff[1], lane, stage);
     return;
   }
-  wave -= D.waves[1];
-  if (wave < D.waves[2]) sweep_line_mix<2, MAG, LIM>(D, wave, lane, stage);
+  if (wave - D.woff[2] < D.waves[2]) {
+    if (tiled) sweep_line_mix<2, MAG, LIM, true>(D, wave - D.woff[2], lane, stage, lds);
+    else sweep_line_mix<2, MAG, LIM>(D, wave - D.woff[2], lane, stage);
+  }
 }
 
 // ---- CFL condition of one cell (timestep.f90:35-57, dti_cell :125-183) ----
@@ -991,6 +1102,26 @@ static inline void klaunch(void (*k)(KA...), unsigned g, unsigned b, hipStream_t
     k<<<g, b, 0, s>>>(KA(args)...);
 }
 
+// Same, with dynamic LDS.
+template <typename... KA, typename... A>
+static inline void klaunch_lds(void (*k)(KA...), unsigned g, unsigned b, size_t lds,
+                               hipStream_t s, const KEv *ev, A... args) {
+  if (ev)
+    hipExtLaunchKernelGGL(k, dim3(g), dim3(b), (uint32_t)lds, s, ev->start, ev->stop, 0,
+                          KA(args)...);
+  else
+    k<<<g, b, lds, s>>>(KA(args)...);
+}
+
+// Bytes of LDS the y/z tile of a sweep block needs, or 0 when the lines of a
+// block do not have consecutive x index (or the tile would not fit).
+size_t sweep_tile_bytes(const Dev &D, int bw) {
+  if (bw < 4 || D.n1 % bw != 0 || !(D.solve[1] || D.solve[2])) return 0;
+  const size_t nf = D.mag ? NWMAX : WEINT + 1;
+  const size_t bytes = nf * 64 * (size_t)(bw + 1) * sizeof(double);
+  return bytes <= (size_t)D.lds_max ? bytes : 0;
+}
+
 void launch_pressure(const Dev &D, hipStream_t s) {
   if (D.mag) k_pressure<true><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
   else k_pressure<false><<<nblk(D.ncell, 256), 256, 0, s>>>(D);
@@ -1015,28 +1146,32 @@ void launch_set_clock(const Dev &D, double time, double t_out, double t_end, dou
 // y/z loads.
 void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev) {
   const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
-  const unsigned g = nblk(D.waves[0] + D.waves[1] + D.waves[2], bw);
+  const unsigned g = nblk(D.woff[2] + D.waves[2], bw);
   const int bs = 64 * bw;
+  // LDS tile for the y/z lines: whole blocks of lines with consecutive x
+  // index (n1 a multiple of the block's line count), 4+ lines per block
+  const size_t tile = sweep_tile_bytes(D, bw);
+  const int tiled = tile > 0 ? 1 : 0;
   if (D.mixed) {
     if (D.mag) {
-      if (D.limiter == LIM_FLAT) klaunch(k_sweep_mix<true, LIM_FLAT>, g, bs, s, ev, D, stage);
-      else if (D.limiter == LIM_MINMOD) klaunch(k_sweep_mix<true, LIM_MINMOD>, g, bs, s, ev, D, stage);
-      else klaunch(k_sweep_mix<true, LIM_MC>, g, bs, s, ev, D, stage);
+      if (D.limiter == LIM_FLAT) klaunch_lds(k_sweep_mix<true, LIM_FLAT>, g, bs, tile, s, ev, D, stage, tiled);
+      else if (D.limiter == LIM_MINMOD) klaunch_lds(k_sweep_mix<true, LIM_MINMOD>, g, bs, tile, s, ev, D, stage, tiled);
+      else klaunch_lds(k_sweep_mix<true, LIM_MC>, g, bs, tile, s, ev, D, stage, tiled);
     } else {
-      if (D.limiter == LIM_FLAT) klaunch(k_sweep_mix<false, LIM_FLAT>, g, bs, s, ev, D, stage);
-      else if (D.limiter == LIM_MINMOD) klaunch(k_sweep_mix<false, LIM_MINMOD>, g, bs, s, ev, D, stage);
-      else klaunch(k_sweep_mix<false, LIM_MC>, g, bs, s, ev, D, stage);
+      if (D.limiter == LIM_FLAT) klaunch_lds(k_sweep_mix<false, LIM_FLAT>, g, bs, tile, s, ev, D, stage, tiled);
+      else if (D.limiter == LIM_MINMOD) klaunch_lds(k_sweep_mix<false, LIM_MINMOD>, g, bs, tile, s, ev, D, stage, tiled);
+      else klaunch_lds(k_sweep_mix<false, LIM_MC>, g, bs, tile, s, ev, D, stage, tiled);
     }
     return;
   }
   if (D.mag) {
-    if (D.limiter == LIM_FLAT) klaunch(k_sweep<true, LIM_FLAT>, g, bs, s, ev, D, stage);
-    else if (D.limiter == LIM_MINMOD) klaunch(k_sweep<true, LIM_MINMOD>, g, bs, s, ev, D, stage);
-    else klaunch(k_sweep<true, LIM_MC>, g, bs, s, ev, D, stage);
+    if (D.limiter == LIM_FLAT) klaunch_lds(k_sweep<true, LIM_FLAT>, g, bs, tile, s, ev, D, stage, tiled);
+    else if (D.limiter == LIM_MINMOD) klaunch_lds(k_sweep<true, LIM_MINMOD>, g, bs, tile, s, ev, D, stage, tiled);
+    else klaunch_lds(k_sweep<true, LIM_MC>, g, bs, tile, s, ev, D, stage, tiled);
   } else {
-    if (D.limiter == LIM_FLAT) klaunch(k_sweep<false, LIM_FLAT>, g, bs, s, ev, D, stage);
-    else if (D.limiter == LIM_MINMOD) klaunch(k_sweep<false, LIM_MINMOD>, g, bs, s, ev, D, stage);
-    else klaunch(k_sweep<false, LIM_MC>, g, bs, s, ev, D, stage);
+    if (D.limiter == LIM_FLAT) klaunch_lds(k_sweep<false, LIM_FLAT>, g, bs, tile, s, ev, D, stage, tiled);
+    else if (D.limiter == LIM_MINMOD) klaunch_lds(k_sweep<false, LIM_MINMOD>, g, bs, tile, s, ev, D, stage, tiled);
+    else klaunch_lds(k_sweep<false, LIM_MC>, g, bs, tile, s, ev, D, stage, tiled);
   }
 }
 

@@ -64,6 +64,8 @@ struct Dev {
   Scalars *sc;
   int nblk_cfl;
   long waves[3];   // sweep wavefronts per direction (0 if not solved)
+  long woff[3];    // first wavefront of each direction in a sweep launch
+  int lds_max;     // LDS bytes one block may use on this device
   int sweep_bw;    // sweep wavefronts per block (1, 2, 4, 8 or 16)
   // hyperbolic self-gravity (gravswitch 3): the potential is stored with a
   // one-cell ghost layer ((n1+2)(n2+2)(n3+2), Dirichlet values), twice for
@@ -132,6 +134,7 @@ struct KEv {
 };
 
 void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev = nullptr);
+size_t sweep_tile_bytes(const Dev &D, int bw);
 // cfl: also the next step's CFL partials; damp: phidamp
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
                bool damp, double damp_factor, hipStream_t s, const KEv *ev = nullptr);
