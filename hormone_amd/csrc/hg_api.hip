@@ -188,8 +188,13 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
     }
   }
   const size_t fb = sizeof(double) * (size_t)D.ncell;
+  // d, e and B are the same numbers in primitive and conserved form
+  // (rungekutta.f90:193-232 copies them), so those W fields alias U and the
+  // RK update writes them once
+  auto aliased = [&](int w) { return w == WD || w == WE || (D.mag && w >= WB1 && w <= WB3); };
   for (int w = 0; w < D.nw; ++w)
-    if ((st = dev_alloc(c, fb, (void **)&D.W[w]))) return fail(st);
+    if (!aliased(w))
+      if ((st = dev_alloc(c, fb, (void **)&D.W[w]))) return fail(st);
   for (int v = 0; v < D.nv; ++v) {
     if ((st = dev_alloc(c, fb, (void **)&D.U[v]))) return fail(st);
     if ((st = dev_alloc(c, fb, (void **)&D.U0[v]))) return fail(st);
@@ -197,6 +202,10 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
       if (D.solve[a])
         if ((st = dev_alloc(c, fb, (void **)&D.DU[a][v]))) return fail(st);
   }
+  D.W[WD] = D.U[0];
+  D.W[WE] = D.U[4];
+  if (D.mag)
+    for (int a = 0; a < 3; ++a) D.W[WB1 + a] = D.U[5 + a];
   for (int w = 0; w < D.nw; ++w)
     if (hipMemsetAsync(D.W[w], 0, fb, c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
   D.nblk_cfl = (int)((D.ncell + 255) / 256);
@@ -471,8 +480,9 @@ extern "C" int hg_profile_read(const hg_ctx *c, int32_t kind, double *ms_total,
     case HG_PROF_SWEEP:  // read the primitives once and write one divergence per direction
       bytes = cell * (double)D.dim * (D.nw + D.nv);
       break;
-    case HG_PROF_RK: {  // read U, U0, DU_d; write U, W (stage 1 writes U0, reads no U0)
-      bytes = cell * (double)(D.nv * (2 + D.dim) + D.nw);
+    case HG_PROF_RK: {  // read U, U0, DU_d; write U and the W fields that are
+                        // not U (stage 1 writes U0, reads no U0)
+      bytes = cell * (double)(D.nv * (2 + D.dim) + D.nw - (D.mag ? 5 : 2));
       break;
     }
     case HG_PROF_CFL:
@@ -815,7 +825,7 @@ static int grow(T **p, size_t *cap, size_t need) {
 
 static void batch_enqueue_step(hg_batch *b) {
   const Batch &B = b->B;
-  const long sw = (B.nsitems + b->sweep_block_waves - 1) / b->sweep_block_waves;
+  const long sw = (B.nsitems + b->sweep_block_waves - 1) / b->sweep_block_waves;  // chunks
   const unsigned gs = b->sweep_grid ? (unsigned)std::min<long>(b->sweep_grid, sw) : (unsigned)sw;
   const unsigned gr = b->rk_grid ? (unsigned)std::min<long>(b->rk_grid, B.ncitems)
                                  : (unsigned)B.ncitems;
@@ -837,7 +847,10 @@ static int batch_rebuild(hg_batch *b) {
   for (size_t q = 0; q < nb; ++q) {
     const Dev &D = b->members[q]->D;
     boxes[q] = D;
-    const long nwv = D.waves[0] + D.waves[1] + D.waves[2];
+    // the box's wavefronts as numbered in its own sweep launch (padded
+    // direction ranges); the count is a multiple of 16, so every box's range
+    // starts on a multiple of 16 and aligned chunks never mix boxes
+    const long nwv = D.woff[2] + (D.waves[2] + 15) / 16 * 16;
     for (long w = 0; w < nwv; ++w) si.push_back(make_int2((int)q, (int)w));
     for (long k = 0; k < D.nblk_cfl; ++k) ci.push_back(make_int2((int)q, (int)k));
   }
@@ -858,6 +871,7 @@ static int batch_rebuild(hg_batch *b) {
   b->B.nbox = (int)nb;
   b->B.sitems = b->d_sitems;
   b->B.nsitems = (long)si.size();
+  b->B.lds_max = nb ? b->members[0]->D.lds_max : 0;
   b->B.citems = b->d_citems;
   b->B.ncitems = (long)ci.size();
   if (b->cap_hsc < (nb ? nb : 1)) {

@@ -739,19 +739,15 @@ __device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double 
   const double d = un[0];
   const double v1 = un[1] / d, v2 = un[2] / d, v3 = un[3] / d;
   const double e = un[4];
-  D.W[WD][m] = d;
+  // W[d], W[e] and W[B] are U[0], U[4] and U[5..7] (same buffers)
   D.W[WV1][m] = v1;
   D.W[WV2][m] = v2;
   D.W[WV3][m] = v3;
-  D.W[WE][m] = e;
   double b1 = 0, b2 = 0, b3 = 0;
   if (MAG) {
     b1 = un[5];
     b2 = un[6];
     b3 = un[7];
-    D.W[WB1][m] = b1;
-    D.W[WB2][m] = b2;
-    D.W[WB3][m] = b3;
     if (D.i9wv >= 0) D.W[WPHI][m] = DAMP ? un[8] * damp : un[8];
   }
   const double eint = get_eint(MAG, e, d, v1, v2, v3, b1, b2, b3);
@@ -952,32 +948,62 @@ __device__ __forceinline__ void step_end(const Dev &D) {
 
 
 // ---- batched boxes: persistent kernels over flattened work-lists ---------
+// Work items are (box, wavefront) with each box's wavefronts numbered as in
+// a single-box sweep launch (x from 0, y from woff[1], z from woff[2]; the
+// ranges padded to multiples of 16 with items that do nothing), and every
+// box's item range starts on a multiple of 16.  A block takes its items in
+// aligned chunks of one per wavefront, so a chunk never mixes boxes or
+// directions and its y/z lines can share an LDS tile (tile_ok: the launch
+// carries the LDS; the box decides by its x extent).
 template <bool MAG, int LIM, bool MIX>
-__global__ __launch_bounds__(1024) void k_bsweep(Batch B, int stage) {
+__device__ __forceinline__ void bsweep_item(const Dev &D, long w, int lane, int stage,
+                                            bool tiled, double *lds) {
+  if (w < D.woff[1]) {
+    if (w < D.waves[0]) {
+      if (MIX) sweep_line_mix<0, MAG, LIM>(D, w, lane, stage);
+      else sweep_line<0, MAG, LIM>(D, w, lane, stage);
+    }
+    return;
+  }
+  if (w < D.woff[2]) {
+    w -= D.woff[1];
+    if (w >= D.waves[1]) return;
+    if (tiled) {
+      if (MIX) sweep_line_mix<1, MAG, LIM, true>(D, w, lane, stage, lds);
+      else sweep_line<1, MAG, LIM, true>(D, w, lane, stage, lds);
+    } else {
+      if (MIX) sweep_line_mix<1, MAG, LIM>(D, w, lane, stage);
+      else sweep_line<1, MAG, LIM>(D, w, lane, stage);
+    }
+    return;
+  }
+  w -= D.woff[2];
+  if (w >= D.waves[2]) return;
+  if (tiled) {
+    if (MIX) sweep_line_mix<2, MAG, LIM, true>(D, w, lane, stage, lds);
+    else sweep_line<2, MAG, LIM, true>(D, w, lane, stage, lds);
+  } else {
+    if (MIX) sweep_line_mix<2, MAG, LIM>(D, w, lane, stage);
+    else sweep_line<2, MAG, LIM>(D, w, lane, stage);
+  }
+}
+
+template <bool MAG, int LIM, bool MIX>
+__global__ __launch_bounds__(1024) void k_bsweep(Batch B, int stage, int tile_ok) {
+  extern __shared__ double lds[];
   const int lane = threadIdx.x & 63;
-  const long nw = (long)gridDim.x * (blockDim.x >> 6);
-  for (long it = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < B.nsitems;
-       it += nw) {
+  const int bw = blockDim.x >> 6;
+  const long nchunk = (B.nsitems + bw - 1) / bw;
+  for (long g = blockIdx.x; g < nchunk; g += gridDim.x) {
+    const long it = g * bw + (threadIdx.x >> 6);
+    if (it >= B.nsitems) continue;  // only in a last, untiled chunk
     // the work item is the same for the whole wavefront: make that visible
     // so the box record is read with scalar loads
     const int2 x = B.sitems[it];
     const Dev &D = B.boxes[__builtin_amdgcn_readfirstlane(x.x)];
-    if (D.sc->stop) continue;
-    long w = __builtin_amdgcn_readfirstlane(x.y);
-    if (w < D.waves[0]) {
-      if (MIX) sweep_line_mix<0, MAG, LIM>(D, w, lane, stage);
-      else sweep_line<0, MAG, LIM>(D, w, lane, stage);
-      continue;
-    }
-    w -= D.waves[0];
-    if (w < D.waves[1]) {
-      if (MIX) sweep_line_mix<1, MAG, LIM>(D, w, lane, stage);
-      else sweep_line<1, MAG, LIM>(D, w, lane, stage);
-      continue;
-    }
-    w -= D.waves[1];
-    if (MIX) sweep_line_mix<2, MAG, LIM>(D, w, lane, stage);
-    else sweep_line<2, MAG, LIM>(D, w, lane, stage);
+    if (D.sc->stop) continue;  // uniform over the chunk (one box)
+    const bool tiled = tile_ok && bw >= 4 && D.n1 % bw == 0;
+    bsweep_item<MAG, LIM, MIX>(D, __builtin_amdgcn_readfirstlane(x.y), lane, stage, tiled, lds);
   }
 }
 
@@ -1223,8 +1249,14 @@ void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
 template <bool MAG, int LIM>
 static void bsweep_mix(const Batch &B, int mixed, int stage, unsigned g, int bs,
                        hipStream_t s) {
-  if (mixed) k_bsweep<MAG, LIM, true><<<g, bs, 0, s>>>(B, stage);
-  else k_bsweep<MAG, LIM, false><<<g, bs, 0, s>>>(B, stage);
+  // LDS for the y/z tile when it fits (the boxes' own x extents decide)
+  const int bw = bs / 64;
+  const size_t nf = MAG ? NWMAX : WEINT + 1;
+  size_t tile = nf * 64 * (size_t)(bw + 1) * sizeof(double);
+  if (bw < 4 || tile > (size_t)B.lds_max) tile = 0;
+  const int ok = tile > 0 ? 1 : 0;
+  if (mixed) k_bsweep<MAG, LIM, true><<<g, bs, tile, s>>>(B, stage, ok);
+  else k_bsweep<MAG, LIM, false><<<g, bs, tile, s>>>(B, stage, ok);
 }
 
 void launch_bsweep_blocks(const Batch &B, int mag, int lim, int mixed, int stage,

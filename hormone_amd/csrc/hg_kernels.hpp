@@ -6,7 +6,8 @@
 // conditions of boundarycondition.f90 on the fly when a stencil reaches
 // past the domain, from the interior cells they mirror or wrap to.
 //
-//   W[NW]   primitive fields  d v1 v2 v3 e eint [b1 b2 b3 phi]
+//   W[NW]   primitive fields  d v1 v2 v3 e eint [b1 b2 b3 phi]; d, e and B
+//           are the same buffers as U[0], U[4], U[5..7]
 //   U[NV]   conserved u(:,:,:,1..ufnmax)      (rungekutta.f90)
 //   U0[NV]  uorg, the RK stage-1 copy
 //   DU[3][NV] per-direction flux divergence  idxi_d * (F(c-1/2) - F(c+1/2))
@@ -105,8 +106,9 @@ __host__ __device__ inline unsigned long long errkey(int stage, int phase, int d
 struct Batch {
   const Dev *boxes;
   int nbox;
-  const int2 *sitems;
+  const int2 *sitems;  // (box, wavefront) in 16-aligned per-box ranges
   long nsitems;
+  int lds_max;         // LDS bytes one block may use
   const int2 *citems;
   long ncitems;
 };
