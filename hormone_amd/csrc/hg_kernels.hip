@@ -378,10 +378,12 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
 // x, then y from woff[1], then z from woff[2] (the y and z ranges start on a
 // multiple of 16 wavefronts, so a block never mixes directions).  Blocks are
 // dispatched in order, so a compute unit mostly runs one direction's code at
-// a time.  tiled: the y/z lines of a block go through the LDS tile (the
-// caller checks that a block's lines have consecutive x index).
-template <bool MAG, int LIM>
-__global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage, int tiled) {
+// a time.  TILE: the y/z lines of a block go through the LDS tile (the
+// launcher checks that a block's lines have consecutive x index); a
+// separate instantiation keeps each kernel's code within the instruction
+// cache.
+template <bool MAG, int LIM, bool TILE>
+__global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
   extern __shared__ double lds[];
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
@@ -392,14 +394,10 @@ __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage, int tiled) {
   }
   if (wave < D.woff[2]) {
     if (wave - D.woff[1] >= D.waves[1]) return;  // padding: whole blocks when tiled
-    if (tiled) sweep_line<1, MAG, LIM, true>(D, wave - D.woff[1], lane, stage, lds);
-    else sweep_line<1, MAG, LIM>(D, wave - D.woff[1], lane, stage);
+    sweep_line<1, MAG, LIM, TILE>(D, wave - D.woff[1], lane, stage, lds);
     return;
   }
-  if (wave - D.woff[2] < D.waves[2]) {
-    if (tiled) sweep_line<2, MAG, LIM, true>(D, wave - D.woff[2], lane, stage, lds);
-    else sweep_line<2, MAG, LIM>(D, wave - D.woff[2], lane, stage);
-  }
+  if (wave - D.woff[2] < D.waves[2]) sweep_line<2, MAG, LIM, TILE>(D, wave - D.woff[2], lane, stage, lds);
 }
 
 // ---- FP32-mixed sweep -------------------------------------------------------
@@ -589,8 +587,8 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   store_du<DIR, MAG, TILE>(D, A, base, seg, c, lane, du, lds);
 }
 
-template <bool MAG, int LIM>
-__global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage, int tiled) {
+template <bool MAG, int LIM, bool TILE>
+__global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
   extern __shared__ double lds[];
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
@@ -601,14 +599,10 @@ __global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage, int tiled)
   }
   if (wave < D.woff[2]) {
     if (wave - D.woff[1] >= D.waves[1]) return;  // padding: whole blocks when tiled
-    if (tiled) sweep_line_mix<1, MAG, LIM, true>(D, wave - D.woff[1], lane, stage, lds);
-    else sweep_line_mix<1, MAG, LIM>(D, wave - D.woff[1], lane, stage);
+    sweep_line_mix<1, MAG, LIM, TILE>(D, wave - D.woff[1], lane, stage, lds);
     return;
   }
-  if (wave - D.woff[2] < D.waves[2]) {
-    if (tiled) sweep_line_mix<2, MAG, LIM, true>(D, wave - D.woff[2], lane, stage, lds);
-    else sweep_line_mix<2, MAG, LIM>(D, wave - D.woff[2], lane, stage);
-  }
+  if (wave - D.woff[2] < D.waves[2]) sweep_line_mix<2, MAG, LIM, TILE>(D, wave - D.woff[2], lane, stage, lds);
 }
 
 // ---- CFL condition of one cell (timestep.f90:35-57, dti_cell :125-183) ----
@@ -1170,6 +1164,18 @@ void launch_set_clock(const Dev &D, double time, double t_out, double t_end, dou
 // a block own neighbouring lines (consecutive i for the y and z sweeps), so
 // with 16 wavefronts a block uses whole 128-byte cache lines of the strided
 // y/z loads.
+template <bool MAG, int LIM>
+static void sweep_variant(const Dev &D, int stage, unsigned g, int bs, size_t tile,
+                          hipStream_t s, const KEv *ev) {
+  if (D.mixed) {
+    if (tile) klaunch_lds(k_sweep_mix<MAG, LIM, true>, g, bs, tile, s, ev, D, stage);
+    else klaunch_lds(k_sweep_mix<MAG, LIM, false>, g, bs, 0, s, ev, D, stage);
+  } else {
+    if (tile) klaunch_lds(k_sweep<MAG, LIM, true>, g, bs, tile, s, ev, D, stage);
+    else klaunch_lds(k_sweep<MAG, LIM, false>, g, bs, 0, s, ev, D, stage);
+  }
+}
+
 void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev) {
   const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
   const unsigned g = nblk(D.woff[2] + D.waves[2], bw);
@@ -1177,27 +1183,14 @@ void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev) {
   // LDS tile for the y/z lines: whole blocks of lines with consecutive x
   // index (n1 a multiple of the block's line count), 4+ lines per block
   const size_t tile = sweep_tile_bytes(D, bw);
-  const int tiled = tile > 0 ? 1 : 0;
-  if (D.mixed) {
-    if (D.mag) {
-      if (D.limiter == LIM_FLAT) klaunch_lds(k_sweep_mix<true, LIM_FLAT>, g, bs, tile, s, ev, D, stage, tiled);
-      else if (D.limiter == LIM_MINMOD) klaunch_lds(k_sweep_mix<true, LIM_MINMOD>, g, bs, tile, s, ev, D, stage, tiled);
-      else klaunch_lds(k_sweep_mix<true, LIM_MC>, g, bs, tile, s, ev, D, stage, tiled);
-    } else {
-      if (D.limiter == LIM_FLAT) klaunch_lds(k_sweep_mix<false, LIM_FLAT>, g, bs, tile, s, ev, D, stage, tiled);
-      else if (D.limiter == LIM_MINMOD) klaunch_lds(k_sweep_mix<false, LIM_MINMOD>, g, bs, tile, s, ev, D, stage, tiled);
-      else klaunch_lds(k_sweep_mix<false, LIM_MC>, g, bs, tile, s, ev, D, stage, tiled);
-    }
-    return;
-  }
   if (D.mag) {
-    if (D.limiter == LIM_FLAT) klaunch_lds(k_sweep<true, LIM_FLAT>, g, bs, tile, s, ev, D, stage, tiled);
-    else if (D.limiter == LIM_MINMOD) klaunch_lds(k_sweep<true, LIM_MINMOD>, g, bs, tile, s, ev, D, stage, tiled);
-    else klaunch_lds(k_sweep<true, LIM_MC>, g, bs, tile, s, ev, D, stage, tiled);
+    if (D.limiter == LIM_FLAT) sweep_variant<true, LIM_FLAT>(D, stage, g, bs, tile, s, ev);
+    else if (D.limiter == LIM_MINMOD) sweep_variant<true, LIM_MINMOD>(D, stage, g, bs, tile, s, ev);
+    else sweep_variant<true, LIM_MC>(D, stage, g, bs, tile, s, ev);
   } else {
-    if (D.limiter == LIM_FLAT) klaunch_lds(k_sweep<false, LIM_FLAT>, g, bs, tile, s, ev, D, stage, tiled);
-    else if (D.limiter == LIM_MINMOD) klaunch_lds(k_sweep<false, LIM_MINMOD>, g, bs, tile, s, ev, D, stage, tiled);
-    else klaunch_lds(k_sweep<false, LIM_MC>, g, bs, tile, s, ev, D, stage, tiled);
+    if (D.limiter == LIM_FLAT) sweep_variant<false, LIM_FLAT>(D, stage, g, bs, tile, s, ev);
+    else if (D.limiter == LIM_MINMOD) sweep_variant<false, LIM_MINMOD>(D, stage, g, bs, tile, s, ev);
+    else sweep_variant<false, LIM_MC>(D, stage, g, bs, tile, s, ev);
   }
 }
 
