@@ -685,8 +685,8 @@ extern "C" int hg_last_error(const hg_ctx *c, int32_t *i, int32_t *j, int32_t *k
 }
 
 extern "C" int hg_kat(int32_t which, int32_t n, const double *in, double *out) {
-  static const int nin[5] = {25, 5, 8, 5, 2}, nout[5] = {9, 1, 1, 1, 2};
-  if (which < 0 || which > 4 || n < 0 || !in || !out) return HG_ERR_ARG;
+  static const int nin[6] = {25, 5, 8, 5, 2, 5}, nout[6] = {9, 1, 1, 1, 2, 4};
+  if (which < 0 || which > 5 || n < 0 || !in || !out) return HG_ERR_ARG;
   if (n == 0) return HG_OK;
   double *din = nullptr, *dout = nullptr;
   const size_t bi = sizeof(double) * (size_t)n * nin[which];

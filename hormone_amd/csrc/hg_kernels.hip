@@ -204,8 +204,8 @@ __device__ __forceinline__ double rcp64(double x) {
 // wavefront shifts (wave_shr:1 = 0x138, wave_shl:1 = 0x130): one VALU move
 // per 32-bit half and no LDS round trip (measured 1-3% faster per sweep
 // than ds_bpermute shuffles).
-__device__ __forceinline__ int wshr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int wshl1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int wshr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int wshl1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true); }
 __device__ __forceinline__ double up1(double x) {
   return __hiloint2double(wshr1(__double2hiint(x)), wshr1(__double2loint(x)));
 }
@@ -213,7 +213,108 @@ __device__ __forceinline__ double dn1(double x) {
   return __hiloint2double(wshl1(__double2hiint(x)), wshl1(__double2loint(x)));
 }
 
-template <int DIR, bool MAG, int LIM, bool TILE = false>
+// Limited slopes of a cell (interpolation.f90, loop c = -1 .. n) and the
+// e_int guard (interpolation.f90:143-163).  qv, ql, qr: d e m1 m2 m3 b1 b2 b3
+// eint of the cell and of its left and right neighbours.  SH: divisions by
+// the grid denominators shared between the variables (qdiv).
+template <bool MAG, int LIM, bool SH>
+__device__ __forceinline__ void cell_slopes(const double qv[9], const double ql[9],
+                                            const double qr[9], double xm, double x0, double xp,
+                                            double xim, double xi0, double ida, double idb,
+                                            double S[8]) {
+  const double hl = x0 - xim, hr = xi0 - x0;
+  // grid factors and denominators shared by every variable of the cell
+  const double dxp = xp - x0;
+  const double cf = dxp / (xi0 - x0), cb = (x0 - xm) / (x0 - xim);
+  const auto Dm = mkden<SH>(LIM == LIM_MINMOD ? ida + idb : x0 - xm);
+  const auto Dp = mkden<SH>(dxp);
+  double fl[8], fr[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    if (!MAG && s >= 5) {
+      S[s] = 0.0;
+      fl[s] = fr[s] = 0.0;
+      continue;
+    }
+    const double du = (LIM == LIM_MINMOD)
+                          ? minmod_d(ql[s], qv[s], qr[s], ida, idb, Dm)
+                          : modified_mc_d(ql[s], qv[s], qr[s], dxp, cf, cb, Dm, Dp);
+    S[s] = du;
+    fl[s] = qv[s] - hl * du;
+    fr[s] = qv[s] + hr * du;
+  }
+  double eintl = fl[1] - 0.5 * (fl[2] * fl[2] + fl[3] * fl[3] + fl[4] * fl[4]) / fl[0];
+  double eintr = fr[1] - 0.5 * (fr[2] * fr[2] + fr[3] * fr[3] + fr[4] * fr[4]) / fr[0];
+  if (MAG) {
+    eintl = eintl - 0.5 * (fl[5] * fl[5] + fl[6] * fl[6] + fl[7] * fl[7]);
+    eintr = eintr - 0.5 * (fr[5] * fr[5] + fr[6] * fr[6] + fr[7] * fr[7]);
+  }
+  const double el_ = ql[8], em_ = qv[8], er_ = qr[8];
+  if (eintl >= mx2(el_, em_) || eintl <= mn2(el_, em_) || eintr >= mx2(em_, er_) ||
+      eintr <= mn2(em_, er_)) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) S[s] = 0.0;
+  }
+}
+
+// Face states, EOS, fast speeds and HLLD/GLM flux t[9] of the face between
+// the lane's cell (left: L = d e m1 m2 m3 b1 b2 b3 phi, slopes S) and the
+// right neighbour (R, RS) -- numflux.f90 with the cyclic rotation normal a =
+// DIR, then b, c (numflux.f90:73-93,246-266,419-439).  Returns the face
+// densities and internal energies for the positivity check.  SH: divisions
+// by dl, dr and the HLLD denominators shared between numerators (qdiv).
+template <int DIR, bool MAG, bool SH>
+__device__ __forceinline__ void face_flux(const double L[9], const double S[8],
+                                          const double R_[9], const double RS[8], double dx1,
+                                          double dx2, double gam, double ch, double t[9],
+                                          double fe[4]) {
+  constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
+  const double dl = L[0] + dx1 * S[0];
+  const double dr = R_[0] - dx2 * RS[0];
+  const double el = L[1] + dx1 * S[1];
+  const double er = R_[1] - dx2 * RS[1];
+  const auto Dl = mkden<SH>(dl), Dr = mkden<SH>(dr);  // every division by dl, dr below
+  const double v1l = qdiv(Dl, L[2 + a_] + dx1 * S[2 + a_]);
+  const double v1r = qdiv(Dr, R_[2 + a_] - dx2 * RS[2 + a_]);
+  const double v2l = qdiv(Dl, L[2 + b_] + dx1 * S[2 + b_]);
+  const double v2r = qdiv(Dr, R_[2 + b_] - dx2 * RS[2 + b_]);
+  const double v3l = qdiv(Dl, L[2 + c_] + dx1 * S[2 + c_]);
+  const double v3r = qdiv(Dr, R_[2 + c_] - dx2 * RS[2 + c_]);
+  double b1l = 0, b1r = 0, b2l = 0, b2r = 0, b3l = 0, b3r = 0, phil = 0, phir = 0;
+  if (MAG) {
+    b1l = L[5 + a_] + dx1 * S[5 + a_];
+    b1r = R_[5 + a_] - dx2 * RS[5 + a_];
+    b2l = L[5 + b_] + dx1 * S[5 + b_];
+    b2r = R_[5 + b_] - dx2 * RS[5 + b_];
+    b3l = L[5 + c_] + dx1 * S[5 + c_];
+    b3r = R_[5 + c_] - dx2 * RS[5 + c_];
+    // dphi is identically zero: phi + dx*0 == phi + 0
+    phil = L[8] + 0.0;
+    phir = R_[8] + 0.0;
+  }
+  const double eil = get_eint(MAG, el, dl, v1l, v2l, v3l, b1l, b2l, b3l);
+  const double eir = get_eint(MAG, er, dr, v1r, v2r, v3r, b1r, b2r, b3r);
+  fe[0] = dl;
+  fe[1] = eil;
+  fe[2] = dr;
+  fe[3] = eir;
+  // eos_p_cs (eos.f90:195-242); the erad term 4*0/(9d) adds +0 to a
+  // positive cs2 and is omitted.
+  double ptl = (gam - 1.0) * eil;
+  double ptr = (gam - 1.0) * eir;
+  const double csl = sqrt(qdiv(Dl, gam * ptl));
+  const double csr = sqrt(qdiv(Dr, gam * ptr));
+  if (MAG) {
+    ptl = ptl + 0.5 * (b1l * b1l + b2l * b2l + b3l * b3l);
+    ptr = ptr + 0.5 * (b1r * b1r + b2r * b2r + b3r * b3r);
+  }
+  const double cfl = get_cf_d(Dl, csl, b1l, b2l, b3l);
+  const double cfr = get_cf_d(Dr, csr, b1r, b2r, b3r);
+  hlld_d<SH>(t, cfl, cfr, v1l, v1r, v2l, v2r, v3l, v3r, dl, dr, el, er, ptl, ptr, b1l, b1r,
+             b2l, b2r, b3l, b3r, phil, phir, ch);
+}
+
+template <int DIR, bool MAG, int LIM, bool TILE = false, bool SH = true>
 __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage,
                                            double *lds = nullptr) {
   const Axis &A = D.ax[DIR];
@@ -237,7 +338,9 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   const int c = seg * SEG - 2 + lane;
   const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, base, seg, c, lds)
                       : load_cell<DIR, MAG>(D, A, base, c);
-  const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
+  // d e m1 m2 m3 b1 b2 b3 and eint (slopes) / phi (face) of the cell
+  const double qv[9] = {q.d, q.e, q.d * q.v[0], q.d * q.v[1], q.d * q.v[2],
+                        q.b[0], q.b[1], q.b[2], q.eint};
 
   // ---- slopes of cell c (interpolation.f90, loop c = -1 .. n) -----------
   double S[8];  // dd de dm1 dm2 dm3 db1 db2 db3 (dphi is always 0)
@@ -246,7 +349,6 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
 #pragma unroll
     for (int s = 0; s < 8; ++s) S[s] = 0.0;
   } else {
-    const double qv[9] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2], q.eint};
     double ql[9], qr[9];
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
@@ -257,109 +359,40 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
     const double xm = A.x[cl - 1], x0 = A.x[cl], xp = A.x[cr + 1];
     const double xim = A.xi[cl - 1], xi0 = A.xi[cl];
     const double ida = A.idx[cl], idb = A.idx[cr + 1];
-    const double hl = x0 - xim, hr = xi0 - x0;
-    double fl[8], fr[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      if (!MAG && s >= 5) {
-        S[s] = 0.0;
-        fl[s] = fr[s] = 0.0;
-        continue;
-      }
-      const double du = (LIM == LIM_MINMOD) ? minmod(ql[s], qv[s], qr[s], ida, idb)
-                                            : modified_mc(ql[s], qv[s], qr[s], xm, x0, xp, xim, xi0);
-      S[s] = du;
-      fl[s] = qv[s] - hl * du;
-      fr[s] = qv[s] + hr * du;
-    }
-    // e_int guard (interpolation.f90:143-163)
-    double eintl = fl[1] - 0.5 * (fl[2] * fl[2] + fl[3] * fl[3] + fl[4] * fl[4]) / fl[0];
-    double eintr = fr[1] - 0.5 * (fr[2] * fr[2] + fr[3] * fr[3] + fr[4] * fr[4]) / fr[0];
-    if (MAG) {
-      eintl = eintl - 0.5 * (fl[5] * fl[5] + fl[6] * fl[6] + fl[7] * fl[7]);
-      eintr = eintr - 0.5 * (fr[5] * fr[5] + fr[6] * fr[6] + fr[7] * fr[7]);
-    }
-    const double el_ = ql[8], em_ = q.eint, er_ = qr[8];
-    if (eintl >= mx2(el_, em_) || eintl <= mn2(el_, em_) || eintr >= mx2(em_, er_) ||
-        eintr <= mn2(em_, er_)) {
-#pragma unroll
-      for (int s = 0; s < 8; ++s) S[s] = 0.0;
-    }
+    cell_slopes<MAG, LIM, SH>(qv, ql, qr, xm, x0, xp, xim, xi0, ida, idb, S);
   }
 
   // ---- face c+1/2 (numflux.f90, faces c = -1 .. n-1) ---------------------
   // right cell's primitives and slopes from lane+1
-  const double rd = dn1(q.d), re = dn1(q.e);
-  const double rm[3] = {dn1(m0), dn1(m1), dn1(m2)};
-  double rb[3] = {0.0, 0.0, 0.0}, rphi = 0.0;
-  if (MAG) {
-    rb[0] = dn1(q.b[0]);
-    rb[1] = dn1(q.b[1]);
-    rb[2] = dn1(q.b[2]);
-    rphi = dn1(q.phi);
-  }
+  double L[9], R_[9];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) L[s] = qv[s];
+  L[8] = q.phi;
+#pragma unroll
+  for (int s = 0; s < 9; ++s) R_[s] = (!MAG && s >= 5) ? 0.0 : dn1(L[s]);
   double RS[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) RS[s] = (!MAG && s >= 5) ? 0.0 : dn1(S[s]);
 
   const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;  // clamp face index
   const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
-  // cyclic rotation: normal a = DIR, then b, c (numflux.f90:73-93,246-266,419-439)
-  constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
-  const double lm[3] = {m0, m1, m2};
-  const double dl = q.d + dx1 * S[0];
-  const double dr = rd - dx2 * RS[0];
-  const double el = q.e + dx1 * S[1];
-  const double er = re - dx2 * RS[1];
-  const double v1l = (lm[a_] + dx1 * S[2 + a_]) / dl;
-  const double v1r = (rm[a_] - dx2 * RS[2 + a_]) / dr;
-  const double v2l = (lm[b_] + dx1 * S[2 + b_]) / dl;
-  const double v2r = (rm[b_] - dx2 * RS[2 + b_]) / dr;
-  const double v3l = (lm[c_] + dx1 * S[2 + c_]) / dl;
-  const double v3r = (rm[c_] - dx2 * RS[2 + c_]) / dr;
-  double b1l = 0, b1r = 0, b2l = 0, b2r = 0, b3l = 0, b3r = 0, phil = 0, phir = 0;
-  if (MAG) {
-    b1l = q.b[a_] + dx1 * S[5 + a_];
-    b1r = rb[a_] - dx2 * RS[5 + a_];
-    b2l = q.b[b_] + dx1 * S[5 + b_];
-    b2r = rb[b_] - dx2 * RS[5 + b_];
-    b3l = q.b[c_] + dx1 * S[5 + c_];
-    b3r = rb[c_] - dx2 * RS[5 + c_];
-    // dphi is identically zero: phi + dx*0 == phi + 0
-    phil = q.phi + 0.0;
-    phir = rphi + 0.0;
-  }
-  const double gam = D.gamma;
-  const double eil = get_eint(MAG, el, dl, v1l, v2l, v3l, b1l, b2l, b3l);
-  const double eir = get_eint(MAG, er, dr, v1r, v2r, v3r, b1r, b2r, b3r);
+  const double gam = D.gamma, ch = D.sc->ch;
+  double t[9], fe[4];
+  face_flux<DIR, MAG, SH>(L, S, R_, RS, dx1, dx2, gam, ch, t, fe);
   // faces between lanes l and l+1 have valid slopes on both sides for l = 1..61
   const bool face_valid = (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
-  if (face_valid && (eil <= 0.0 || dl <= 0.0 || eir <= 0.0 || dr <= 0.0)) {
+  if (face_valid && (fe[1] <= 0.0 || fe[0] <= 0.0 || fe[3] <= 0.0 || fe[2] <= 0.0)) {
     int ii = ci, jj = cj, kk = ck;
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
     atomicMin(&D.sc->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
   }
-  // eos_p_cs (eos.f90:195-242); the erad term 4*0/(9d) adds +0 to a
-  // positive cs2 and is omitted.
-  double ptl = (gam - 1.0) * eil;
-  double ptr = (gam - 1.0) * eir;
-  const double csl = sqrt(gam * ptl / dl);
-  const double csr = sqrt(gam * ptr / dr);
-  if (MAG) {
-    ptl = ptl + 0.5 * (b1l * b1l + b2l * b2l + b3l * b3l);
-    ptr = ptr + 0.5 * (b1r * b1r + b2r * b2r + b3r * b3r);
-  }
-  const double cfl = get_cf(dl, csl, b1l, b2l, b3l);
-  const double cfr = get_cf(dr, csr, b1r, b2r, b3r);
-  double t[9];
-  hlld(t, cfl, cfr, v1l, v1r, v2l, v2r, v3l, v3r, dl, dr, el, er, ptl, ptr, b1l, b1r,
-       b2l, b2r, b3l, b3r, phil, phir, D.sc->ch);
 
   // ---- flux difference of cell c (rungekutta.f90:42-55, one direction) --
   double tm[9];
 #pragma unroll
   for (int s = 0; s < 9; ++s) tm[s] = up1(t[s]);
   // equation order: icnt imo1 imo2 imo3 iene img1 img2 img3 i9wv
+  constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
   const double ix = A.idxi[cc];
   double du[NVMAX];
   du[0] = ix * (tm[0] - t[0]);
@@ -1092,6 +1125,10 @@ __global__ void k_kat(int which, int n, const double *in, double *out) {
     const double *a = in + 2 * (long)t;
     out[2 * (long)t] = a[0] / a[1];
     out[2 * (long)t + 1] = sqrt(a[0]);
+  } else if (which == 5) {  // division by a shared denominator: b a0 a1 a2 a3 -> ai/b
+    const double *a = in + 5 * (long)t;
+    const Den64 Db = mkden<true>(a[0]);
+    for (int s = 0; s < 4; ++s) out[4 * (long)t + s] = qdiv(Db, a[1 + s]);
   }
 }
 

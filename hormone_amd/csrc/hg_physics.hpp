@@ -32,31 +32,95 @@ __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_a
 __device__ __forceinline__ double abs_(double a) { return fabs(a); }
 __device__ __forceinline__ float abs_(float a) { return fabsf(a); }
 
+// ---- division by a shared denominator ---------------------------------------
+// FP64 (SH = true): q = a / b by the same instruction sequence as the
+// compiler's IEEE division -- v_div_scale of b and of a, v_rcp_f64 of the
+// scaled b refined by two Newton steps, v_div_fmas, v_div_fixup -- with the
+// reciprocal of b computed once and shared by every numerator.
+// v_div_scale leaves b unscaled except for operands of extreme exponent (b
+// denormal or beyond 2^1022, a tiny, or a/b near the overflow or denormal
+// range); then the shared reciprocal is not the one a / b would use, and
+// qdiv recomputes that lane's quotient with a plain division.  Where the
+// scaled b is NaN (a or b zero or NaN) v_div_fixup returns the special
+// result (0, inf or NaN) without reading the reciprocal.  Every quotient is
+// therefore bit-identical to a / b.
+struct Den64 {
+  double b, r;
+};
+template <class R> struct DenX {  // plain division (FP64 IEEE, FP32 approximate)
+  R b;
+};
+template <bool SH, class R> __device__ __forceinline__ auto mkden(R b) {
+  if constexpr (SH) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    return Den64{b, r};
+  } else {
+    return DenX<R>{b};
+  }
+}
+__device__ __forceinline__ double qdiv(const Den64 &D, double a) {
+  bool flag, vcc;
+  const double sb = __builtin_amdgcn_div_scale(a, D.b, false, &flag);  // scaled b
+  const double n = __builtin_amdgcn_div_scale(a, D.b, true, &vcc);    // scaled a
+  const double m = n * D.r;
+  const double t = fma(-D.b, m, n);
+  double q = __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(t, D.r, m, vcc), D.b, a);
+  if (__builtin_islessgreater(sb, D.b)) {
+    // rare: keep this a branch (the compiler would otherwise speculate the
+    // full division on every lane and select)
+    double x = a;
+    asm volatile("" : "+v"(x));
+    q = x / D.b;
+  }
+  return q;
+}
+template <class R> __device__ __forceinline__ R qdiv(const DenX<R> &D, R a) {
+  return fdiv(a, D.b);
+}
+
 enum { LIM_FLAT = 0, LIM_MINMOD = 1, LIM_MC = 2 };
 
 // fluxlimiter.f90:40-56 (generalised minmod); dx = idx(i:i+1)
-template <class R>
-__device__ __forceinline__ R minmod(R u0, R u1, R u2, R dxa, R dxb) {
+// Dab: the denominator dxa + dxb (a grid function, shared by every variable
+// of the cell).
+template <class R, class DN>
+__device__ __forceinline__ R minmod_d(R u0, R u1, R u2, R dxa, R dxb, const DN &Dab) {
   R x = R(1.4) * (u2 - u1) * dxb;
   R y = R(1.4) * (u1 - u0) * dxa;
-  R z = fdiv((u2 - u0) * dxa * dxb, dxa + dxb);
+  R z = qdiv(Dab, (u2 - u0) * dxa * dxb);
   R s = sgn(R(1), x);
   return s * mx2(R(0), mn3(abs_(x), s * y, s * z));
+}
+template <class R>
+__device__ __forceinline__ R minmod(R u0, R u1, R u2, R dxa, R dxb) {
+  return minmod_d(u0, u1, u2, dxa, dxb, mkden<false>(dxa + dxb));
 }
 
 // fluxlimiter.f90:58-75 (modified MC).  cf, cb and the centre spacings are
 // pure grid functions; they are evaluated here from the same coordinate
 // operands as in the reference so they round identically.
+//
+// modified_mc_d takes the per-cell grid factors: dxp = xp - x0,
+// cf = (xp - x0)/(xi0 - x0), cb = (x0 - xm)/(x0 - xim) and the shared
+// denominators Dm = x0 - xm and Dp = xp - x0.
+template <class R, class DN>
+__device__ __forceinline__ R modified_mc_d(R u0, R u1, R u2, R dxp, R cf, R cb, const DN &Dm,
+                                           const DN &Dp) {
+  R top = qdiv(Dm, (u1 - u0) * dxp);
+  R bot = u2 - u1;
+  R sig = sgn(R(1), bot);
+  return qdiv(Dp, sig * mx2(R(0), mn3(sig * R(0.5) * (bot + top), sig * cf * bot, sig * cb * top)));
+}
 template <class R>
 __device__ __forceinline__ R modified_mc(R u0, R u1, R u2, R xm, R x0, R xp,
                                          R xim, R xi0) {
   R cf = fdiv(xp - x0, xi0 - x0);
   R cb = fdiv(x0 - xm, x0 - xim);
-  R top = fdiv((u1 - u0) * (xp - x0), x0 - xm);
-  R bot = u2 - u1;
-  R sig = sgn(R(1), bot);
-  return fdiv(sig * mx2(R(0), mn3(sig * R(0.5) * (bot + top), sig * cf * bot, sig * cb * top)),
-              xp - x0);
+  return modified_mc_d(u0, u1, u2, xp - x0, cf, cb, mkden<false>(x0 - xm), mkden<false>(xp - x0));
 }
 
 // get_eint, eos.f90:846-865
@@ -70,24 +134,29 @@ __device__ __forceinline__ R get_eint(bool mag, R etot, R d, R v1, R v2, R v3, R
 }
 
 // get_cf, eos.f90:351-365
-template <class R>
-__device__ __forceinline__ R get_cf(R d, R cs, R b1, R b2, R b3) {
+// get_cf_d takes the density as a shared denominator.
+template <class R, class DN>
+__device__ __forceinline__ R get_cf_d(const DN &Dd, R cs, R b1, R b2, R b3) {
   R asq = cs * cs;
-  R bb1 = fdiv(b1 * b1, d);
-  R bb2 = fdiv(b2 * b2, d);
-  R bb3 = fdiv(b3 * b3, d);
+  R bb1 = qdiv(Dd, b1 * b1);
+  R bb2 = qdiv(Dd, b2 * b2);
+  R bb3 = qdiv(Dd, b3 * b3);
   R bbsq = bb1 + bb2 + bb3;
   return rsqrt_(R(0.5) * (asq + bbsq + rsqrt_((asq + bbsq) * (asq + bbsq) - R(4) * asq * bb1)));
+}
+template <class R>
+__device__ __forceinline__ R get_cf(R d, R cs, R b1, R b2, R b3) {
+  return get_cf_d(mkden<false>(d), cs, b1, b2, b3);
 }
 
 // hlldflux + fstar, hlldflux.f90:19-179, without the radiation component
 // (erl = err = 0 on this path; its flux only feeds irad).  f[0..8] are
-// tmpflux(1..9).
-template <class R>
-__device__ __forceinline__ void hlld(R f[9], R cfl, R cfr, R v1l, R v1r, R v2l, R v2r,
-                                     R v3l, R v3r, R dl, R dr, R el, R er, R ptl,
-                                     R ptr, R b1l, R b1r, R b2l, R b2r, R b3l, R b3r,
-                                     R phil, R phir, R ch) {
+// tmpflux(1..9).  SH: FP64 divisions by shared denominators (qdiv).
+template <bool SH, class R>
+__device__ __forceinline__ void hlld_d(R f[9], R cfl, R cfr, R v1l, R v1r, R v2l, R v2r,
+                                       R v3l, R v3r, R dl, R dr, R el, R er, R ptl,
+                                       R ptr, R b1l, R b1r, R b2l, R b2r, R b3l, R b3r,
+                                       R phil, R phir, R ch) {
   const R h = R(0.5);
   R b1 = h * ((b1l + b1r) - fdiv(phir - phil, ch));
   R phi = h * ((phil + phir) - (b1r - b1l) * ch);
@@ -96,48 +165,48 @@ __device__ __forceinline__ void hlld(R f[9], R cfl, R cfr, R v1l, R v1r, R v2l, 
   R sr = mx3(v1r + cfr, v1l + cfl, R(0));
 
   R slv = sl - v1l, srv = sr - v1r;
-  R den = srv * dr - slv * dl;
-  R sm = fdiv(srv * dr * v1r - slv * dl * v1l - ptr + ptl, den);
+  const auto Dden = mkden<SH>(srv * dr - slv * dl);
+  R sm = qdiv(Dden, srv * dr * v1r - slv * dl * v1l - ptr + ptl);
+  R pta = qdiv(Dden, srv * dr * ptl - slv * dl * ptr + dl * dr * srv * slv * (v1r - v1l));
 
   R slm = sl - sm, srm = sr - sm;
-  R dla = fdiv(dl * slv, slm);
-  R dra = fdiv(dr * srv, srm);
+  const auto Dslm = mkden<SH>(slm), Dsrm = mkden<SH>(srm);
+  R dla = qdiv(Dslm, dl * slv);
+  R dra = qdiv(Dsrm, dr * srv);
 
   R sqla = rsqrt_(dla), sqra = rsqrt_(dra);
   R sla = sm - fdiv(abs_(b1), sqla);
   R sra = sm + fdiv(abs_(b1), sqra);
 
-  R pta = fdiv(srv * dr * ptl - slv * dl * ptr + dl * dr * srv * slv * (v1r - v1l), den);
-
   R b1b1 = b1 * b1;
-  R denl = dl * slv * slm - b1b1;
-  R denr = dr * srv * srm - b1b1;
-  R v2la = v2l - fdiv(b1 * b2l * (sm - v1l), denl);
-  R v2ra = v2r - fdiv(b1 * b2r * (sm - v1r), denr);
-  R v3la = v3l - fdiv(b1 * b3l * (sm - v1l), denl);
-  R v3ra = v3r - fdiv(b1 * b3r * (sm - v1r), denr);
+  const auto Dl = mkden<SH>(dl * slv * slm - b1b1);
+  const auto Dr = mkden<SH>(dr * srv * srm - b1b1);
+  R v2la = v2l - qdiv(Dl, b1 * b2l * (sm - v1l));
+  R v2ra = v2r - qdiv(Dr, b1 * b2r * (sm - v1r));
+  R v3la = v3l - qdiv(Dl, b1 * b3l * (sm - v1l));
+  R v3ra = v3r - qdiv(Dr, b1 * b3r * (sm - v1r));
 
   R numl = dl * slv * slv - b1b1;
   R numr = dr * srv * srv - b1b1;
-  R b2la = fdiv(b2l * numl, denl);
-  R b2ra = fdiv(b2r * numr, denr);
-  R b3la = fdiv(b3l * numl, denl);
-  R b3ra = fdiv(b3r * numr, denr);
+  R b2la = qdiv(Dl, b2l * numl);
+  R b2ra = qdiv(Dr, b2r * numr);
+  R b3la = qdiv(Dl, b3l * numl);
+  R b3ra = qdiv(Dr, b3r * numr);
 
-  R ela = fdiv(slv * el - ptl * v1l + pta * sm +
-                   b1 * ((v1l * b1 + v2l * b2l + v3l * b3l) - (sm * b1 + v2la * b2la + v3la * b3la)),
-               slm);
-  R era = fdiv(srv * er - ptr * v1r + pta * sm +
-                   b1 * ((v1r * b1 + v2r * b2r + v3r * b3r) - (sm * b1 + v2ra * b2ra + v3ra * b3ra)),
-               srm);
+  R ela = qdiv(Dslm, slv * el - ptl * v1l + pta * sm +
+                         b1 * ((v1l * b1 + v2l * b2l + v3l * b3l) -
+                               (sm * b1 + v2la * b2la + v3la * b3la)));
+  R era = qdiv(Dsrm, srv * er - ptr * v1r + pta * sm +
+                         b1 * ((v1r * b1 + v2r * b2r + v3r * b3r) -
+                               (sm * b1 + v2ra * b2ra + v3ra * b3ra)));
 
   R sb1 = sgn(R(1), b1);
-  R sqs = sqla + sqra;
-  R v2aa = fdiv(sqla * v2la + sqra * v2ra + (b2ra - b2la) * sb1, sqs);
-  R v3aa = fdiv(sqla * v3la + sqra * v3ra + (b3ra - b3la) * sb1, sqs);
+  const auto Dsqs = mkden<SH>(sqla + sqra);
+  R v2aa = qdiv(Dsqs, sqla * v2la + sqra * v2ra + (b2ra - b2la) * sb1);
+  R v3aa = qdiv(Dsqs, sqla * v3la + sqra * v3ra + (b3ra - b3la) * sb1);
   R sqp = rsqrt_(dla * dra);
-  R b2aa = fdiv(sqla * b2ra + sqra * b2la + sqp * (v2ra - v2la) * sb1, sqs);
-  R b3aa = fdiv(sqla * b3ra + sqra * b3la + sqp * (v3ra - v3la) * sb1, sqs);
+  R b2aa = qdiv(Dsqs, sqla * b2ra + sqra * b2la + sqp * (v2ra - v2la) * sb1);
+  R b3aa = qdiv(Dsqs, sqla * b3ra + sqra * b3la + sqp * (v3ra - v3la) * sb1);
 
   R vbaa = v2aa * b2aa + v3aa * b3aa;
   R eraa = era + sb1 * sqra * ((v2ra * b2ra + v3ra * b3ra) - vbaa);
@@ -167,6 +236,14 @@ __device__ __forceinline__ void hlld(R f[9], R cfl, R cfr, R v1l, R v1r, R v2l, 
   f[6] = b2_ * sm - b1 * v2_;
   f[7] = b3_ * sm - b1 * v3_;
   f[8] = ch * ch * b1;
+}
+template <class R>
+__device__ __forceinline__ void hlld(R f[9], R cfl, R cfr, R v1l, R v1r, R v2l, R v2r,
+                                     R v3l, R v3r, R dl, R dr, R el, R er, R ptl,
+                                     R ptr, R b1l, R b1r, R b2l, R b2r, R b3l, R b3r,
+                                     R phil, R phir, R ch) {
+  hlld_d<false>(f, cfl, cfr, v1l, v1r, v2l, v2r, v3l, v3r, dl, dr, el, er, ptl, ptr, b1l, b1r,
+                b2l, b2r, b3l, b3r, phil, phir, ch);
 }
 
 // ---- FP32-mixed HLLD in perturbation form ----------------------------------

@@ -237,8 +237,9 @@ int hg_last_error(const hg_ctx *ctx, int32_t *i, int32_t *j, int32_t *k,
    which: 0 hlldflux (25 inputs in hlldflux.f90:19-21 argument order
    cfl..ch,erl,err -> 9 outputs tmpflux(1:9)), 1 get_cf (d cs b1 b2 b3 -> cf),
    2 modified_mc (u(1:3) x(1:3) xi(1:2) -> slope), 3 minmod (u(1:3) dx(1:2)
-   -> slope), 4 IEEE a/b and sqrt(a) (a b -> a/b sqrt(a)).  n records,
-   row-major.  For tests of bit-exactness against the reference. */
+   -> slope), 4 IEEE a/b and sqrt(a) (a b -> a/b sqrt(a)), 5 the sweep's
+   division by a shared denominator (b a0 a1 a2 a3 -> a0/b .. a3/b).  n
+   records, row-major.  For tests of bit-exactness against the reference. */
 int hg_kat(int32_t which, int32_t n, const double *in, double *out);
 
 #ifdef __cplusplus

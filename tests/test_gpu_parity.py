@@ -76,6 +76,40 @@ def test_kat_div_sqrt_correctly_rounded(hiplib):
     assert np.array_equal(out[:, 1].view(np.uint64), np.sqrt(a).view(np.uint64))
 
 
+def _division_operands(rng, n):
+    """Denominators and numerators over the whole double range: ordinary
+    values, zeros of both signs, denormals, values next to the overflow and
+    denormal thresholds, infinities and NaNs, and quotients that overflow or
+    underflow -- the operands for which v_div_scale rescales."""
+    special = np.array([0.0, -0.0, 5e-324, -5e-324, 1e-310, 2.0 ** -1022, 2.0 ** -1000,
+                        2.0 ** -970, 2.0 ** -960, 1.0, -1.0, 3.0, 2.0 ** 767, 2.0 ** 1000,
+                        2.0 ** 1022, 1.7e308, np.inf, -np.inf, np.nan])
+    wide = np.exp(rng.uniform(-740, 709, n)) * rng.choice([-1.0, 1.0], n)
+    b = np.where(rng.random(n) < 0.15, rng.choice(special, n), wide)
+    a = np.exp(rng.uniform(-740, 709, (n, 4))) * rng.choice([-1.0, 1.0], (n, 4))
+    pick = rng.random((n, 4))
+    a = np.where(pick < 0.15, rng.choice(special, (n, 4)), a)
+    with np.errstate(all="ignore"):
+        near = b[:, None] * np.exp2(rng.choice([-1080.0, -1060.0, -1030.0, 760.0, 800.0, 1100.0],
+                                               (n, 4)))
+    a = np.where((pick >= 0.15) & (pick < 0.25), near, a)
+    a = np.where((pick >= 0.25) & (pick < 0.35), rng.uniform(-4, 4, (n, 4)), a)
+    return b, a
+
+
+def test_kat_shared_denominator_division_is_plain_division(hiplib):
+    """The sweep divides several numerators by one denominator with a shared
+    reciprocal; every quotient must equal the correctly rounded a / b."""
+    rng = np.random.default_rng(11)
+    b, a = _division_operands(rng, 200_000)
+    out = _kat(hiplib, 5, np.concatenate([b[:, None], a], 1), 4)
+    with np.errstate(all="ignore"):
+        ref = a / b[:, None]
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(out), nan)
+    assert np.array_equal(out[~nan].view(np.uint64), ref[~nan].view(np.uint64))
+
+
 @pytest.mark.parametrize("flat", [False, True], ids=["muscl", "flat"])
 @pytest.mark.parametrize("name", ONE_D)
 def test_gpu_1d_goldens(hiplib, name, flat):
