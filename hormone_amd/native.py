@@ -114,6 +114,7 @@ def _load():
     lib.hg_gravity.argtypes = [vp, C.c_double, C.c_double, _dp, C.c_double, _dp, _dp,
                                C.POINTER(C.c_int32)]
     lib.hg_set_launch.argtypes = [vp, C.c_int32]
+    lib.hg_time_sweep.argtypes = [vp, C.c_int32, _dp]
     return lib
 
 
@@ -132,7 +133,7 @@ def exported_symbols():
     return ["hg_version", "hg_create", "hg_destroy", "hg_ufnmax", "hg_upload",
             "hg_download", "hg_timestep", "hg_hydro_step", "hg_phidamp",
             "hg_last_error", "hg_run", "hg_profile", "hg_profile_read", "hg_kat",
-            "hg_gravity", "hg_set_launch"]
+            "hg_gravity", "hg_set_launch", "hg_time_sweep"]
 
 
 STOP_END, STOP_OUTPUT, STOP_MAXSTEPS, STOP_ERROR = 1, 2, 3, 4
@@ -222,6 +223,13 @@ class Hydro:
 
     def profile(self, enable=True):
         self._check(self._lib.hg_profile(self.h, int(enable)), "hg_profile")
+
+    def time_sweep(self, launches):
+        """Milliseconds of `launches` back-to-back sweeps of the current state
+        (no per-dispatch events); the state is unchanged."""
+        ms = C.c_double()
+        self._check(self._lib.hg_time_sweep(self.h, int(launches), C.byref(ms)), "hg_time_sweep")
+        return ms.value
 
     def profile_read(self, kind):
         ms, n, b = C.c_double(), C.c_int64(), C.c_double()

@@ -193,6 +193,13 @@ int hg_profile(hg_ctx *ctx, int32_t enable);
 int hg_profile_read(const hg_ctx *ctx, int32_t kind, double *ms_total,
                     int64_t *launches, double *bytes_per_launch);
 
+/* The sweep kernel's duration without per-dispatch events: `launches`
+   back-to-back sweeps of the current state between two events on the
+   context's stream; *ms_total = their elapsed milliseconds.  The state is
+   unchanged (the sweep overwrites only the flux divergences).  No
+   counterpart in the reference; for measurement. */
+int hg_time_sweep(hg_ctx *ctx, int32_t launches, double *ms_total);
+
 /* ---- batches of independent boxes (no counterpart in the reference) ------
    Many contexts of possibly different sizes and dimensions, sharing one
    kernel variant (mag_on, flux_limiter, precision, rktype; no gravity),
