@@ -448,6 +448,8 @@ def main():
         advance_to(r64, run.tim)
         ref = D.fetch(eng64, g64, fields + (D.F_P,))
         extra["rel_l1_vs_fp64"] = D.rel_l1_errors(mine, ref)
+        extra["reference_norms_vs_fp64"] = D.reference_norms(
+            mine, {f: ref[f] for f in fields}, g64, prm.gamma, bool(prm.mag_on))
         # velocity error in units of the local sound speed (the relevant
         # scale for a near-hydrostatic flow, where |v| itself is noise)
         cs = np.sqrt(prm.gamma * ref[D.F_P] / ref[D.F_D])

@@ -82,6 +82,20 @@ def rel_l1_errors(mine: dict, ref: dict) -> dict:
     return out
 
 
+def reference_norms(mine: dict, ref: dict, g: Grid, gamma: float, mag_on: bool) -> dict:
+    """The reference's own test norms (tests.f90:198-298: max, "L1", and
+    discontinuity-weighted L2) of a snapshot against another run, keyed
+    {norm: {variable: error}}; see hormone_amd/testnorms.py."""
+    from . import testnorms as T
+    names = {F_D: "d", F_V1: "v1", F_V2: "v2", F_V3: "v3", F_E: "e", F_B1: "b1",
+             F_B2: "b2", F_B3: "b3", F_PHI: "phi"}
+    sh = g.shape
+    a = {names[f]: np.asarray(v).reshape(sh) for f, v in mine.items() if f in names}
+    b = {names[f]: np.asarray(v).reshape(sh) for f, v in ref.items() if f in names}
+    _, errs, _ = T.reference_criterion(a, b, gamma, T.cell_volumes(g), 0.0, mag_on, g.dim)
+    return errs
+
+
 @dataclass
 class RunResult:
     tn: int

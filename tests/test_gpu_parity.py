@@ -12,8 +12,8 @@ import pytest
 
 from hormone_amd.native import Hydro
 from oracle.pyoracle import Oracle, lib as oracle_lib
-from tests.helpers import (EXTRA_CASES, compare_to_meta, compare_to_npz, digest, meta,
-                           run_extra, run_problem)
+from tests.helpers import (EXTRA_CASES, GOLDEN, compare_to_meta, compare_to_npz, digest,
+                           meta, run_extra, run_problem)
 
 pytestmark = pytest.mark.gpu
 
@@ -116,6 +116,16 @@ def test_gpu_1d_goldens(hiplib, name, flat):
     print(f"case {name} flat={flat}", flush=True)
     res, state, _ = run_problem(Hydro, name, flat=flat)
     compare_to_npz(res, state, name + ("_flat" if flat else ""))
+    # and the reference's own test criterion (tests.f90:48-178) says Identical
+    from hormone_amd import problems as P
+    from hormone_amd import testnorms as T
+    prm, g, _ = P.make_problem(name)
+    z = np.load(f"{GOLDEN}/{name}{'_flat' if flat else ''}.npz")
+    golden = {k: z[k].reshape(g.shape) for k in state}
+    mine = {k: v.reshape(g.shape) for k, v in state.items()}
+    passed, errs, identical = T.reference_criterion(
+        mine, golden, prm.gamma, T.cell_volumes(g), T.tolerance_for(name), prm.mag_on, g.dim)
+    assert passed and identical, errs
 
 
 @pytest.mark.parametrize("flat", [False, True], ids=["muscl", "flat"])
@@ -299,3 +309,32 @@ def test_gpu_reflective_briowu_stops_like_the_reference(hiplib):
         D.run(eng, prm, g, max_steps=1)
     assert ei.value.code == HG_ERR_NEG_EINT
     assert list(ei.value.cell) == fail["cell"]
+
+
+def test_time_sweep_leaves_the_state_unchanged(hiplib):
+    """hg_time_sweep (the benchmark's kernel timing) replays sweeps between
+    steps: the run continues bit for bit as if it had not been called, also
+    after a replay on a run that has reached its end."""
+    import bench as B
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+
+    def start():
+        prm, g, st = P.make_problem("orszagtang_xy", n=64, outstyle=2, endstyle=2,
+                                    tnlim=30, tn_out=30)
+        eng = Hydro(**D.desc_fields(prm, g))
+        D.load_initial_state(eng, prm, g, st)
+        return g, eng, B.Runner(eng, prm, g)
+
+    g, e1, r1 = start()
+    r1.advance(10)
+    assert e1.time_sweep(7) > 0.0
+    r1.advance(20)
+    e1.time_sweep(3)  # at the end of the run (stop flag set)
+    g, e2, r2 = start()
+    r2.advance(30)
+    assert (r1.tn, r1.tim) == (r2.tn, r2.tim)
+    a = D.fetch(e1, g, D.SNAP_FIELDS_MAG)
+    b = D.fetch(e2, g, D.SNAP_FIELDS_MAG)
+    for f in D.SNAP_FIELDS_MAG:
+        assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
