@@ -57,7 +57,9 @@ static int dev_alloc(hg_ctx *c, size_t bytes, void **p) {
 
 static int validate(const hg_desc *d) {
   if (!d) return HG_ERR_ARG;
-  if (d->crdnt != 0 || d->eostype != 0 || d->compswitch != 0) return HG_ERR_UNSUPPORTED;
+  if (d->crdnt != 0 || d->eostype != 0 || (d->compswitch != 0 && d->compswitch != 2))
+    return HG_ERR_UNSUPPORTED;
+  if (d->compswitch == 2 && (d->spn < 1 || d->spn > HG_SPN_MAX)) return HG_ERR_ARG;
   if (d->ie < d->is || d->je < d->js || d->ke < d->ks) return HG_ERR_ARG;
   if (d->rktype < 1 || d->rktype > 3) return HG_ERR_ARG;
   if (d->flux_limiter < 0 || d->flux_limiter > 2) return HG_ERR_ARG;
@@ -202,6 +204,20 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
       if (D.solve[a])
         if ((st = dev_alloc(c, fb, (void **)&D.DU[a][v]))) return fail(st);
   }
+  D.nspc = desc->compswitch == 2 ? desc->spn : 0;
+  if (D.nspc) {
+    const size_t sb = fb * (size_t)D.nspc;
+    if ((st = dev_alloc(c, sb, (void **)&D.SPC))) return fail(st);
+    if ((st = dev_alloc(c, sb, (void **)&D.SPC0))) return fail(st);
+    if ((st = dev_alloc(c, sb * 3, (void **)&D.DUS))) return fail(st);
+    if (hipMemsetAsync(D.SPC, 0, sb, c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
+    for (int a = 0; a < 3; ++a)
+      if (D.solve[a]) {
+        const long nl = D.ncell / nn[a];
+        if ((st = dev_alloc(c, sizeof(double) * (size_t)nl * (nn[a] + 1), (void **)&D.FM[a])))
+          return fail(st);
+      }
+  }
   D.W[WD] = D.U[0];
   D.W[WE] = D.U[4];
   if (D.mag)
@@ -302,6 +318,8 @@ static double *field_dev(hg_ctx *c, int field, bool upload) {
     case HG_F_GRVPSI: return D.grav ? D.gpsi : nullptr;
     default:
       if (field >= HG_F_U && field < HG_F_U + D.nv) return D.U[field - HG_F_U];
+      if (field >= HG_F_SPC && field < HG_F_SPC + D.nspc)
+        return D.SPC + (size_t)(field - HG_F_SPC) * D.ncell;
       return nullptr;
   }
 }
@@ -616,6 +634,7 @@ static void enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
     double faco, fact, facn;
     runge_coeff(stage, rk, &faco, &fact, &facn);
     launch_sweep(D, stage, c->stream, prof_ev(c, HG_PROF_SWEEP));
+    launch_species(D, c->stream);
     const bool last = stage == rk;
     launch_rk(D, stage, faco, fact, facn, fuse_tail && last, fuse_tail && last && damp > 0.0,
               damp, c->stream, prof_ev(c, HG_PROF_RK));
@@ -808,7 +827,7 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
                             double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
                             int64_t tn_out) {
   if (!b || !c) return HG_ERR_ARG;
-  if (c->device != b->device || c->D.grav) return HG_ERR_UNSUPPORTED;
+  if (c->device != b->device || c->D.grav || c->D.nspc) return HG_ERR_UNSUPPORTED;
   for (hg_ctx *m : b->members)
     if (m == c) return HG_ERR_ARG;
   if (!b->members.empty() &&

@@ -314,7 +314,19 @@ __device__ __forceinline__ void face_flux(const double L[9], const double S[8],
              b2l, b2r, b3l, b3r, phil, phir, ch);
 }
 
-template <int DIR, bool MAG, int LIM, bool TILE = false, bool SH = true>
+// Mass flux of the face c+1/2 for the species pass (compswitch 2): FM[d]
+// holds n+1 faces per line, face c+1/2 at line*(n+1) + c+1 (c = -1..n-1).
+// Lanes 1..61 cover their segment's faces; the face two segments share is
+// written twice with the same value.
+template <int DIR>
+__device__ __forceinline__ void store_mass_flux(const Dev &D, long line, int c, int lane,
+                                                double f0) {
+  const int n = D.ax[DIR].n;
+  if (lane >= 1 && lane <= 61 && c >= -1 && c <= n - 1)
+    D.FM[DIR][line * (n + 1) + (c + 1)] = f0;
+}
+
+template <int DIR, bool MAG, int LIM, bool TILE = false, bool SPC = false>
 __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage,
                                            double *lds = nullptr) {
   const Axis &A = D.ax[DIR];
@@ -359,7 +371,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
     const double xm = A.x[cl - 1], x0 = A.x[cl], xp = A.x[cr + 1];
     const double xim = A.xi[cl - 1], xi0 = A.xi[cl];
     const double ida = A.idx[cl], idb = A.idx[cr + 1];
-    cell_slopes<MAG, LIM, SH>(qv, ql, qr, xm, x0, xp, xim, xi0, ida, idb, S);
+    cell_slopes<MAG, LIM, true>(qv, ql, qr, xm, x0, xp, xim, xi0, ida, idb, S);
   }
 
   // ---- face c+1/2 (numflux.f90, faces c = -1 .. n-1) ---------------------
@@ -378,7 +390,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
   const double gam = D.gamma, ch = D.sc->ch;
   double t[9], fe[4];
-  face_flux<DIR, MAG, SH>(L, S, R_, RS, dx1, dx2, gam, ch, t, fe);
+  face_flux<DIR, MAG, true>(L, S, R_, RS, dx1, dx2, gam, ch, t, fe);
   // faces between lanes l and l+1 have valid slopes on both sides for l = 1..61
   const bool face_valid = (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
   if (face_valid && (fe[1] <= 0.0 || fe[0] <= 0.0 || fe[3] <= 0.0 || fe[2] <= 0.0)) {
@@ -405,6 +417,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   du[5 + c_] = ix * (tm[7] - t[7]);
   du[8] = ix * (tm[8] - t[8]);
   store_du<DIR, MAG, TILE>(D, A, base, seg, c, lane, du, lds);
+  if (SPC) store_mass_flux<DIR>(D, line, c, lane, t[0]);
 }
 
 // One launch covers every solved direction: wavefronts [0, waves[0]) sweep
@@ -415,22 +428,23 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
 // launcher checks that a block's lines have consecutive x index); a
 // separate instantiation keeps each kernel's code within the instruction
 // cache.
-template <bool MAG, int LIM, bool TILE>
+template <bool MAG, int LIM, bool TILE, bool SPC = false>
 __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
   extern __shared__ double lds[];
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave < D.woff[1]) {
-    if (wave < D.waves[0]) sweep_line<0, MAG, LIM>(D, wave, lane, stage);
+    if (wave < D.waves[0]) sweep_line<0, MAG, LIM, false, SPC>(D, wave, lane, stage);
     return;
   }
   if (wave < D.woff[2]) {
     if (wave - D.woff[1] >= D.waves[1]) return;  // padding: whole blocks when tiled
-    sweep_line<1, MAG, LIM, TILE>(D, wave - D.woff[1], lane, stage, lds);
+    sweep_line<1, MAG, LIM, TILE, SPC>(D, wave - D.woff[1], lane, stage, lds);
     return;
   }
-  if (wave - D.woff[2] < D.waves[2]) sweep_line<2, MAG, LIM, TILE>(D, wave - D.woff[2], lane, stage, lds);
+  if (wave - D.woff[2] < D.waves[2])
+    sweep_line<2, MAG, LIM, TILE, SPC>(D, wave - D.woff[2], lane, stage, lds);
 }
 
 // ---- FP32-mixed sweep -------------------------------------------------------
@@ -466,7 +480,7 @@ __device__ __forceinline__ double minmod_geo(double dm, double dp, double ida, d
   return s * mx2(0.0, mn3(fabs(x), s * y, s * z));
 }
 
-template <int DIR, bool MAG, int LIM, bool TILE = false>
+template <int DIR, bool MAG, int LIM, bool TILE = false, bool SPC = false>
 __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane, int stage,
                                                double *lds = nullptr) {
   const Axis &A = D.ax[DIR];
@@ -618,24 +632,106 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   du[5 + c_] = ix * (tm[7] - t[7]);
   du[8] = ix * (tm[8] - t[8]);
   store_du<DIR, MAG, TILE>(D, A, base, seg, c, lane, du, lds);
+  if (SPC) store_mass_flux<DIR>(D, line, c, lane, t[0]);
 }
 
-template <bool MAG, int LIM, bool TILE>
+template <bool MAG, int LIM, bool TILE, bool SPC = false>
 __global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
   extern __shared__ double lds[];
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave < D.woff[1]) {
-    if (wave < D.waves[0]) sweep_line_mix<0, MAG, LIM>(D, wave, lane, stage);
+    if (wave < D.waves[0]) sweep_line_mix<0, MAG, LIM, false, SPC>(D, wave, lane, stage);
     return;
   }
   if (wave < D.woff[2]) {
     if (wave - D.woff[1] >= D.waves[1]) return;  // padding: whole blocks when tiled
-    sweep_line_mix<1, MAG, LIM, TILE>(D, wave - D.woff[1], lane, stage, lds);
+    sweep_line_mix<1, MAG, LIM, TILE, SPC>(D, wave - D.woff[1], lane, stage, lds);
     return;
   }
-  if (wave - D.woff[2] < D.waves[2]) sweep_line_mix<2, MAG, LIM, TILE>(D, wave - D.woff[2], lane, stage, lds);
+  if (wave - D.woff[2] < D.waves[2])
+    sweep_line_mix<2, MAG, LIM, TILE, SPC>(D, wave - D.woff[2], lane, stage, lds);
+}
+
+// ---- advected species (compswitch 2) -----------------------------------------
+// One wavefront per 64-cell segment of a line, as the sweeps: the species'
+// limited slopes (interpolation.f90:135-141; the e_int guard does not reset
+// them), face values, upwinding with the face's mass flux (stored by the
+// sweep) and renormalisation by the upwind sum (numflux.f90:200-212), and
+// the flux divergence of each cell, idxi*(F(c-1/2) - F(c+1/2))
+// (rungekutta.f90:66-77 with detg = 1).  FP64 in both precision modes.
+template <int DIR, int LIM>
+__device__ __forceinline__ void species_line(const Dev &D, long wave, int lane) {
+  const Axis &A = D.ax[DIR];
+  const int n = A.n;
+  const long nlines = D.ncell / n;
+  const long line = wave % nlines;
+  const int seg = (int)(wave / nlines);
+  long base;
+  if (DIR == 0) base = line * D.n1;
+  else if (DIR == 1) base = line % D.n1 + (line / D.n1) * (long)D.n1 * D.n2;
+  else base = line;
+  const int c = seg * SEG - 2 + lane;
+  const int cc = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
+  long ms, mv;
+  cell_sources(A, base, c, ms, mv);
+  const int cl = cc > -1 ? cc : -1, cr = cc < n ? cc : n;  // as the hydro slopes
+  const double xm = A.x[cl - 1], x0 = A.x[cl], xp = A.x[cr + 1];
+  const double xim = A.xi[cl - 1], xi0 = A.xi[cl];
+  const double ida = A.idx[cl], idb = A.idx[cr + 1];
+  const double dxp = xp - x0;
+  const double cf = dxp / (xi0 - x0), cb = (x0 - xm) / (x0 - xim);
+  const auto Dm = mkden<true>(LIM == LIM_MINMOD ? ida + idb : x0 - xm);
+  const auto Dp = mkden<true>(dxp);
+  const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;  // face index, as the sweep
+  const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
+  const long N = D.ncell;
+  double sl[HG_SPN_MAX], sr[HG_SPN_MAX];
+  double suml = 0.0, sumr = 0.0;  // sum(spcl(1:spn)), sum(spcr(1:spn))
+#pragma unroll
+  for (int q = 0; q < HG_SPN_MAX; ++q) {
+    if (q >= D.nspc) break;
+    const double v = D.SPC[q * N + ms];
+    double sp = 0.0;
+    if (LIM != LIM_FLAT) {
+      const double vl = up1(v), vr = dn1(v);
+      sp = LIM == LIM_MINMOD ? minmod_d(vl, v, vr, ida, idb, Dm)
+                             : modified_mc_d(vl, v, vr, dxp, cf, cb, Dm, Dp);
+    }
+    sl[q] = v + dx1 * sp;
+    sr[q] = dn1(v) - dx2 * dn1(sp);
+    suml = suml + sl[q];
+    sumr = sumr + sr[q];
+  }
+  const bool face = c >= -1 && c <= n - 1;
+  const double f0 = face ? D.FM[DIR][line * (n + 1) + (c + 1)] : 0.0;
+  const double sg = sgn(0.5, f0);
+  const double fix = 1.0 / (suml * (0.5 + sg) + sumr * (0.5 - sg));
+  const double ix = A.idxi[cc];
+  const bool own = lane >= 2 && lane <= 61 && c >= 0 && c < n;
+  const long m = base + (long)c * A.stride;
+#pragma unroll
+  for (int q = 0; q < HG_SPN_MAX; ++q) {
+    if (q >= D.nspc) break;
+    const double fl = fix * f0 * (sl[q] * (0.5 + sg) + sr[q] * (0.5 - sg));
+    const double dv = ix * (up1(fl) - fl);
+    if (own) D.DUS[((long)DIR * D.nspc + q) * N + m] = dv;
+  }
+}
+
+template <int LIM>
+__global__ __launch_bounds__(256) void k_species(Dev D) {
+  if (D.sc->stop) return;
+  const int lane = threadIdx.x & 63;
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave < D.woff[1]) {
+    if (wave < D.waves[0]) species_line<0, LIM>(D, wave, lane);
+  } else if (wave < D.woff[2]) {
+    if (wave - D.woff[1] < D.waves[1]) species_line<1, LIM>(D, wave - D.woff[1], lane);
+  } else if (wave - D.woff[2] < D.waves[2]) {
+    species_line<2, LIM>(D, wave - D.woff[2], lane);
+  }
 }
 
 // ---- CFL condition of one cell (timestep.f90:35-57, dti_cell :125-183) ----
@@ -743,6 +839,7 @@ __device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double 
   double src[NVMAX];
   if (GRAV) grav_source(D, m, src);
   double un[NVMAX];
+  double dold = 0.0;  // stage-start density (primitive d = u(icnt))
 #pragma unroll
   for (int v = 0; v < NVMAX; ++v) {
     if (v >= D.nv) break;
@@ -761,6 +858,26 @@ __device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double 
     }
     un[v] = faco * u0 + facn * u + ft * acc;
     D.U[v][m] = un[v];
+    if (v == 0) dold = u;
+  }
+  // species (rungekutta.f90:61-85): spcorg = spc*uorg(icnt) at stage 1;
+  // spc = (faco*spcorg + facn*spc*d + fact*dt*(divergences)) / u(icnt)
+  const long N = D.ncell;
+#pragma unroll
+  for (int q = 0; q < HG_SPN_MAX; ++q) {
+    if (q >= D.nspc) break;
+    const double x = D.SPC[q * N + m];
+    double x0;
+    if (stage == 1) {
+      x0 = x * dold;
+      D.SPC0[q * N + m] = x0;
+    } else {
+      x0 = D.SPC0[q * N + m];
+    }
+    const double a = D.solve[0] ? D.DUS[(0L * D.nspc + q) * N + m] : 0.0;
+    const double b = D.solve[1] ? D.DUS[(1L * D.nspc + q) * N + m] : 0.0;
+    const double cz = D.solve[2] ? D.DUS[(2L * D.nspc + q) * N + m] : 0.0;
+    D.SPC[q * N + m] = (faco * x0 + facn * x * dold + ft * ((a + b) + cz)) / un[0];
   }
   // primitive (rungekutta.f90:193-232) + internalenergy (eos.f90:805-842)
   const double d = un[0];
@@ -1205,10 +1322,12 @@ template <bool MAG, int LIM>
 static void sweep_variant(const Dev &D, int stage, unsigned g, int bs, size_t tile,
                           hipStream_t s, const KEv *ev) {
   if (D.mixed) {
-    if (tile) klaunch_lds(k_sweep_mix<MAG, LIM, true>, g, bs, tile, s, ev, D, stage);
+    if (D.nspc) klaunch_lds(k_sweep_mix<MAG, LIM, false, true>, g, bs, 0, s, ev, D, stage);
+    else if (tile) klaunch_lds(k_sweep_mix<MAG, LIM, true>, g, bs, tile, s, ev, D, stage);
     else klaunch_lds(k_sweep_mix<MAG, LIM, false>, g, bs, 0, s, ev, D, stage);
   } else {
-    if (tile) klaunch_lds(k_sweep<MAG, LIM, true>, g, bs, tile, s, ev, D, stage);
+    if (D.nspc) klaunch_lds(k_sweep<MAG, LIM, false, true>, g, bs, 0, s, ev, D, stage);
+    else if (tile) klaunch_lds(k_sweep<MAG, LIM, true>, g, bs, tile, s, ev, D, stage);
     else klaunch_lds(k_sweep<MAG, LIM, false>, g, bs, 0, s, ev, D, stage);
   }
 }
@@ -1229,6 +1348,14 @@ void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev) {
     else if (D.limiter == LIM_MINMOD) sweep_variant<false, LIM_MINMOD>(D, stage, g, bs, tile, s, ev);
     else sweep_variant<false, LIM_MC>(D, stage, g, bs, tile, s, ev);
   }
+}
+
+void launch_species(const Dev &D, hipStream_t s) {
+  if (!D.nspc) return;
+  const unsigned g = nblk(D.woff[2] + D.waves[2], 4);
+  if (D.limiter == LIM_FLAT) klaunch(k_species<LIM_FLAT>, g, 256, s, nullptr, D);
+  else if (D.limiter == LIM_MINMOD) klaunch(k_species<LIM_MINMOD>, g, 256, s, nullptr, D);
+  else klaunch(k_species<LIM_MC>, g, 256, s, nullptr, D);
 }
 
 template <bool MAG, bool GRAV>

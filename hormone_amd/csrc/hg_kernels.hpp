@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/hormone_hip.h"
+
 namespace hg {
 
 enum { WD = 0, WV1, WV2, WV3, WE, WEINT, WB1, WB2, WB3, WPHI, NWMAX };
@@ -61,6 +63,12 @@ struct Dev {
   double *W[NWMAX];
   double *U[NVMAX], *U0[NVMAX];
   double *DU[3][NVMAX];
+  // advected species (compswitch 2): mass fractions SPC[n*ncell + m], the
+  // stage-1 partial densities SPC0 (spcorg) and the per-direction species
+  // flux divergence DUS[(d*nspc + n)*ncell + m]
+  int nspc;
+  double *SPC, *SPC0, *DUS;
+  double *FM[3];  // face mass flux per direction, (n+1) faces per line
   double *part;    // CFL partials: 2 doubles per block of 256 cells
   Scalars *sc;
   int nblk_cfl;
@@ -136,6 +144,7 @@ struct KEv {
 };
 
 void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev = nullptr);
+void launch_species(const Dev &D, hipStream_t s);
 size_t sweep_tile_bytes(const Dev &D, int bw);
 // cfl: also the next step's CFL partials; damp: phidamp
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,

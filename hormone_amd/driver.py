@@ -22,7 +22,7 @@ from . import problems as P
 from .grid import Grid
 
 F_D, F_V1, F_V2, F_V3, F_E, F_B1, F_B2, F_B3, F_PHI, F_EINT, F_P = range(11)
-F_GRVPHI, F_GRVPSI, F_U = 11, 12, 16
+F_GRVPHI, F_GRVPSI, F_U, F_SPC = 11, 12, 16, 32
 
 
 def desc_fields(prm: P.Params, g: Grid) -> dict:
@@ -33,7 +33,8 @@ def desc_fields(prm: P.Params, g: Grid) -> dict:
                 alpha9wave=prm.alpha9wave, gravswitch=prm.gravswitch,
                 grktype=prm.grktype, HGfac=prm.HGfac, hgcfl=prm.hgcfl,
                 alphagrv=prm.alphagrv, G=prm.G, maxtngrv=prm.maxtngrv,
-                precision=1 if prm.precision == "fp32mix" else 0)
+                precision=1 if prm.precision == "fp32mix" else 0,
+                compswitch=prm.compswitch, spn=prm.spn)
 
 
 def load_initial_state(eng, prm: P.Params, g: Grid, st: P.State):
@@ -47,6 +48,9 @@ def load_initial_state(eng, prm: P.Params, g: Grid, st: P.State):
         eng.upload(f, g.pad(a))
     for n, a in enumerate(u):
         eng.upload(F_U + n, g.pad(a))
+    if prm.compswitch == 2:
+        for n, a in enumerate(st.extra["spc"]):
+            eng.upload(F_SPC + n, g.pad(a))
     if prm.gravswitch == 3:
         eng.upload(F_GRVPHI, st.extra["grvphi_padded"])
         eng.upload(F_GRVPSI, np.zeros(g.padded_shape))

@@ -174,7 +174,7 @@ program hg_driver
  desc%gravswitch = gravswitch ; desc%grktype = grktype
  desc%HGfac = HGfac ; desc%hgcfl = hgcfl ; desc%alphagrv = alphagrv ; desc%G = gconst
  desc%maxtngrv = maxtngrv ; desc%precision = HG_PREC_FP64 ; desc%device = device
- desc%reserved = 0
+ desc%spn = spn
 
  st = hg_create(desc, ctx)
  call hg_check(ctx, st, 'hg_create')

@@ -33,7 +33,7 @@ module hormone_hip_mod
   real(c_double):: gamma, courant, alpha9wave
   integer(c_int32_t):: gravswitch, grktype
   real(c_double):: HGfac, hgcfl, alphagrv, G
-  integer(c_int32_t):: maxtngrv, precision, device, reserved
+  integer(c_int32_t):: maxtngrv, precision, device, spn
  end type hg_desc
 
  interface

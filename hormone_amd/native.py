@@ -37,7 +37,7 @@ class HgDesc(C.Structure):
         ("gravswitch", C.c_int32), ("grktype", C.c_int32),
         ("HGfac", C.c_double), ("hgcfl", C.c_double), ("alphagrv", C.c_double),
         ("G", C.c_double), ("maxtngrv", C.c_int32), ("precision", C.c_int32),
-        ("device", C.c_int32), ("reserved", C.c_int32),
+        ("device", C.c_int32), ("spn", C.c_int32),
     ]
 
 
@@ -54,7 +54,7 @@ class HgError(RuntimeError):
 def make_desc(grid, mag_on=0, flux_limiter=2, rktype=3, bc=(0,) * 12,
               gamma=5.0 / 3.0, courant=0.9, alpha9wave=0.1, gravswitch=0,
               grktype=2, HGfac=5.0, hgcfl=0.9, alphagrv=0.1, G=6.67430e-8,
-              maxtngrv=-1, precision=PREC_FP64, device=0):
+              maxtngrv=-1, precision=PREC_FP64, device=0, compswitch=0, spn=0):
     """Build an HgDesc; returns (desc, keepalive) -- keep both referenced."""
     d = HgDesc()
     keep = []
@@ -69,7 +69,8 @@ def make_desc(grid, mag_on=0, flux_limiter=2, rktype=3, bc=(0,) * 12,
             setattr(d, f"{nm}{a + 1}", arr.ctypes.data_as(_dp))
     d.crdnt = 0
     d.eostype = 0
-    d.compswitch = 0
+    d.compswitch = int(compswitch)
+    d.spn = int(spn)
     d.mag_on = int(mag_on)
     d.flux_limiter = int(flux_limiter)
     d.rktype = int(rktype)

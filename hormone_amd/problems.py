@@ -50,6 +50,8 @@ class Params:
     G: float = 6.67430e-8
     is_test: bool = True
     precision: str = "fp64"   # or "fp32mix"
+    compswitch: int = 0       # 2: spn advected species (eos_con)
+    spn: int = 0
 
     @property
     def solve(self):
@@ -112,6 +114,14 @@ def briowu_params(direction: str, n: int = 500, **kw) -> Params:
 def orszagtang_params(plane: str, n: int = 128, **kw) -> Params:
     p = Params(n=_axis_n(plane, n), t_end=500.0, dt_out=500.0, dt_unit="ms",
                mag_on=True, bc=(0,) * 12, tnlim=100000)
+    return replace(p, **kw)
+
+
+def khinstability_params(n: int = 128, **kw) -> Params:
+    """para/parameters_KHinstability: 2D hydro shear layer with two advected
+    species, periodic, t_end 1000 ms."""
+    p = Params(n=(n, n, 1), lo=(-0.5, -0.5, 0.0), hi=(0.5, 0.5, 1.0), t_end=1000.0,
+               dt_out=100.0, dt_unit="ms", bc=(0,) * 12, compswitch=2, spn=2, tnlim=100000)
     return replace(p, **kw)
 
 
@@ -201,6 +211,44 @@ def orszagtang(g: Grid, gamma: float) -> State:
     b[ax1] = along(bs2, ax2)
     b[ax2] = along(s4a, ax1)
     return State(d=d, p=p, v=v, b=b, phi=_zeros(g))
+
+
+def khinstability(g: Grid) -> State:
+    """src/init/KHtests.f90:14-56: tanh-like shear layer |y| = 1/4 of width
+    0.025 (exp profiles), species 1 outside and 2 inside the layer,
+    vy = 0.01 sin(4 pi x), p = 2.5.  exp and sin are the C library's, as the
+    reference's."""
+    pi = math.acos(-1.0)
+    rho1, rho2 = 1.0, 2.0
+    rhom = 0.5 * (rho1 - rho2)
+    vv1, vv2 = 0.5, -0.5
+    vvm = 0.5 * (vv1 - vv2)
+    ptb, width = 1e-2, 0.025
+    ys = g.centres(1)
+    v1y, dy, s1y = [], [], []
+    for y in ys:
+        ay = abs(float(y))
+        if ay > 0.25:
+            ex = math.exp((-ay + 0.25) / width)
+            v1y.append(vv1 - vvm * ex)
+            dy.append(rho1 - rhom * ex)
+            s1y.append(1.0)
+        else:
+            ex = math.exp((ay - 0.25) / width)
+            v1y.append(vv2 + vvm * ex)
+            dy.append(rho2 + rhom * ex)
+            s1y.append(0.0)
+    sx = np.array([ptb * math.sin(4.0 * pi * float(x)) for x in g.centres(0)])
+
+    def along_y(a):
+        return np.broadcast_to(np.asarray(a)[None, :, None], g.shape).copy()
+
+    s1 = along_y(s1y)
+    z = np.zeros(g.shape)
+    return State(d=along_y(dy), p=np.full(g.shape, 2.5),
+                 v=[along_y(v1y), np.broadcast_to(sx[None, None, :], g.shape).copy(), z.copy()],
+                 b=[z.copy(), z.copy(), z.copy()], phi=z.copy(),
+                 extra={"spc": [s1, 1.0 - s1], "species": ["scalar1", "scalar2"]})
 
 
 def initial_energy(st: State, gamma: float):
@@ -360,6 +408,10 @@ def make_problem(name: str, **kw):
         prm = orszagtang_params(name.split("_")[1], **kw)
         g = cartesian_grid(prm.n, prm.lo, prm.hi)
         return prm, g, orszagtang(g, prm.gamma)
+    if name == "KHinstability":
+        prm = khinstability_params(**kw)
+        g = cartesian_grid(prm.n, prm.lo, prm.hi)
+        return prm, g, khinstability(g)
     if name == "kh3d":
         seed = kw.pop("seed", 2024)
         prm = kh_params(**kw)

@@ -2,7 +2,8 @@
 
 Layout (src/main/output.f90:697-795, io.F90:345-363; stream access,
 little-endian, legacy int4(-1) record markers): [-1][tn i4][time f8][-1]
-[-1] d v1 v2 v3 e [-1] then optional groups; for mag_on [-1] b1 b2 b3 phi [-1].
+[-1] d v1 v2 v3 e [-1] then optional groups; for compswitch >= 2 [-1] spn
+spc(1:spn,...) names [-1]; for mag_on [-1] b1 b2 b3 phi [-1].
 Each field is nx*ny*nz float64, x fastest.
 """
 from __future__ import annotations
@@ -41,8 +42,20 @@ def read_bin(path: str, shape_zyx, mag_on: bool, eostype: int = 0,
         assert i4() == -1
         out["T"] = f8(n).reshape(shape_zyx)
         assert i4() == -1
-    if gravswitch >= 2 or gravswitch == 3 or compswitch:
-        raise NotImplementedError("only hydro/MHD snapshots are read here")
+    if gravswitch >= 2 or compswitch == 1:
+        raise NotImplementedError("gravity and compswitch 1 groups are not read here")
+    if compswitch >= 2:
+        # [-1] spn, spc(1:spn, cells) with the species index fastest,
+        # species names (character(10) each) [-1]  (output.f90:756-761)
+        assert i4() == -1
+        spn = i4()
+        spc = f8(spn * n).reshape(n, spn)
+        for q in range(spn):
+            out[f"spc{q + 1}"] = spc[:, q].copy().reshape(shape_zyx)
+        out["species"] = [raw[off + 10 * q:off + 10 * q + 10].decode().strip()
+                          for q in range(spn)]
+        off += 10 * spn
+        assert i4() == -1
     if mag_on:
         assert i4() == -1
         for nm in ("b1", "b2", "b3", "phi"):

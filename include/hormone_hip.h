@@ -33,7 +33,7 @@ extern "C" {
 enum {
   HG_OK = 0,
   HG_ERR_ARG = 1,         /* bad argument / descriptor                       */
-  HG_ERR_UNSUPPORTED = 2, /* crdnt/=0, eostype/=0, compswitch/=0, bc 4/5/9/10,
+  HG_ERR_UNSUPPORTED = 2, /* crdnt/=0, eostype/=0, compswitch 1 or >2, bc 4/5/9/10,
                              gravswitch other than 0 and 3                   */
   HG_ERR_DEVICE = 3,      /* a HIP runtime call failed                       */
   HG_ERR_FACE_EOS = 4,    /* numflux.f90:579-593 error_flux: rho<=0 or
@@ -58,6 +58,8 @@ enum { HG_PREC_FP64 = 0, HG_PREC_FP32MIX = 1 };
    physval arrays (modules.f90:106-138); HG_F_U + n (n = 0..ufnmax-1) is the
    conserved variable u(:,:,:,n+1) in the reference's equation order
    icnt,imo1,imo2,imo3,iene[,img1,img2,img3[,i9wv]] (checksetup.f90:62-75). */
+enum { HG_SPN_MAX = 8 }; /* species advected per context (compswitch 2) */
+
 enum {
   HG_F_D = 0, HG_F_V1 = 1, HG_F_V2 = 2, HG_F_V3 = 3, HG_F_E = 4,
   HG_F_B1 = 5, HG_F_B2 = 6, HG_F_B3 = 7, HG_F_PHI = 8,
@@ -65,7 +67,9 @@ enum {
   HG_F_P = 10,        /* download only: gas pressure (gamma-1)*eint       */
   HG_F_GRVPHI = 11,   /* gravitational potential (gravswitch 3)           */
   HG_F_GRVPSI = 12,   /* its time derivative variable psi                 */
-  HG_F_U = 16         /* + n: conserved variable n                        */
+  HG_F_U = 16,        /* + n: conserved variable n                        */
+  HG_F_SPC = 32       /* + n: mass fraction of species n+1, spc(n+1,:,:,:)
+                         (compswitch 2)                                    */
 };
 
 /* Everything the hot path reads from the reference's settings, grid and
@@ -81,7 +85,7 @@ typedef struct hg_desc {
   const double *x3, *xi3, *dx3, *dxi3, *idx3, *idxi3;
   int32_t crdnt;        /* must be 0 (Cartesian)                            */
   int32_t eostype;      /* must be 0 (ideal gas)                            */
-  int32_t compswitch;   /* must be 0 (uniform composition)                  */
+  int32_t compswitch;   /* 0 (uniform composition) or 2 (advected species)  */
   int32_t mag_on;       /* 0/1                                              */
   int32_t flux_limiter; /* HG_LIMITER_*                                     */
   int32_t rktype;       /* 1, 2 or 3                                        */
@@ -97,7 +101,8 @@ typedef struct hg_desc {
   int32_t maxtngrv;     /* settings: maxtngrv (<=0: no limit)               */
   int32_t precision;    /* HG_PREC_*                                        */
   int32_t device;       /* HIP device ordinal                               */
-  int32_t reserved;
+  int32_t spn;          /* settings: spn, species advected when compswitch=2
+                           (1..HG_SPN_MAX)                                  */
 } hg_desc;
 
 typedef struct hg_ctx hg_ctx;
@@ -202,7 +207,7 @@ int hg_time_sweep(hg_ctx *ctx, int32_t launches, double *ms_total);
 
 /* ---- batches of independent boxes (no counterpart in the reference) ------
    Many contexts of possibly different sizes and dimensions, sharing one
-   kernel variant (mag_on, flux_limiter, precision, rktype; no gravity),
+   kernel variant (mag_on, flux_limiter, precision, rktype; no gravity, no species),
    advance together: each step is one launch per kernel over flattened
    work-lists of (box, sweep wavefront) and (box, cell block) items, replayed
    from a captured hipGraph.  Each box keeps its own dt, clock and stop

@@ -50,6 +50,11 @@ struct or_ctx {
   /* gravity */
   double *grvphi, *grvpsi, *grvphiorg[2], *lapphi, *hgsrc, *gsrc;
   double *lapc[5]; /* lap_coeff(-1:3,...) for crdnt 0 (no reference counterpart) */
+  /* advected species (compswitch 2): spc, spcorg, dspc, spcflx
+     (allocations.f90:294-300) */
+  int spn;
+  double *spc[HG_SPN_MAX], *spcorg[HG_SPN_MAX];
+  double *dspc[3][HG_SPN_MAX], *spcflx[3][HG_SPN_MAX];
   double *dti;
   double ch;
   int err_i, err_j, err_k, err_dir;
@@ -84,8 +89,12 @@ or_ctx *or_create(const hg_desc *desc, int *status) {
   int st = HG_OK;
   or_ctx *c = NULL;
   if (!desc) { st = HG_ERR_ARG; goto fail; }
-  if (desc->crdnt != 0 || desc->eostype != 0 || desc->compswitch != 0) {
+  if (desc->crdnt != 0 || desc->eostype != 0 ||
+      (desc->compswitch != 0 && desc->compswitch != 2)) {
     st = HG_ERR_UNSUPPORTED; goto fail;
+  }
+  if (desc->compswitch == 2 && (desc->spn < 1 || desc->spn > HG_SPN_MAX)) {
+    st = HG_ERR_ARG; goto fail;
   }
   for (int b = 0; b < 12; ++b)
     if (desc->bc[b] < 0 || desc->bc[b] > 3) { st = HG_ERR_UNSUPPORTED; goto fail; }
@@ -151,6 +160,17 @@ or_ctx *or_create(const hg_desc *desc, int *status) {
     }
     for (int d = 0; d < 3; ++d)
       for (int a = 0; a < 9; ++a) c->sl[d][a] = alloc_field(N);
+    c->spn = desc->compswitch == 2 ? desc->spn : 0;
+    for (int n = 0; n < c->spn; ++n) {
+      /* zero-filled: the flux arrays of an unsolved direction stay 0 and
+         enter the species update as idxi*(0-0) */
+      c->spc[n] = alloc_field(N);
+      c->spcorg[n] = alloc_field(N);
+      for (int d = 0; d < 3; ++d) {
+        c->dspc[d][n] = alloc_field(N);
+        c->spcflx[d][n] = alloc_field(N);
+      }
+    }
     if (desc->gravswitch == 3) {
       c->grvphi = alloc_field(N); c->grvpsi = alloc_field(N);
       c->grvphiorg[0] = alloc_field(N); c->grvphiorg[1] = alloc_field(N);
@@ -200,6 +220,10 @@ void or_destroy(or_ctx *c) {
   }
   for (int d = 0; d < 3; ++d)
     for (int a = 0; a < 9; ++a) free(c->sl[d][a]);
+  for (int n = 0; n < HG_SPN_MAX; ++n) {
+    free(c->spc[n]); free(c->spcorg[n]);
+    for (int d = 0; d < 3; ++d) { free(c->dspc[d][n]); free(c->spcflx[d][n]); }
+  }
   free(c->mem_coord);
   free(c);
 }
@@ -223,6 +247,7 @@ static double *field_ptr(or_ctx *c, int field) {
   case HG_F_GRVPSI: return c->grvpsi;
   default:
     if (field >= HG_F_U && field < HG_F_U + c->ufnmax) return c->u[field - HG_F_U];
+    if (field >= HG_F_SPC && field < HG_F_SPC + c->spn) return c->spc[field - HG_F_SPC];
     return NULL;
   }
 }
@@ -490,6 +515,7 @@ static void apply_face(or_ctx *c, int axis, int inner) {
         c->d[G] = c->d[S];
         c->p[G] = c->p[S];
         if (c->prm.mag_on) c->phi[G] = c->phi[S];
+        for (int n = 0; n < c->spn; ++n) c->spc[n][G] = c->spc[n][S];
         /* vectors */
         double *v[3] = {c->v1, c->v2, c->v3};
         double *b[3] = {c->b1, c->b2, c->b3};
@@ -573,6 +599,12 @@ static void interpolation(or_ctx *c) {
             b1l = b1r = b2l = b2r = b3l = b3r = 0.0;
           }
 #undef SLOPE
+          /* species slopes (interpolation.f90:135-141): not reset by the
+             e_int guard below */
+          for (int n = 0; n < c->spn; ++n) {
+            uu[0] = c->spc[n][l]; uu[1] = c->spc[n][m]; uu[2] = c->spc[n][r];
+            c->dspc[dir][n][m] = limiter(lim, uu, dx, x, xi);
+          }
           double eintl = elv - 0.5 * (m1l * m1l + m2l * m2l + m3l * m3l) / dlv;
           double eintr = erv - 0.5 * (m1r * m1r + m2r * m2r + m3r * m3r) / drv;
           if (mag) {
@@ -672,6 +704,21 @@ static int numflux(or_ctx *c) {
             F[img[B]][L] = t[6];
             F[img[C]][L] = t[7];
             if (c->dim > 1) F[c->i9wv][L] = t[8];
+          }
+          if (c->spn) {
+            /* species upwinded with the mass flux, renormalised by the
+               upwind sum (numflux.f90:200-212) */
+            double spcl[HG_SPN_MAX], spcr[HG_SPN_MAX], suml = 0.0, sumr = 0.0;
+            for (int n = 0; n < c->spn; ++n) {
+              spcl[n] = c->spc[n][L] + dx1 * c->dspc[dir][n][L];
+              spcr[n] = c->spc[n][R] - dx2 * c->dspc[dir][n][R];
+            }
+            for (int n = 0; n < c->spn; ++n) { suml = suml + spcl[n]; sumr = sumr + spcr[n]; }
+            double sgn = fsign(0.5, F[c->icnt][L]);
+            double fix = 1.0 / (suml * (0.5 + sgn) + sumr * (0.5 - sgn));
+            for (int n = 0; n < c->spn; ++n)
+              c->spcflx[dir][n][L] = fix * F[c->icnt][L] *
+                                     (spcl[n] * (0.5 + sgn) + spcr[n] * (0.5 - sgn));
           }
         }
   }
@@ -782,6 +829,26 @@ static int rungekutta(or_ctx *c, int rungen, double dt) {
                      (I1[i] * (1.0 * F1[m - sx] - 1.0 * F1[m]) +
                       I2[j] * (1.0 * F2[m - sy] - 1.0 * F2[m]) +
                       I3[k] * (F3[m - sz] - F3[m]) + S[m]);
+        }
+  }
+  /* species (rungekutta.f90:61-85): mass fractions from the partial
+     densities, with the stage-start primitive density d and the new u(icnt) */
+  for (int n = 0; n < c->spn; ++n) {
+    double *X = c->spc[n], *X0 = c->spcorg[n];
+    double *F1 = c->spcflx[0][n], *F2 = c->spcflx[1][n], *F3 = c->spcflx[2][n];
+    const double *D0 = c->uorg[c->icnt], *UN = c->u[c->icnt];
+#pragma omp parallel for collapse(2) num_threads(NT)
+    for (int k = c->ks; k <= c->ke; ++k)
+      for (int j = c->js; j <= c->je; ++j)
+        for (int i = c->is; i <= c->ie; ++i) {
+          size_t m = ID(c, i, j, k);
+          if (rungen == 1) X0[m] = X[m] * D0[m];
+          X[m] = (faco * X0[m] + facn * X[m] * c->d[m] +
+                  fact * dt *
+                      (I1[i] * (1.0 * F1[m - sx] - 1.0 * F1[m]) +
+                       I2[j] * (1.0 * F2[m - sy] - 1.0 * F2[m]) +
+                       I3[k] * (F3[m - sz] - F3[m]))) /
+                 UN[m];
         }
   }
   /* primitive */

@@ -14,7 +14,7 @@ Two sources, both data produced by the reference itself:
    full run to t=0.5).  The binary runs in a scratch directory under /tmp
    whose ../para points at the reference's parameter files.
 
-Usage: python tests/golden/make_fixtures.py [--skip-long | --extra-only]
+Usage: python tests/golden/make_fixtures.py [--skip-long | --extra-only | --kh-only]
 """
 from __future__ import annotations
 
@@ -80,6 +80,18 @@ def convert_reference_goldens(meta: dict):
             snap = read_bin(os.path.join(REF, "tests", name + ".bin"), shape_of(nn), True)
             meta[name] = summary(snap, FIELDS_MAG)
             meta[name]["source"] = f"reference tests/{name}.bin"
+    convert_kh_goldens(meta)
+
+
+def convert_kh_goldens(meta: dict):
+    """KHinstability{,_flat}: 2D hydro, 128^2, two advected species."""
+    for flat in ("", "_flat"):
+        name = f"KHinstability{flat}"
+        snap = read_bin(os.path.join(REF, "tests", name + ".bin"), (1, 128, 128), False,
+                        compswitch=2)
+        meta[name] = summary(snap, FIELDS + ("spc1", "spc2"))
+        meta[name]["species"] = snap["species"]
+        meta[name]["source"] = f"reference tests/{name}.bin"
 
 
 def run_reference(simtype: str, params: str, threads: int = 8) -> str:
@@ -221,6 +233,14 @@ def reference_runs_extra(meta: dict):
 
 def main():
     long = "--skip-long" not in sys.argv
+    if "--kh-only" in sys.argv:
+        path = os.path.join(HERE, "golden_meta.json")
+        meta = json.load(open(path))
+        convert_kh_goldens(meta)
+        with open(path, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+        print("wrote", path)
+        return
     if "--extra-only" in sys.argv:
         path = os.path.join(HERE, "golden_meta.json")
         meta = json.load(open(path))

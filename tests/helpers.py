@@ -31,6 +31,10 @@ def run_problem(engine_cls, name, flat=False, max_steps=None, **kw):
     fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
     out = D.fetch(eng, g, fields)
     state = {NAMES[i]: out[f] for i, f in enumerate(fields)}
+    if prm.compswitch == 2:
+        spc = D.fetch(eng, g, tuple(D.F_SPC + q for q in range(prm.spn)))
+        for q in range(prm.spn):
+            state[f"spc{q + 1}"] = spc[D.F_SPC + q]
     return res, state, eng
 
 

@@ -136,6 +136,15 @@ def test_gpu_orszagtang_goldens(hiplib, plane, flat):
     compare_to_meta(res, state, meta()[f"orszagtang_{plane}" + ("_flat" if flat else "")])
 
 
+@pytest.mark.parametrize("flat", [False, True], ids=["muscl", "flat"])
+def test_gpu_kh_goldens(hiplib, flat):
+    """Kelvin-Helmholtz with two advected species (compswitch 2): the
+    reference's KHinstability goldens bit for bit, species included."""
+    print(f"case KHinstability flat={flat}", flush=True)
+    res, state, _ = run_problem(Hydro, "KHinstability", flat=flat)
+    compare_to_meta(res, state, meta()["KHinstability" + ("_flat" if flat else "")])
+
+
 def test_gpu_sod1024_courant04(hiplib):
     res, state, _ = run_problem(Hydro, "sodshock_x", n=1024, courant=0.4)
     compare_to_npz(res, state, "sod1024_c04")

@@ -68,3 +68,11 @@ def test_oracle_reflective_briowu_stops_like_the_reference():
     with pytest.raises(HgError) as ei:
         D.run(eng, prm, g, max_steps=1)
     assert list(ei.value.cell) == fail["cell"]
+
+
+@pytest.mark.parametrize("flat", [False, True], ids=["muscl", "flat"])
+def test_oracle_kh_goldens(flat):
+    """2D hydro Kelvin-Helmholtz with two advected species (compswitch 2):
+    the reference's KHinstability goldens, every field and both species."""
+    res, state, _ = run_problem(Oracle, "KHinstability", flat=flat)
+    compare_to_meta(res, state, meta()["KHinstability" + ("_flat" if flat else "")])
