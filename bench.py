@@ -81,6 +81,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--serving-lanes", default="50000,500000",
+                    help="serving: cell-count bounds of the concurrently served size classes "
+                         "(comma separated; empty = one batch)")
     ap.add_argument("--sweep-waves", type=int, default=0,
                     help="sweep wavefronts per block (0: autotune during warm-up)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -267,8 +270,9 @@ def serving_bench(a):
     by_size = sorted(reqs, key=lambda r: r.cells)
     check = {by_size[0].rid, by_size[-1].rid}
     torch.cuda.synchronize()
+    lanes = tuple(int(x) for x in a.serving_lanes.split(",") if x) if a.serving_lanes else ()
     rep = S.serve(reqs, hbm_budget=a.hbm_budget_gib * 2 ** 30, steps_per_round=4,
-                  digest_ids=check)
+                  digest_ids=check, lanes=lanes)
     torch.cuda.synchronize()
     sm = rep.summary()
     done = [r for r in reqs if r.finished is not None and r.error is None]
@@ -285,7 +289,8 @@ def serving_bench(a):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded request generator)",
         "config": {"workload": f"serving: {a.requests} Poisson(50/s) MHD Riemann/blast requests, "
                                f"64^2..192^3 cells, t_end 0.1, HBM budget {a.hbm_budget_gib:g} GiB",
-                   "requests": a.requests},
+                   "requests": a.requests,
+                   "size_classes": [f"<= {b} cells" for b in lanes] + ["larger"]},
         "roofline": None, "cpu_baseline": None,
         "p50_time_to_solution_s": sm["p50_s"], "p99_time_to_solution_s": sm["p99_s"],
         "completed": sm["completed"], "failed": sm["failed"], "makespan_s": sm["makespan_s"],

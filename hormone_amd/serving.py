@@ -3,8 +3,9 @@
 Requests (2D/3D Riemann or blast problems of different grid sizes) arrive
 over time; the loop keeps their state resident on the device, admits a
 queued request as soon as it has arrived and its memory fits the HBM budget,
-advances all admitted requests together through one :class:`Batch` (their
-work overlaps on the device in the same launches), retires each one when it
+advances the admitted requests of one size class together through one
+:class:`Batch` (their work overlaps on the device in the same launches; the
+classes run concurrently on their own streams), retires each one when it
 reaches its end time and frees its memory for the next.  It reports p50/p99
 time-to-solution (arrival to completion) and aggregate zone-updates/s.
 
@@ -15,6 +16,7 @@ from __future__ import annotations
 
 import hashlib
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -164,81 +166,124 @@ class ServeReport:
         }
 
 
-def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int = 4,
-          tune: bool = True, keep_digest: bool = False, digest_ids=None) -> ServeReport:
-    """Continuous batching over arriving requests (arrival times are honoured
-    in wall-clock time from the start of the call).  A final-state digest is
-    kept for every request (keep_digest) or for the ids in digest_ids."""
-    queue = sorted(requests, key=lambda r: r.arrival)
+class _Shared:
+    """State the size-class lanes share: the HBM budget, the zone-update
+    count and the event log."""
+
+    def __init__(self, budget: float):
+        self.lock = threading.Lock()
+        self.budget = budget
+        self.used = 0
+        self.zu = 0
+        self.events = []
+        self.tuned = []
+
+
+def _serve_lane(lane: int, queue: list, sh: _Shared, t0: float, steps_per_round: int,
+                tune: bool, keep_digest: bool, digest_ids):
+    """One size class: its own Batch (hence its own stream and hipGraph),
+    admitting its requests in arrival order as the shared budget allows."""
     qi = 0
     batch = Batch()
     active = {}   # id(eng) -> (req, eng, g)
-    used = 0
-    zu = 0
-    tuned = None
     tuned_class = None
-    tunings = 0
-    events = []
-    t0 = time.perf_counter()
-    while qi < len(queue) or active:
-        now = time.perf_counter() - t0
-        # admission: arrived requests, in order, while the budget holds
-        admitted = False
-        while qi < len(queue) and queue[qi].arrival <= now:
-            req = queue[qi]
-            need = device_bytes(req)
-            if used + need > hbm_budget and active:
-                break
-            prm, g, st = setup(req)
-            eng = Hydro(**D.desc_fields(prm, g))
-            D.load_initial_state(eng, prm, g, st)
-            t_end, dt_out, t_out = prm.times()
-            batch.add(eng, t_end=t_end, t_out=t_out, dt_out=dt_out)
-            req.admitted = time.perf_counter() - t0
-            active[id(eng)] = (req, eng, g)
-            used += need
-            qi += 1
-            admitted = True
-        if not active:
-            if qi < len(queue):
-                time.sleep(max(0.0, min(0.01, queue[qi].arrival - now)))
-            continue
-        # (re)tune the launch shapes whenever the mix has moved to another
-        # size class (tuned shapes are cached per class)
-        if tune and len(active) >= 2:
-            cls = shape_class(batch)
-            if cls != tuned_class:
-                tuned, _ = autotune(batch)
-                tuned_class = cls
-                tunings += 1
-        try:
-            batch.step(steps_per_round)
-        except HgError:
-            pass
-        events.append({"t": round(time.perf_counter() - t0, 4), "active": len(active),
-                       "queued": sum(1 for r in queue[qi:] if r.arrival <= now),
-                       "hbm_bytes": used})
-        for key, (req, eng, g) in list(active.items()):
+    try:
+        while qi < len(queue) or active:
+            now = time.perf_counter() - t0
+            while qi < len(queue) and queue[qi].arrival <= now:
+                req = queue[qi]
+                need = device_bytes(req)
+                with sh.lock:
+                    # over the budget, wait for memory unless nothing at all
+                    # is in flight (a request larger than the budget still runs)
+                    if sh.used + need > sh.budget and sh.used > 0:
+                        break
+                    sh.used += need
+                prm, g, st = setup(req)
+                eng = Hydro(**D.desc_fields(prm, g))
+                D.load_initial_state(eng, prm, g, st)
+                t_end, dt_out, t_out = prm.times()
+                batch.add(eng, t_end=t_end, t_out=t_out, dt_out=dt_out)
+                req.admitted = time.perf_counter() - t0
+                active[id(eng)] = (req, eng, g)
+                qi += 1
+            if not active:
+                if qi < len(queue):
+                    time.sleep(max(0.0, min(0.005, queue[qi].arrival - now)))
+                continue
+            # (re)tune the launch shapes whenever the mix has moved to another
+            # size class (tuned shapes are cached per class)
+            if tune and len(active) >= 2:
+                cls = shape_class(batch)
+                if cls != tuned_class:
+                    tuned, _ = autotune(batch)
+                    tuned_class = cls
+                    with sh.lock:
+                        sh.tuned.append((lane, tuned))
             try:
-                tm, tn, stop = batch.state(eng)
-            except HgError as err:
-                req.error = str(err)
-                tm, tn, stop = None, 0, 4
-            if stop != 0:
-                req.finished = time.perf_counter() - t0
-                req.steps = tn
-                if stop == 1 and req.error is None:
-                    zu += req.cells * tn
-                    if keep_digest or (digest_ids is not None and req.rid in digest_ids):
-                        batch.remove(eng)
-                        req.digest = state_digest(eng, g)
-                    else:
-                        batch.remove(eng)
-                else:
+                batch.step(steps_per_round)
+            except HgError:
+                pass
+            with sh.lock:
+                sh.events.append({"t": round(time.perf_counter() - t0, 4), "lane": lane,
+                                  "active": len(active),
+                                  "queued": sum(1 for r in queue[qi:] if r.arrival <= now),
+                                  "hbm_bytes": sh.used})
+            for key, (req, eng, g) in list(active.items()):
+                try:
+                    tm, tn, stop = batch.state(eng)
+                except HgError as err:
+                    req.error = str(err)
+                    tm, tn, stop = None, 0, 4
+                if stop != 0:
+                    req.finished = time.perf_counter() - t0
+                    req.steps = tn
                     batch.remove(eng)
-                eng.close()
-                used -= device_bytes(req)
-                del active[key]
+                    if stop == 1 and req.error is None:
+                        with sh.lock:
+                            sh.zu += req.cells * tn
+                        if keep_digest or (digest_ids is not None and req.rid in digest_ids):
+                            req.digest = state_digest(eng, g)
+                    eng.close()
+                    with sh.lock:
+                        sh.used -= device_bytes(req)
+                    del active[key]
+    finally:
+        batch.close()
+
+
+def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int = 4,
+          tune: bool = True, keep_digest: bool = False, digest_ids=None,
+          lanes=(50_000, 500_000)) -> ServeReport:
+    """Continuous batching over arriving requests (arrival times are honoured
+    in wall-clock time from the start of the call).  A final-state digest is
+    kept for every request (keep_digest) or for the ids in digest_ids.
+
+    lanes: upper cell-count bounds of the size classes below the largest.
+    Each class has its own batch -- own stream, hipGraph and host thread --
+    so small requests advance at their own step rate instead of one step per
+    step of the largest box in flight, while all classes share the GPU and
+    the HBM budget.  lanes=() serves everything from one batch."""
+    bounds = sorted(lanes) + [math.inf]
+    queues = [[] for _ in bounds]
+    for r in sorted(requests, key=lambda r: r.arrival):
+        queues[next(k for k, b in enumerate(bounds) if r.cells <= b)].append(r)
+    sh = _Shared(hbm_budget)
+    errors = []
+
+    def worker(lane, queue):
+        try:
+            _serve_lane(lane, queue, sh, t0, steps_per_round, tune, keep_digest, digest_ids)
+        except BaseException as err:  # re-raised below
+            errors.append(err)
+
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=worker, args=(k, q)) for k, q in enumerate(queues) if q]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
     makespan = time.perf_counter() - t0
-    batch.close()
-    return ServeReport(requests, makespan, zu, tuned, events)
+    if errors:
+        raise errors[0]
+    return ServeReport(requests, makespan, sh.zu, tuple(sh.tuned), sh.events)

@@ -73,9 +73,13 @@ def test_autotune_picks_a_shape(hiplib):
     assert best in timings and len(timings) >= 4
 
 
-def test_serving_results_equal_alone_and_report(hiplib):
+@pytest.mark.parametrize("lanes", [(), (2000,)], ids=["one-batch", "two-size-classes"])
+def test_serving_results_equal_alone_and_report(hiplib, lanes):
+    """Every request ends as it does alone, whether it shares one batch with
+    all others or the size classes run concurrently on their own streams."""
     reqs = small_requests(10, seed=21)
-    rep = S.serve(reqs, hbm_budget=2 ** 30, steps_per_round=4, tune=True, keep_digest=True)
+    rep = S.serve(reqs, hbm_budget=2 ** 30, steps_per_round=4, tune=True, keep_digest=True,
+                  lanes=lanes)
     summ = rep.summary()
     assert summ["completed"] == 10 and summ["failed"] == 0
     assert summ["p50_s"] is not None and summ["p99_s"] >= summ["p50_s"]
