@@ -153,6 +153,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
       A.idx = dp[4];
       A.idxi = dp[5];
       A.n = nn[a];
+      A.nlines = (unsigned)(D.ncell / nn[a]);
       A.stride = a == 0 ? 1 : (a == 1 ? (long)D.n1 : (long)D.n1 * D.n2);
       A.sc_lo = desc->bc[2 * a];
       A.sc_hi = desc->bc[2 * a + 1];

@@ -111,24 +111,51 @@ __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base
 }
 
 // ---- LDS-staged lines for the strided (y, z) sweeps ------------------------
-// A block's wavefronts own lines of consecutive x index over the same
-// 64-cell segment, so the block's cells form a tile [64 cells][bw lines]
-// whose rows are contiguous in HBM (bw doubles each).  Loading the tile row
-// by row (one element per thread and field) replaces 64 lane-strided
-// accesses per instruction -- one cache line and, along z, one page per
-// lane -- by bw/64-row coalesced ones; the flux divergences go back out the
-// same way.  LDS layout [field][cell row][line], rows padded to bw+1.
+// A block's bw wavefronts own bw consecutive waves of one direction, i.e.
+// (mostly) neighbouring lines of consecutive x index over the same 64-cell
+// segment, so the block's cells form a tile [64 cells][bw lines] whose rows
+// are (mostly) contiguous in HBM.  Loading the tile row by row (one element
+// per thread and field) replaces 64 lane-strided accesses per instruction --
+// one cache line and, along z, one page per lane -- by bw/64-row coalesced
+// ones; the flux divergences go back out the same way.  Each column finds
+// its own line and segment, so a block may span the end of an x row or of a
+// segment; columns past the last wave of the direction (padding of a
+// partial block) work on a copy of the last line and store nothing.  LDS
+// layout [field][cell row][line], rows padded to bw+1, then each column's
+// line base and segment (-1 for padding) for the stores.
 __device__ __forceinline__ int tile_ld(int bw) { return bw + 1; }
 
+// line base index and segment of wave wc of direction DIR
+template <int DIR>
+__device__ __forceinline__ void wave_line(const Dev &D, const Axis &A, unsigned wc,
+                                          long &base, int &seg) {
+  const unsigned nl = A.nlines;
+  const unsigned line = wc % nl;
+  seg = (int)(wc / nl);
+  if (DIR == 0) base = (long)line * D.n1;
+  else if (DIR == 1) base = (long)(line % (unsigned)D.n1) + (long)(line / (unsigned)D.n1) * D.n1 * D.n2;
+  else base = line;
+}
+
 template <int DIR, bool MAG>
-__device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long base, int seg,
-                                          int c, double *lds) {
+__device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave0, int c,
+                                          double *lds) {
   const int bw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ld = tile_ld(bw), fs = 64 * ld;
-  {  // cooperative load: thread t -> (row r, line col)
+  {  // cooperative load: thread t -> (row r, column col)
     const int t = threadIdx.x, r = t / bw, col = t - r * bw;
+    const long last = D.waves[DIR] - 1;
+    const long wc = wave0 + col < last ? wave0 + col : last;
+    long bc;
+    int sg;
+    wave_line<DIR>(D, A, (unsigned)wc, bc, sg);
+    if (r == 0) {  // the column's line and segment for the stores (-1: padding)
+      long *cinfo = reinterpret_cast<long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
+      cinfo[2 * col] = bc;
+      cinfo[2 * col + 1] = wave0 + col < D.waves[DIR] ? sg : -1;
+    }
     long ms, mv;
-    cell_sources(A, base - w + col, seg * SEG - 2 + r, ms, mv);
+    cell_sources(A, bc, sg * SEG - 2 + r, ms, mv);
     double *p = lds + r * ld + col;
     p[WD * fs] = D.W[WD][ms];
     p[WE * fs] = D.W[WE][ms];
@@ -153,9 +180,8 @@ __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long base
 // Flux-divergence stores of a line: du[v] is the value of conserved
 // variable v for this lane's cell c (lanes 2..61 with 0 <= c < n own one).
 template <int DIR, bool MAG, bool TILE>
-__device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base, int seg,
-                                         int c, int lane, const double du[NVMAX],
-                                         double *lds) {
+__device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base, int c,
+                                         int lane, const double du[NVMAX], double *lds) {
   const int n = A.n;
   double *const *DU = D.DU[DIR];
   const int nv = MAG ? (D.i9wv >= 0 ? 9 : 8) : 5;
@@ -179,9 +205,13 @@ __device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base,
   }
   __syncthreads();
   const int t = threadIdx.x, r = t / bw, col = t - r * bw;
-  const int cr = seg * SEG - 2 + r;
+  const long *cinfo = reinterpret_cast<const long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
+  const long bc = cinfo[2 * col];
+  const int sg = (int)cinfo[2 * col + 1];
+  if (sg < 0) return;  // padding column
+  const int cr = sg * SEG - 2 + r;
   if (r >= 2 && r <= 61 && cr >= 0 && cr < n) {
-    const long m = base - w + col + (long)cr * A.stride;
+    const long m = bc + (long)cr * A.stride;
     const double *p = lds + r * ld + col;
 #pragma unroll
     for (int v = 0; v < NVMAX; ++v)
@@ -332,6 +362,12 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   const Axis &A = D.ax[DIR];
   const int n = A.n;
   const long nlines = D.ncell / n;
+  // a tiled block's padding wavefronts (past the direction's last wave) take
+  // part in the block's loads and barriers on a copy of the last line and
+  // store and report nothing
+  const bool valid = !TILE || wave < D.waves[DIR];
+  const long wave0 = wave - (threadIdx.x >> 6);  // first wave of the block (tiled)
+  if (!valid) wave = D.waves[DIR] - 1;
   const long line = wave % nlines;
   const int seg = (int)(wave / nlines);
   long base;
@@ -348,7 +384,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
     ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
   }
   const int c = seg * SEG - 2 + lane;
-  const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, base, seg, c, lds)
+  const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, wave0, c, lds)
                       : load_cell<DIR, MAG>(D, A, base, c);
   // d e m1 m2 m3 b1 b2 b3 and eint (slopes) / phi (face) of the cell
   const double qv[9] = {q.d, q.e, q.d * q.v[0], q.d * q.v[1], q.d * q.v[2],
@@ -392,7 +428,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   double t[9], fe[4];
   face_flux<DIR, MAG, true>(L, S, R_, RS, dx1, dx2, gam, ch, t, fe);
   // faces between lanes l and l+1 have valid slopes on both sides for l = 1..61
-  const bool face_valid = (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
+  const bool face_valid = valid && (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
   if (face_valid && (fe[1] <= 0.0 || fe[0] <= 0.0 || fe[3] <= 0.0 || fe[2] <= 0.0)) {
     int ii = ci, jj = cj, kk = ck;
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
@@ -416,7 +452,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   du[5 + b_] = ix * (tm[6] - t[6]);
   du[5 + c_] = ix * (tm[7] - t[7]);
   du[8] = ix * (tm[8] - t[8]);
-  store_du<DIR, MAG, TILE>(D, A, base, seg, c, lane, du, lds);
+  store_du<DIR, MAG, TILE>(D, A, base, c, lane, du, lds);
   if (SPC) store_mass_flux<DIR>(D, line, c, lane, t[0]);
 }
 
@@ -438,13 +474,19 @@ __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
     if (wave < D.waves[0]) sweep_line<0, MAG, LIM, false, SPC>(D, wave, lane, stage);
     return;
   }
+  // tiled: a block past the direction's last wave has nothing to do; a
+  // partly filled one runs its padding wavefronts too (barriers)
   if (wave < D.woff[2]) {
-    if (wave - D.woff[1] >= D.waves[1]) return;  // padding: whole blocks when tiled
-    sweep_line<1, MAG, LIM, TILE, SPC>(D, wave - D.woff[1], lane, stage, lds);
+    const long w1 = wave - D.woff[1], w0 = w1 - (threadIdx.x >> 6);
+    if (TILE ? w0 >= D.waves[1] : w1 >= D.waves[1]) return;
+    sweep_line<1, MAG, LIM, TILE, SPC>(D, w1, lane, stage, lds);
     return;
   }
-  if (wave - D.woff[2] < D.waves[2])
-    sweep_line<2, MAG, LIM, TILE, SPC>(D, wave - D.woff[2], lane, stage, lds);
+  {
+    const long w2 = wave - D.woff[2], w0 = w2 - (threadIdx.x >> 6);
+    if (TILE ? w0 >= D.waves[2] : w2 >= D.waves[2]) return;
+    sweep_line<2, MAG, LIM, TILE, SPC>(D, w2, lane, stage, lds);
+  }
 }
 
 // ---- FP32-mixed sweep -------------------------------------------------------
@@ -486,6 +528,12 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   const Axis &A = D.ax[DIR];
   const int n = A.n;
   const long nlines = D.ncell / n;
+  // a tiled block's padding wavefronts (past the direction's last wave) take
+  // part in the block's loads and barriers on a copy of the last line and
+  // store and report nothing
+  const bool valid = !TILE || wave < D.waves[DIR];
+  const long wave0 = wave - (threadIdx.x >> 6);  // first wave of the block (tiled)
+  if (!valid) wave = D.waves[DIR] - 1;
   const long line = wave % nlines;
   const int seg = (int)(wave / nlines);
   long base;
@@ -502,7 +550,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
   }
   const int c = seg * SEG - 2 + lane;
-  const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, base, seg, c, lds)
+  const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, wave0, c, lds)
                       : load_cell<DIR, MAG>(D, A, base, c);
   const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
   const double qd[8] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2]};
@@ -578,7 +626,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     eil = eil - bbl;
     eir = eir - bbr;
   }
-  const bool face_valid = (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
+  const bool face_valid = valid && (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
   if (face_valid && (eil <= 0.0 || L[0] <= 0.0 || eir <= 0.0 || R_[0] <= 0.0)) {
     int ii = ci, jj = cj, kk = ck;
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
@@ -631,7 +679,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   du[5 + b_] = ix * (tm[6] - t[6]);
   du[5 + c_] = ix * (tm[7] - t[7]);
   du[8] = ix * (tm[8] - t[8]);
-  store_du<DIR, MAG, TILE>(D, A, base, seg, c, lane, du, lds);
+  store_du<DIR, MAG, TILE>(D, A, base, c, lane, du, lds);
   if (SPC) store_mass_flux<DIR>(D, line, c, lane, t[0]);
 }
 
@@ -646,12 +694,16 @@ __global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
     return;
   }
   if (wave < D.woff[2]) {
-    if (wave - D.woff[1] >= D.waves[1]) return;  // padding: whole blocks when tiled
-    sweep_line_mix<1, MAG, LIM, TILE, SPC>(D, wave - D.woff[1], lane, stage, lds);
+    const long w1 = wave - D.woff[1], w0 = w1 - (threadIdx.x >> 6);
+    if (TILE ? w0 >= D.waves[1] : w1 >= D.waves[1]) return;
+    sweep_line_mix<1, MAG, LIM, TILE, SPC>(D, w1, lane, stage, lds);
     return;
   }
-  if (wave - D.woff[2] < D.waves[2])
-    sweep_line_mix<2, MAG, LIM, TILE, SPC>(D, wave - D.woff[2], lane, stage, lds);
+  {
+    const long w2 = wave - D.woff[2], w0 = w2 - (threadIdx.x >> 6);
+    if (TILE ? w0 >= D.waves[2] : w2 >= D.waves[2]) return;
+    sweep_line_mix<2, MAG, LIM, TILE, SPC>(D, w2, lane, stage, lds);
+  }
 }
 
 // ---- advected species (compswitch 2) -----------------------------------------
@@ -1109,9 +1161,11 @@ __device__ __forceinline__ void bsweep_item(const Dev &D, long w, int lane, int 
     }
     return;
   }
+  // tiled: padding waves of a partly filled chunk join its barriers
+  const long wid = threadIdx.x >> 6;
   if (w < D.woff[2]) {
     w -= D.woff[1];
-    if (w >= D.waves[1]) return;
+    if (tiled ? w - wid >= D.waves[1] : w >= D.waves[1]) return;
     if (tiled) {
       if (MIX) sweep_line_mix<1, MAG, LIM, true>(D, w, lane, stage, lds);
       else sweep_line<1, MAG, LIM, true>(D, w, lane, stage, lds);
@@ -1122,7 +1176,7 @@ __device__ __forceinline__ void bsweep_item(const Dev &D, long w, int lane, int 
     return;
   }
   w -= D.woff[2];
-  if (w >= D.waves[2]) return;
+  if (tiled ? w - wid >= D.waves[2] : w >= D.waves[2]) return;
   if (tiled) {
     if (MIX) sweep_line_mix<2, MAG, LIM, true>(D, w, lane, stage, lds);
     else sweep_line<2, MAG, LIM, true>(D, w, lane, stage, lds);
@@ -1146,7 +1200,7 @@ __global__ __launch_bounds__(1024) void k_bsweep(Batch B, int stage, int tile_ok
     const int2 x = B.sitems[it];
     const Dev &D = B.boxes[__builtin_amdgcn_readfirstlane(x.x)];
     if (D.sc->stop) continue;  // uniform over the chunk (one box)
-    const bool tiled = tile_ok && bw >= 4 && D.n1 % bw == 0;
+    const bool tiled = tile_ok && bw >= 4;
     bsweep_item<MAG, LIM, MIX>(D, __builtin_amdgcn_readfirstlane(x.y), lane, stage, tiled, lds);
   }
 }
@@ -1290,9 +1344,9 @@ static inline void klaunch_lds(void (*k)(KA...), unsigned g, unsigned b, size_t 
 // Bytes of LDS the y/z tile of a sweep block needs, or 0 when the lines of a
 // block do not have consecutive x index (or the tile would not fit).
 size_t sweep_tile_bytes(const Dev &D, int bw) {
-  if (bw < 4 || D.n1 % bw != 0 || !(D.solve[1] || D.solve[2])) return 0;
+  if (bw < 4 || !(D.solve[1] || D.solve[2])) return 0;
   const size_t nf = D.mag ? NWMAX : WEINT + 1;
-  const size_t bytes = nf * 64 * (size_t)(bw + 1) * sizeof(double);
+  const size_t bytes = (nf * 64 * (size_t)(bw + 1) + 2 * (size_t)bw) * sizeof(double);
   return bytes <= (size_t)D.lds_max ? bytes : 0;
 }
 
@@ -1409,7 +1463,7 @@ static void bsweep_mix(const Batch &B, int mixed, int stage, unsigned g, int bs,
   // LDS for the y/z tile when it fits (the boxes' own x extents decide)
   const int bw = bs / 64;
   const size_t nf = MAG ? NWMAX : WEINT + 1;
-  size_t tile = nf * 64 * (size_t)(bw + 1) * sizeof(double);
+  size_t tile = (nf * 64 * (size_t)(bw + 1) + 2 * (size_t)bw) * sizeof(double);
   if (bw < 4 || tile > (size_t)B.lds_max) tile = 0;
   const int ok = tile > 0 ? 1 : 0;
   if (mixed) k_bsweep<MAG, LIM, true><<<g, bs, tile, s>>>(B, stage, ok);

@@ -46,6 +46,7 @@ struct Axis {
   const double4 *limg;
   const double *mmw;
   int n;
+  unsigned nlines;   // lines along this axis (ncell / n)
   long stride;       // element stride of this axis in the SoA arrays
   int sc_lo, sc_hi;  // scalar BC codes (bcNis, bcNos)
   int vc_lo, vc_hi;  // vector BC codes (bcNiv, bcNov)

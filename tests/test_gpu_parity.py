@@ -347,3 +347,31 @@ def test_time_sweep_leaves_the_state_unchanged(hiplib):
     b = D.fetch(e2, g, D.SNAP_FIELDS_MAG)
     for f in D.SNAP_FIELDS_MAG:
         assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
+
+
+@pytest.mark.parametrize("bw", [4, 8, 16])
+@pytest.mark.parametrize("name,kw,steps", [
+    ("orszagtang_xy", dict(n=90), 15),
+    ("orszagtang_yz", dict(n=70), 10),
+    ("kh3d", dict(n=28, steps=6), 6),
+])
+def test_gpu_tiled_sweep_any_width_matches_oracle(hiplib, name, kw, steps, bw):
+    """The LDS tile of the y/z sweeps with x extents that are not a multiple
+    of the block's line count (blocks spanning the end of an x row or of a
+    segment, padding wavefronts in the last block): bitwise the oracle."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print(f"case tiled {name} {kw} bw={bw}", flush=True)
+    prm, g, st = P.make_problem(name, **kw)
+    e1 = Hydro(**D.desc_fields(prm, g))
+    e1.set_launch(bw)
+    D.load_initial_state(e1, prm, g, st)
+    r1 = D.run(e1, prm, g, max_steps=steps)
+    e2 = Oracle(**D.desc_fields(prm, g))
+    D.load_initial_state(e2, prm, g, st)
+    r2 = D.run(e2, prm, g, max_steps=steps)
+    assert (r1.tn, r1.time) == (r2.tn, r2.time)
+    fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+    a, b = D.fetch(e1, g, fields), D.fetch(e2, g, fields)
+    for f in fields:
+        assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
