@@ -56,6 +56,31 @@ def load_initial_state(eng, prm: P.Params, g: Grid, st: P.State):
         eng.upload(F_GRVPSI, np.zeros(g.padded_shape))
 
 
+def load_restart_state(eng, prm: P.Params, g: Grid, snap: dict):
+    """Restart from a snapshot (restart.f90:15-64 -> readbin.f90:16-115,
+    then hormone.f90:86-90): the primitives and total energy as read, the
+    conserved variables from them (conserve), e_int from them (pressure, by
+    the engine).  Returns (tn, time) of the snapshot."""
+    sh = g.shape
+    d = np.asarray(snap["d"]).reshape(sh)
+    v = [np.asarray(snap[k]).reshape(sh) for k in ("v1", "v2", "v3")]
+    e = np.asarray(snap["e"]).reshape(sh)
+    z = np.zeros(sh)
+    b = [np.asarray(snap[k]).reshape(sh) for k in ("b1", "b2", "b3")] if prm.mag_on else [z, z, z]
+    phi = np.asarray(snap["phi"]).reshape(sh) if prm.mag_on else z
+    st = P.State(d=d, p=z, v=v, b=b, phi=phi)
+    prims = {F_D: d, F_V1: v[0], F_V2: v[1], F_V3: v[2], F_E: e,
+             F_B1: b[0], F_B2: b[1], F_B3: b[2], F_PHI: phi}
+    for f, a in prims.items():
+        eng.upload(f, g.pad(a))
+    for n, a in enumerate(P.conserve(st, e, prm.mag_on, g.dim)):
+        eng.upload(F_U + n, g.pad(a))
+    if prm.compswitch == 2:
+        for q in range(prm.spn):
+            eng.upload(F_SPC + q, g.pad(np.asarray(snap[f"spc{q + 1}"]).reshape(sh)))
+    return int(snap["tn"]), float(snap["time"])
+
+
 def rel_l1_errors(mine: dict, ref: dict) -> dict:
     """Relative L1 error of each snapshot field against a reference run.
 
@@ -111,18 +136,18 @@ class RunResult:
 
 
 def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
-        record_dt: bool = False, device_loop: bool = False) -> RunResult:
+        record_dt: bool = False, device_loop: bool = False, start=(0, 0.0)) -> RunResult:
     """Main loop of hormone.f90:124-179 restricted to the hydro path.
 
     ``device_loop`` runs it on the device (hg_run) when the engine offers it;
-    the state is the same bit for bit.
+    the state is the same bit for bit.  ``start`` = (tn, time) of a restart
+    (load_restart_state); max_steps counts from step 0 like tnlim.
     """
     t_end, dt_out, t_out = prm.times()
     if (device_loop and hasattr(eng, "run_loop") and prm.fixed_dt <= 0 and not record_dt
             and prm.gravswitch == 0):
-        return _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out)
-    tim = 0.0
-    tn = 0
+        return _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out, start)
+    tn, tim = int(start[0]), float(start[1])
     damp = math.exp(-prm.alpha9wave * prm.courant)  # source.f90:253
     dts = []
     nsubs = []
@@ -166,9 +191,9 @@ def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
     return RunResult(tn, tim, wall, tn, dts, nsubs)
 
 
-def _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out) -> RunResult:
+def _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out, start=(0, 0.0)) -> RunResult:
     damp = math.exp(-prm.alpha9wave * prm.courant) if (prm.mag_on and g.dim > 1) else 0.0
-    tim, tn = 0.0, 0
+    tn, tim = int(start[0]), float(start[1])
     budget = max_steps if max_steps is not None else 1 << 62
     t0 = _time.perf_counter()
     while True:

@@ -63,3 +63,28 @@ def read_bin(path: str, shape_zyx, mag_on: bool, eostype: int = 0,
         assert i4() == -1
     assert off == len(raw), (off, len(raw))
     return out
+
+
+def write_bin(snap: dict, mag_on: bool, compswitch: int = 0) -> bytes:
+    """The bytes write_bin would write for this state (output.f90:697-795,
+    io.F90:345-363): stream access, little-endian, int4(-1) record markers.
+    snap: tn, time, d v1 v2 v3 e (nz, ny, nx) arrays, b1 b2 b3 phi when
+    mag_on, spc1..spcN and species names when compswitch >= 2."""
+    m1 = np.int32(-1).tobytes()
+
+    def f8(a):
+        return np.ascontiguousarray(a, dtype="<f8").tobytes()
+
+    out = [m1, np.int32(snap["tn"]).tobytes(), f8(snap["time"]), m1, m1]
+    out += [f8(snap[k]) for k in ("d", "v1", "v2", "v3", "e")]
+    out.append(m1)
+    if compswitch >= 2:
+        names = snap["species"]
+        spn = len(names)
+        spc = np.stack([np.asarray(snap[f"spc{q + 1}"]).reshape(-1) for q in range(spn)], 1)
+        out += [m1, np.int32(spn).tobytes(), f8(spc)]
+        out += [nm.encode().ljust(10)[:10] for nm in names]
+        out.append(m1)
+    if mag_on:
+        out += [m1] + [f8(snap[k]) for k in ("b1", "b2", "b3", "phi")] + [m1]
+    return b"".join(out)

@@ -14,7 +14,7 @@ Two sources, both data produced by the reference itself:
    full run to t=0.5).  The binary runs in a scratch directory under /tmp
    whose ../para points at the reference's parameter files.
 
-Usage: python tests/golden/make_fixtures.py [--skip-long | --extra-only | --kh-only]
+Usage: python tests/golden/make_fixtures.py [--skip-long | --extra-only | --kh-only | --restart-only]
 """
 from __future__ import annotations
 
@@ -111,6 +111,52 @@ def run_reference(simtype: str, params: str, threads: int = 8) -> str:
     bins = glob.glob(os.path.join(run, "work", "data", "bin*.dat"))
     # the final snapshot is the one with the largest step count
     return max(bins, key=lambda b: read_header(b)[0])
+
+
+def restart_reference(simtype: str, params: str, start: int, threads: int = 8) -> str:
+    """Restart the reference program in the scratch directory of the last
+    run_reference from its snapshot at step `start` (startfile start=N,
+    restart.f90:15-64); returns the last snapshot written."""
+    run = "/tmp/hormone_fixture_run"
+    with open(os.path.join(run, "work", "startfile"), "w") as f:
+        f.write(f"&startcon\n start={start}\n simtype='{simtype}'\n parafile='parameters'\n/\n")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    before = set(glob.glob(os.path.join(run, "work", "data", "bin*.dat")))
+    with open(os.path.join(run, "log_restart.txt"), "w") as log:
+        subprocess.run([BIN], cwd=os.path.join(run, "work"), env=env, check=True,
+                       stdout=log, stderr=subprocess.STDOUT)
+    bins = glob.glob(os.path.join(run, "work", "data", "bin*.dat"))
+    return max(bins, key=lambda b: read_header(b)[0])
+
+
+RESTART_PARAMS = ("&gridcon ie=64 je=64 /\n&out_con outstyle=2 endstyle=2 tnlim=100 "
+                  "tn_out=50 output_ascii=.false. /\n")
+
+
+def reference_restart_runs(meta: dict):
+    """Orszag-Tang 64^2: a run to step 100 with snapshots every 50 steps,
+    then a restart from the step-50 snapshot to step 100.  Stored: the
+    step-50 snapshot (input of the restart) as arrays plus its file digest,
+    and digests of both step-100 states (uninterrupted and restarted)."""
+    b100 = run_reference("orszagtang_xy", RESTART_PARAMS)
+    snap100 = read_bin(b100, (1, 64, 64), True)
+    b50 = [b for b in glob.glob(os.path.join(os.path.dirname(b100), "bin*.dat"))
+           if read_header(b)[0] == 50][0]
+    raw50 = open(b50, "rb").read()
+    snap50 = read_bin(b50, (1, 64, 64), True)
+    np.savez_compressed(os.path.join(HERE, "ot64_step50.npz"), tn=snap50["tn"],
+                        time=snap50["time"], **{k: snap50[k].reshape(-1) for k in FIELDS_MAG})
+    meta["ot64_step50"] = summary(snap50, FIELDS_MAG)
+    meta["ot64_step50"]["file_sha256"] = hashlib.sha256(raw50).hexdigest()
+    meta["ot64_step50"]["file_bytes"] = len(raw50)
+    meta["ot64_step50"]["source"] = "reference run: orszagtang_xy ie=je=64, snapshot at tn=50"
+    meta["ot64_step100"] = summary(snap100, FIELDS_MAG)
+    meta["ot64_step100"]["source"] = "reference run: orszagtang_xy ie=je=64, tnlim=100"
+    br = restart_reference("orszagtang_xy", RESTART_PARAMS, 50)
+    snapr = read_bin(br, (1, 64, 64), True)
+    meta["ot64_restart50_step100"] = summary(snapr, FIELDS_MAG)
+    meta["ot64_restart50_step100"]["source"] = (
+        "reference run: orszagtang_xy ie=je=64 restarted (start=50) from its tn=50 snapshot, to tnlim=100")
 
 
 def read_header(path):
@@ -233,6 +279,14 @@ def reference_runs_extra(meta: dict):
 
 def main():
     long = "--skip-long" not in sys.argv
+    if "--restart-only" in sys.argv:
+        path = os.path.join(HERE, "golden_meta.json")
+        meta = json.load(open(path))
+        reference_restart_runs(meta)
+        with open(path, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+        print("wrote", path)
+        return
     if "--kh-only" in sys.argv:
         path = os.path.join(HERE, "golden_meta.json")
         meta = json.load(open(path))
@@ -254,6 +308,7 @@ def main():
     convert_reference_goldens(meta)
     reference_runs(meta, long)
     reference_runs_extra(meta)
+    reference_restart_runs(meta)
     with open(path, "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print("wrote", path)
