@@ -37,7 +37,11 @@ def test_create_rejects_unsupported(hiplib):
     g = cartesian_grid((8, 1, 1))
     for kw, code in ((dict(bc=(4,) * 12), native.HG_ERR_UNSUPPORTED),
                      (dict(rktype=4), native.HG_ERR_ARG),
-                     (dict(bc=(0, 3) + (3,) * 10), native.HG_ERR_ARG)):
+                     (dict(bc=(0, 3) + (3,) * 10), native.HG_ERR_ARG),
+                     (dict(compswitch=1), native.HG_ERR_UNSUPPORTED),
+                     (dict(compswitch=3, spn=2), native.HG_ERR_UNSUPPORTED),
+                     (dict(compswitch=2, spn=0), native.HG_ERR_ARG),
+                     (dict(compswitch=2, spn=9), native.HG_ERR_ARG)):
         d, keep = native.make_desc(g, **kw)
         h = ctypes.c_void_p()
         assert hiplib.hg_create(ctypes.byref(d), ctypes.byref(h)) == code
