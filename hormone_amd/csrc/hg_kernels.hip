@@ -1060,18 +1060,28 @@ __global__ __launch_bounds__(256) void k_cfl(Dev D) {
 
 // Final CFL reduction and dt (timestep.f90:74-89): dt = courant*min(dti),
 // clipped to t_out / t_end as the reference does; ch = max fast speed.
+// One block of up to 1024 threads; each thread folds every blockDim-th
+// partial pair, four loads in flight.
 __device__ __forceinline__ void cfl_final(const Dev &D) {
   Scalars *sc = D.sc;
   if (sc->stop) return;
-  __shared__ double smin[4], smax[4];
+  __shared__ double smin[16], smax[16];
   double a = __builtin_inf(), b = 0.0;
-  for (int q = threadIdx.x; q < D.nblk_cfl; q += blockDim.x) {
-    a = mn2(a, D.part[2 * q]);
-    b = mx2(b, D.part[2 * q + 1]);
+  const double2 *P = reinterpret_cast<const double2 *>(D.part);
+  const int nb = D.nblk_cfl, st = blockDim.x;
+  int q = threadIdx.x;
+  for (; q + 3 * st < nb; q += 4 * st) {
+    const double2 p0 = P[q], p1 = P[q + st], p2 = P[q + 2 * st], p3 = P[q + 3 * st];
+    a = mn2(mn2(mn2(mn2(a, p0.x), p1.x), p2.x), p3.x);
+    b = mx2(mx2(mx2(mx2(b, p0.y), p1.y), p2.y), p3.y);
+  }
+  for (; q < nb; q += st) {
+    a = mn2(a, P[q].x);
+    b = mx2(b, P[q].y);
   }
   a = wave_min(a);
   b = wave_max(b);
-  const int w = threadIdx.x >> 6;
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     smin[w] = a;
     smax[w] = b;
@@ -1080,7 +1090,7 @@ __device__ __forceinline__ void cfl_final(const Dev &D) {
   if (threadIdx.x == 0) {
     a = smin[0];
     b = smax[0];
-    for (int q = 1; q < 4; ++q) {
+    for (int q = 1; q < nw; ++q) {
       a = mn2(a, smin[q]);
       b = mx2(b, smax[q]);
     }
@@ -1102,7 +1112,7 @@ __device__ __forceinline__ void cfl_final(const Dev &D) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_cfl_final(Dev D) { cfl_final(D); }
+__global__ __launch_bounds__(1024) void k_cfl_final(Dev D) { cfl_final(D); }
 
 // End of one main-loop iteration (hormone.f90:147-177): advance the clock,
 // then the end and output checks in the reference's order.
@@ -1229,7 +1239,7 @@ __global__ __launch_bounds__(256) void k_brk(Batch B, int stage, double faco, do
 
 // The end of one step and the dt of the next in one launch: step_end on one
 // thread, then (if the run goes on) the CFL reduction of cfl_final.
-__global__ __launch_bounds__(256) void k_step_end_cfl(Dev D) {
+__global__ __launch_bounds__(1024) void k_step_end_cfl(Dev D) {
   if (threadIdx.x == 0) step_end(D);
   __syncthreads();
   cfl_final(D);
@@ -1450,7 +1460,15 @@ void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev) {
   else klaunch(k_cfl<false>, (unsigned)D.nblk_cfl, 256, s, ev, D);
 }
 
-void launch_cfl_final(const Dev &D, hipStream_t s) { k_cfl_final<<<1, 256, 0, s>>>(D); }
+// threads of the final CFL reduction: enough to keep many partial loads in
+// flight on large grids, one wavefront on small ones
+static inline int cfl_final_threads(const Dev &D) {
+  return D.nblk_cfl > 16384 ? 1024 : (D.nblk_cfl > 2048 ? 512 : 256);
+}
+
+void launch_cfl_final(const Dev &D, hipStream_t s) {
+  k_cfl_final<<<1, cfl_final_threads(D), 0, s>>>(D);
+}
 
 
 void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
@@ -1495,7 +1513,9 @@ void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, do
   }
 }
 
-void launch_step_end_cfl(const Dev &D, hipStream_t s) { k_step_end_cfl<<<1, 256, 0, s>>>(D); }
+void launch_step_end_cfl(const Dev &D, hipStream_t s) {
+  k_step_end_cfl<<<1, cfl_final_threads(D), 0, s>>>(D);
+}
 
 void launch_bstep_end_cfl(const Batch &B, hipStream_t s) {
   k_bstep_end_cfl<<<B.nbox, 256, 0, s>>>(B);
