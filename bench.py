@@ -86,6 +86,9 @@ def parse():
                          "(comma separated; empty = one batch)")
     ap.add_argument("--sweep-waves", type=int, default=0,
                     help="sweep wavefronts per block (0: autotune during warm-up)")
+    ap.add_argument("--fused", default="auto", choices=["auto", "on", "off"],
+                    help="last sweep direction fused with the RK update (auto: time both "
+                         "during warm-up)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (the reference kernel for "
                          "scaling rocprof FETCH_SIZE/WRITE_SIZE to bytes)")
@@ -124,14 +127,20 @@ def measured_traffic(workload, precision, kernel):
     return None if e is None else e.get("bytes_per_launch")
 
 
+# steps the warm-up may spend on launch tuning beyond --warmup (the run's
+# step limit leaves room for them)
+TUNE_STEPS = 64
+
+
 def setup(a, precision):
     from hormone_amd import problems as P
+    limit = a.warmup + a.steps + TUNE_STEPS
     if a.workload == "ot512":
         return P.make_problem("orszagtang_xy", n=a.n or 512, precision=precision)
     if a.workload == "polytrope256":
-        return P.make_problem("polytrope3d", n=a.n or 256, steps=a.warmup + a.steps,
-                              mag_on=a.mag, precision=precision)
-    return P.make_problem("kh3d", n=a.n or 384, steps=a.warmup + a.steps, precision=precision)
+        return P.make_problem("polytrope3d", n=a.n or 256, steps=limit, mag_on=a.mag,
+                              precision=precision)
+    return P.make_problem("kh3d", n=a.n or 384, steps=limit, precision=precision)
 
 
 def cpu_baseline(a, seconds: float):
@@ -355,6 +364,19 @@ def main():
         left = a.warmup - per * len(times)
         if left > 0:
             run.advance(left)
+    fused = False
+    if a.fused == "on":
+        eng.set_fused(True)
+        fused = True
+        tuned = dict(tuned or {})
+        tuned["fused_last_sweep"] = True
+    elif a.fused == "auto":
+        from hormone_amd.batch import tune_fused
+        fused, ftimes = tune_fused(eng, run.advance, steps=max(2, min(12, a.warmup // 4)))
+        tuned = dict(tuned or {})
+        tuned["fused_last_sweep"] = fused
+        tuned["ms_per_step"] = {("fused" if k else "separate"): round(v * 1e3, 3)
+                                for k, v in ftimes.items()}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ok = run.advance(a.steps)

@@ -185,3 +185,29 @@ def tune_single(eng: native.Hydro, advance, candidates=(2, 4, 8, 16), steps: int
     best = min(timings, key=timings.get) if timings else 4
     eng.set_launch(best)
     return best, timings
+
+
+def tune_fused(eng: native.Hydro, advance, steps: int = 4):
+    """Choose between the separate RK update and the last sweep direction
+    fused with it (hg_set_fused) by timing whole steps of each, on the host
+    clock; ``advance(k)`` must take k steps and return after they finish.
+    Returns (fused, {False: s/step, True: s/step}); (False, {}) when the
+    context does not support the fused form.  Both forms give the same
+    bits, so the tuning steps are ordinary steps of the run."""
+    try:
+        eng.set_fused(True)
+    except native.HgError as err:
+        if err.code != native.HG_ERR_UNSUPPORTED:
+            raise
+        return False, {}
+    timings = {}
+    for on in (True, False, True, False):
+        eng.set_fused(on)
+        advance(1)
+        t0 = time.perf_counter()
+        advance(steps)
+        dt = (time.perf_counter() - t0) / steps
+        timings[on] = min(dt, timings.get(on, dt))
+    best = timings[True] < timings[False]
+    eng.set_fused(best)
+    return best, timings

@@ -37,6 +37,8 @@ struct hg_ctx {
   std::vector<int> ev_kind;     // kind of each recorded pair
   int ev_used = 0;
   KEv kev{};
+  bool fused = false;   // last sweep direction fused with the RK update
+  long part_cap = 0;    // CFL partial slots allocated
 };
 
 #define HIPCHK(x)                                   \
@@ -55,6 +57,19 @@ static int dev_alloc(hg_ctx *c, size_t bytes, void **p) {
   return HG_OK;
 }
 
+// The codes each face accepts (the cases of boundarycondition.f90's select
+// blocks; any other code stops the reference with "Error from x? ...
+// boundary condition"): 0-3, 9 and 10 everywhere; linear 4/5 at x1 outer
+// (:302, :419, :441), x3 inner (:963, :1074, :1096) and x3 outer, where
+// the scalar code takes 4 only (:1222; vectors :1335, :1357).  b indexes
+// hg_desc.bc: 0..5 scalar codes (1is 1os 2is 2os 3is 3os), 6..11 vector.
+static bool bc_accepted(int code, int b) {
+  if ((code >= 0 && code <= 3) || code == 9 || code == 10) return true;
+  const int face = b % 6;
+  if (code == 4 || code == 5) return face == 1 || face == 4 || (face == 5 && (b >= 6 || code == 4));
+  return false;
+}
+
 static int validate(const hg_desc *d) {
   if (!d) return HG_ERR_ARG;
   if (d->crdnt != 0 || d->eostype != 0 || (d->compswitch != 0 && d->compswitch != 2))
@@ -64,7 +79,12 @@ static int validate(const hg_desc *d) {
   if (d->rktype < 1 || d->rktype > 3) return HG_ERR_ARG;
   if (d->flux_limiter < 0 || d->flux_limiter > 2) return HG_ERR_ARG;
   for (int b = 0; b < 12; ++b)
-    if (d->bc[b] < 0 || d->bc[b] > 3) return HG_ERR_UNSUPPORTED;
+    if (!bc_accepted(d->bc[b], b)) return HG_ERR_UNSUPPORTED;
+  // species ghosts under Dirichlet (spc0) and flux (spcflx) boundaries are
+  // not carried
+  if (d->compswitch == 2)
+    for (int b = 0; b < 12; ++b)
+      if (d->bc[b] >= 9) return HG_ERR_UNSUPPORTED;
   // checksetup.f90:82-107: periodic must pair with periodic
   for (int q = 0; q < 6; ++q) {
     int a = d->bc[2 * q], b = d->bc[2 * q + 1];
@@ -159,6 +179,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
       A.sc_hi = desc->bc[2 * a + 1];
       A.vc_lo = desc->bc[6 + 2 * a];
       A.vc_hi = desc->bc[6 + 2 * a + 1];
+      A.ext = A.sc_lo >= 4 || A.sc_hi >= 4 || A.vc_lo >= 4 || A.vc_hi >= 4;
     }
     if (hipMemcpyAsync(c->coord, h.data(), tot * sizeof(double), hipMemcpyHostToDevice,
                        c->stream) != hipSuccess ||
@@ -242,6 +263,8 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
     D.lds_max = lds;
   }
   if ((st = dev_alloc(c, sizeof(double) * 2 * D.nblk_cfl, (void **)&D.part))) return fail(st);
+  c->part_cap = D.nblk_cfl;
+  D.fdir = -1;
   if (desc->gravswitch == 3) {
     D.grav = 1;
     D.fourpig = 4.0 * std::acos(-1.0) * desc->G;
@@ -270,6 +293,32 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
           m = v < m ? v : m;
         }
     c->dtg_unit = m;
+  }
+  // ghost values and boundary fluxes of the faces with codes 4, 5, 9, 10:
+  // G starts at the reference's allocation values (allocations.f90:71-78:
+  // d = p = 1, v = B = phi = 0), FX at zero
+  for (int a = 0; a < 3; ++a) {
+    Axis &A = D.ax[a];
+    for (int side = 0; side < 2; ++side) {
+      const int sc = side ? A.sc_hi : A.sc_lo, vc = side ? A.vc_hi : A.vc_lo;
+      if (sc < 4 && vc < 4) continue;
+      const size_t nl = A.nlines, ng = (size_t)NGF * 2 * nl;
+      std::vector<double> g(ng, 0.0);
+      std::fill(g.begin() + GF_D * 2 * nl, g.begin() + (GF_D + 1) * 2 * nl, 1.0);
+      std::fill(g.begin() + GF_P * 2 * nl, g.begin() + (GF_P + 1) * 2 * nl, 1.0);
+      double *gp = nullptr, *fx = nullptr;
+      if ((st = dev_alloc(c, sizeof(double) * ng, (void **)&gp))) return fail(st);
+      if ((st = dev_alloc(c, sizeof(double) * NVMAX, (void **)&fx))) return fail(st);
+      const double zero[NVMAX] = {};
+      if (hipMemcpyAsync(gp, g.data(), sizeof(double) * ng, hipMemcpyHostToDevice,
+                         c->stream) != hipSuccess ||
+          hipMemcpyAsync(fx, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream) !=
+              hipSuccess ||
+          hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(HG_ERR_DEVICE);
+      A.G[side] = gp;
+      A.FX[side] = fx;
+    }
   }
   if ((st = dev_alloc(c, sizeof(Scalars), (void **)&D.sc))) return fail(st);
   // Every initialisation write goes to the context's own (non-blocking)
@@ -464,6 +513,15 @@ static const KEv *prof_ev(hg_ctx *c, int kind) {
   return &c->kev;
 }
 
+// prof_ev's pair copied into `slot`: prof_ev hands out one KEv per context,
+// which the next call overwrites
+static const KEv *prof_ev_into(hg_ctx *c, int kind, KEv &slot) {
+  const KEv *e = prof_ev(c, kind);
+  if (!e) return nullptr;
+  slot = *e;
+  return &slot;
+}
+
 // after the stream has been synchronised: fold recorded pairs into totals
 static void prof_collect(hg_ctx *c) {
   for (int q = 0; q < c->ev_used; ++q) {
@@ -628,18 +686,36 @@ extern "C" int hg_gravity(hg_ctx *c, double time, double dt, double *grvtime, do
   return HG_OK;
 }
 
-static void enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
+// The RK stages of one step.  Returns the number of CFL partials the last
+// stage wrote (fuse_tail): one per 256 cells, or one per block of the fused
+// last sweep.
+static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
   const Dev &D = c->D;
   const int rk = c->desc.rktype;
+  int npart = D.nblk_cfl;
   for (int stage = 1; stage <= rk; ++stage) {
     double faco, fact, facn;
     runge_coeff(stage, rk, &faco, &fact, &facn);
+    if (c->fused) {
+      const bool last = stage == rk;
+      RKArgs a{faco, fact, facn, damp, fuse_tail && last, fuse_tail && last && damp > 0.0,
+               D.grav};
+      // a pair for the sweep of the directions before the fused one only
+      // when there are such directions (every pair taken must be recorded)
+      KEv k0{}, k1{};
+      const KEv *e0 = D.woff[D.fdir] > 0 ? prof_ev_into(c, HG_PROF_SWEEP, k0) : nullptr;
+      const KEv *e1 = prof_ev_into(c, HG_PROF_RK, k1);
+      const int g = launch_fused_stage(D, stage, a, c->stream, e0, e1);
+      if (last && fuse_tail) npart = g;
+      continue;
+    }
     launch_sweep(D, stage, c->stream, prof_ev(c, HG_PROF_SWEEP));
     launch_species(D, c->stream);
     const bool last = stage == rk;
     launch_rk(D, stage, faco, fact, facn, fuse_tail && last, fuse_tail && last && damp > 0.0,
               damp, c->stream, prof_ev(c, HG_PROF_RK));
   }
+  return npart;
 }
 
 extern "C" int hg_hydro_step(hg_ctx *c, double dt, double ch) {
@@ -696,17 +772,20 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
     }
     const int ev_base = c->ev_used;
     for (int s = 0; s < nb; ++s) {
-      enqueue_stages(c, true, damp ? damp_factor : 0.0);
-      launch_step_end_cfl(D, c->stream);
+      Dev De = D;  // the step end reduces the partials the last stage wrote
+      De.nblk_cfl = enqueue_stages(c, true, damp ? damp_factor : 0.0);
+      launch_step_end_cfl(De, c->stream);
     }
     left -= nb;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_sc, D.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    // time only the launches of steps that ran (2 timed launches per stage)
+    // time only the launches of steps that ran (every step takes the same
+    // number of event pairs)
     const int64_t ran = c->h_sc->tn - tn_before;
     tn_before = c->h_sc->tn;
-    if (c->prof) c->ev_used = ev_base + (int)(std::min<int64_t>(ran, nb) * 2 * c->desc.rktype);
+    if (c->prof)
+      c->ev_used = ev_base + (int)(std::min<int64_t>(ran, nb) * ((c->ev_used - ev_base) / nb));
     prof_collect(c);
     reason = c->h_sc->stop;
     if (reason != STOP_NONE) break;
@@ -727,6 +806,97 @@ extern "C" int hg_set_launch(hg_ctx *c, int32_t sweep_block_waves) {
   if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) return HG_ERR_ARG;
   HIPCHK(hipStreamSynchronize(c->stream));
   c->D.sweep_bw = w;
+  return HG_OK;
+}
+
+extern "C" int hg_set_ghosts(hg_ctx *c, int32_t field, const double *padded) {
+  if (!c || !padded) return HG_ERR_ARG;
+  int gf;
+  switch (field) {
+    case HG_F_D: gf = GF_D; break;
+    case HG_F_P: gf = GF_P; break;
+    case HG_F_V1: case HG_F_V2: case HG_F_V3: gf = GF_V1 + (field - HG_F_V1); break;
+    case HG_F_B1: case HG_F_B2: case HG_F_B3: gf = GF_B1 + (field - HG_F_B1); break;
+    case HG_F_PHI: gf = GF_PHI; break;
+    default: return HG_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  Dev &D = c->D;
+  const long nx = c->nxg, ny = c->nyg;
+  auto at = [&](long i, long j, long k) { return padded[(i + 2) + nx * ((j + 2) + ny * (k + 2))]; };
+  std::vector<double> h;
+  for (int a = 0; a < 3; ++a) {
+    const Axis &A = D.ax[a];
+    for (int side = 0; side < 2; ++side) {
+      if (!A.G[side]) continue;
+      const long nl = A.nlines;
+      h.assign(2 * (size_t)nl, 0.0);
+      for (int layer = 0; layer < 2; ++layer) {
+        const long g = side ? A.n + layer : -1 - layer;  // ghost index along the axis
+        for (long line = 0; line < nl; ++line) {
+          double v;
+          if (a == 0) v = at(g, line % D.n2, line / D.n2);
+          else if (a == 1) v = at(line % D.n1, g, line / D.n1);
+          else v = at(line % D.n1, line / D.n1, g);
+          h[(size_t)layer * nl + line] = v;
+        }
+      }
+      HIPCHK(hipMemcpyAsync(const_cast<double *>(A.G[side]) + (size_t)gf * 2 * nl, h.data(),
+                            sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  }
+  return HG_OK;
+}
+
+extern "C" int hg_set_boundary_flux(hg_ctx *c, int32_t face, const double *flux) {
+  if (!c || !flux || face < 0 || face > 5) return HG_ERR_ARG;
+  const Axis &A = c->D.ax[face / 2];
+  const int side = face % 2;
+  if ((side ? A.sc_hi : A.sc_lo) != 10) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  double f[NVMAX] = {};
+  for (int v = 0; v < c->D.nv; ++v) f[v] = flux[v];
+  HIPCHK(hipMemcpyAsync(const_cast<double *>(A.FX[side]), f, sizeof(f), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return HG_OK;
+}
+
+extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
+  if (!c) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  Dev &D = c->D;
+  if (!on) {
+    c->fused = false;
+    D.fdir = -1;
+    return HG_OK;
+  }
+  // FP64 only, no species; the last solved direction must be y or z
+  int L = -1;
+  for (int a = 0; a < 3; ++a)
+    if (D.solve[a]) L = a;
+  if (D.mixed || D.nspc || L < 1) return HG_ERR_UNSUPPORTED;
+  for (int a = 0; a < 3; ++a)
+    if (D.ax[a].ext) return HG_ERR_UNSUPPORTED;  // the halo copy knows codes 0-3 only
+  if (!D.H) {
+    const int K = (D.ax[L].n + SEG - 1) / SEG;
+    D.hsz = (long)D.ax[L].nlines * (K + 1) * 4;
+    int st = dev_alloc(c, sizeof(double) * (size_t)D.hsz * D.nw, (void **)&D.H);
+    if (st) return st;
+  }
+  // one CFL partial per block of the fused launch (one wavefront per block
+  // at most)
+  if (D.waves[L] > c->part_cap) {
+    double *p = nullptr;
+    int st = dev_alloc(c, sizeof(double) * 2 * (size_t)D.waves[L], (void **)&p);
+    if (st) return st;
+    D.part = p;
+    c->part_cap = D.waves[L];
+  }
+  D.fdir = L;
+  c->fused = true;
   return HG_OK;
 }
 
@@ -828,7 +998,9 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
                             double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
                             int64_t tn_out) {
   if (!b || !c) return HG_ERR_ARG;
-  if (c->device != b->device || c->D.grav || c->D.nspc) return HG_ERR_UNSUPPORTED;
+  if (c->device != b->device || c->D.grav || c->D.nspc || c->D.ax[0].ext || c->D.ax[1].ext ||
+      c->D.ax[2].ext)
+    return HG_ERR_UNSUPPORTED;
   for (hg_ctx *m : b->members)
     if (m == c) return HG_ERR_ARG;
   if (!b->members.empty() &&

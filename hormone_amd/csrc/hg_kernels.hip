@@ -19,7 +19,6 @@
 
 namespace hg {
 
-static constexpr int SEG = 60;  // outputs per 64-lane wavefront
 
 struct Cell {
   double d, v[3], e, eint, b[3], phi;
@@ -37,8 +36,10 @@ __device__ __forceinline__ int ghost_src(int code, bool lo, int c, int n) {
 // One cell of a sweep line, interior or ghost, from the values stored at its
 // source cells: f[] holds W[d v1 v2 v3 e eint b1 b2 b3 phi], the scalars
 // (d, e, eint, phi) read at the scalar-code source and v, B at the
-// vector-code source.  Ghost cells (boundarycondition.f90, codes 0..3)
-// transform the normal components, then rebuild eint and e (:221-243).
+// vector-code source.  Ghost cells (boundarycondition.f90) transform the
+// normal components (codes 0..3), then rebuild eint and e (:221-243) from
+// the ghost pressure: gm1*eint of the source cell, or, for scalar codes 4,
+// 5, 9 and 10, the pressure ghost_ext left in f[WEINT].
 template <int DIR, bool MAG>
 __device__ __forceinline__ Cell finish_cell(const Dev &D, const Axis &A, int c,
                                             const double f[NWMAX]) {
@@ -47,13 +48,14 @@ __device__ __forceinline__ Cell finish_cell(const Dev &D, const Axis &A, int c,
   const bool ghost = c < 0 || c >= n;
   const bool lo = c < 0;
   const int vc = lo ? A.vc_lo : A.vc_hi;
+  const int sc = lo ? A.sc_lo : A.sc_hi;
   q.d = f[WD];
   q.phi = MAG ? f[WPHI] : 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     double v = f[WV1 + a];
     double b = MAG ? f[WB1 + a] : 0.0;
-    if (ghost && a == DIR) {
+    if (ghost && a == DIR && vc <= 3) {
       if (vc == 1) {
         v = -v;
         b = -b;
@@ -70,7 +72,7 @@ __device__ __forceinline__ Cell finish_cell(const Dev &D, const Axis &A, int c,
     q.e = f[WE];
   } else {
     const double gm1 = D.gamma - 1.0;
-    const double p = gm1 * f[WEINT];
+    const double p = sc >= 4 ? f[WEINT] : gm1 * f[WEINT];
     q.eint = p / gm1;
     q.e = q.eint + 0.5 * (q.d * (q.v[0] * q.v[0] + q.v[1] * q.v[1] + q.v[2] * q.v[2]) +
                           q.b[0] * q.b[0] + q.b[1] * q.b[1] + q.b[2] * q.b[2]);
@@ -91,9 +93,115 @@ __device__ __forceinline__ void cell_sources(const Axis &A, long base, int c, lo
   }
 }
 
-// One cell read straight from HBM by its own lane.
+// Ghost cell c of line `line` at a face whose scalar or vector code is 4, 5
+// (linear extrapolation), 9 (Dirichlet) or 10 (flux): overwrites the raw
+// values the code defines.  Scalars: d and the pressure (left in f[WEINT])
+// and phi; vectors: v and B.
+// * 4/5, boundarycondition.f90:302-326 (x1 outer), :963-985 (x3 inner),
+//   :1222-1244 (x3 outer): q(ghost 1) and q(ghost 2) extrapolated from the
+//   edge cell and its inner neighbour with the cell-width ratios, d and p
+//   reset to the edge value when the outer ghost is not positive, phi (and
+//   species) copied from the edge.  Vectors :419-468, :1074-1120,
+//   :1335-1382; 5 also multiplies the normal velocity by the outgoing plug
+//   of the edge velocity.  Below the x3 inner boundary the reference never
+//   writes v2 and b2 of the outer ghost (:1081,1088,1104,1114 assign v3/b3
+//   there), which keep the values of G.
+// * 9: d0, p0, v0, b0 of the ghost cells (:104-116, :196-212, ...), here G;
+//   phi is not written by the reference, G again.
+// * 10: the reference writes no ghost value at all: G.
 template <int DIR, bool MAG>
-__device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base, int c) {
+__device__ __forceinline__ void ghost_ext(const Dev &D, const Axis &A, long base, long line,
+                                          int c, double f[NWMAX]) {
+  const int n = A.n;
+  const bool lo = c < 0;
+  const int sc = lo ? A.sc_lo : A.sc_hi, vc = lo ? A.vc_lo : A.vc_hi;
+  if (sc < 4 && vc < 4) return;
+  const int layer = (c == -1 || c == n) ? 0 : 1;  // lanes past the ghosts: unused
+  const long nl = A.nlines;
+  const double *G = A.G[lo ? 0 : 1] + (long)layer * nl + line;  // field g at G[g * 2 * nl]
+  const long me = base + (long)(lo ? 0 : n - 1) * A.stride;      // edge cell
+  const long mn = base + (long)(lo ? 1 : n - 2) * A.stride;      // its inner neighbour
+  const double r1 = A.dx[lo ? 0 : n - 1], r1d = A.dx[lo ? 1 : n - 2];
+  const double r2 = A.dx[lo ? -1 : n], r2d = A.dx[lo ? 0 : n - 1];
+  auto lin = [&](double qe, double qn, double &q1, double &q2) {
+    if (lo) {
+      q1 = qe - (qn - qe) * r1 / r1d;
+      q2 = q1 - (qe - q1) * r2 / r2d;
+    } else {
+      q1 = qe + (qe - qn) * r1 / r1d;
+      q2 = q1 + (q1 - qe) * r2 / r2d;
+    }
+  };
+  if (sc == 4 || sc == 5) {
+    const double gm1 = D.gamma - 1.0;
+    double q1, q2;
+    const double de = D.W[WD][me];
+    lin(de, D.W[WD][mn], q1, q2);
+    f[WD] = q2 <= 0.0 ? de : (layer ? q2 : q1);
+    const double pe = gm1 * D.W[WEINT][me];
+    lin(pe, gm1 * D.W[WEINT][mn], q1, q2);
+    f[WEINT] = q2 <= 0.0 ? pe : (layer ? q2 : q1);
+    if (MAG) f[WPHI] = D.W[WPHI][me];
+  } else if (sc >= 9) {
+    f[WD] = G[GF_D * 2 * nl];
+    f[WEINT] = G[GF_P * 2 * nl];
+    if (MAG) f[WPHI] = G[GF_PHI * 2 * nl];
+  }
+  if (vc == 4 || vc == 5) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double q1, q2;
+      lin(D.W[WV1 + a][me], D.W[WV1 + a][mn], q1, q2);
+      f[WV1 + a] = layer ? q2 : q1;
+      if (MAG) {
+        lin(D.W[WB1 + a][me], D.W[WB1 + a][mn], q1, q2);
+        f[WB1 + a] = layer ? q2 : q1;
+      }
+    }
+    if (DIR == 2 && lo && layer == 1) {
+      f[WV2] = G[GF_V2 * 2 * nl];
+      if (MAG) f[WB2] = G[GF_B2 * 2 * nl];
+    }
+    if (vc == 5) {
+      const double vn = D.W[WV1 + DIR][me];
+      f[WV1 + DIR] = f[WV1 + DIR] * (lo ? 0.5 - sgn(0.5, vn) : 0.5 + sgn(0.5, vn));
+    }
+  } else if (vc >= 9) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      f[WV1 + a] = G[(GF_V1 + a) * 2 * nl];
+      if (MAG) f[WB1 + a] = G[(GF_B1 + a) * 2 * nl];
+    }
+  }
+}
+
+// Raw values of cell c of a line (interior, or ghost by its face's code).
+template <int DIR, bool MAG>
+__device__ __forceinline__ void raw_cell(const Dev &D, const Axis &A, long base, long line,
+                                         int c, double f[NWMAX]) {
+  long ms, mv;
+  cell_sources(A, base, c, ms, mv);
+  f[WD] = D.W[WD][ms];
+  f[WE] = D.W[WE][ms];
+  f[WEINT] = D.W[WEINT][ms];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) f[WV1 + a] = D.W[WV1 + a][mv];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) f[WB1 + a] = MAG ? D.W[WB1 + a][mv] : 0.0;
+  f[WPHI] = MAG ? D.W[WPHI][ms] : 0.0;
+  if (A.ext && (c < 0 || c >= A.n)) ghost_ext<DIR, MAG>(D, A, base, line, c, f);
+}
+
+// One cell read straight from HBM by its own lane.
+// EXT = false: the caller's boxes have no code 4, 5, 9 or 10 (batches).
+template <int DIR, bool MAG, bool EXT = true>
+__device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base, long line,
+                                          int c) {
+  if (EXT && A.ext) {
+    double f[NWMAX];
+    raw_cell<DIR, MAG>(D, A, base, line, c, f);
+    return finish_cell<DIR, MAG>(D, A, c, f);
+  }
   long ms, mv;
   cell_sources(A, base, c, ms, mv);
   double f[NWMAX];
@@ -137,7 +245,85 @@ __device__ __forceinline__ void wave_line(const Dev &D, const Axis &A, unsigned 
   else base = line;
 }
 
+// ---- halo of the last direction (the sweep fused with the RK update) --------
+// The fused launch updates each cell in place right after its own line
+// segment has swept it, so a segment must not read its neighbours' cells
+// from the state (they may already hold new values).  Before the launch,
+// k_halo copies, for every line of that direction, the cells a segment
+// reads but does not own -- the two cells on either side of each segment
+// boundary and the ghosts' source cells -- into H: per line K+1 groups of 4
+// (K segments), group g holding cells 60g-2 .. 60g+1 and group K the
+// ghosts n, n+1 (raw source values; the loader applies the ghost rules).
+// H is field-major, H[f * hsz + entry], f the primitive index.
+__device__ __forceinline__ long halo_entry(int n, long line, int seg, int c) {
+  const int K = (n + SEG - 1) / SEG;
+  int g, slot;
+  if (c >= n) {
+    g = K;
+    slot = c - n < 1 ? c - n : 1;  // n, n+1 (lanes past n+1 are not used)
+  } else if (c < seg * SEG) {
+    g = seg;
+    slot = c - (seg * SEG - 2);
+  } else {
+    g = seg + 1;
+    slot = c - ((seg + 1) * SEG - 2);
+  }
+  return (line * (K + 1) + g) * 4 + slot;
+}
+
+__device__ __forceinline__ bool owned(int n, int seg, int c) {
+  return c >= seg * SEG && c < seg * SEG + SEG && c < n;
+}
+
+// raw primitives of cell c of a line (own cells from the state, the others
+// from the halo copy)
+template <bool MAG>
+__device__ __forceinline__ void fused_raw(const Dev &D, const Axis &A, long base, long line,
+                                          int seg, int c, double f[NWMAX]) {
+  const int nf = MAG ? NWMAX : WEINT + 1;
+  if (owned(A.n, seg, c)) {
+    const long m = base + (long)c * A.stride;
+#pragma unroll
+    for (int k = 0; k < NWMAX; ++k) f[k] = k < nf ? D.W[k][m] : 0.0;
+  } else {
+    const long e = halo_entry(A.n, line, seg, c);
+#pragma unroll
+    for (int k = 0; k < NWMAX; ++k) f[k] = k < nf ? D.H[k * D.hsz + e] : 0.0;
+  }
+}
+
 template <int DIR, bool MAG>
+__global__ __launch_bounds__(256) void k_halo(Dev D) {
+  if (D.sc->stop) return;
+  const Axis &A = D.ax[DIR];
+  const int n = A.n, K = (n + SEG - 1) / SEG;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)A.nlines * (K + 1) * 4) return;
+  const int slot = (int)(e & 3);
+  const long lg = e >> 2;
+  const int g = (int)(lg % (K + 1));
+  const long line = lg / (K + 1);
+  const int c = g < K ? g * SEG - 2 + slot : n + slot;
+  if (c > n + 1) return;  // group K holds two cells
+  long base;
+  int sg;
+  wave_line<DIR>(D, A, (unsigned)line, base, sg);  // line < nlines: segment 0
+  long ms, mv;
+  cell_sources(A, base, c, ms, mv);
+  const long h = D.hsz;
+  D.H[WD * h + e] = D.W[WD][ms];
+  D.H[WE * h + e] = D.W[WE][ms];
+  D.H[WEINT * h + e] = D.W[WEINT][ms];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) D.H[(WV1 + a) * h + e] = D.W[WV1 + a][mv];
+  if (MAG) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) D.H[(WB1 + a) * h + e] = D.W[WB1 + a][mv];
+    D.H[WPHI * h + e] = D.W[WPHI][ms];
+  }
+}
+
+template <int DIR, bool MAG, bool FUSED = false, bool EXT = true>
 __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave0, int c,
                                           double *lds) {
   const int bw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -154,18 +340,34 @@ __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave
       cinfo[2 * col] = bc;
       cinfo[2 * col + 1] = wave0 + col < D.waves[DIR] ? sg : -1;
     }
-    long ms, mv;
-    cell_sources(A, bc, sg * SEG - 2 + r, ms, mv);
     double *p = lds + r * ld + col;
-    p[WD * fs] = D.W[WD][ms];
-    p[WE * fs] = D.W[WE][ms];
-    p[WEINT * fs] = D.W[WEINT][ms];
+    const int cr = sg * SEG - 2 + r;
+    if (FUSED && !owned(A.n, sg, cr)) {
+      const long e = halo_entry(A.n, (long)((unsigned)wc % A.nlines), sg, cr);
+      const int nf = MAG ? NWMAX : WEINT + 1;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) p[(WV1 + a) * fs] = D.W[WV1 + a][mv];
-    if (MAG) {
+      for (int k = 0; k < NWMAX; ++k)
+        if (k < nf) p[k * fs] = D.H[k * D.hsz + e];
+    } else if (EXT && A.ext && (cr < 0 || cr >= A.n)) {
+      double f[NWMAX];
+      raw_cell<DIR, MAG>(D, A, bc, (long)((unsigned)wc % A.nlines), cr, f);
+      const int nf = MAG ? NWMAX : WEINT + 1;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) p[(WB1 + a) * fs] = D.W[WB1 + a][mv];
-      p[WPHI * fs] = D.W[WPHI][ms];
+      for (int k = 0; k < NWMAX; ++k)
+        if (k < nf) p[k * fs] = f[k];
+    } else {
+      long ms, mv;
+      cell_sources(A, bc, cr, ms, mv);
+      p[WD * fs] = D.W[WD][ms];
+      p[WE * fs] = D.W[WE][ms];
+      p[WEINT * fs] = D.W[WEINT][ms];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) p[(WV1 + a) * fs] = D.W[WV1 + a][mv];
+      if (MAG) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) p[(WB1 + a) * fs] = D.W[WB1 + a][mv];
+        p[WPHI * fs] = D.W[WPHI][ms];
+      }
     }
   }
   __syncthreads();
@@ -356,9 +558,45 @@ __device__ __forceinline__ void store_mass_flux(const Dev &D, long line, int c, 
     D.FM[DIR][line * (n + 1) + (c + 1)] = f0;
 }
 
-template <int DIR, bool MAG, int LIM, bool TILE = false, bool SPC = false>
+// Flux boundary (scalar code 10): the reference skips the Riemann solve of
+// the boundary face (numflux.f90:59-62, :232-235, :405-408) and takes the
+// flux fluxboundary sets there (fluxboundary.f90:66-78; the flux array
+// keeps its zero initial value at the faces it does not set).  Face c+1/2
+// of the line is such a face:
+__device__ __forceinline__ bool flux_face(const Axis &A, int c) {
+  return (c == -1 && A.sc_lo == 10) || (c == A.n - 1 && A.sc_hi == 10);
+}
+
+// the flux given for the face (conserved-variable order) in the sweep's
+// face-local order
+template <int DIR>
+__device__ __forceinline__ void boundary_flux(const Axis &A, int c, double t[9]) {
+  const double *F = A.FX[c < 0 ? 0 : 1];
+  constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
+  t[0] = F[0];
+  t[1] = F[1 + a_];
+  t[2] = F[1 + b_];
+  t[3] = F[1 + c_];
+  t[4] = F[4];
+  t[5] = F[5 + a_];
+  t[6] = F[5 + b_];
+  t[7] = F[5 + c_];
+  t[8] = F[8];
+}
+
+// RK update of the cells a fused sweep has just finished (defined with the RK
+// kernels below)
+template <int DIR, bool MAG, bool TILE>
+__device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base, int c,
+                                         int lane, const double du[NVMAX], double *lds,
+                                         int stage, const RKArgs &rk, double &dti,
+                                         double &cfm);
+
+template <int DIR, bool MAG, int LIM, bool TILE = false, bool SPC = false, bool FUSED = false,
+          bool EXT = true>
 __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, int stage,
-                                           double *lds = nullptr) {
+                                           double *lds = nullptr, const RKArgs *rk = nullptr,
+                                           double *dti = nullptr, double *cfm = nullptr) {
   const Axis &A = D.ax[DIR];
   const int n = A.n;
   const long nlines = D.ncell / n;
@@ -384,8 +622,16 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
     ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
   }
   const int c = seg * SEG - 2 + lane;
-  const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, wave0, c, lds)
-                      : load_cell<DIR, MAG>(D, A, base, c);
+  Cell q;
+  if (TILE) {
+    q = tile_cell<DIR, MAG, FUSED, EXT>(D, A, wave0, c, lds);
+  } else if (FUSED) {
+    double f[NWMAX];
+    fused_raw<MAG>(D, A, base, line, seg, c, f);
+    q = finish_cell<DIR, MAG>(D, A, c, f);
+  } else {
+    q = load_cell<DIR, MAG, EXT>(D, A, base, line, c);
+  }
   // d e m1 m2 m3 b1 b2 b3 and eint (slopes) / phi (face) of the cell
   const double qv[9] = {q.d, q.e, q.d * q.v[0], q.d * q.v[1], q.d * q.v[2],
                         q.b[0], q.b[1], q.b[2], q.eint};
@@ -427,8 +673,10 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   const double gam = D.gamma, ch = D.sc->ch;
   double t[9], fe[4];
   face_flux<DIR, MAG, true>(L, S, R_, RS, dx1, dx2, gam, ch, t, fe);
+  const bool fxb = EXT && A.ext && flux_face(A, c);
+  if (fxb) boundary_flux<DIR>(A, c, t);
   // faces between lanes l and l+1 have valid slopes on both sides for l = 1..61
-  const bool face_valid = valid && (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
+  const bool face_valid = valid && (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61 && !fxb;
   if (face_valid && (fe[1] <= 0.0 || fe[0] <= 0.0 || fe[3] <= 0.0 || fe[2] <= 0.0)) {
     int ii = ci, jj = cj, kk = ck;
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
@@ -452,6 +700,10 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   du[5 + b_] = ix * (tm[6] - t[6]);
   du[5 + c_] = ix * (tm[7] - t[7]);
   du[8] = ix * (tm[8] - t[8]);
+  if (FUSED) {
+    fused_rk<DIR, MAG, TILE>(D, A, base, c, lane, du, lds, stage, *rk, *dti, *cfm);
+    return;
+  }
   store_du<DIR, MAG, TILE>(D, A, base, c, lane, du, lds);
   if (SPC) store_mass_flux<DIR>(D, line, c, lane, t[0]);
 }
@@ -464,14 +716,14 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
 // launcher checks that a block's lines have consecutive x index); a
 // separate instantiation keeps each kernel's code within the instruction
 // cache.
-template <bool MAG, int LIM, bool TILE, bool SPC = false>
+template <bool MAG, int LIM, bool TILE, bool SPC = false, bool EXT = false>
 __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
   extern __shared__ double lds[];
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave < D.woff[1]) {
-    if (wave < D.waves[0]) sweep_line<0, MAG, LIM, false, SPC>(D, wave, lane, stage);
+    if (wave < D.waves[0]) sweep_line<0, MAG, LIM, false, SPC, false, EXT>(D, wave, lane, stage);
     return;
   }
   // tiled: a block past the direction's last wave has nothing to do; a
@@ -479,13 +731,13 @@ __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
   if (wave < D.woff[2]) {
     const long w1 = wave - D.woff[1], w0 = w1 - (threadIdx.x >> 6);
     if (TILE ? w0 >= D.waves[1] : w1 >= D.waves[1]) return;
-    sweep_line<1, MAG, LIM, TILE, SPC>(D, w1, lane, stage, lds);
+    sweep_line<1, MAG, LIM, TILE, SPC, false, EXT>(D, w1, lane, stage, lds);
     return;
   }
   {
     const long w2 = wave - D.woff[2], w0 = w2 - (threadIdx.x >> 6);
     if (TILE ? w0 >= D.waves[2] : w2 >= D.waves[2]) return;
-    sweep_line<2, MAG, LIM, TILE, SPC>(D, w2, lane, stage, lds);
+    sweep_line<2, MAG, LIM, TILE, SPC, false, EXT>(D, w2, lane, stage, lds);
   }
 }
 
@@ -522,7 +774,7 @@ __device__ __forceinline__ double minmod_geo(double dm, double dp, double ida, d
   return s * mx2(0.0, mn3(fabs(x), s * y, s * z));
 }
 
-template <int DIR, bool MAG, int LIM, bool TILE = false, bool SPC = false>
+template <int DIR, bool MAG, int LIM, bool TILE = false, bool SPC = false, bool EXT = true>
 __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane, int stage,
                                                double *lds = nullptr) {
   const Axis &A = D.ax[DIR];
@@ -550,8 +802,8 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
   }
   const int c = seg * SEG - 2 + lane;
-  const Cell q = TILE ? tile_cell<DIR, MAG>(D, A, wave0, c, lds)
-                      : load_cell<DIR, MAG>(D, A, base, c);
+  const Cell q = TILE ? tile_cell<DIR, MAG, false, EXT>(D, A, wave0, c, lds)
+                      : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
   const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
   const double qd[8] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2]};
   const int cc = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
@@ -626,7 +878,8 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     eil = eil - bbl;
     eir = eir - bbr;
   }
-  const bool face_valid = valid && (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61;
+  const bool fxb = EXT && A.ext && flux_face(A, c);
+  const bool face_valid = valid && (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61 && !fxb;
   if (face_valid && (eil <= 0.0 || L[0] <= 0.0 || eir <= 0.0 || R_[0] <= 0.0)) {
     int ii = ci, jj = cj, kk = ck;
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
@@ -662,6 +915,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   hlld_delta(t, fl, fr, b1, cfl, cfr);
   t[5] = phi;
   t[8] = ch * ch * b1;
+  if (fxb) boundary_flux<DIR>(A, c, t);
 
   // flux difference in FP64
   double tm[9];
@@ -683,26 +937,26 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   if (SPC) store_mass_flux<DIR>(D, line, c, lane, t[0]);
 }
 
-template <bool MAG, int LIM, bool TILE, bool SPC = false>
+template <bool MAG, int LIM, bool TILE, bool SPC = false, bool EXT = false>
 __global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
   extern __shared__ double lds[];
   if (D.sc->stop) return;
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave < D.woff[1]) {
-    if (wave < D.waves[0]) sweep_line_mix<0, MAG, LIM, false, SPC>(D, wave, lane, stage);
+    if (wave < D.waves[0]) sweep_line_mix<0, MAG, LIM, false, SPC, EXT>(D, wave, lane, stage);
     return;
   }
   if (wave < D.woff[2]) {
     const long w1 = wave - D.woff[1], w0 = w1 - (threadIdx.x >> 6);
     if (TILE ? w0 >= D.waves[1] : w1 >= D.waves[1]) return;
-    sweep_line_mix<1, MAG, LIM, TILE, SPC>(D, w1, lane, stage, lds);
+    sweep_line_mix<1, MAG, LIM, TILE, SPC, EXT>(D, w1, lane, stage, lds);
     return;
   }
   {
     const long w2 = wave - D.woff[2], w0 = w2 - (threadIdx.x >> 6);
     if (TILE ? w0 >= D.waves[2] : w2 >= D.waves[2]) return;
-    sweep_line_mix<2, MAG, LIM, TILE, SPC>(D, w2, lane, stage, lds);
+    sweep_line_mix<2, MAG, LIM, TILE, SPC, EXT>(D, w2, lane, stage, lds);
   }
 }
 
@@ -827,13 +1081,13 @@ __device__ __forceinline__ double wave_max(double x) {
   return x;
 }
 
-// block (256 threads) reduction of (min dti, max cf) into D.part[blk]
+// block (up to 1024 threads) reduction of (min dti, max cf) into D.part[blk]
 __device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, double cfm,
                                                  long blk) {
-  __shared__ double smin[4], smax[4];
+  __shared__ double smin[16], smax[16];
   dti = wave_min(dti);
   cfm = wave_max(cfm);
-  const int w = threadIdx.x >> 6;
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     smin[w] = dti;
     smax[w] = cfm;
@@ -841,7 +1095,7 @@ __device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, doubl
   __syncthreads();
   if (threadIdx.x == 0) {
     double a = smin[0], b = smax[0];
-    for (int q = 1; q < 4; ++q) {
+    for (int q = 1; q < nw; ++q) {
       a = mn2(a, smin[q]);
       b = mx2(b, smax[q]);
     }
@@ -883,10 +1137,14 @@ __device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVM
   src[4] = gr[0] * D.W[WV1][m] + gr[1] * D.W[WV2][m] + gr[2] * D.W[WV3][m];
 }
 
-template <bool MAG, bool CFL, bool DAMP, bool GRAV>
-__device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double faco,
-                                        double fact, double facn, double damp, double &dti,
-                                        double &cfm) {
+// LD >= 0: direction LD's flux divergence comes from duL[] (the fused last
+// sweep) instead of memory.  The flags are plain arguments so that one
+// inlined copy serves every variant; k_rk passes constants.
+template <bool MAG, int LD = -1>
+__device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, double faco,
+                                           double fact, double facn, double damp, bool CFL,
+                                           bool DAMP, bool GRAV, double &dti, double &cfm,
+                                           const double *duL = nullptr) {
   const double ft = fact * D.sc->dt;
   double src[NVMAX];
   if (GRAV) grav_source(D, m, src);
@@ -896,9 +1154,9 @@ __device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double 
   for (int v = 0; v < NVMAX; ++v) {
     if (v >= D.nv) break;
     // inactive directions contribute idxi*(0-0) = +0 (allocations.f90:214-240)
-    const double a = D.solve[0] ? D.DU[0][v][m] : 0.0;
-    const double b = D.solve[1] ? D.DU[1][v][m] : 0.0;
-    const double c = D.solve[2] ? D.DU[2][v][m] : 0.0;
+    const double a = LD == 0 ? duL[v] : (D.solve[0] ? D.DU[0][v][m] : 0.0);
+    const double b = LD == 1 ? duL[v] : (D.solve[1] ? D.DU[1][v][m] : 0.0);
+    const double c = LD == 2 ? duL[v] : (D.solve[2] ? D.DU[2][v][m] : 0.0);
     const double acc = ((a + b) + c) + (GRAV ? src[v] : 0.0);  // + src
     const double u = D.U[v][m];
     double u0;
@@ -956,6 +1214,13 @@ __device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double 
 }
 
 template <bool MAG, bool CFL, bool DAMP, bool GRAV>
+__device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double faco,
+                                        double fact, double facn, double damp, double &dti,
+                                        double &cfm) {
+  rk_cell_rt<MAG>(D, m, stage, faco, fact, facn, damp, CFL, DAMP, GRAV, dti, cfm);
+}
+
+template <bool MAG, bool CFL, bool DAMP, bool GRAV>
 __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
                                             double facn, double damp) {
   if (D.sc->stop) return;
@@ -964,6 +1229,64 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
   if (m < D.ncell)
     rk_cell<MAG, CFL, DAMP, GRAV>(D, m, stage, faco, fact, facn, damp, dti, cfm);
   if (CFL) cfl_block_reduce(D, dti, cfm, blockIdx.x);
+}
+
+// ---- the last sweep direction fused with the RK update --------------------
+// The divergence of direction DIR comes straight from the sweep (registers,
+// or the LDS tile when tiled); the other directions' from memory.  Each cell
+// is updated by the thread that holds its divergence; no other thread of the
+// launch reads that cell afterwards (other segments read the halo copy).
+template <int DIR, bool MAG, bool TILE>
+__device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base, int c,
+                                         int lane, const double du[NVMAX], double *lds,
+                                         int stage, const RKArgs &rk, double &dti,
+                                         double &cfm) {
+  const int n = A.n;
+  const int nv = MAG ? (D.i9wv >= 0 ? 9 : 8) : 5;
+  if (!TILE) {
+    if (lane >= 2 && lane <= 61 && c >= 0 && c < n)
+      rk_cell_rt<MAG, DIR>(D, base + (long)c * A.stride, stage, rk.faco, rk.fact, rk.facn,
+                           rk.damp, rk.cfl, rk.dampf, rk.grav, dti, cfm, du);
+    return;
+  }
+  const int bw = blockDim.x >> 6, w = threadIdx.x >> 6;
+  const int ld = tile_ld(bw), fs = 64 * ld;
+  __syncthreads();  // every wavefront has read its cells out of the tile
+  {
+    double *p = lds + lane * ld + w;
+#pragma unroll
+    for (int v = 0; v < NVMAX; ++v)
+      if (v < nv) p[v * fs] = du[v];
+  }
+  __syncthreads();
+  const int t = threadIdx.x, r = t / bw, col = t - r * bw;
+  const long *cinfo = reinterpret_cast<const long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
+  const long bc = cinfo[2 * col];
+  const int sg = (int)cinfo[2 * col + 1];
+  const int cr = sg * SEG - 2 + r;
+  if (sg >= 0 && r >= 2 && r <= 61 && cr >= 0 && cr < n) {
+    const double *p = lds + r * ld + col;
+    double d2[NVMAX];
+#pragma unroll
+    for (int v = 0; v < NVMAX; ++v) d2[v] = v < nv ? p[v * fs] : 0.0;
+    rk_cell_rt<MAG, DIR>(D, bc + (long)cr * A.stride, stage, rk.faco, rk.fact, rk.facn,
+                         rk.damp, rk.cfl, rk.dampf, rk.grav, dti, cfm, d2);
+  }
+}
+
+// One wavefront per segment of a DIR line, as k_sweep; CFL partials (last
+// stage) one per block.
+template <int DIR, bool MAG, int LIM, bool TILE>
+__global__ __launch_bounds__(1024) void k_sweep_rk(Dev D, int stage, RKArgs rk) {
+  extern __shared__ double lds[];
+  if (D.sc->stop) return;
+  const int lane = threadIdx.x & 63;
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  double dti = __builtin_inf(), cfm = 0.0;
+  if (TILE || wave < D.waves[DIR])
+    sweep_line<DIR, MAG, LIM, TILE, false, true, false>(D, wave, lane, stage, lds, &rk, &dti,
+                                                         &cfm);
+  if (rk.cfl) cfl_block_reduce(D, dti, cfm, blockIdx.x);
 }
 
 // ---- pressure(): eint from e, d, v, B after an upload ---------------------
@@ -1166,8 +1489,8 @@ __device__ __forceinline__ void bsweep_item(const Dev &D, long w, int lane, int 
                                             bool tiled, double *lds) {
   if (w < D.woff[1]) {
     if (w < D.waves[0]) {
-      if (MIX) sweep_line_mix<0, MAG, LIM>(D, w, lane, stage);
-      else sweep_line<0, MAG, LIM>(D, w, lane, stage);
+      if (MIX) sweep_line_mix<0, MAG, LIM, false, false, false>(D, w, lane, stage);
+      else sweep_line<0, MAG, LIM, false, false, false, false>(D, w, lane, stage);
     }
     return;
   }
@@ -1177,22 +1500,22 @@ __device__ __forceinline__ void bsweep_item(const Dev &D, long w, int lane, int 
     w -= D.woff[1];
     if (tiled ? w - wid >= D.waves[1] : w >= D.waves[1]) return;
     if (tiled) {
-      if (MIX) sweep_line_mix<1, MAG, LIM, true>(D, w, lane, stage, lds);
-      else sweep_line<1, MAG, LIM, true>(D, w, lane, stage, lds);
+      if (MIX) sweep_line_mix<1, MAG, LIM, true, false, false>(D, w, lane, stage, lds);
+      else sweep_line<1, MAG, LIM, true, false, false, false>(D, w, lane, stage, lds);
     } else {
-      if (MIX) sweep_line_mix<1, MAG, LIM>(D, w, lane, stage);
-      else sweep_line<1, MAG, LIM>(D, w, lane, stage);
+      if (MIX) sweep_line_mix<1, MAG, LIM, false, false, false>(D, w, lane, stage);
+      else sweep_line<1, MAG, LIM, false, false, false, false>(D, w, lane, stage);
     }
     return;
   }
   w -= D.woff[2];
   if (tiled ? w - wid >= D.waves[2] : w >= D.waves[2]) return;
   if (tiled) {
-    if (MIX) sweep_line_mix<2, MAG, LIM, true>(D, w, lane, stage, lds);
-    else sweep_line<2, MAG, LIM, true>(D, w, lane, stage, lds);
+    if (MIX) sweep_line_mix<2, MAG, LIM, true, false, false>(D, w, lane, stage, lds);
+    else sweep_line<2, MAG, LIM, true, false, false, false>(D, w, lane, stage, lds);
   } else {
-    if (MIX) sweep_line_mix<2, MAG, LIM>(D, w, lane, stage);
-    else sweep_line<2, MAG, LIM>(D, w, lane, stage);
+    if (MIX) sweep_line_mix<2, MAG, LIM, false, false, false>(D, w, lane, stage);
+    else sweep_line<2, MAG, LIM, false, false, false, false>(D, w, lane, stage);
   }
 }
 
@@ -1351,8 +1674,10 @@ static inline void klaunch_lds(void (*k)(KA...), unsigned g, unsigned b, size_t 
     k<<<g, b, lds, s>>>(KA(args)...);
 }
 
-// Bytes of LDS the y/z tile of a sweep block needs, or 0 when the lines of a
-// block do not have consecutive x index (or the tile would not fit).
+// Bytes of LDS the y/z tile of a sweep block needs (the fields of 64 cells
+// of bw lines, then each column's line base and segment), or 0 when the
+// block has fewer than 4 lines, no y/z direction is solved or the tile
+// would not fit.
 size_t sweep_tile_bytes(const Dev &D, int bw) {
   if (bw < 4 || !(D.solve[1] || D.solve[2])) return 0;
   const size_t nf = D.mag ? NWMAX : WEINT + 1;
@@ -1382,26 +1707,37 @@ void launch_set_clock(const Dev &D, double time, double t_out, double t_end, dou
 // a block own neighbouring lines (consecutive i for the y and z sweeps), so
 // with 16 wavefronts a block uses whole 128-byte cache lines of the strided
 // y/z loads.
+template <bool MAG, int LIM, bool EXT>
+static void sweep_variant_x(const Dev &D, int stage, unsigned g, int bs, size_t tile,
+                            hipStream_t s, const KEv *ev) {
+  if (D.mixed) {
+    if (D.nspc) klaunch_lds(k_sweep_mix<MAG, LIM, false, true, EXT>, g, bs, 0, s, ev, D, stage);
+    else if (tile) klaunch_lds(k_sweep_mix<MAG, LIM, true, false, EXT>, g, bs, tile, s, ev, D, stage);
+    else klaunch_lds(k_sweep_mix<MAG, LIM, false, false, EXT>, g, bs, 0, s, ev, D, stage);
+  } else {
+    if (D.nspc) klaunch_lds(k_sweep<MAG, LIM, false, true, EXT>, g, bs, 0, s, ev, D, stage);
+    else if (tile) klaunch_lds(k_sweep<MAG, LIM, true, false, EXT>, g, bs, tile, s, ev, D, stage);
+    else klaunch_lds(k_sweep<MAG, LIM, false, false, EXT>, g, bs, 0, s, ev, D, stage);
+  }
+}
+
+// boxes with a code 4, 5, 9 or 10 run their own instantiation, so that the
+// others keep the sweep without those branches
 template <bool MAG, int LIM>
 static void sweep_variant(const Dev &D, int stage, unsigned g, int bs, size_t tile,
                           hipStream_t s, const KEv *ev) {
-  if (D.mixed) {
-    if (D.nspc) klaunch_lds(k_sweep_mix<MAG, LIM, false, true>, g, bs, 0, s, ev, D, stage);
-    else if (tile) klaunch_lds(k_sweep_mix<MAG, LIM, true>, g, bs, tile, s, ev, D, stage);
-    else klaunch_lds(k_sweep_mix<MAG, LIM, false>, g, bs, 0, s, ev, D, stage);
-  } else {
-    if (D.nspc) klaunch_lds(k_sweep<MAG, LIM, false, true>, g, bs, 0, s, ev, D, stage);
-    else if (tile) klaunch_lds(k_sweep<MAG, LIM, true>, g, bs, tile, s, ev, D, stage);
-    else klaunch_lds(k_sweep<MAG, LIM, false>, g, bs, 0, s, ev, D, stage);
-  }
+  if (D.ax[0].ext || D.ax[1].ext || D.ax[2].ext)
+    sweep_variant_x<MAG, LIM, true>(D, stage, g, bs, tile, s, ev);
+  else
+    sweep_variant_x<MAG, LIM, false>(D, stage, g, bs, tile, s, ev);
 }
 
 void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev) {
   const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
   const unsigned g = nblk(D.woff[2] + D.waves[2], bw);
   const int bs = 64 * bw;
-  // LDS tile for the y/z lines: whole blocks of lines with consecutive x
-  // index (n1 a multiple of the block's line count), 4+ lines per block
+  // LDS tile for the y/z lines with 4+ lines per block (each column finds
+  // its own line, so any x extent works)
   const size_t tile = sweep_tile_bytes(D, bw);
   if (D.mag) {
     if (D.limiter == LIM_FLAT) sweep_variant<true, LIM_FLAT>(D, stage, g, bs, tile, s, ev);
@@ -1420,6 +1756,61 @@ void launch_species(const Dev &D, hipStream_t s) {
   if (D.limiter == LIM_FLAT) klaunch(k_species<LIM_FLAT>, g, 256, s, nullptr, D);
   else if (D.limiter == LIM_MINMOD) klaunch(k_species<LIM_MINMOD>, g, 256, s, nullptr, D);
   else klaunch(k_species<LIM_MC>, g, 256, s, nullptr, D);
+}
+
+template <int DIR, bool MAG, int LIM>
+static void sweep_rk_variant(const Dev &D, int stage, const RKArgs &rk, unsigned g, int bs,
+                             size_t tile, hipStream_t s, const KEv *ev) {
+  if (tile) klaunch_lds(k_sweep_rk<DIR, MAG, LIM, true>, g, bs, tile, s, ev, D, stage, rk);
+  else klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false>, g, bs, 0, s, ev, D, stage, rk);
+}
+
+template <int DIR>
+static void sweep_rk_dir(const Dev &D, int stage, const RKArgs &rk, unsigned g, int bs,
+                         size_t tile, hipStream_t s, const KEv *ev) {
+  if (D.mag) {
+    if (D.limiter == LIM_FLAT) sweep_rk_variant<DIR, true, LIM_FLAT>(D, stage, rk, g, bs, tile, s, ev);
+    else if (D.limiter == LIM_MINMOD) sweep_rk_variant<DIR, true, LIM_MINMOD>(D, stage, rk, g, bs, tile, s, ev);
+    else sweep_rk_variant<DIR, true, LIM_MC>(D, stage, rk, g, bs, tile, s, ev);
+  } else {
+    if (D.limiter == LIM_FLAT) sweep_rk_variant<DIR, false, LIM_FLAT>(D, stage, rk, g, bs, tile, s, ev);
+    else if (D.limiter == LIM_MINMOD) sweep_rk_variant<DIR, false, LIM_MINMOD>(D, stage, rk, g, bs, tile, s, ev);
+    else sweep_rk_variant<DIR, false, LIM_MC>(D, stage, rk, g, bs, tile, s, ev);
+  }
+}
+
+int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
+                       const KEv *ev, const KEv *ev_fused) {
+  const int L = D.fdir;
+  const Axis &A = D.ax[L];
+  const int K = (A.n + SEG - 1) / SEG;
+  const long nh = (long)A.nlines * (K + 1) * 4;
+  if (L == 1) {
+    if (D.mag) k_halo<1, true><<<nblk(nh, 256), 256, 0, s>>>(D);
+    else k_halo<1, false><<<nblk(nh, 256), 256, 0, s>>>(D);
+  } else {
+    if (D.mag) k_halo<2, true><<<nblk(nh, 256), 256, 0, s>>>(D);
+    else k_halo<2, false><<<nblk(nh, 256), 256, 0, s>>>(D);
+  }
+  const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
+  const int bs = 64 * bw;
+  const size_t tile = sweep_tile_bytes(D, bw);
+  if (D.woff[L] > 0) {  // the directions before L, as launch_sweep does them
+    const unsigned g0 = nblk(D.woff[L], bw);
+    if (D.mag) {
+      if (D.limiter == LIM_FLAT) sweep_variant<true, LIM_FLAT>(D, stage, g0, bs, tile, s, ev);
+      else if (D.limiter == LIM_MINMOD) sweep_variant<true, LIM_MINMOD>(D, stage, g0, bs, tile, s, ev);
+      else sweep_variant<true, LIM_MC>(D, stage, g0, bs, tile, s, ev);
+    } else {
+      if (D.limiter == LIM_FLAT) sweep_variant<false, LIM_FLAT>(D, stage, g0, bs, tile, s, ev);
+      else if (D.limiter == LIM_MINMOD) sweep_variant<false, LIM_MINMOD>(D, stage, g0, bs, tile, s, ev);
+      else sweep_variant<false, LIM_MC>(D, stage, g0, bs, tile, s, ev);
+    }
+  }
+  const unsigned g = nblk(D.waves[L], bw);
+  if (L == 1) sweep_rk_dir<1>(D, stage, rk, g, bs, tile, s, ev_fused);
+  else sweep_rk_dir<2>(D, stage, rk, g, bs, tile, s, ev_fused);
+  return (int)g;
 }
 
 template <bool MAG, bool GRAV>

@@ -22,6 +22,7 @@ namespace hg {
 
 enum { WD = 0, WV1, WV2, WV3, WE, WEINT, WB1, WB2, WB3, WPHI, NWMAX };
 enum { NVMAX = 9 };
+static constexpr int SEG = 60;  // cells a 64-lane sweep wavefront updates (2-cell halos)
 
 // stop reasons of the device-resident main loop
 enum { STOP_NONE = 0, STOP_END = 1, STOP_OUTPUT = 2, STOP_MAXSTEPS = 3, STOP_ERROR = 4 };
@@ -50,7 +51,19 @@ struct Axis {
   long stride;       // element stride of this axis in the SoA arrays
   int sc_lo, sc_hi;  // scalar BC codes (bcNis, bcNos)
   int vc_lo, vc_hi;  // vector BC codes (bcNiv, bcNov)
+  // codes 4, 5 (linear), 9 (Dirichlet) and 10 (flux) on this axis: ext is
+  // set when any of the four codes is one of them.  G[side] holds, for a
+  // face with such a code, the ghost values the boundary code does not
+  // compute from the interior, [GF_*][layer 0 (next to the face), 1][line];
+  // FX[side] the flux of a flux-boundary face (conserved-variable order,
+  // NVMAX values).
+  int ext;
+  const double *G[2];
+  const double *FX[2];
 };
+
+// fields of the ghost-value buffers
+enum { GF_D = 0, GF_P, GF_V1, GF_V2, GF_V3, GF_B1, GF_B2, GF_B3, GF_PHI, NGF };
 
 struct Dev {
   int n1, n2, n3;  // interior extents
@@ -70,6 +83,11 @@ struct Dev {
   int nspc;
   double *SPC, *SPC0, *DUS;
   double *FM[3];  // face mass flux per direction, (n+1) faces per line
+  // last sweep direction fused with the RK update: halo copy of the cells a
+  // segment reads but does not own (see k_halo), hsz entries per field
+  int fdir;        // the fused direction (1 or 2), -1 if none
+  double *H;
+  long hsz;
   double *part;    // CFL partials: 2 doubles per block of 256 cells
   Scalars *sc;
   int nblk_cfl;
@@ -145,6 +163,16 @@ struct KEv {
 };
 
 void launch_sweep(const Dev &D, int stage, hipStream_t s, const KEv *ev = nullptr);
+// RK coefficients and options of the fused last sweep
+struct RKArgs {
+  double faco, fact, facn, damp;
+  int cfl, dampf, grav;
+};
+// the fused path of one RK stage: halo copy, the sweeps of the directions
+// before D.fdir, then D.fdir's sweep with the RK update; returns the number
+// of CFL partials the fused launch wrote (rk.cfl)
+int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
+                       const KEv *ev_sweep = nullptr, const KEv *ev_fused = nullptr);
 void launch_species(const Dev &D, hipStream_t s);
 size_t sweep_tile_bytes(const Dev &D, int bw);
 // cfl: also the next step's CFL partials; damp: phidamp

@@ -131,6 +131,27 @@ module hormone_hip_mod
    integer(c_int):: st
   end function hg_set_launch
 
+  ! ghost values of the faces with codes 4, 5, 9, 10 (for Dirichlet faces:
+  ! the d0, p0, v10.. arrays, after each dirichletbound)
+  function hg_set_ghosts(ctx, field, padded) bind(C, name='hg_set_ghosts') result(st)
+   import:: c_int, c_int32_t, c_ptr
+   type(c_ptr),value:: ctx
+   integer(c_int32_t),value:: field
+   type(c_ptr),value:: padded
+   integer(c_int):: st
+  end function hg_set_ghosts
+
+  ! flux of a flux-boundary face (face 0..5 = x1i x1o x2i x2o x3i x3o;
+  ! ufnmax values in conserved-variable order)
+  function hg_set_boundary_flux(ctx, face, flux) bind(C, name='hg_set_boundary_flux') &
+           result(st)
+   import:: c_int, c_int32_t, c_ptr, c_double
+   type(c_ptr),value:: ctx
+   integer(c_int32_t),value:: face
+   real(c_double),intent(in):: flux(*)
+   integer(c_int):: st
+  end function hg_set_boundary_flux
+
   function hg_last_error(ctx, i, j, k, dir) bind(C, name='hg_last_error') result(st)
    import:: c_int, c_int32_t, c_ptr
    type(c_ptr),value:: ctx

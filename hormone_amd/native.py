@@ -116,6 +116,9 @@ def _load():
                                C.POINTER(C.c_int32)]
     lib.hg_set_launch.argtypes = [vp, C.c_int32]
     lib.hg_time_sweep.argtypes = [vp, C.c_int32, _dp]
+    lib.hg_set_fused.argtypes = [vp, C.c_int32]
+    lib.hg_set_ghosts.argtypes = [vp, C.c_int32, _dp]
+    lib.hg_set_boundary_flux.argtypes = [vp, C.c_int32, _dp]
     return lib
 
 
@@ -134,7 +137,8 @@ def exported_symbols():
     return ["hg_version", "hg_create", "hg_destroy", "hg_ufnmax", "hg_upload",
             "hg_download", "hg_timestep", "hg_hydro_step", "hg_phidamp",
             "hg_last_error", "hg_run", "hg_profile", "hg_profile_read", "hg_kat",
-            "hg_gravity", "hg_set_launch", "hg_time_sweep"]
+            "hg_gravity", "hg_set_launch", "hg_time_sweep", "hg_set_fused",
+            "hg_set_ghosts", "hg_set_boundary_flux"]
 
 
 STOP_END, STOP_OUTPUT, STOP_MAXSTEPS, STOP_ERROR = 1, 2, 3, 4
@@ -182,6 +186,20 @@ class Hydro:
         a = np.ascontiguousarray(padded, dtype=np.float64)
         self._check(self._lib.hg_upload(self.h, field, a.ctypes.data_as(_dp)), "hg_upload")
 
+    def set_ghosts(self, field, padded):
+        """Ghost values of `field` at the faces with codes 4, 5, 9, 10
+        (hg_set_ghosts); `padded` is shaped like an upload."""
+        a = np.ascontiguousarray(padded, dtype=np.float64)
+        self._check(self._lib.hg_set_ghosts(self.h, field, a.ctypes.data_as(_dp)),
+                    "hg_set_ghosts")
+
+    def set_boundary_flux(self, face, flux):
+        """Flux of flux-boundary face `face` (0..5 = x1i x1o x2i x2o x3i x3o)."""
+        a = np.ascontiguousarray(flux, dtype=np.float64)
+        assert a.size >= self.ufnmax()
+        self._check(self._lib.hg_set_boundary_flux(self.h, face, a.ctypes.data_as(_dp)),
+                    "hg_set_boundary_flux")
+
     def download(self, field, padded):
         assert padded.flags.c_contiguous and padded.dtype == np.float64
         self._check(self._lib.hg_download(self.h, field, padded.ctypes.data_as(_dp)), "hg_download")
@@ -224,6 +242,10 @@ class Hydro:
 
     def profile(self, enable=True):
         self._check(self._lib.hg_profile(self.h, int(enable)), "hg_profile")
+
+    def set_fused(self, on: bool = True):
+        """Fuse the last sweep direction with the RK update (bit-identical)."""
+        self._check(self._lib.hg_set_fused(self.h, int(on)), "hg_set_fused")
 
     def time_sweep(self, launches):
         """Milliseconds of `launches` back-to-back sweeps of the current state

@@ -90,7 +90,12 @@ typedef struct hg_desc {
   int32_t flux_limiter; /* HG_LIMITER_*                                     */
   int32_t rktype;       /* 1, 2 or 3                                        */
   int32_t bc[12];       /* bc1is bc1os bc2is bc2os bc3is bc3os
-                           bc1iv bc1ov bc2iv bc2ov bc3iv bc3ov              */
+                           bc1iv bc1ov bc2iv bc2ov bc3iv bc3ov
+                           0 periodic, 1 reflective, 2 outgoing, 3 free,
+                           4 linear, 5 linear + outgoing (x1 outer, x3
+                           inner, x3 outer; scalar 5 not at x3 outer),
+                           9 Dirichlet (hg_set_ghosts), 10 flux
+                           (hg_set_boundary_flux); 9/10 not with species */
   double gamma;         /* physval: gamma                                   */
   double courant;       /* settings: courant                                */
   double alpha9wave;    /* settings: alpha9wave                             */
@@ -197,6 +202,41 @@ enum { HG_PROF_SWEEP = 0, HG_PROF_RK = 1, HG_PROF_CFL = 2, HG_PROF_GRAV = 3,
 int hg_profile(hg_ctx *ctx, int32_t enable);
 int hg_profile_read(const hg_ctx *ctx, int32_t kind, double *ms_total,
                     int64_t *launches, double *bytes_per_launch);
+
+/* Ghost values of one field at the faces whose scalar or vector code is 4,
+   5, 9 or 10: the values those codes do not compute from the interior.
+   Replaces the d0/p0/v10../b10.. arrays a Dirichlet face copies into its
+   ghost cells (boundarycondition.f90:104-116, 196-212 and the other faces;
+   set by dirichletbound, dirichlet.f90:14-20) and, where the reference
+   leaves a ghost cell unwritten (phi under 9, everything under 10, v2 and
+   b2 of the outer ghost below the x3 inner boundary under 4/5,
+   boundarycondition.f90:1081,1088,1104,1114), the value that cell holds.
+   field: HG_F_D, HG_F_P, HG_F_V1..V3, HG_F_B1..B3, HG_F_PHI; padded: a
+   padded array like hg_upload's, of which only the two ghost layers of
+   those faces are read.  Before any call the values are the reference's
+   allocation values (allocations.f90:71-78: d = p = 1, v = B = phi = 0).
+   Call again between steps to change them (time-dependent Dirichlet
+   data). */
+int hg_set_ghosts(hg_ctx *ctx, int32_t field, const double *padded);
+
+/* The flux through a flux-boundary face (scalar code 10), which the
+   reference sets in fluxboundary (fluxboundary.f90:17-78) after skipping
+   that face's Riemann solve (numflux.f90:59-62, 566).  face: 0..5 = x1
+   inner, x1 outer, x2 inner, x2 outer, x3 inner, x3 outer; flux: ufnmax
+   values in conserved-variable order, the same at every cell of the face.
+   Until set it is zero, the value the reference's flux array keeps where
+   fluxboundary does not write it. */
+int hg_set_boundary_flux(hg_ctx *ctx, int32_t face, const double *flux);
+
+/* Fuse the sweep of the last solved direction (y or z) with the RK update:
+   one launch per RK stage does that direction's reconstruction, Riemann
+   solves and flux difference and updates each cell in place, so that
+   direction's flux divergence is never written to or read back from HBM;
+   the cells a line segment reads but does not own come from a halo copy
+   made just before.  Results are bit-identical either way; FP64 without
+   species only (HG_ERR_UNSUPPORTED otherwise).  Off by default.  No
+   counterpart in the reference (a launch-shape choice). */
+int hg_set_fused(hg_ctx *ctx, int32_t on);
 
 /* The sweep kernel's duration without per-dispatch events: `launches`
    back-to-back sweeps of the current state between two events on the

@@ -58,6 +58,9 @@ struct or_ctx {
   double *dti;
   double ch;
   int err_i, err_j, err_k, err_dir;
+  /* flux of each flux-boundary face (code 10), faces x1i x1o x2i x2o x3i
+     x3o, conserved-variable order; zero until set */
+  double fxb[6][NV];
 };
 
 static int g_threads = 0;
@@ -96,8 +99,17 @@ or_ctx *or_create(const hg_desc *desc, int *status) {
   if (desc->compswitch == 2 && (desc->spn < 1 || desc->spn > HG_SPN_MAX)) {
     st = HG_ERR_ARG; goto fail;
   }
-  for (int b = 0; b < 12; ++b)
-    if (desc->bc[b] < 0 || desc->bc[b] > 3) { st = HG_ERR_UNSUPPORTED; goto fail; }
+  for (int b = 0; b < 12; ++b) {
+    /* the cases of boundarycondition.f90's select blocks: 0-3, 9, 10 on
+       every face; 4/5 at x1 outer (:302,:419,:441), x3 inner
+       (:963,:1074,:1096), x3 outer (scalar 4 only :1222; vectors
+       :1335,:1357) */
+    int code = desc->bc[b], face = b % 6, ok = 0;
+    if ((code >= 0 && code <= 3) || code == 9 || code == 10) ok = 1;
+    else if (code == 4 || code == 5)
+      ok = face == 1 || face == 4 || (face == 5 && (b >= 6 || code == 4));
+    if (!ok || (desc->compswitch == 2 && code >= 9)) { st = HG_ERR_UNSUPPORTED; goto fail; }
+  }
   if (desc->ie < desc->is || desc->je < desc->js || desc->ke < desc->ks) {
     st = HG_ERR_ARG; goto fail;
   }
@@ -152,6 +164,10 @@ or_ctx *or_create(const hg_desc *desc, int *status) {
                     &c->b1, &c->b2, &c->b3, &c->phi, &c->ptot, &c->cs,
                     &c->dti};
     for (size_t a = 0; a < sizeof(f) / sizeof(f[0]); ++a) *f[a] = alloc_field(N);
+    /* allocations.f90:71-78: d = p = e = eint = ptot = 1 in every cell
+       (ghost cells no boundary code writes keep them) */
+    for (size_t m = 0; m < N; ++m)
+      c->d[m] = c->p[m] = c->e[m] = c->eint[m] = c->ptot[m] = 1.0;
     for (int n = 0; n < c->ufnmax; ++n) {
       c->u[n] = alloc_field(N);
       c->uorg[n] = alloc_field(N);
@@ -263,6 +279,42 @@ int or_upload(or_ctx *c, int field, const double *a) {
   for (int k = c->ks; k <= c->ke; ++k)
     for (int j = c->js; j <= c->je; ++j)
       for (int i = c->is; i <= c->ie; ++i) f[ID(c, i, j, k)] = a[ID(c, i, j, k)];
+  return HG_OK;
+}
+
+/* ghost layers of the faces with codes 4, 5, 9, 10 (hg_set_ghosts) */
+int or_set_ghosts(or_ctx *c, int field, const double *a) {
+  double *f = NULL;
+  switch (field) {
+    case HG_F_D: f = c->d; break;
+    case HG_F_P: f = c->p; break;
+    case HG_F_V1: f = c->v1; break;
+    case HG_F_V2: f = c->v2; break;
+    case HG_F_V3: f = c->v3; break;
+    case HG_F_B1: f = c->b1; break;
+    case HG_F_B2: f = c->b2; break;
+    case HG_F_B3: f = c->b3; break;
+    case HG_F_PHI: f = c->phi; break;
+    default: return HG_ERR_ARG;
+  }
+  if (!a) return HG_ERR_ARG;
+  int s_[3] = {c->is, c->js, c->ks}, e_[3] = {c->ie, c->je, c->ke};
+  for (int axis = 0; axis < 3; ++axis)
+    for (int side = 0; side < 2; ++side) {
+      if (c->prm.bc[2 * axis + side] < 4 && c->prm.bc[6 + 2 * axis + side] < 4) continue;
+      int lo[3] = {c->is, c->js, c->ks}, hi[3] = {c->ie, c->je, c->ke};
+      lo[axis] = side ? e_[axis] + 1 : s_[axis] - 2;
+      hi[axis] = side ? e_[axis] + 2 : s_[axis] - 1;
+      for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+          for (int i = lo[0]; i <= hi[0]; ++i) f[ID(c, i, j, k)] = a[ID(c, i, j, k)];
+    }
+  return HG_OK;
+}
+
+int or_set_boundary_flux(or_ctx *c, int face, const double *flux) {
+  if (!flux || face < 0 || face > 5 || c->prm.bc[face] != 10) return HG_ERR_ARG;
+  for (int v = 0; v < NV; ++v) c->fxb[face][v] = v < c->ufnmax ? flux[v] : 0.0;
   return HG_OK;
 }
 
@@ -475,6 +527,69 @@ static inline void ghost_energy(or_ctx *c, size_t g) {
                    c->b3[g] * c->b3[g]);
 }
 
+/* Linear extrapolation of q into the two ghosts of one face line:
+   boundarycondition.f90:302-326 (outer, edge e: q(e+1) = q(e) +
+   (q(e)-q(e-1))*dx(e)/dx(e-1), q(e+2) = q(e+1) + (q(e+1)-q(e))*dx(e+1)/dx(e))
+   and :963-985 (inner, edge s, mirrored).  E/N/G1/G2: edge, its inner
+   neighbour, the ghosts next to and beyond the face; dx indexed by the
+   axis index, eg = edge index. */
+static inline void lin2(double *q, size_t E, size_t N, size_t G1, size_t G2, const double *dx,
+                        int eg, int inner) {
+  if (inner) {
+    q[G1] = q[E] - (q[N] - q[E]) * dx[eg] / dx[eg + 1];
+    q[G2] = q[G1] - (q[E] - q[G1]) * dx[eg - 1] / dx[eg];
+  } else {
+    q[G1] = q[E] + (q[E] - q[N]) * dx[eg] / dx[eg - 1];
+    q[G2] = q[G1] + (q[G1] - q[E]) * dx[eg + 1] / dx[eg];
+  }
+}
+
+/* codes 4/5 for one line of a face (scalars :302-326, :963-985,
+   :1222-1244; vectors :419-468, :1074-1120, :1335-1382) */
+static void linear_line(or_ctx *c, int axis, int inner, int sc, int vc, size_t E, size_t N,
+                        size_t G1, size_t G2) {
+  const double *dx = c->dxs[axis];
+  int eg = inner ? (axis == 0 ? c->is : axis == 1 ? c->js : c->ks)
+                 : (axis == 0 ? c->ie : axis == 1 ? c->je : c->ke);
+  const int mag = c->prm.mag_on;
+  if (sc == 4 || sc == 5) {
+    lin2(c->d, E, N, G1, G2, dx, eg, inner);
+    lin2(c->p, E, N, G1, G2, dx, eg, inner);
+    if (c->d[G2] <= 0.0) c->d[G1] = c->d[G2] = c->d[E];
+    if (c->p[G2] <= 0.0) c->p[G1] = c->p[G2] = c->p[E];
+    if (mag) c->phi[G1] = c->phi[G2] = c->phi[E];
+    for (int n = 0; n < c->spn; ++n) c->spc[n][G1] = c->spc[n][G2] = c->spc[n][E];
+  }
+  if (vc == 4 || vc == 5) {
+    double *v[3] = {c->v1, c->v2, c->v3}, *b[3] = {c->b1, c->b2, c->b3};
+    double vn = v[axis][E];
+    double plug = inner ? 0.5 - fsign(0.5, vn) : 0.5 + fsign(0.5, vn);
+    if (axis == 2 && inner) {
+      /* :1078-1083 and :1085-1090 statement by statement: the second
+         statement for v2 (b2) writes v3(ks-2) (b3(ks-2)) instead, from
+         v2 (b2) differences; v2(ks-2) and b2(ks-2) keep their values */
+      for (int w = 0; w < (mag ? 2 : 1); ++w) {
+        double **q = w ? b : v;
+        q[0][G1] = q[0][E] - (q[0][N] - q[0][E]) * dx[eg] / dx[eg + 1];
+        q[0][G2] = q[0][G1] - (q[0][E] - q[0][G1]) * dx[eg - 1] / dx[eg];
+        q[1][G1] = q[1][E] - (q[1][N] - q[1][E]) * dx[eg] / dx[eg + 1];
+        q[2][G2] = q[2][G1] - (q[1][E] - q[1][G1]) * dx[eg - 1] / dx[eg];
+        q[2][G1] = q[2][E] - (q[2][N] - q[2][E]) * dx[eg] / dx[eg + 1];
+        q[2][G2] = q[2][G1] - (q[2][E] - q[2][G1]) * dx[eg - 1] / dx[eg];
+      }
+    } else {
+      for (int a = 0; a < 3; ++a) {
+        lin2(v[a], E, N, G1, G2, dx, eg, inner);
+        if (mag) lin2(b[a], E, N, G1, G2, dx, eg, inner);
+      }
+    }
+    if (vc == 5) {
+      v[axis][G1] = v[axis][G1] * plug;
+      v[axis][G2] = v[axis][G2] * plug;
+    }
+  }
+}
+
 /* The reference applies the scalar and the vector codes separately; for
    the supported codes (0..3) both copy from the same source cell except
    that reflective mirrors and periodic wraps.  We require the scalar and
@@ -511,11 +626,15 @@ static void apply_face(or_ctx *c, int axis, int inner) {
         size_t G = ID(c, ijk_g[0], ijk_g[1], ijk_g[2]);
         size_t S = ID(c, ijk_s[0], ijk_s[1], ijk_s[2]);
         size_t V = ID(c, ijk_v[0], ijk_v[1], ijk_v[2]);
-        /* scalars */
-        c->d[G] = c->d[S];
-        c->p[G] = c->p[S];
-        if (c->prm.mag_on) c->phi[G] = c->phi[S];
-        for (int n = 0; n < c->spn; ++n) c->spc[n][G] = c->spc[n][S];
+        /* scalars (codes 0-3; 4/5 below; 9/10 keep the ghost values
+           or_set_ghosts put there) */
+        if (sc <= 3) {
+          c->d[G] = c->d[S];
+          c->p[G] = c->p[S];
+          if (c->prm.mag_on) c->phi[G] = c->phi[S];
+          for (int n = 0; n < c->spn; ++n) c->spc[n][G] = c->spc[n][S];
+        }
+        if (vc > 3) continue;
         /* vectors */
         double *v[3] = {c->v1, c->v2, c->v3};
         double *b[3] = {c->b1, c->b2, c->b3};
@@ -535,6 +654,17 @@ static void apply_face(or_ctx *c, int axis, int inner) {
           v[a][G] = vv;
           if (c->prm.mag_on) b[a][G] = (a == axis && vc == 1) ? -b[a][V] : b[a][V];
         }
+      }
+      if ((sc == 4 || sc == 5) || (vc == 4 || vc == 5)) {
+        int ie_[4] = {inner ? s : e, inner ? s + 1 : e - 1, inner ? s - 1 : e + 1,
+                      inner ? s - 2 : e + 2};
+        size_t m4[4];
+        for (int t = 0; t < 4; ++t) {
+          int ijk[3];
+          ijk[axis] = ie_[t]; ijk[oa] = r; ijk[ob] = q;
+          m4[t] = ID(c, ijk[0], ijk[1], ijk[2]);
+        }
+        linear_line(c, axis, inner, sc, vc, m4[0], m4[1], m4[2], m4[3]);
       }
       for (int gl = 0; gl < 2; ++gl) {
         int g = inner ? s - 2 + gl : e + 1 + gl;
@@ -573,6 +703,11 @@ static void interpolation(or_ctx *c) {
       for (int j = lo[1]; j <= hi[1]; ++j)
         for (int i = lo[0]; i <= hi[0]; ++i) {
           int ii = dir == 0 ? i : dir == 1 ? j : k;
+          /* interpolation.f90:59,62 (and y, z): no slopes in the ghost cell
+             of a flux-boundary face */
+          if ((ii == lo[dir] && c->prm.bc[2 * dir] == 10) ||
+              (ii == hi[dir] && c->prm.bc[2 * dir + 1] == 10))
+            continue;
           size_t m = ID(c, i, j, k), l = m - st, r = m + st;
           double dx[2] = {IDX[ii], IDX[ii + 1]};
           double x[3] = {X[ii - 1], X[ii], X[ii + 1]};
@@ -648,6 +783,11 @@ static int numflux(or_ctx *c) {
       for (int j = lo[1]; j <= hi[1]; ++j)
         for (int i = lo[0]; i <= hi[0]; ++i) {
           int ii = dir == 0 ? i : dir == 1 ? j : k;
+          /* numflux.f90:59-62 (and y, z): flux-boundary faces are not
+             solved; their flux is set below */
+          if ((ii == lo[dir] && c->prm.bc[2 * dir] == 10) ||
+              (ii == hi[dir] && c->prm.bc[2 * dir + 1] == 10))
+            continue;
           size_t L = ID(c, i, j, k), R = L + st;
           double dx1 = XI[ii] - X[ii], dx2 = X[ii + 1] - XI[ii];
           double dl = c->d[L] + dx1 * S[0][L];
@@ -721,6 +861,22 @@ static int numflux(or_ctx *c) {
                                      (spcl[n] * (0.5 + sgn) + spcr[n] * (0.5 - sgn));
           }
         }
+  }
+  /* fluxboundary (numflux.f90:566, fluxboundary.f90:66-78): the given
+     flux on every face of a flux boundary */
+  for (int dir = 0; dir < 3; ++dir) {
+    if (!c->solve[dir]) continue;
+    for (int side = 0; side < 2; ++side) {
+      if (c->prm.bc[2 * dir + side] != 10) continue;
+      int lo[3] = {c->is, c->js, c->ks}, hi[3] = {c->ie, c->je, c->ke};
+      int f = side ? hi[dir] : lo[dir] - 1;
+      lo[dir] = hi[dir] = f;
+      for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+          for (int i = lo[0]; i <= hi[0]; ++i)
+            for (int v = 0; v < c->ufnmax; ++v)
+              c->flux[dir][v][ID(c, i, j, k)] = c->fxb[2 * dir + side][v];
+    }
   }
   return err ? HG_ERR_FACE_EOS : HG_OK;
 }

@@ -44,6 +44,8 @@ def lib():
         L.or_phidamp.argtypes = [vp, C.c_double]
         L.or_last_error.argtypes = [vp] + [C.POINTER(C.c_int)] * 4
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_ghosts.argtypes = [vp, C.c_int, _dp]
+        L.or_set_boundary_flux.argtypes = [vp, C.c_int, _dp]
         L.or_gravity.argtypes = [vp, C.c_double, C.c_double, _dp, C.c_double, _dp, _dp,
                                  C.POINTER(C.c_int)]
         L.or_kat_hlldflux.argtypes = [_dp, _dp]
@@ -104,6 +106,16 @@ class Oracle:
         self._check(self._lib.or_upload(self.h, field, a.ctypes.data_as(_dp)), "upload")
         if field < 9 or field >= 16:
             self._dirty = True
+
+    def set_ghosts(self, field, padded):
+        a = np.ascontiguousarray(padded, dtype=np.float64)
+        self._check(self._lib.or_set_ghosts(self.h, field, a.ctypes.data_as(_dp)), "set_ghosts")
+
+    def set_boundary_flux(self, face, flux):
+        a = np.ascontiguousarray(flux, dtype=np.float64)
+        assert a.size >= self.ufnmax()
+        self._check(self._lib.or_set_boundary_flux(self.h, face, a.ctypes.data_as(_dp)),
+                    "set_boundary_flux")
 
     def download(self, field, padded):
         self._ensure()

@@ -250,6 +250,61 @@ EXTRA = {
 }
 
 
+# Linear boundary codes 4/5 where the reference implements them (x1 outer,
+# x3 inner, x3 outer).  Orszag-Tang in the yz plane has nonzero v2 and b2 at
+# the x3 inner boundary, where the reference leaves v2 and b2 of the outer
+# ghost unset (boundarycondition.f90:1081,1088,1104,1114).
+_OUT = "&out_con t_end=300d0 dt_out=300d0 output_ascii=.false. /\n"
+# Brio-Wu along z with linear + outgoing codes at x3 inner turns to NaN in
+# the reference after about 600 steps (the x3 outer inflow grows without
+# bound); the pinned state is at 450, after both boundaries have acted
+LINEAR_BRIOWU_Z_STEPS = 450
+LINEAR = {
+    "sod_x_linear": ("sodshock_x", "&gridcon ie=256 /\n&bouncon bc1is=2 bc1iv=2 bc1os=4 "
+                     "bc1ov=4 /\n" + _OUT),
+    "briowu_x_linear5": ("briowushock_x", "&gridcon ie=256 /\n&bouncon bc1is=3 bc1iv=3 "
+                         "bc1os=5 bc1ov=5 /\n" + _OUT),
+    "briowu_z_linear5": ("briowushock_z", "&gridcon ke=256 /\n&bouncon bc3is=5 bc3iv=5 "
+                         "bc3os=4 bc3ov=4 /\n&out_con outstyle=2 endstyle=2 tnlim=TN "
+                         "tn_out=TN output_ascii=.false. /\n"),
+    # the run stops during step 50 (negative e_int next to the x3 inner
+    # boundary): the state after 40 steps and the stop are both pinned
+    "ot_yz_linear": ("orszagtang_yz", "&gridcon je=64 ke=64 /\n&bouncon bc3is=4 bc3iv=4 "
+                     "bc3os=4 bc3ov=5 /\n&out_con outstyle=2 endstyle=2 tnlim=40 "
+                     "tn_out=40 output_ascii=.false. /\n"),
+    "ot_yz_linear_failure": ("orszagtang_yz", "&gridcon je=64 ke=64 /\n&bouncon bc3is=4 "
+                             "bc3iv=4 bc3os=4 bc3ov=5 /\n&out_con outstyle=2 endstyle=2 "
+                             "tnlim=100 tn_out=100 output_ascii=.false. /\n"),
+}
+
+
+def reference_runs_linear(meta: dict):
+    import re
+    for name, (simtype, params) in LINEAR.items():
+        params = params.replace("TN", str(LINEAR_BRIOWU_Z_STEPS))
+        b = run_reference(simtype, params)
+        if name.endswith("_failure"):
+            log = open("/tmp/hormone_fixture_run/log.txt").read()
+            steps = [int(m) for m in re.findall(r"tn =\s+(\d+)", log)]
+            cell = re.search(r"i=\s*(\d+), j=\s*(\d+), k=\s*(\d+)", log)
+            meta[name] = {
+                "stops": "Negative internal energy" in log,
+                "completed_steps": steps[-1] if steps else None,
+                "cell": [int(cell.group(q)) for q in (1, 2, 3)] if cell else None,
+                "source": f"reference run: {simtype} with " + " ".join(params.split())}
+            continue
+        if simtype.startswith("orszagtang"):
+            shape = (64, 64, 1)
+        else:
+            shape = {"x": (1, 1, 256), "y": (1, 256, 1), "z": (256, 1, 1)}[simtype[-1]]
+        mag = not simtype.startswith("sodshock")
+        snap = read_bin(b, shape, mag)
+        meta[name] = summary(snap, FIELDS_MAG if mag else FIELDS)
+        meta[name]["source"] = f"reference run: {simtype} with " + " ".join(params.split())
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), tn=snap["tn"], time=snap["time"],
+                            **{k: snap[k].reshape(-1) for k in (FIELDS_MAG if mag else FIELDS)})
+
+
 def reference_runs_extra(meta: dict):
     import re
     meta.pop("briowu_x_reflective", None)
@@ -295,6 +350,14 @@ def main():
             json.dump(meta, f, indent=1, sort_keys=True)
         print("wrote", path)
         return
+    if "--linear-only" in sys.argv:
+        path = os.path.join(HERE, "golden_meta.json")
+        meta = json.load(open(path))
+        reference_runs_linear(meta)
+        with open(path, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+        print("wrote", path)
+        return
     if "--extra-only" in sys.argv:
         path = os.path.join(HERE, "golden_meta.json")
         meta = json.load(open(path))
@@ -308,6 +371,7 @@ def main():
     convert_reference_goldens(meta)
     reference_runs(meta, long)
     reference_runs_extra(meta)
+    reference_runs_linear(meta)
     reference_restart_runs(meta)
     with open(path, "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)

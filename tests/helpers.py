@@ -84,7 +84,17 @@ EXTRA_CASES = {
     "sod_x_flat_rk1": ("sodshock_x", dict(n=256, flux_limiter="flat", rktype=1)),
     "ot64_minmod_rk2": ("orszagtang_xy", dict(n=64, flux_limiter="minmod", rktype=2, outstyle=2,
                                               endstyle=2, tnlim=100, tn_out=100)),
+    # linear boundary codes 4/5 (make_fixtures.py, LINEAR)
+    "sod_x_linear": ("sodshock_x", dict(n=256, bc=(2, 4, 3, 3, 3, 3, 2, 4, 3, 3, 3, 3),
+                                        t_end=300.0, dt_out=300.0)),
+    "briowu_x_linear5": ("briowushock_x", dict(n=256, bc=(3, 5, 3, 3, 3, 3, 3, 5, 3, 3, 3, 3),
+                                               t_end=300.0, dt_out=300.0)),
+    "briowu_z_linear5": ("briowushock_z", dict(n=256, bc=(3, 3, 3, 3, 5, 4, 3, 3, 3, 3, 5, 4),
+                                               outstyle=2, endstyle=2, tnlim=450, tn_out=450)),
+    "ot_yz_linear": ("orszagtang_yz", dict(n=64, bc=(0, 0, 0, 0, 4, 4, 0, 0, 0, 0, 4, 5),
+                                           outstyle=2, endstyle=2, tnlim=40, tn_out=40)),
 }
+LINEAR_CASES = ("sod_x_linear", "briowu_x_linear5", "briowu_z_linear5", "ot_yz_linear")
 
 
 def run_extra(engine_cls, case, max_steps=None):

@@ -46,3 +46,41 @@ def test_create_rejects_unsupported(hiplib):
         h = ctypes.c_void_p()
         assert hiplib.hg_create(ctypes.byref(d), ctypes.byref(h)) == code
         assert not h.value
+
+
+# The codes each face accepts: the cases of boundarycondition.f90's select
+# blocks (x1 inner :54-122 / :124-219, x1 outer :253-347 / :349-493, x2
+# :527-598 / :600-692 and :723-787 / :789-877, x3 :911-1004 / :1006-1144 and
+# :1175-1265 / :1267-1406); any other code stops the reference.
+_ACCEPTED = {
+    0: {1, 2, 3, 9, 10}, 1: {1, 2, 3, 4, 5, 9, 10}, 2: {1, 2, 3, 9, 10}, 3: {1, 2, 3, 9, 10},
+    4: {1, 2, 3, 4, 5, 9, 10}, 5: {1, 2, 3, 4, 9, 10},                # scalar codes
+    6: {1, 2, 3, 9, 10}, 7: {1, 2, 3, 4, 5, 9, 10}, 8: {1, 2, 3, 9, 10}, 9: {1, 2, 3, 9, 10},
+    10: {1, 2, 3, 4, 5, 9, 10}, 11: {1, 2, 3, 4, 5, 9, 10},           # vector codes
+}
+
+
+def test_boundary_codes_accepted_per_face(hiplib):
+    """hg_create and the oracle accept exactly the codes the reference
+    implements on each face (one slot changed at a time from outgoing)."""
+    from hormone_amd import native
+    from hormone_amd.grid import cartesian_grid
+    from oracle.pyoracle import lib as olib
+    g = cartesian_grid((8, 8, 8))
+    for b in range(12):
+        for code in range(1, 12):
+            bc = [2] * 12
+            bc[b] = code
+            for kw in (dict(), dict(compswitch=2, spn=2)):
+                d, keep = native.make_desc(g, bc=tuple(bc), **kw)
+                h = ctypes.c_void_p()
+                st = hiplib.hg_create(ctypes.byref(d), ctypes.byref(h))
+                if h.value:
+                    hiplib.hg_destroy(h)
+                ost = ctypes.c_int(0)
+                oh = olib().or_create(ctypes.byref(d), ctypes.byref(ost))
+                if oh:
+                    olib().or_destroy(oh)
+                want = code in _ACCEPTED[b] and not (kw and code >= 9)
+                assert (st != native.HG_ERR_UNSUPPORTED) == want, (b, code, kw, st)
+                assert (ost.value != native.HG_ERR_UNSUPPORTED) == want, (b, code, kw)

@@ -320,6 +320,26 @@ def test_gpu_reflective_briowu_stops_like_the_reference(hiplib):
     assert list(ei.value.cell) == fail["cell"]
 
 
+def test_gpu_linear_boundary_stops_like_the_reference(hiplib):
+    """Orszag-Tang yz with linear codes at the x3 boundaries stops during
+    step 50 at the reference's cell (negative internal energy)."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    from hormone_amd.native import HG_ERR_NEG_EINT, HgError
+    print("case ot_yz_linear_failure", flush=True)
+    fail = meta()["ot_yz_linear_failure"]
+    name, kw = EXTRA_CASES["ot_yz_linear"]
+    prm, g, st = P.make_problem(name, **dict(kw, tnlim=100, tn_out=100))
+    eng = Hydro(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, max_steps=fail["completed_steps"])
+    assert res.tn == fail["completed_steps"]
+    with pytest.raises(HgError) as ei:
+        D.run(eng, prm, g, max_steps=1)
+    assert ei.value.code == HG_ERR_NEG_EINT
+    assert list(ei.value.cell) == fail["cell"]
+
+
 def test_time_sweep_leaves_the_state_unchanged(hiplib):
     """hg_time_sweep (the benchmark's kernel timing) replays sweeps between
     steps: the run continues bit for bit as if it had not been called, also
@@ -375,3 +395,201 @@ def test_gpu_tiled_sweep_any_width_matches_oracle(hiplib, name, kw, steps, bw):
     a, b = D.fetch(e1, g, fields), D.fetch(e2, g, fields)
     for f in fields:
         assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
+
+
+@pytest.mark.parametrize("bw", [1, 4, 16])
+@pytest.mark.parametrize("name,kw,steps", [
+    ("orszagtang_xy", dict(n=90), 15),
+    ("orszagtang_yz", dict(n=70), 10),
+    ("briowushock_y", dict(), 40),
+    ("kh3d", dict(n=28, steps=6), 6),
+])
+def test_gpu_fused_last_sweep_matches_oracle(hiplib, name, kw, steps, bw):
+    """The last direction's sweep fused with the RK update (in-place update
+    from a halo copy), lane lines (bw=1) and LDS tiles with padding columns:
+    bitwise the oracle."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print(f"case fused {name} {kw} bw={bw}", flush=True)
+    prm, g, st = P.make_problem(name, **kw)
+    e1 = Hydro(**D.desc_fields(prm, g))
+    e1.set_launch(bw)
+    e1.set_fused(True)
+    D.load_initial_state(e1, prm, g, st)
+    r1 = D.run(e1, prm, g, max_steps=steps)
+    e2 = Oracle(**D.desc_fields(prm, g))
+    D.load_initial_state(e2, prm, g, st)
+    r2 = D.run(e2, prm, g, max_steps=steps)
+    assert (r1.tn, r1.time) == (r2.tn, r2.time)
+    fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+    a, b = D.fetch(e1, g, fields), D.fetch(e2, g, fields)
+    for f in fields:
+        assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
+
+
+@pytest.mark.parametrize("bw", [1, 8])
+@pytest.mark.parametrize("name", ["briowushock_y", "orszagtang_xy", "orszagtang_yz"])
+def test_gpu_fused_device_loop_goldens(hiplib, name, bw):
+    """hg_run with the fused last sweep: the CFL partials come one per block
+    of the fused launch, and the goldens still hold bit for bit."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print(f"case fused device-loop {name} bw={bw}", flush=True)
+    prm, g, st = P.make_problem(name)
+    eng = Hydro(**D.desc_fields(prm, g))
+    eng.set_launch(bw)
+    eng.set_fused(True)
+    eng.profile(bw == 8)  # per-launch events around the fused launches too
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, device_loop=True)
+    if bw == 8:
+        from hormone_amd.native import PROF_RK
+        ms, n, _ = eng.profile_read(PROF_RK)
+        assert n == prm.rktype * res.tn and ms > 0.0
+    fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+    out = D.fetch(eng, g, fields)
+    names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
+    state = {names[i]: out[f] for i, f in enumerate(fields)}
+    if name.startswith("orszagtang"):
+        compare_to_meta(res, state, meta()[name])
+    else:
+        compare_to_npz(res, state, name)
+
+
+def test_gpu_fused_polytrope_gravity_matches_oracle(hiplib):
+    """Fused z sweep with the gravity source in the RK update: every field and
+    the potential equal the oracle after 12 steps at 32^3."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print("case fused polytrope32", flush=True)
+    res = []
+    for cls in (Hydro, Oracle):
+        prm, g, st = P.make_problem("polytrope3d", n=32, steps=12)
+        eng = cls(**D.desc_fields(prm, g))
+        if cls is Hydro:
+            eng.set_fused(True)
+        D.load_initial_state(eng, prm, g, st)
+        r = D.run(eng, prm, g)
+        res.append((r, D.fetch(eng, g, D.SNAP_FIELDS + (D.F_GRVPHI, D.F_GRVPSI))))
+    (r1, s1), (r2, s2) = res
+    assert r1.tn == r2.tn == 12 and r1.time == r2.time
+    for f in s2:
+        assert np.array_equal(s1[f].view(np.uint64), s2[f].view(np.uint64)), f
+
+
+def test_gpu_fused_rejects_what_it_does_not_cover(hiplib):
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    from hormone_amd.native import HG_ERR_UNSUPPORTED, HgError
+    for name in ("sodshock_x", "KHinstability"):  # x only; species
+        prm, g, st = P.make_problem(name)
+        eng = Hydro(**D.desc_fields(prm, g))
+        with pytest.raises(HgError) as ei:
+            eng.set_fused(True)
+        assert ei.value.code == HG_ERR_UNSUPPORTED
+        eng.set_fused(False)
+
+
+@pytest.mark.parametrize("axis", ["x", "z"])
+@pytest.mark.parametrize("mag", [False, True], ids=["hydro", "mhd"])
+def test_gpu_dirichlet_and_flux_boundary_closed_forms(hiplib, axis, mag):
+    """Dirichlet ghosts holding a uniform flow keep it uniform bit for bit;
+    flux boundaries change every conserved total by dt*(F_in - F_out)*area."""
+    from hormone_amd import driver as D
+    from tests import boundary_cases as B
+    print(f"case boundary closed forms {axis} mag={mag}", flush=True)
+    prm, g, eng = B.uniform(Hydro, axis, 9, mag=mag)
+    B.set_uniform_ghosts(eng, g, mag)
+    f0 = D.fetch(eng, g, B.fields(prm))
+    D.run(eng, prm, g, max_steps=10)
+    f1 = D.fetch(eng, g, B.fields(prm))
+    for f in f0:
+        assert np.array_equal(f0[f], f1[f]), f
+    got, want = B.flux_budget(Hydro, axis, mag)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-15)
+
+
+def _ext_case(kind):
+    """A smooth 2D MHD state on the yz plane (64^2 cells of Orszag-Tang's
+    grid, periodic y) with the extended codes on the z faces; the caller
+    sets Dirichlet ghosts and boundary fluxes near that state."""
+    from hormone_amd import problems as P
+    bc = {"dirichlet": (0, 0, 0, 0, 9, 9, 0, 0, 0, 0, 9, 9),
+          "flux": (0, 0, 0, 0, 10, 10, 0, 0, 0, 0, 10, 10),
+          "mixed": (0, 0, 0, 0, 10, 4, 0, 0, 0, 0, 9, 5)}[kind]
+    prm, g, st = P.make_problem("orszagtang_yz", n=64, bc=bc, outstyle=2, endstyle=2,
+                                tnlim=1000, tn_out=1000)
+    y = g.centres(1)[None, :, None]
+    z = g.centres(2)[:, None, None]
+    s = np.sin(2 * np.pi * y) * np.sin(2 * np.pi * z)
+    c = np.cos(2 * np.pi * y) * np.cos(2 * np.pi * z)
+    st.d[...] = 1.0 + 0.1 * s
+    st.p[...] = 0.6 + 0.05 * c
+    for a in range(3):
+        st.v[a][...] = 0.05 * (s if a != 1 else c)
+        st.b[a][...] = 0.1 * (c if a != 2 else s) + (0.3 if a == 2 else 0.0)
+    return prm, g, st
+
+
+def _ext_boundary_values(eng, g, phase):
+    from hormone_amd import driver as D
+    rng = np.random.default_rng(7 + phase)
+    sh = g.padded_shape
+    eng.set_ghosts(D.F_D, 1.0 + 0.1 * rng.random(sh))
+    eng.set_ghosts(D.F_P, 0.6 + 0.05 * rng.random(sh))
+    for a in range(3):
+        eng.set_ghosts(D.F_V1 + a, 0.05 * (rng.random(sh) - 0.5))
+        eng.set_ghosts(D.F_B1 + a, 0.1 * (rng.random(sh) - 0.5) + (0.3 if a == 2 else 0.0))
+    eng.set_ghosts(D.F_PHI, 0.01 * rng.random(sh))
+    # z faces: normal momentum flux (imo3) near the total pressure
+    flux = np.array([0.001, 0.0, 0.002, 0.6, 0.01, 0.0, 0.001, -0.001, 0.0]) * (1 + 0.1 * phase)
+    for face in (4, 5):
+        try:
+            eng.set_boundary_flux(face, flux * (1.0 if face == 4 else 0.5))
+        except Exception:  # not a flux face
+            pass
+
+
+@pytest.mark.parametrize("bw", [1, 8])
+@pytest.mark.parametrize("kind", ["dirichlet", "flux", "mixed"])
+def test_gpu_extended_boundaries_match_oracle(hiplib, kind, bw):
+    """Dirichlet, flux and linear codes on the z faces, lane lines (bw=1) and
+    LDS tiles: bitwise the oracle, with the boundary data changed between
+    steps (time-dependent Dirichlet data)."""
+    from hormone_amd import driver as D
+    print(f"case extended boundaries {kind} bw={bw}", flush=True)
+    out = []
+    for cls in (Hydro, Oracle):
+        prm, g, st = _ext_case(kind)
+        eng = cls(**D.desc_fields(prm, g))
+        if cls is Hydro:
+            eng.set_launch(bw)
+        D.load_initial_state(eng, prm, g, st)
+        _ext_boundary_values(eng, g, 0)
+        r1 = D.run(eng, prm, g, max_steps=8)
+        _ext_boundary_values(eng, g, 1)
+        r2 = D.run(eng, prm, g, max_steps=16, start=(r1.tn, r1.time))
+        out.append((r2.tn, r2.time, D.fetch(eng, g, D.SNAP_FIELDS_MAG)))
+    (n1, t1, a), (n2, t2, b) = out
+    assert (n1, t1) == (n2, t2) == (16, t2)
+    for f in D.SNAP_FIELDS_MAG:
+        assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
+
+
+@pytest.mark.parametrize("bw", [1, 16])
+def test_gpu_linear_boundaries_any_launch_shape(hiplib, bw):
+    """The linear codes (and the unwritten v2/b2 ghost below the x3 inner
+    boundary) in lane lines and in 16-line tiles: the reference's run."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print(f"case ot_yz_linear bw={bw}", flush=True)
+    name, kw = EXTRA_CASES["ot_yz_linear"]
+    prm, g, st = P.make_problem(name, **kw)
+    eng = Hydro(**D.desc_fields(prm, g))
+    eng.set_launch(bw)
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g)
+    out = D.fetch(eng, g, D.SNAP_FIELDS_MAG)
+    names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
+    compare_to_meta(res, {names[i]: out[f] for i, f in enumerate(D.SNAP_FIELDS_MAG)},
+                    meta()["ot_yz_linear"])

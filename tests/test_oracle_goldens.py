@@ -70,6 +70,25 @@ def test_oracle_reflective_briowu_stops_like_the_reference():
     assert list(ei.value.cell) == fail["cell"]
 
 
+def test_oracle_linear_boundary_stops_like_the_reference():
+    """Orszag-Tang yz with linear codes at the x3 boundaries: the reference
+    stops during step 50 with a negative internal energy next to the x3
+    inner boundary; the oracle stops at the same step and cell."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    from oracle.pyoracle import HgError
+    fail = meta()["ot_yz_linear_failure"]
+    name, kw = EXTRA_CASES["ot_yz_linear"]
+    prm, g, st = P.make_problem(name, **dict(kw, tnlim=100, tn_out=100))
+    eng = Oracle(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    res = D.run(eng, prm, g, max_steps=fail["completed_steps"])
+    assert res.tn == fail["completed_steps"]
+    with pytest.raises(HgError) as ei:
+        D.run(eng, prm, g, max_steps=1)
+    assert list(ei.value.cell) == fail["cell"]
+
+
 @pytest.mark.parametrize("flat", [False, True], ids=["muscl", "flat"])
 def test_oracle_kh_goldens(flat):
     """2D hydro Kelvin-Helmholtz with two advected species (compswitch 2):
