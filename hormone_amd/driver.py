@@ -56,11 +56,15 @@ def load_initial_state(eng, prm: P.Params, g: Grid, st: P.State):
         eng.upload(F_GRVPSI, np.zeros(g.padded_shape))
 
 
-def load_restart_state(eng, prm: P.Params, g: Grid, snap: dict):
+def load_restart_state(eng, prm: P.Params, g: Grid, snap: dict, grav_boundary=None):
     """Restart from a snapshot (restart.f90:15-64 -> readbin.f90:16-115,
     then hormone.f90:86-90): the primitives and total energy as read, the
     conserved variables from them (conserve), e_int from them (pressure, by
-    the engine).  Returns (tn, time) of the snapshot."""
+    the engine).  Returns (tn, time) of the snapshot, and cgrav_old as a
+    third element for hyperbolic gravity (gravswitch 3), whose potential
+    and psi come from the snapshot too; grav_boundary is then the padded
+    potential the run started with, whose ghost layer holds the boundary
+    values."""
     sh = g.shape
     d = np.asarray(snap["d"]).reshape(sh)
     v = [np.asarray(snap[k]).reshape(sh) for k in ("v1", "v2", "v3")]
@@ -78,6 +82,12 @@ def load_restart_state(eng, prm: P.Params, g: Grid, snap: dict):
     if prm.compswitch == 2:
         for q in range(prm.spn):
             eng.upload(F_SPC + q, g.pad(np.asarray(snap[f"spc{q + 1}"]).reshape(sh)))
+    if prm.gravswitch == 3:
+        phi = np.array(grav_boundary, dtype=np.float64, copy=True)
+        phi[2:-2, 2:-2, 2:-2] = np.asarray(snap["grvphi"]).reshape(sh)
+        eng.upload(F_GRVPHI, phi)
+        eng.upload(F_GRVPSI, g.pad(np.asarray(snap["grvpsi"]).reshape(sh)))
+        return int(snap["tn"]), float(snap["time"]), float(snap["cgrav_old"])
     return int(snap["tn"]), float(snap["time"])
 
 
@@ -133,6 +143,7 @@ class RunResult:
     steps_timed: int
     dts: list = field(default_factory=list)
     nsub: list = field(default_factory=list)  # gravity sub-steps per step
+    cgrav_old: float = 0.0  # gravmod's cgrav_old at the end (gravswitch 3)
 
 
 def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
@@ -153,6 +164,9 @@ def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
     nsubs = []
     grav = prm.gravswitch == 3
     grvtime, cgrav_old = 0.0, 0.0  # gravmod module values at start-up
+    if grav and len(start) > 2:
+        # restart: cgrav_old from the snapshot, grvtime = time (restart.f90:59)
+        grvtime, cgrav_old = tim, float(start[2])
     t0 = _time.perf_counter()
     while True:
         if grav:
@@ -188,7 +202,7 @@ def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
         if max_steps is not None and tn >= max_steps:
             break
     wall = _time.perf_counter() - t0
-    return RunResult(tn, tim, wall, tn, dts, nsubs)
+    return RunResult(tn, tim, wall, tn, dts, nsubs, cgrav_old)
 
 
 def _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out, start=(0, 0.0)) -> RunResult:
@@ -218,3 +232,23 @@ def fetch(eng, g: Grid, fields) -> dict:
 
 SNAP_FIELDS = (F_D, F_V1, F_V2, F_V3, F_E)
 SNAP_FIELDS_MAG = SNAP_FIELDS + (F_B1, F_B2, F_B3, F_PHI)
+
+
+def snapshot(eng, g: Grid, prm: P.Params, res: RunResult, species=()) -> dict:
+    """The state write_bin writes (output.f90:697-795) at the end of run
+    result `res`: for snapshot.write_bin and load_restart_state."""
+    names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
+    fields = SNAP_FIELDS_MAG if prm.mag_on else SNAP_FIELDS
+    out = fetch(eng, g, fields)
+    snap = {names[q]: out[f] for q, f in enumerate(fields)}
+    snap["tn"], snap["time"] = res.tn, res.time
+    if prm.gravswitch == 3:
+        gv = fetch(eng, g, (F_GRVPHI, F_GRVPSI))
+        snap["grvphi"], snap["grvpsi"] = gv[F_GRVPHI], gv[F_GRVPSI]
+        snap["cgrav_old"] = res.cgrav_old
+    if prm.compswitch == 2:
+        sp = fetch(eng, g, tuple(F_SPC + q for q in range(prm.spn)))
+        for q in range(prm.spn):
+            snap[f"spc{q + 1}"] = sp[F_SPC + q]
+        snap["species"] = list(species) or [f"spc{q + 1}" for q in range(prm.spn)]
+    return snap

@@ -305,6 +305,21 @@ def reference_runs_linear(meta: dict):
                             **{k: snap[k].reshape(-1) for k in (FIELDS_MAG if mag else FIELDS)})
 
 
+def reference_grid_file(meta: dict):
+    """gridfile.bin the reference writes at start-up (write_grid_bin,
+    output.f90:432-452) for Orszag-Tang 64^2."""
+    run_reference("orszagtang_xy", "&gridcon ie=64 je=64 /\n&out_con outstyle=2 endstyle=2 "
+                  "tnlim=1 tn_out=1 output_ascii=.false. /\n")
+    # dx(is-2) of each axis is never set by gridset (gridset.f90:54-59 starts
+    # at is-1), so the file holds whatever that memory held: keep the file
+    raw = open("/tmp/hormone_fixture_run/work/data/gridfile.bin", "rb").read()
+    with open(os.path.join(HERE, "gridfile_ot64.bin"), "wb") as f:
+        f.write(raw)
+    meta["gridfile_ot64"] = {"file_bytes": len(raw),
+                             "source": "reference run: orszagtang_xy with ie=64 je=64, "
+                                       "data/gridfile.bin (tests/golden/gridfile_ot64.bin)"}
+
+
 def reference_runs_extra(meta: dict):
     import re
     meta.pop("briowu_x_reflective", None)
@@ -354,6 +369,7 @@ def main():
         path = os.path.join(HERE, "golden_meta.json")
         meta = json.load(open(path))
         reference_runs_linear(meta)
+        reference_grid_file(meta)
         with open(path, "w") as f:
             json.dump(meta, f, indent=1, sort_keys=True)
         print("wrote", path)
@@ -372,6 +388,7 @@ def main():
     reference_runs(meta, long)
     reference_runs_extra(meta)
     reference_runs_linear(meta)
+    reference_grid_file(meta)
     reference_restart_runs(meta)
     with open(path, "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)

@@ -75,3 +75,103 @@ def test_gpu_uninterrupted_ot64_matches_the_reference_step100(hiplib):
     from tests.helpers import run_problem
     res, state, _ = run_problem(Hydro, "orszagtang_xy", **RESTART)
     compare_to_meta(res, state, meta()["ot64_step100"])
+
+
+def test_grid_file_matches_the_reference_file():
+    """gridfile.bin (write_grid_bin, output.f90:432-452) for Orszag-Tang
+    64^2: the reference's own file, element for element, except dx(is-2) of
+    each axis, which the reference never sets (gridset.f90:54-59 starts at
+    is-1) and so writes whatever that memory held."""
+    from hormone_amd.snapshot import read_grid_bin, write_grid_bin
+    prm, g, _ = P.make_problem("orszagtang_xy", n=64)
+    raw = write_grid_bin(g)
+    ref = open(os.path.join(GOLDEN, "gridfile_ot64.bin"), "rb").read()
+    assert len(raw) == len(ref) == meta()["gridfile_ot64"]["file_bytes"]
+    assert raw[:4] == ref[:4] and raw[-4:] == ref[-4:]  # record length markers
+    a, b = read_grid_bin(raw, (64, 64, 1)), read_grid_bin(ref, (64, 64, 1))
+    for k in a:
+        lo = 1 if k.startswith("dx") and not k.startswith("dxi") else 0
+        assert np.array_equal(a[k][lo:], b[k][lo:]), k
+
+
+def _gravity_run(engine_cls, steps, snap=None):
+    """Polytrope 16^3 with hyperbolic gravity: `steps` steps from the start,
+    or from snapshot `snap` to step `steps`."""
+    prm, g, st = P.make_problem("polytrope3d", n=16, steps=steps)
+    eng = engine_cls(**D.desc_fields(prm, g))
+    if snap is None:
+        D.load_initial_state(eng, prm, g, st)
+        res = D.run(eng, prm, g)
+    else:
+        start = D.load_restart_state(eng, prm, g, snap, st.extra["grvphi_padded"])
+        res = D.run(eng, prm, g, start=start)
+    return prm, g, eng, res
+
+
+def test_gravity_snapshot_round_trip_and_restart():
+    """Snapshots with the gravity groups (output.f90:735-746: grvphi, then
+    grvpsi and cgrav_old) read back exactly, and a restart from one (the
+    potential, psi and cgrav_old restored, grvtime = time, restart.f90:59)
+    on a fresh engine continues bit for bit like the engine that wrote the
+    snapshot restarted from it in place: the snapshot carries all the
+    state the step needs.  Dropping psi or cgrav_old changes the result.
+    (An uninterrupted run is not the comparison: the restart rebuilds the
+    momenta from the primitives, an ulp off, and grvtime = time where the
+    running clock trails by an ulp; the polytrope's limiter and e_int-guard
+    branches turn that into differences of 1e-4, as in the reference's
+    own restart.)"""
+    from oracle.pyoracle import Oracle
+    from hormone_amd.snapshot import read_bin
+    prm, g, eng, res = _gravity_run(Oracle, 6)
+    snap = D.snapshot(eng, g, prm, res)
+    raw = write_bin(snap, mag_on=False, gravswitch=3)
+    n = g.ncells
+    assert len(raw) == 4 * 4 + 4 + 8 + 5 * 8 * n + 4 * 4 + 2 * 8 * n + 8
+    path = "/tmp/hormone_grav_snapshot.bin"
+    with open(path, "wb") as f:
+        f.write(raw)
+    back = read_bin(path, g.shape, False, gravswitch=3)
+    os.remove(path)
+    assert back["cgrav_old"] == snap["cgrav_old"] and back["cgrav_old"] > 0.0
+    for k in ("d", "e", "grvphi", "grvpsi"):
+        assert np.array_equal(back[k], snap[k]), k
+    assert write_bin(back, mag_on=False, gravswitch=3) == raw
+    # the engine that wrote the snapshot, restarted from it in place
+    prm12, g12, st12 = P.make_problem("polytrope3d", n=16, steps=12)
+    start = D.load_restart_state(eng, prm12, g12, back, st12.extra["grvphi_padded"])
+    D.run(eng, prm12, g12, start=start)
+    cont = D.fetch(eng, g, (D.F_D, D.F_E, D.F_GRVPHI, D.F_GRVPSI))
+    _, _, e_rest, r_rest = _gravity_run(Oracle, 12, back)
+    assert r_rest.tn == 12
+    rest = D.fetch(e_rest, g, tuple(cont))
+    for f in cont:
+        assert np.array_equal(rest[f].view(np.uint64), cont[f].view(np.uint64)), f
+    for k, v in (("grvpsi", np.zeros_like(back["grvpsi"])), ("cgrav_old", 0.5 * back["cgrav_old"])):
+        _, _, e_bad, _ = _gravity_run(Oracle, 12, dict(back, **{k: v}))
+        assert not np.array_equal(D.fetch(e_bad, g, (D.F_GRVPHI,))[D.F_GRVPHI],
+                                  cont[D.F_GRVPHI]), k
+
+
+@pytest.mark.gpu
+def test_gpu_gravity_snapshot_and_restart_match_oracle(hiplib):
+    """The GPU's snapshot of a gravity run (potential, psi, cgrav_old) is the
+    oracle's byte for byte, and restarting from it gives the oracle's
+    restarted run bit for bit."""
+    from hormone_amd.native import Hydro
+    from oracle.pyoracle import Oracle
+    print("case gravity snapshot/restart polytrope16", flush=True)
+    raws, states = [], []
+    for cls in (Hydro, Oracle):
+        prm, g, eng, res = _gravity_run(cls, 6)
+        raws.append(write_bin(D.snapshot(eng, g, prm, res), mag_on=False, gravswitch=3))
+    assert raws[0] == raws[1]
+    path = "/tmp/hormone_grav_snapshot_gpu.bin"
+    with open(path, "wb") as f:
+        f.write(raws[0])
+    back = read_bin(path, g.shape, False, gravswitch=3)
+    os.remove(path)
+    for cls in (Hydro, Oracle):
+        _, _, e, r = _gravity_run(cls, 12, back)
+        states.append(D.fetch(e, g, (D.F_D, D.F_E, D.F_V3, D.F_GRVPHI, D.F_GRVPSI)))
+    for f in states[1]:
+        assert np.array_equal(states[0][f].view(np.uint64), states[1][f].view(np.uint64)), f
