@@ -86,9 +86,9 @@ def parse():
                          "(comma separated; empty = one batch)")
     ap.add_argument("--sweep-waves", type=int, default=0,
                     help="sweep wavefronts per block (0: autotune during warm-up)")
-    ap.add_argument("--fused", default="auto", choices=["auto", "on", "off"],
-                    help="last sweep direction fused with the RK update (auto: time both "
-                         "during warm-up)")
+    ap.add_argument("--fused", default="auto", choices=["auto", "last", "x", "off"],
+                    help="a sweep direction fused with the RK update: the last solved one, "
+                         "x, or none (auto: time each during warm-up, keep the fastest)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 256 MiB (the reference kernel for "
                          "scaling rocprof FETCH_SIZE/WRITE_SIZE to bytes)")
@@ -365,18 +365,18 @@ def main():
         if left > 0:
             run.advance(left)
     fused = False
-    if a.fused == "on":
-        eng.set_fused(True)
-        fused = True
+    names = {0: "separate", 1: "last", 2: "x"}
+    if a.fused in ("last", "x"):
+        fused = {"last": 1, "x": 2}[a.fused]
+        eng.set_fused(fused)
         tuned = dict(tuned or {})
-        tuned["fused_last_sweep"] = True
+        tuned["fused_sweep"] = a.fused
     elif a.fused == "auto":
         from hormone_amd.batch import tune_fused
-        fused, ftimes = tune_fused(eng, run.advance, steps=max(2, min(12, a.warmup // 4)))
+        fused, ftimes = tune_fused(eng, run.advance, steps=max(2, min(8, a.warmup // 4)))
         tuned = dict(tuned or {})
-        tuned["fused_last_sweep"] = fused
-        tuned["ms_per_step"] = {("fused" if k else "separate"): round(v * 1e3, 3)
-                                for k, v in ftimes.items()}
+        tuned["fused_sweep"] = names[fused]
+        tuned["ms_per_step"] = {names[k]: round(v * 1e3, 3) for k, v in ftimes.items()}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ok = run.advance(a.steps)
@@ -416,9 +416,18 @@ def main():
                 "timing": "back-to-back launches between two HIP events, after the timed region",
                 "algorithmic_bytes_per_launch": bpl,
                 "share_of_timed_region": round(avg_s * a.steps * prm.rktype / wall, 3)}
-        if ev_n:
-            roof["event_avg_launch_us"] = round(ev_ms / ev_n * 1e3, 2)
-        if rk_n:
+        if fused:
+            # the timed region ran one direction's sweep fused with the RK
+            # update (k_sweep_rk) and the other directions in k_sweep; the
+            # replay above times the all-direction k_sweep
+            roof["timed_region_form"] = {1: "last", 2: "x"}[fused] + " direction fused with RK"
+            roof.pop("share_of_timed_region")
+        if ev_n:  # fused: the sweep of the other directions only
+            roof["other_directions_sweep_avg_launch_us" if fused else "event_avg_launch_us"] = \
+                round(ev_ms / ev_n * 1e3, 2)
+        if rk_n and fused:  # the RK events time the fused sweep + RK launch
+            roof["fused_sweep_rk_avg_launch_us"] = round(rk_ms / rk_n * 1e3, 2)
+        elif rk_n:
             roof["rk_avg_launch_us"] = round(rk_ms / rk_n * 1e3, 2)
             roof["rk_achieved_GBs"] = round(rk_b / (rk_ms / rk_n / 1e3) / 1e9, 1)
         if gr_n:

@@ -188,26 +188,33 @@ def tune_single(eng: native.Hydro, advance, candidates=(2, 4, 8, 16), steps: int
 
 
 def tune_fused(eng: native.Hydro, advance, steps: int = 4):
-    """Choose between the separate RK update and the last sweep direction
-    fused with it (hg_set_fused) by timing whole steps of each, on the host
-    clock; ``advance(k)`` must take k steps and return after they finish.
-    Returns (fused, {False: s/step, True: s/step}); (False, {}) when the
-    context does not support the fused form.  Both forms give the same
-    bits, so the tuning steps are ordinary steps of the run."""
-    try:
-        eng.set_fused(True)
-    except native.HgError as err:
-        if err.code != native.HG_ERR_UNSUPPORTED:
-            raise
-        return False, {}
+    """Choose between the separate RK update and a sweep direction fused
+    with it (hg_set_fused: 1 the last solved direction, 2 x) by timing
+    whole steps of each, on the host clock; ``advance(k)`` must take k steps
+    and return after they finish.  Returns (mode, {mode: s/step}) with mode
+    0 for the separate update; the modes the context does not support are
+    skipped.  Every form gives the same bits, so the tuning steps are
+    ordinary steps of the run."""
+    modes = [0]
+    for m in (1, 2):
+        try:
+            eng.set_fused(m)
+            modes.append(m)
+        except native.HgError as err:
+            if err.code != native.HG_ERR_UNSUPPORTED:
+                raise
+    eng.set_fused(0)
+    if len(modes) == 1:
+        return 0, {}
     timings = {}
-    for on in (True, False, True, False):
-        eng.set_fused(on)
-        advance(1)
-        t0 = time.perf_counter()
-        advance(steps)
-        dt = (time.perf_counter() - t0) / steps
-        timings[on] = min(dt, timings.get(on, dt))
-    best = timings[True] < timings[False]
+    for _ in range(2):
+        for m in modes:
+            eng.set_fused(m)
+            advance(1)
+            t0 = time.perf_counter()
+            advance(steps)
+            dt = (time.perf_counter() - t0) / steps
+            timings[m] = min(dt, timings.get(m, dt))
+    best = min(timings, key=timings.get)
     eng.set_fused(best)
     return best, timings

@@ -39,6 +39,7 @@ struct hg_ctx {
   KEv kev{};
   bool fused = false;   // last sweep direction fused with the RK update
   long part_cap = 0;    // CFL partial slots allocated
+  long h_cap = 0;       // halo entries per field allocated (fused sweep)
 };
 
 #define HIPCHK(x)                                   \
@@ -703,7 +704,8 @@ static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
       // a pair for the sweep of the directions before the fused one only
       // when there are such directions (every pair taken must be recorded)
       KEv k0{}, k1{};
-      const KEv *e0 = D.woff[D.fdir] > 0 ? prof_ev_into(c, HG_PROF_SWEEP, k0) : nullptr;
+      const long pre = D.fdir == 0 ? D.woff[2] - D.woff[1] + D.waves[2] : D.woff[D.fdir];
+      const KEv *e0 = pre > 0 ? prof_ev_into(c, HG_PROF_SWEEP, k0) : nullptr;
       const KEv *e1 = prof_ev_into(c, HG_PROF_RK, k1);
       const int g = launch_fused_stage(D, stage, a, c->stream, e0, e1);
       if (last && fuse_tail) npart = g;
@@ -873,18 +875,29 @@ extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
     D.fdir = -1;
     return HG_OK;
   }
-  // FP64 only, no species; the last solved direction must be y or z
+  // FP64 only, no species; on = 1: the last solved direction (y or z),
+  // on = 2: x
   int L = -1;
-  for (int a = 0; a < 3; ++a)
-    if (D.solve[a]) L = a;
-  if (D.mixed || D.nspc || L < 1) return HG_ERR_UNSUPPORTED;
+  if (on == 2) {
+    L = D.solve[0] ? 0 : -1;
+  } else if (on == 1) {
+    for (int a = 1; a < 3; ++a)
+      if (D.solve[a]) L = a;
+  } else {
+    return HG_ERR_ARG;
+  }
+  if (D.mixed || D.nspc || L < 0) return HG_ERR_UNSUPPORTED;
   for (int a = 0; a < 3; ++a)
     if (D.ax[a].ext) return HG_ERR_UNSUPPORTED;  // the halo copy knows codes 0-3 only
-  if (!D.H) {
+  {
     const int K = (D.ax[L].n + SEG - 1) / SEG;
-    D.hsz = (long)D.ax[L].nlines * (K + 1) * 4;
-    int st = dev_alloc(c, sizeof(double) * (size_t)D.hsz * D.nw, (void **)&D.H);
-    if (st) return st;
+    const long hsz = (long)D.ax[L].nlines * (K + 1) * 4;
+    if (hsz > c->h_cap) {
+      int st = dev_alloc(c, sizeof(double) * (size_t)hsz * D.nw, (void **)&D.H);
+      if (st) return st;
+      c->h_cap = hsz;
+    }
+    D.hsz = hsz;
   }
   // one CFL partial per block of the fused launch (one wavefront per block
   // at most)

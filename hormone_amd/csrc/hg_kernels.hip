@@ -1761,8 +1761,13 @@ void launch_species(const Dev &D, hipStream_t s) {
 template <int DIR, bool MAG, int LIM>
 static void sweep_rk_variant(const Dev &D, int stage, const RKArgs &rk, unsigned g, int bs,
                              size_t tile, hipStream_t s, const KEv *ev) {
-  if (tile) klaunch_lds(k_sweep_rk<DIR, MAG, LIM, true>, g, bs, tile, s, ev, D, stage, rk);
-  else klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false>, g, bs, 0, s, ev, D, stage, rk);
+  if constexpr (DIR > 0) {  // x lines are lane lines, never tiled
+    if (tile) {
+      klaunch_lds(k_sweep_rk<DIR, MAG, LIM, true>, g, bs, tile, s, ev, D, stage, rk);
+      return;
+    }
+  }
+  klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false>, g, bs, 0, s, ev, D, stage, rk);
 }
 
 template <int DIR>
@@ -1785,7 +1790,10 @@ int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
   const Axis &A = D.ax[L];
   const int K = (A.n + SEG - 1) / SEG;
   const long nh = (long)A.nlines * (K + 1) * 4;
-  if (L == 1) {
+  if (L == 0) {
+    if (D.mag) k_halo<0, true><<<nblk(nh, 256), 256, 0, s>>>(D);
+    else k_halo<0, false><<<nblk(nh, 256), 256, 0, s>>>(D);
+  } else if (L == 1) {
     if (D.mag) k_halo<1, true><<<nblk(nh, 256), 256, 0, s>>>(D);
     else k_halo<1, false><<<nblk(nh, 256), 256, 0, s>>>(D);
   } else {
@@ -1795,20 +1803,32 @@ int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
   const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
   const int bs = 64 * bw;
   const size_t tile = sweep_tile_bytes(D, bw);
-  if (D.woff[L] > 0) {  // the directions before L, as launch_sweep does them
-    const unsigned g0 = nblk(D.woff[L], bw);
+  // the other directions first, as launch_sweep does them: those before L
+  // (L = 1, 2), or y and z with the x range of the wave numbering emptied
+  // (L = 0; the y/z ranges start on multiples of 16 wavefronts, so blocks
+  // still never mix directions)
+  Dev Do = D;
+  long nwo = D.woff[L];
+  if (L == 0) {
+    Do.woff[2] = D.woff[2] - D.woff[1];
+    Do.woff[1] = 0;
+    nwo = Do.woff[2] + D.waves[2];
+  }
+  if (nwo > 0) {
+    const unsigned g0 = nblk(nwo, bw);
     if (D.mag) {
-      if (D.limiter == LIM_FLAT) sweep_variant<true, LIM_FLAT>(D, stage, g0, bs, tile, s, ev);
-      else if (D.limiter == LIM_MINMOD) sweep_variant<true, LIM_MINMOD>(D, stage, g0, bs, tile, s, ev);
-      else sweep_variant<true, LIM_MC>(D, stage, g0, bs, tile, s, ev);
+      if (D.limiter == LIM_FLAT) sweep_variant<true, LIM_FLAT>(Do, stage, g0, bs, tile, s, ev);
+      else if (D.limiter == LIM_MINMOD) sweep_variant<true, LIM_MINMOD>(Do, stage, g0, bs, tile, s, ev);
+      else sweep_variant<true, LIM_MC>(Do, stage, g0, bs, tile, s, ev);
     } else {
-      if (D.limiter == LIM_FLAT) sweep_variant<false, LIM_FLAT>(D, stage, g0, bs, tile, s, ev);
-      else if (D.limiter == LIM_MINMOD) sweep_variant<false, LIM_MINMOD>(D, stage, g0, bs, tile, s, ev);
-      else sweep_variant<false, LIM_MC>(D, stage, g0, bs, tile, s, ev);
+      if (D.limiter == LIM_FLAT) sweep_variant<false, LIM_FLAT>(Do, stage, g0, bs, tile, s, ev);
+      else if (D.limiter == LIM_MINMOD) sweep_variant<false, LIM_MINMOD>(Do, stage, g0, bs, tile, s, ev);
+      else sweep_variant<false, LIM_MC>(Do, stage, g0, bs, tile, s, ev);
     }
   }
   const unsigned g = nblk(D.waves[L], bw);
-  if (L == 1) sweep_rk_dir<1>(D, stage, rk, g, bs, tile, s, ev_fused);
+  if (L == 0) sweep_rk_dir<0>(D, stage, rk, g, bs, 0, s, ev_fused);
+  else if (L == 1) sweep_rk_dir<1>(D, stage, rk, g, bs, tile, s, ev_fused);
   else sweep_rk_dir<2>(D, stage, rk, g, bs, tile, s, ev_fused);
   return (int)g;
 }

@@ -85,7 +85,7 @@ struct Dev {
   double *FM[3];  // face mass flux per direction, (n+1) faces per line
   // last sweep direction fused with the RK update: halo copy of the cells a
   // segment reads but does not own (see k_halo), hsz entries per field
-  int fdir;        // the fused direction (1 or 2), -1 if none
+  int fdir;        // the direction fused with the RK update, -1 if none
   double *H;
   long hsz;
   double *part;    // CFL partials: 2 doubles per block of 256 cells
@@ -168,8 +168,8 @@ struct RKArgs {
   double faco, fact, facn, damp;
   int cfl, dampf, grav;
 };
-// the fused path of one RK stage: halo copy, the sweeps of the directions
-// before D.fdir, then D.fdir's sweep with the RK update; returns the number
+// the fused path of one RK stage: halo copy, the sweeps of the other
+// directions, then D.fdir's sweep with the RK update; returns the number
 // of CFL partials the fused launch wrote (rk.cfl)
 int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
                        const KEv *ev_sweep = nullptr, const KEv *ev_fused = nullptr);

@@ -243,8 +243,9 @@ class Hydro:
     def profile(self, enable=True):
         self._check(self._lib.hg_profile(self.h, int(enable)), "hg_profile")
 
-    def set_fused(self, on: bool = True):
-        """Fuse the last sweep direction with the RK update (bit-identical)."""
+    def set_fused(self, on=True):
+        """Fuse one sweep direction with the RK update (bit-identical):
+        0/False off, 1/True the last solved direction (y or z), 2 x."""
         self._check(self._lib.hg_set_fused(self.h, int(on)), "hg_set_fused")
 
     def time_sweep(self, launches):

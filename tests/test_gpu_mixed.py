@@ -68,6 +68,10 @@ def test_mixed_error_at_rounding_floor(hiplib, name, steps, kw):
 @pytest.mark.parametrize("name,steps,kw,tol", [
     ("orszagtang_xy", 20, dict(n=128), 2e-4),
     ("sodshock_x", 300, dict(n=1024, courant=0.4), 1e-3),
+    # linear boundary codes on the z faces (the mixed sweep's own
+    # instantiation for codes 4, 5, 9, 10)
+    ("orszagtang_yz", 30, dict(n=64, bc=(0, 0, 0, 0, 4, 4, 0, 0, 0, 0, 4, 5), outstyle=2,
+                               endstyle=2, tnlim=40, tn_out=40), 1e-3),
 ])
 def test_mixed_close_to_fp64(hiplib, name, steps, kw, tol):
     print(f"case mixed {name}", flush=True)

@@ -398,23 +398,26 @@ def test_gpu_tiled_sweep_any_width_matches_oracle(hiplib, name, kw, steps, bw):
 
 
 @pytest.mark.parametrize("bw", [1, 4, 16])
-@pytest.mark.parametrize("name,kw,steps", [
-    ("orszagtang_xy", dict(n=90), 15),
-    ("orszagtang_yz", dict(n=70), 10),
-    ("briowushock_y", dict(), 40),
-    ("kh3d", dict(n=28, steps=6), 6),
+@pytest.mark.parametrize("name,kw,steps,mode", [
+    ("orszagtang_xy", dict(n=90), 15, 1),
+    ("orszagtang_xy", dict(n=90), 15, 2),
+    ("orszagtang_yz", dict(n=70), 10, 1),
+    ("briowushock_y", dict(), 40, 1),
+    ("sodshock_x", dict(), 40, 2),
+    ("kh3d", dict(n=28, steps=6), 6, 1),
+    ("kh3d", dict(n=28, steps=6), 6, 2),
 ])
-def test_gpu_fused_last_sweep_matches_oracle(hiplib, name, kw, steps, bw):
-    """The last direction's sweep fused with the RK update (in-place update
-    from a halo copy), lane lines (bw=1) and LDS tiles with padding columns:
-    bitwise the oracle."""
+def test_gpu_fused_last_sweep_matches_oracle(hiplib, name, kw, steps, mode, bw):
+    """A direction's sweep fused with the RK update (in-place update from a
+    halo copy; mode 1 the last direction, 2 x), lane lines (bw=1) and LDS
+    tiles with padding columns: bitwise the oracle."""
     from hormone_amd import driver as D
     from hormone_amd import problems as P
-    print(f"case fused {name} {kw} bw={bw}", flush=True)
+    print(f"case fused {name} {kw} mode={mode} bw={bw}", flush=True)
     prm, g, st = P.make_problem(name, **kw)
     e1 = Hydro(**D.desc_fields(prm, g))
     e1.set_launch(bw)
-    e1.set_fused(True)
+    e1.set_fused(mode)
     D.load_initial_state(e1, prm, g, st)
     r1 = D.run(e1, prm, g, max_steps=steps)
     e2 = Oracle(**D.desc_fields(prm, g))
@@ -428,17 +431,19 @@ def test_gpu_fused_last_sweep_matches_oracle(hiplib, name, kw, steps, bw):
 
 
 @pytest.mark.parametrize("bw", [1, 8])
-@pytest.mark.parametrize("name", ["briowushock_y", "orszagtang_xy", "orszagtang_yz"])
-def test_gpu_fused_device_loop_goldens(hiplib, name, bw):
+@pytest.mark.parametrize("name,mode", [("briowushock_y", 1), ("orszagtang_xy", 1),
+                                       ("orszagtang_yz", 1), ("orszagtang_xy", 2),
+                                       ("sodshock_x", 2)])
+def test_gpu_fused_device_loop_goldens(hiplib, name, mode, bw):
     """hg_run with the fused last sweep: the CFL partials come one per block
     of the fused launch, and the goldens still hold bit for bit."""
     from hormone_amd import driver as D
     from hormone_amd import problems as P
-    print(f"case fused device-loop {name} bw={bw}", flush=True)
+    print(f"case fused device-loop {name} mode={mode} bw={bw}", flush=True)
     prm, g, st = P.make_problem(name)
     eng = Hydro(**D.desc_fields(prm, g))
     eng.set_launch(bw)
-    eng.set_fused(True)
+    eng.set_fused(mode)
     eng.profile(bw == 8)  # per-launch events around the fused launches too
     D.load_initial_state(eng, prm, g, st)
     res = D.run(eng, prm, g, device_loop=True)
@@ -456,18 +461,19 @@ def test_gpu_fused_device_loop_goldens(hiplib, name, bw):
         compare_to_npz(res, state, name)
 
 
-def test_gpu_fused_polytrope_gravity_matches_oracle(hiplib):
-    """Fused z sweep with the gravity source in the RK update: every field and
-    the potential equal the oracle after 12 steps at 32^3."""
+@pytest.mark.parametrize("mode", [1, 2])
+def test_gpu_fused_polytrope_gravity_matches_oracle(hiplib, mode):
+    """Fused z (or x) sweep with the gravity source in the RK update: every
+    field and the potential equal the oracle after 12 steps at 32^3."""
     from hormone_amd import driver as D
     from hormone_amd import problems as P
-    print("case fused polytrope32", flush=True)
+    print(f"case fused polytrope32 mode={mode}", flush=True)
     res = []
     for cls in (Hydro, Oracle):
         prm, g, st = P.make_problem("polytrope3d", n=32, steps=12)
         eng = cls(**D.desc_fields(prm, g))
         if cls is Hydro:
-            eng.set_fused(True)
+            eng.set_fused(mode)
         D.load_initial_state(eng, prm, g, st)
         r = D.run(eng, prm, g)
         res.append((r, D.fetch(eng, g, D.SNAP_FIELDS + (D.F_GRVPHI, D.F_GRVPSI))))
@@ -481,11 +487,12 @@ def test_gpu_fused_rejects_what_it_does_not_cover(hiplib):
     from hormone_amd import driver as D
     from hormone_amd import problems as P
     from hormone_amd.native import HG_ERR_UNSUPPORTED, HgError
-    for name in ("sodshock_x", "KHinstability"):  # x only; species
+    for name, mode in (("sodshock_x", 1), ("briowushock_y", 2), ("KHinstability", 1),
+                       ("KHinstability", 2)):  # no y/z; no x; species
         prm, g, st = P.make_problem(name)
         eng = Hydro(**D.desc_fields(prm, g))
         with pytest.raises(HgError) as ei:
-            eng.set_fused(True)
+            eng.set_fused(mode)
         assert ei.value.code == HG_ERR_UNSUPPORTED
         eng.set_fused(False)
 
