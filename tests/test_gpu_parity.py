@@ -600,3 +600,25 @@ def test_gpu_linear_boundaries_any_launch_shape(hiplib, bw):
     names = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
     compare_to_meta(res, {names[i]: out[f] for i, f in enumerate(D.SNAP_FIELDS_MAG)},
                     meta()["ot_yz_linear"])
+
+
+def test_gpu_species_with_linear_boundary_matches_oracle(hiplib):
+    """Advected species next to a linear (code 4) x1 outer face, whose
+    species ghosts copy the edge cell (boundarycondition.f90:317-320):
+    bitwise the oracle."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print("case KH species + linear x1 outer", flush=True)
+    bc = (2, 4, 0, 0, 0, 0, 2, 4, 0, 0, 0, 0)
+    out = []
+    for cls in (Hydro, Oracle):
+        prm, g, st = P.make_problem("KHinstability", bc=bc)
+        eng = cls(**D.desc_fields(prm, g))
+        D.load_initial_state(eng, prm, g, st)
+        r = D.run(eng, prm, g, max_steps=30)
+        flds = D.SNAP_FIELDS + tuple(D.F_SPC + q for q in range(prm.spn))
+        out.append((r.tn, r.time, D.fetch(eng, g, flds)))
+    (n1, t1, a), (n2, t2, b) = out
+    assert (n1, t1) == (n2, t2)
+    for f in b:
+        assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
