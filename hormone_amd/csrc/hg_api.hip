@@ -39,7 +39,10 @@ struct hg_ctx {
   KEv kev{};
   bool fused = false;   // last sweep direction fused with the RK update
   long part_cap = 0;    // CFL partial slots allocated
-  long h_cap = 0;       // halo entries per field allocated (fused sweep)
+  // the second state buffer set of the fused form (W aliasing U as in D)
+  double *alt_U[NVMAX] = {}, *alt_W[NWMAX] = {};
+  bool alt_ready = false;
+  bool in_batch = false;
 };
 
 #define HIPCHK(x)                                   \
@@ -707,7 +710,13 @@ static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
       const long pre = D.fdir == 0 ? D.woff[2] - D.woff[1] + D.waves[2] : D.woff[D.fdir];
       const KEv *e0 = pre > 0 ? prof_ev_into(c, HG_PROF_SWEEP, k0) : nullptr;
       const KEv *e1 = prof_ev_into(c, HG_PROF_RK, k1);
-      const int g = launch_fused_stage(D, stage, a, c->stream, e0, e1);
+      Dev Df = D;
+      for (int v = 0; v < D.nv; ++v) Df.Uo[v] = c->alt_U[v];
+      for (int w = 0; w < D.nw; ++w) Df.Wo[w] = c->alt_W[w];
+      const int g = launch_fused_stage(Df, stage, a, c->stream, e0, e1);
+      // the new state is in the other set: it becomes the current one
+      for (int v = 0; v < D.nv; ++v) std::swap(c->D.U[v], c->alt_U[v]);
+      for (int w = 0; w < D.nw; ++w) std::swap(c->D.W[w], c->alt_W[w]);
       if (last && fuse_tail) npart = g;
       continue;
     }
@@ -888,16 +897,25 @@ extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
   }
   if (D.mixed || D.nspc || L < 0) return HG_ERR_UNSUPPORTED;
   for (int a = 0; a < 3; ++a)
-    if (D.ax[a].ext) return HG_ERR_UNSUPPORTED;  // the halo copy knows codes 0-3 only
-  {
-    const int K = (D.ax[L].n + SEG - 1) / SEG;
-    const long hsz = (long)D.ax[L].nlines * (K + 1) * 4;
-    if (hsz > c->h_cap) {
-      int st = dev_alloc(c, sizeof(double) * (size_t)hsz * D.nw, (void **)&D.H);
+    if (D.ax[a].ext) return HG_ERR_UNSUPPORTED;  // fused kernels carry codes 0-3 only
+  if (c->in_batch) return HG_ERR_UNSUPPORTED;
+  // the second buffer set: the fused launch reads one and writes the
+  // other, then the two swap (d, e and B alias U in both, as in hg_create;
+  // a phi the update does not write, 1D MHD, is shared)
+  if (!c->alt_ready) {
+    const size_t fb = sizeof(double) * (size_t)D.ncell;
+    for (int v = 0; v < D.nv; ++v) {
+      int st = dev_alloc(c, fb, (void **)&c->alt_U[v]);
       if (st) return st;
-      c->h_cap = hsz;
     }
-    D.hsz = hsz;
+    for (int w = 0; w < D.nw; ++w) {
+      if (w == WD) c->alt_W[w] = c->alt_U[0];
+      else if (w == WE) c->alt_W[w] = c->alt_U[4];
+      else if (D.mag && w >= WB1 && w <= WB3) c->alt_W[w] = c->alt_U[5 + (w - WB1)];
+      else if (w == WPHI && D.i9wv < 0) c->alt_W[w] = D.W[w];
+      else if (int st = dev_alloc(c, fb, (void **)&c->alt_W[w])) return st;
+    }
+    c->alt_ready = true;
   }
   // one CFL partial per block of the fused launch (one wavefront per block
   // at most)
@@ -1012,7 +1030,7 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
                             int64_t tn_out) {
   if (!b || !c) return HG_ERR_ARG;
   if (c->device != b->device || c->D.grav || c->D.nspc || c->D.ax[0].ext || c->D.ax[1].ext ||
-      c->D.ax[2].ext)
+      c->D.ax[2].ext || c->fused)
     return HG_ERR_UNSUPPORTED;
   for (hg_ctx *m : b->members)
     if (m == c) return HG_ERR_ARG;
@@ -1035,6 +1053,7 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
   b->mixed = c->D.mixed;
   b->rktype = c->desc.rktype;
   b->members.push_back(c);
+  c->in_batch = true;
   b->dirty = true;
   return HG_OK;
 }
@@ -1045,6 +1064,7 @@ extern "C" int hg_batch_remove(hg_batch *b, hg_ctx *c) {
     if (b->members[q] == c) {
       HIPCHK(hipStreamSynchronize(b->stream));
       b->members.erase(b->members.begin() + q);
+      c->in_batch = false;
       // keep the cached member states aligned with the member list
       for (size_t r = q; r < b->members.size(); ++r) b->h_sc[r] = b->h_sc[r + 1];
       b->dirty = true;

@@ -245,85 +245,7 @@ __device__ __forceinline__ void wave_line(const Dev &D, const Axis &A, unsigned 
   else base = line;
 }
 
-// ---- halo of the last direction (the sweep fused with the RK update) --------
-// The fused launch updates each cell in place right after its own line
-// segment has swept it, so a segment must not read its neighbours' cells
-// from the state (they may already hold new values).  Before the launch,
-// k_halo copies, for every line of that direction, the cells a segment
-// reads but does not own -- the two cells on either side of each segment
-// boundary and the ghosts' source cells -- into H: per line K+1 groups of 4
-// (K segments), group g holding cells 60g-2 .. 60g+1 and group K the
-// ghosts n, n+1 (raw source values; the loader applies the ghost rules).
-// H is field-major, H[f * hsz + entry], f the primitive index.
-__device__ __forceinline__ long halo_entry(int n, long line, int seg, int c) {
-  const int K = (n + SEG - 1) / SEG;
-  int g, slot;
-  if (c >= n) {
-    g = K;
-    slot = c - n < 1 ? c - n : 1;  // n, n+1 (lanes past n+1 are not used)
-  } else if (c < seg * SEG) {
-    g = seg;
-    slot = c - (seg * SEG - 2);
-  } else {
-    g = seg + 1;
-    slot = c - ((seg + 1) * SEG - 2);
-  }
-  return (line * (K + 1) + g) * 4 + slot;
-}
-
-__device__ __forceinline__ bool owned(int n, int seg, int c) {
-  return c >= seg * SEG && c < seg * SEG + SEG && c < n;
-}
-
-// raw primitives of cell c of a line (own cells from the state, the others
-// from the halo copy)
-template <bool MAG>
-__device__ __forceinline__ void fused_raw(const Dev &D, const Axis &A, long base, long line,
-                                          int seg, int c, double f[NWMAX]) {
-  const int nf = MAG ? NWMAX : WEINT + 1;
-  if (owned(A.n, seg, c)) {
-    const long m = base + (long)c * A.stride;
-#pragma unroll
-    for (int k = 0; k < NWMAX; ++k) f[k] = k < nf ? D.W[k][m] : 0.0;
-  } else {
-    const long e = halo_entry(A.n, line, seg, c);
-#pragma unroll
-    for (int k = 0; k < NWMAX; ++k) f[k] = k < nf ? D.H[k * D.hsz + e] : 0.0;
-  }
-}
-
-template <int DIR, bool MAG>
-__global__ __launch_bounds__(256) void k_halo(Dev D) {
-  if (D.sc->stop) return;
-  const Axis &A = D.ax[DIR];
-  const int n = A.n, K = (n + SEG - 1) / SEG;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)A.nlines * (K + 1) * 4) return;
-  const int slot = (int)(e & 3);
-  const long lg = e >> 2;
-  const int g = (int)(lg % (K + 1));
-  const long line = lg / (K + 1);
-  const int c = g < K ? g * SEG - 2 + slot : n + slot;
-  if (c > n + 1) return;  // group K holds two cells
-  long base;
-  int sg;
-  wave_line<DIR>(D, A, (unsigned)line, base, sg);  // line < nlines: segment 0
-  long ms, mv;
-  cell_sources(A, base, c, ms, mv);
-  const long h = D.hsz;
-  D.H[WD * h + e] = D.W[WD][ms];
-  D.H[WE * h + e] = D.W[WE][ms];
-  D.H[WEINT * h + e] = D.W[WEINT][ms];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) D.H[(WV1 + a) * h + e] = D.W[WV1 + a][mv];
-  if (MAG) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) D.H[(WB1 + a) * h + e] = D.W[WB1 + a][mv];
-    D.H[WPHI * h + e] = D.W[WPHI][ms];
-  }
-}
-
-template <int DIR, bool MAG, bool FUSED = false, bool EXT = true>
+template <int DIR, bool MAG, bool EXT = true>
 __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave0, int c,
                                           double *lds) {
   const int bw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -342,13 +264,7 @@ __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave
     }
     double *p = lds + r * ld + col;
     const int cr = sg * SEG - 2 + r;
-    if (FUSED && !owned(A.n, sg, cr)) {
-      const long e = halo_entry(A.n, (long)((unsigned)wc % A.nlines), sg, cr);
-      const int nf = MAG ? NWMAX : WEINT + 1;
-#pragma unroll
-      for (int k = 0; k < NWMAX; ++k)
-        if (k < nf) p[k * fs] = D.H[k * D.hsz + e];
-    } else if (EXT && A.ext && (cr < 0 || cr >= A.n)) {
+    if (EXT && A.ext && (cr < 0 || cr >= A.n)) {
       double f[NWMAX];
       raw_cell<DIR, MAG>(D, A, bc, (long)((unsigned)wc % A.nlines), cr, f);
       const int nf = MAG ? NWMAX : WEINT + 1;
@@ -622,16 +538,8 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
     ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
   }
   const int c = seg * SEG - 2 + lane;
-  Cell q;
-  if (TILE) {
-    q = tile_cell<DIR, MAG, FUSED, EXT>(D, A, wave0, c, lds);
-  } else if (FUSED) {
-    double f[NWMAX];
-    fused_raw<MAG>(D, A, base, line, seg, c, f);
-    q = finish_cell<DIR, MAG>(D, A, c, f);
-  } else {
-    q = load_cell<DIR, MAG, EXT>(D, A, base, line, c);
-  }
+  const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, c, lds)
+                      : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
   // d e m1 m2 m3 b1 b2 b3 and eint (slopes) / phi (face) of the cell
   const double qv[9] = {q.d, q.e, q.d * q.v[0], q.d * q.v[1], q.d * q.v[2],
                         q.b[0], q.b[1], q.b[2], q.eint};
@@ -802,7 +710,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
   }
   const int c = seg * SEG - 2 + lane;
-  const Cell q = TILE ? tile_cell<DIR, MAG, false, EXT>(D, A, wave0, c, lds)
+  const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, c, lds)
                       : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
   const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
   const double qd[8] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2]};
@@ -1145,6 +1053,10 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
                                            double fact, double facn, double damp, bool CFL,
                                            bool DAMP, bool GRAV, double &dti, double &cfm,
                                            const double *duL = nullptr) {
+  // the fused form writes the new state to the other buffer set (the
+  // launch's neighbours still read this one); k_rk updates in place
+  double *const *Uo = LD >= 0 ? D.Uo : D.U;
+  double *const *Wo = LD >= 0 ? D.Wo : D.W;
   const double ft = fact * D.sc->dt;
   double src[NVMAX];
   if (GRAV) grav_source(D, m, src);
@@ -1167,7 +1079,7 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
       u0 = D.U0[v][m];
     }
     un[v] = faco * u0 + facn * u + ft * acc;
-    D.U[v][m] = un[v];
+    Uo[v][m] = un[v];
     if (v == 0) dold = u;
   }
   // species (rungekutta.f90:61-85): spcorg = spc*uorg(icnt) at stage 1;
@@ -1194,18 +1106,18 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
   const double v1 = un[1] / d, v2 = un[2] / d, v3 = un[3] / d;
   const double e = un[4];
   // W[d], W[e] and W[B] are U[0], U[4] and U[5..7] (same buffers)
-  D.W[WV1][m] = v1;
-  D.W[WV2][m] = v2;
-  D.W[WV3][m] = v3;
+  Wo[WV1][m] = v1;
+  Wo[WV2][m] = v2;
+  Wo[WV3][m] = v3;
   double b1 = 0, b2 = 0, b3 = 0;
   if (MAG) {
     b1 = un[5];
     b2 = un[6];
     b3 = un[7];
-    if (D.i9wv >= 0) D.W[WPHI][m] = DAMP ? un[8] * damp : un[8];
+    if (D.i9wv >= 0) Wo[WPHI][m] = DAMP ? un[8] * damp : un[8];
   }
   const double eint = get_eint(MAG, e, d, v1, v2, v3, b1, b2, b3);
-  D.W[WEINT][m] = eint;
+  Wo[WEINT][m] = eint;
   if (eint <= 0.0) {
     const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
     atomicMin(&D.sc->err, errkey(stage, 1, 0, i, j, k, D));
@@ -1231,11 +1143,12 @@ __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, doubl
   if (CFL) cfl_block_reduce(D, dti, cfm, blockIdx.x);
 }
 
-// ---- the last sweep direction fused with the RK update --------------------
+// ---- one sweep direction fused with the RK update ---------------------------
 // The divergence of direction DIR comes straight from the sweep (registers,
 // or the LDS tile when tiled); the other directions' from memory.  Each cell
-// is updated by the thread that holds its divergence; no other thread of the
-// launch reads that cell afterwards (other segments read the halo copy).
+// is updated by the thread that holds its divergence.  The launch reads the
+// state from one buffer set and writes the new state to the other
+// (Dev::Uo/Wo), so neighbouring segments still read the old values.
 template <int DIR, bool MAG, bool TILE>
 __device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base, int c,
                                          int lane, const double du[NVMAX], double *lds,
@@ -1279,7 +1192,19 @@ __device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base,
 template <int DIR, bool MAG, int LIM, bool TILE>
 __global__ __launch_bounds__(1024) void k_sweep_rk(Dev D, int stage, RKArgs rk) {
   extern __shared__ double lds[];
-  if (D.sc->stop) return;
+  if (D.sc->stop) {
+    // stages enqueued after the run stopped: the host still makes the
+    // output set the current one, so hand the state over unchanged
+    const long nt = (long)gridDim.x * blockDim.x;
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < D.ncell; m += nt) {
+      for (int v = 0; v < D.nv; ++v) D.Uo[v][m] = D.U[v][m];
+      for (int w = 0; w < D.nw; ++w) {  // d, e, B alias U; a shared phi stays
+        const bool aliased = w == WD || w == WE || (MAG && w >= WB1 && w <= WB3);
+        if (!aliased && D.Wo[w] != D.W[w]) D.Wo[w][m] = D.W[w][m];
+      }
+    }
+    return;
+  }
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   double dti = __builtin_inf(), cfm = 0.0;
@@ -1787,19 +1712,6 @@ static void sweep_rk_dir(const Dev &D, int stage, const RKArgs &rk, unsigned g, 
 int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
                        const KEv *ev, const KEv *ev_fused) {
   const int L = D.fdir;
-  const Axis &A = D.ax[L];
-  const int K = (A.n + SEG - 1) / SEG;
-  const long nh = (long)A.nlines * (K + 1) * 4;
-  if (L == 0) {
-    if (D.mag) k_halo<0, true><<<nblk(nh, 256), 256, 0, s>>>(D);
-    else k_halo<0, false><<<nblk(nh, 256), 256, 0, s>>>(D);
-  } else if (L == 1) {
-    if (D.mag) k_halo<1, true><<<nblk(nh, 256), 256, 0, s>>>(D);
-    else k_halo<1, false><<<nblk(nh, 256), 256, 0, s>>>(D);
-  } else {
-    if (D.mag) k_halo<2, true><<<nblk(nh, 256), 256, 0, s>>>(D);
-    else k_halo<2, false><<<nblk(nh, 256), 256, 0, s>>>(D);
-  }
   const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
   const int bs = 64 * bw;
   const size_t tile = sweep_tile_bytes(D, bw);

@@ -83,11 +83,10 @@ struct Dev {
   int nspc;
   double *SPC, *SPC0, *DUS;
   double *FM[3];  // face mass flux per direction, (n+1) faces per line
-  // last sweep direction fused with the RK update: halo copy of the cells a
-  // segment reads but does not own (see k_halo), hsz entries per field
+  // one sweep direction fused with the RK update: that launch writes the
+  // new state to the other buffer set Uo/Wo (W aliasing U as above)
   int fdir;        // the direction fused with the RK update, -1 if none
-  double *H;
-  long hsz;
+  double *Uo[NVMAX], *Wo[NWMAX];
   double *part;    // CFL partials: 2 doubles per block of 256 cells
   Scalars *sc;
   int nblk_cfl;
@@ -168,8 +167,8 @@ struct RKArgs {
   double faco, fact, facn, damp;
   int cfl, dampf, grav;
 };
-// the fused path of one RK stage: halo copy, the sweeps of the other
-// directions, then D.fdir's sweep with the RK update; returns the number
+// the fused path of one RK stage: the sweeps of the other directions, then
+// D.fdir's sweep with the RK update into D.Uo/D.Wo; returns the number
 // of CFL partials the fused launch wrote (rk.cfl)
 int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
                        const KEv *ev_sweep = nullptr, const KEv *ev_fused = nullptr);

@@ -231,13 +231,14 @@ int hg_set_boundary_flux(hg_ctx *ctx, int32_t face, const double *flux);
 /* Fuse one direction's sweep with the RK update: on = 1 the last solved
    direction (y or z), on = 2 the x direction, 0 off (the default).  One
    launch per RK stage does that direction's reconstruction, Riemann solves
-   and flux difference and updates each cell in place, so that direction's
-   flux divergence is never written to or read back from HBM; the cells a
-   line segment reads but does not own come from a halo copy made just
-   before, and the other directions are swept first.  Results are
+   and flux difference and updates each cell, so that direction's flux
+   divergence is never written to or read back from HBM; the other
+   directions are swept first.  Results are
    bit-identical either way; FP64 without species and without boundary
-   codes 4, 5, 9, 10 only (HG_ERR_UNSUPPORTED otherwise).  No counterpart
-   in the reference (a launch-shape choice). */
+   codes 4, 5, 9, 10 only, and not while the context is in a batch
+   (HG_ERR_UNSUPPORTED otherwise).  The first call allocates a second copy
+   of the state (the launch reads one and writes the other).  No
+   counterpart in the reference (a launch-shape choice). */
 int hg_set_fused(hg_ctx *ctx, int32_t on);
 
 /* The sweep kernel's duration without per-dispatch events: `launches`
