@@ -884,7 +884,7 @@ extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
     D.fdir = -1;
     return HG_OK;
   }
-  // FP64 only, no species; on = 1: the last solved direction (y or z),
+  // no species; on = 1: the last solved direction (y or z),
   // on = 2: x
   int L = -1;
   if (on == 2) {
@@ -895,7 +895,7 @@ extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
   } else {
     return HG_ERR_ARG;
   }
-  if (D.mixed || D.nspc || L < 0) return HG_ERR_UNSUPPORTED;
+  if (D.nspc || L < 0) return HG_ERR_UNSUPPORTED;
   for (int a = 0; a < 3; ++a)
     if (D.ax[a].ext) return HG_ERR_UNSUPPORTED;  // fused kernels carry codes 0-3 only
   if (c->in_batch) return HG_ERR_UNSUPPORTED;

@@ -682,9 +682,11 @@ __device__ __forceinline__ double minmod_geo(double dm, double dp, double ida, d
   return s * mx2(0.0, mn3(fabs(x), s * y, s * z));
 }
 
-template <int DIR, bool MAG, int LIM, bool TILE = false, bool SPC = false, bool EXT = true>
+template <int DIR, bool MAG, int LIM, bool TILE = false, bool SPC = false, bool EXT = true,
+          bool FUSED = false>
 __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane, int stage,
-                                               double *lds = nullptr) {
+                                               double *lds = nullptr, const RKArgs *rk = nullptr,
+                                               double *dti = nullptr, double *cfm = nullptr) {
   const Axis &A = D.ax[DIR];
   const int n = A.n;
   const long nlines = D.ncell / n;
@@ -841,6 +843,10 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   du[5 + b_] = ix * (tm[6] - t[6]);
   du[5 + c_] = ix * (tm[7] - t[7]);
   du[8] = ix * (tm[8] - t[8]);
+  if (FUSED) {
+    fused_rk<DIR, MAG, TILE>(D, A, base, c, lane, du, lds, stage, *rk, *dti, *cfm);
+    return;
+  }
   store_du<DIR, MAG, TILE>(D, A, base, c, lane, du, lds);
   if (SPC) store_mass_flux<DIR>(D, line, c, lane, t[0]);
 }
@@ -1189,7 +1195,7 @@ __device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base,
 
 // One wavefront per segment of a DIR line, as k_sweep; CFL partials (last
 // stage) one per block.
-template <int DIR, bool MAG, int LIM, bool TILE>
+template <int DIR, bool MAG, int LIM, bool TILE, bool MIX = false>
 __global__ __launch_bounds__(1024) void k_sweep_rk(Dev D, int stage, RKArgs rk) {
   extern __shared__ double lds[];
   if (D.sc->stop) {
@@ -1208,9 +1214,14 @@ __global__ __launch_bounds__(1024) void k_sweep_rk(Dev D, int stage, RKArgs rk) 
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   double dti = __builtin_inf(), cfm = 0.0;
-  if (TILE || wave < D.waves[DIR])
-    sweep_line<DIR, MAG, LIM, TILE, false, true, false>(D, wave, lane, stage, lds, &rk, &dti,
-                                                         &cfm);
+  if (TILE || wave < D.waves[DIR]) {
+    if (MIX)
+      sweep_line_mix<DIR, MAG, LIM, TILE, false, false, true>(D, wave, lane, stage, lds, &rk,
+                                                               &dti, &cfm);
+    else
+      sweep_line<DIR, MAG, LIM, TILE, false, true, false>(D, wave, lane, stage, lds, &rk, &dti,
+                                                           &cfm);
+  }
   if (rk.cfl) cfl_block_reduce(D, dti, cfm, blockIdx.x);
 }
 
@@ -1688,11 +1699,13 @@ static void sweep_rk_variant(const Dev &D, int stage, const RKArgs &rk, unsigned
                              size_t tile, hipStream_t s, const KEv *ev) {
   if constexpr (DIR > 0) {  // x lines are lane lines, never tiled
     if (tile) {
-      klaunch_lds(k_sweep_rk<DIR, MAG, LIM, true>, g, bs, tile, s, ev, D, stage, rk);
+      if (D.mixed) klaunch_lds(k_sweep_rk<DIR, MAG, LIM, true, true>, g, bs, tile, s, ev, D, stage, rk);
+      else klaunch_lds(k_sweep_rk<DIR, MAG, LIM, true>, g, bs, tile, s, ev, D, stage, rk);
       return;
     }
   }
-  klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false>, g, bs, 0, s, ev, D, stage, rk);
+  if (D.mixed) klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false, true>, g, bs, 0, s, ev, D, stage, rk);
+  else klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false>, g, bs, 0, s, ev, D, stage, rk);
 }
 
 template <int DIR>

@@ -233,10 +233,9 @@ int hg_set_boundary_flux(hg_ctx *ctx, int32_t face, const double *flux);
    launch per RK stage does that direction's reconstruction, Riemann solves
    and flux difference and updates each cell, so that direction's flux
    divergence is never written to or read back from HBM; the other
-   directions are swept first.  Results are
-   bit-identical either way; FP64 without species and without boundary
-   codes 4, 5, 9, 10 only, and not while the context is in a batch
-   (HG_ERR_UNSUPPORTED otherwise).  The first call allocates a second copy
+   directions are swept first.  Results are bit-identical either way, in
+   FP64 and FP32-mixed.  Not with species, boundary codes 4, 5, 9, 10 or
+   while the context is in a batch (HG_ERR_UNSUPPORTED).  The first call allocates a second copy
    of the state (the launch reads one and writes the other).  No
    counterpart in the reference (a launch-shape choice). */
 int hg_set_fused(hg_ctx *ctx, int32_t on);

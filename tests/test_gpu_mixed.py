@@ -89,3 +89,22 @@ def test_mixed_conserves_mass_to_fp64_rounding(hiplib):
     m0 = float(st.d.sum())
     m1 = float(s[D.F_D].sum())
     assert abs(m1 - m0) / m0 < 1e-12
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("name,kw,steps", [("kh3d", dict(n=28, steps=8), 8),
+                                           ("orszagtang_xy", dict(n=90), 12)])
+def test_mixed_fused_equals_mixed_separate(hiplib, name, kw, steps, mode):
+    """The FP32-mixed sweep fused with the (FP64) RK update gives the
+    separate mixed launches' result bit for bit."""
+    print(f"case mixed fused {name} mode={mode}", flush=True)
+    out = []
+    for fused in (0, mode):
+        prm, g, eng, r = _start(name, "fp32mix", kw)
+        eng.set_fused(fused)
+        r.advance(steps)
+        out.append((r.tn, r.tim, D.fetch(eng, g, _fields(prm))))
+    (n1, t1, a), (n2, t2, b) = out
+    assert (n1, t1) == (n2, t2)
+    for f in a:
+        assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
