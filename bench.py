@@ -9,7 +9,11 @@ A step is one full timestep (CFL reduction + 3 RK stages + phidamp), and a
 zone-update is one interior cell advanced by one step.
 
 Timed region: K steps after W warm-up steps, through the device-resident
-main loop (hg_run), bracketed by torch.cuda.synchronize().  Afterwards the
+main loop (hg_run), bracketed by torch.cuda.synchronize().  The warm-up
+picks the sweep's block size and whether one direction's sweep is fused
+with the RK update (none, x or the last direction; --fused) by timing real
+steps; every choice gives the same bits.  Tuning steps beyond W are
+ordinary steps of the run.  Afterwards the
 run continues, untimed, to step 2200, where the state is compared bit for bit
 with the reference program's own run (tests/golden/golden_meta.json, entry
 ot512_2200steps).  It then runs on until the reference's own FP64 run stops
