@@ -622,3 +622,26 @@ def test_gpu_species_with_linear_boundary_matches_oracle(hiplib):
     assert (n1, t1) == (n2, t2)
     for f in b:
         assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_gpu_fused_device_loop_across_output_stops(hiplib, mode):
+    """hg_run stops at every output time (outstyle 1) with steps already
+    enqueued behind the stop; the fused form swaps its two state buffers at
+    each of those stages too, so they must hand the state over unchanged.
+    Many stops, then the end: bit for bit the separate RK update."""
+    from hormone_amd import driver as D
+    from hormone_amd import problems as P
+    print(f"case fused device loop output stops mode={mode}", flush=True)
+    out = []
+    for fused in (0, mode):
+        prm, g, st = P.make_problem("orszagtang_xy", n=64, t_end=100.0, dt_out=7.0)
+        eng = Hydro(**D.desc_fields(prm, g))
+        eng.set_fused(fused)
+        D.load_initial_state(eng, prm, g, st)
+        res = D.run(eng, prm, g, device_loop=True)
+        out.append((res.tn, res.time, D.fetch(eng, g, D.SNAP_FIELDS_MAG)))
+    (n1, t1, a), (n2, t2, b) = out
+    assert (n1, t1) == (n2, t2) and n1 > 20
+    for f in a:
+        assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), f
