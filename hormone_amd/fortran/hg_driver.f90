@@ -54,6 +54,7 @@ program hg_driver
  character(len=256):: outfile='final.bin', initfile=''
  logical:: is_test=.true.
  integer:: device=0
+ integer:: fused=0             ! hg_set_fused: 0 off, 1 last direction, 2 x
  real(8):: gconst=6.67430d-8   ! G (a parameter of the reference, G=1 for test set-ups)
 
  namelist /gridcon/ xi1s,xi1e,xi2s,xi2e,xi3s,xi3e,is,ie,js,je,ks,ke, &
@@ -71,7 +72,7 @@ program hg_driver
  namelist /gravcon/ gravswitch,grvsrctype,grverr,cgerr,HGfac,hgcfl,maxtngrv,gbtype, &
                     grktype,alphagrv,grav_init_other,grav_init_relax,include_extgrv, &
                     gis,gie,gjs,gje,gks,gke
- namelist /gpucon/ simtype,outfile,initfile,is_test,device,gconst
+ namelist /gpucon/ simtype,outfile,initfile,is_test,device,gconst,fused
 
  real(8),allocatable,target:: x1(:),xi1(:),dx1(:),dxi1(:),idx1(:),idxi1(:)
  real(8),allocatable,target:: x2(:),xi2(:),dx2(:),dxi2(:),idx2(:),idxi2(:)
@@ -178,6 +179,9 @@ program hg_driver
 
  st = hg_create(desc, ctx)
  call hg_check(ctx, st, 'hg_create')
+ if(fused/=0)then
+  st = hg_set_fused(ctx, int(fused, c_int32_t)) ; call hg_check(ctx, st, 'hg_set_fused')
+ end if
  st = hg_upload(ctx, HG_F_D, c_loc(d))   ; call hg_check(ctx, st, 'upload')
  st = hg_upload(ctx, HG_F_V1, c_loc(v1)) ; call hg_check(ctx, st, 'upload')
  st = hg_upload(ctx, HG_F_V2, c_loc(v2)) ; call hg_check(ctx, st, 'upload')

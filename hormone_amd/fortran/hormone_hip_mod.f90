@@ -131,6 +131,15 @@ module hormone_hip_mod
    integer(c_int):: st
   end function hg_set_launch
 
+  ! one sweep direction fused with the RK update: 0 off, 1 the last solved
+  ! direction, 2 x (bit-identical; a launch-shape choice)
+  function hg_set_fused(ctx, on) bind(C, name='hg_set_fused') result(st)
+   import:: c_int, c_int32_t, c_ptr
+   type(c_ptr),value:: ctx
+   integer(c_int32_t),value:: on
+   integer(c_int):: st
+  end function hg_set_fused
+
   ! ghost values of the faces with codes 4, 5, 9, 10 (for Dirichlet faces:
   ! the d0, p0, v10.. arrays, after each dirichletbound)
   function hg_set_ghosts(ctx, field, padded) bind(C, name='hg_set_ghosts') result(st)

@@ -42,13 +42,16 @@ SHAPES = {"sodshock_x": ((1, 1, 500), False), "briowushock_z": ((500, 1, 1), Tru
           "orszagtang_xy": ((1, 128, 128), True)}
 
 
-@pytest.mark.parametrize("simtype", list(PARAMS))
-def test_fortran_driver_matches_golden(simtype, tmp_path, hiplib):
+@pytest.mark.parametrize("simtype,fused", [(s, 0) for s in PARAMS] + [
+    ("sodshock_x", 2), ("briowushock_z", 1), ("orszagtang_xy", 2)])
+def test_fortran_driver_matches_golden(simtype, fused, tmp_path, hiplib):
+    """The Fortran host (&gpucon fused: hg_set_fused) reproduces the goldens."""
     assert os.path.exists(DRIVER), "hg_driver not built (needs amdflang at build time)"
     par = tmp_path / "parameters"
     out = tmp_path / "final.bin"
-    par.write_text(PARAMS[simtype] + f"&gpucon simtype='{simtype}' outfile='{out}' /\n")
-    print(f"case fortran driver {simtype}", flush=True)
+    par.write_text(PARAMS[simtype]
+                   + f"&gpucon simtype='{simtype}' outfile='{out}' fused={fused} /\n")
+    print(f"case fortran driver {simtype} fused={fused}", flush=True)
     r = subprocess.run([DRIVER, str(par)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     shape, mag = SHAPES[simtype]
