@@ -1244,3 +1244,8 @@ double or_kat_modified_mc(const double *u3, const double *x3, const double *xi2)
 }
 
 double or_kat_minmod(const double *u3, const double *dx2) { return minmod(u3, dx2); }
+
+double or_kat_get_eint(int mag_on, double etot, double d, double v1, double v2, double v3,
+                       double b1, double b2, double b3) {
+  return get_eint(mag_on, etot, d, v1, v2, v3, b1, b2, b3);
+}

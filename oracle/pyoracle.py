@@ -56,6 +56,8 @@ def lib():
         L.or_kat_modified_mc.restype = C.c_double
         L.or_kat_minmod.argtypes = [_dp, _dp]
         L.or_kat_minmod.restype = C.c_double
+        L.or_kat_get_eint.argtypes = [C.c_int] + [C.c_double] * 8
+        L.or_kat_get_eint.restype = C.c_double
         _LIB = L
     return _LIB
 

@@ -8,12 +8,27 @@ forms instead of reference data:
   * one explicit sub-step obeys psi = dt*c*(lap phi - 4 pi G rho)*damp with
     damp = exp(-(1 - c_old/c + alphagrv*hgcfl)) (sign and factor of the
     source term, gravity_hyperbolic.f90:97-116);
+  * one two-stage (grktype 2) sub-step from a quadratic potential, uniform
+    density and uniform psi0 equals the exact two-stage result
+    phi1 = phi0 + c dt psi0 + (c dt)^2/2 (L - 4 pi G rho),
+    psi1 = (psi0 + c dt (L - 4 pi G rho)) damp
+    (gravity_hyperbolic.f90:73-116 with get_runge_coeff, rungekutta.f90:
+    146-183: the Laplacian of each stage is taken before that stage's
+    update, and the stage-2 potential update uses the stage-1 psi);
+  * repeated sub-steps with the density held fixed drive the discrete
+    Poisson residual |L_h phi - 4 pi G rho| down to rounding, and the
+    potential they converge to is the n=1 polytrope's to second order;
+  * the polytrope's initial state follows from mass, hydrostatic equilibrium
+    and Poisson's equation (checked by quadrature and finite differences,
+    not by re-evaluating its formulas);
   * an n=1 polytrope in its closed-form potential stays near hydrostatic
     equilibrium (a flipped force or source sign makes it collapse or explode).
 """
 import math
 
 import numpy as np
+import pytest
+from scipy import integrate
 
 from hormone_amd import driver as D
 from hormone_amd import problems as P
@@ -21,9 +36,9 @@ from hormone_amd.grid import cartesian_grid
 from oracle.pyoracle import Oracle
 
 
-def quadratic_setup(engine_cls, n=12, rho0=0.7):
+def quadratic_setup(engine_cls, n=12, rho0=0.7, grktype=1, psi0=0.0):
     prm = P.Params(n=(n, n, n), lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0), gamma=2.0,
-                   bc=(3,) * 12, gravswitch=3, G=1.0, grktype=1, is_test=False)
+                   bc=(3,) * 12, gravswitch=3, G=1.0, grktype=grktype, is_test=False)
     g = cartesian_grid(prm.n, prm.lo, prm.hi, gravity=True)
     z = np.zeros(g.shape)
     st = P.State(d=np.full(g.shape, rho0), p=np.full(g.shape, 1.0),
@@ -36,6 +51,9 @@ def quadratic_setup(engine_cls, n=12, rho0=0.7):
     st.extra["grvphi_padded"] = (x * x + 2.0 * y * y + 3.0 * zz * zz).transpose(2, 1, 0).copy()
     eng = engine_cls(**D.desc_fields(prm, g))
     D.load_initial_state(eng, prm, g, st)
+    if psi0:
+        eng.upload(D.F_GRVPSI, np.full(g.padded_shape, psi0))
+    eng.phi0 = st.extra["grvphi_padded"]
     return prm, g, eng
 
 
@@ -71,3 +89,131 @@ def test_polytrope_stays_hydrostatic():
     c = np.unravel_index(np.argmax(d), d.shape)
     assert all(abs(q - 15.5) <= 1.5 for q in c)
     assert out[D.F_GRVPHI].min() < -6.0
+
+
+def two_stage_substep_errors(engine_cls):
+    """One grktype-2 sub-step from phi0 = x^2 + 2y^2 + 3z^2 (Dirichlet ghosts
+    from the same quadratic), rho = 0.7, psi0 = 0.3, with c_old != c; returns
+    the largest deviations of phi and psi from the exact two-stage result in
+    the cells at least two cells from the boundary (the ghost layer holds
+    phi0, not the stage-1 potential, so the first layer's psi differs), that
+    first layer's psi deviation, and the size of the (c dt)^2/2 term."""
+    rho0, psi0, c, c_old, dt = 0.7, 0.3, 2.0, 1.5, 1e-2
+    prm, g, eng = quadratic_setup(engine_cls, n=12, rho0=rho0, grktype=2, psi0=psi0)
+    grvtime, cold, dtg2, nsub = eng.gravity(0.0, dt, 0.0, c, c_old, dt)
+    assert nsub == 1 and cold == c
+    out = D.fetch(eng, g, [D.F_GRVPHI, D.F_GRVPSI])
+    phi0 = g.interior(eng.phi0)
+    S = 12.0 - 4.0 * math.pi * prm.G * rho0
+    damp = math.exp(-(1.0 - c_old / c + prm.alphagrv * prm.hgcfl))
+    phi_x = phi0 + c * dt * psi0 + 0.5 * (c * dt) ** 2 * S
+    psi_x = (psi0 + c * dt * S) * damp
+    core = (slice(2, -2),) * 3
+    ephi = np.max(np.abs(out[D.F_GRVPHI] - phi_x)[core] / np.maximum(1.0, np.abs(phi_x[core])))
+    epsi = np.max(np.abs(out[D.F_GRVPSI] - psi_x)[core]) / abs(psi_x)
+    # the first layer's stage-2 Laplacian does see the fixed ghost values
+    edge = np.max(np.abs(out[D.F_GRVPSI] - psi_x)[0, 2:-2, 2:-2])
+    return ephi, epsi, edge, (c * dt) ** 2 * 0.5 * abs(S)
+
+
+def test_two_stage_substep_matches_closed_form():
+    ephi, epsi, edge, term = two_stage_substep_errors(Oracle)
+    assert ephi < 1e-13 and epsi < 1e-13, (ephi, epsi)
+    # the (c dt)^2/2 term is resolved: dropping it would miss by 6.4e-4
+    assert term > 1e-4 > 1e3 * ephi
+    assert edge > 1e-9  # and the boundary layer is really different
+
+
+def discrete_residual(phi_padded, rho, dx, G=1.0):
+    p = phi_padded[1:-1, 1:-1, 1:-1]
+    lap = sum(np.roll(p, 1, a) + np.roll(p, -1, a) for a in range(3)) - 6.0 * p
+    lap = lap[1:-1, 1:-1, 1:-1] / (dx * dx)
+    return lap - 4.0 * math.pi * G * rho
+
+
+def relax(n, blocks, per_block=100):
+    """The polytrope's density held fixed (no noise), the potential started
+    at zero inside the closed-form Dirichlet boundary, then blocks of
+    sub-steps of the wave equation (c = 1, the largest stable sub-step).
+    Returns the rms discrete Poisson residual after each block and the final
+    largest deviation from the closed-form potential relative to its depth."""
+    prm, g, st = P.make_problem("polytrope3d", n=n, steps=1)
+    pol = P.Polytrope()
+    st.d = np.maximum(pol.rho(P._radius(g)), pol.rho_bg)
+    phi_bd = st.extra["grvphi_padded"].copy()
+    st.extra["grvphi_padded"][2:-2, 2:-2, 2:-2] = 0.0
+    eng = Oracle(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    dx = 2.0 / n
+    full = phi_bd.copy()
+
+    def residual():
+        full[2:-2, 2:-2, 2:-2] = D.fetch(eng, g, [D.F_GRVPHI])[D.F_GRVPHI]
+        return float(np.sqrt(np.mean(discrete_residual(full, st.d, dx) ** 2)))
+
+    c = 1.0
+    dtg = prm.hgcfl * (dx / 6.0) / c   # dtg_unit of a uniform 3D grid is dx/6
+    res = [residual()]
+    tim, gt, co = 0.0, 0.0, c
+    for _ in range(blocks):
+        for _ in range(per_block):
+            gt, co, _, ns = eng.gravity(tim, dtg, gt, c, co, dtg)
+            tim += dtg
+        res.append(residual())
+    exact = g.interior(phi_bd)
+    return res, float(np.max(np.abs(full[2:-2, 2:-2, 2:-2] - exact)) / np.max(np.abs(exact)))
+
+
+def test_substeps_converge_to_discrete_poisson_solution():
+    """The damped wave equation relaxes to the discrete Poisson solution:
+    the residual falls monotonically over blocks of sub-steps, by more than
+    six decades (a wrong source sign or factor makes it grow or stall)."""
+    res, _ = relax(16, 12)
+    assert all(b < a for a, b in zip(res, res[1:])), res
+    assert res[-1] < 1e-6 * res[0], res
+
+
+def test_relaxed_potential_is_the_polytrope_potential_to_second_order():
+    """The potential the sub-steps converge to approaches the closed-form
+    n=1 polytrope potential at second order in the cell size (the star's
+    radius is 2.4 cells at 16^3 and 4.8 at 32^3)."""
+    _, e16 = relax(16, 6)
+    _, e32 = relax(32, 6, per_block=400)
+    assert e16 < 0.06 and e16 / e32 > 2.5, (e16, e32)
+
+
+def test_polytrope_initial_state_from_first_principles():
+    """Mass by quadrature, hydrostatic equilibrium dp/dr = -rho dphi/dr with
+    p = K rho^2, Poisson (1/r^2)(r^2 phi')' = 4 pi G rho inside and Gauss's
+    law phi' = G M(r)/r^2 everywhere, and the grid state built from them."""
+    pol = P.Polytrope()
+    G, M, R = pol.G, pol.M, pol.R
+    rho = lambda r: float(pol.rho(np.array(r)))  # noqa: E731
+    phi = lambda r: float(pol.potential(np.array(r)))  # noqa: E731
+    mass, _ = integrate.quad(lambda r: 4 * math.pi * r * r * rho(r), 0.0, R)
+    assert abs(mass - M) < 1e-10 * M
+    h = 1e-5
+    for r in np.linspace(0.02, 0.98, 9) * R:
+        dphi = (phi(r + h) - phi(r - h)) / (2 * h)
+        dp = (pol.K * rho(r + h) ** 2 - pol.K * rho(r - h) ** 2) / (2 * h)
+        assert abs(dp + rho(r) * dphi) < 1e-6 * abs(rho(r) * dphi)
+        lap = ((r + h) ** 2 * (phi(r + 2 * h) - phi(r)) / (2 * h)
+               - (r - h) ** 2 * (phi(r) - phi(r - 2 * h)) / (2 * h)) / (2 * h) / (r * r)
+        assert abs(lap - 4 * math.pi * G * rho(r)) < 1e-4 * 4 * math.pi * G * pol.rho_c
+    for r in (0.1 * R, 0.5 * R, 0.9 * R, 1.2 * R, 2.0 * R, 3.0):
+        menc, _ = integrate.quad(lambda s: 4 * math.pi * s * s * rho(s), 0.0, min(r, R))
+        dphi = (phi(r + h) - phi(r - h)) / (2 * h)
+        assert abs(dphi - G * menc / (r * r)) < 1e-6 * G * menc / (r * r)
+    # potential zero at infinity: phi(r) = -int_r^inf G M(s)/s^2 ds outside
+    for r in (1.5 * R, 3.0):
+        tail, _ = integrate.quad(lambda s: G * M / (s * s), r, np.inf)
+        assert abs(phi(r) + tail) < 1e-12 * tail
+    # the grid state: density within the noise of the profile, the
+    # potential the closed form at the centres and in the ghost layer
+    prm, g, st = P.make_problem("polytrope3d", n=24, steps=1)
+    r = P._radius(g)
+    prof = np.maximum(pol.rho(r), pol.rho_bg)
+    assert np.max(np.abs(st.d / prof - 1.0)) <= pol.noise * (1 + 1e-12)
+    assert np.std(st.d / prof - 1.0) > 0.3 * pol.noise
+    mass_grid = float(st.d.sum()) * (2.0 / 24) ** 3
+    assert abs(mass_grid - M) < 0.05 * M
