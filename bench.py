@@ -1,47 +1,51 @@
 #!/usr/bin/env python3
 """Zone-updates/s of HORMONE's explicit hydro step on one MI355X.
 
-Default workload (the configuration the headline metric is quoted on):
-2D Orszag-Tang MHD vortex, 512^2 periodic on [0,1]^2, gamma 5/3, MUSCL
-(modified MC) + HLLD with GLM cleaning, SSP-RK3, courant 0.9, FP64.  The
-initial data is the closed-form vortex (synthetic); the state stays in HBM.
-A step is one full timestep (CFL reduction + 3 RK stages + phidamp), and a
-zone-update is one interior cell advanced by one step.
+Default workload: n=1 polytrope (M=1, R=0.3, G=1, gamma=2) on a 256^3
+Cartesian grid over [-1,1]^3 with background density 1e-6 and seeded +-1e-3
+density noise, MHD with B=0 (9 conserved variables: the GLM cleaning field
+rides along), Cartesian hyperbolic self-gravity (gravswitch 3, HGfac 5,
+hgcfl 0.9, RK2 sub-steps), MUSCL (modified MC) + HLLD, SSP-RK3, courant
+0.9, FP64.  The initial state is closed form (synthetic); the state stays
+in HBM.  A step is one full timestep (CFL reduction + gravity sub-steps +
+3 RK stages + phidamp), a zone-update one interior cell advanced by one
+step.  Steps run through the device-resident main loop (hg_run_gravity).
 
-Timed region: K steps after W warm-up steps, through the device-resident
-main loop (hg_run), bracketed by torch.cuda.synchronize().  The warm-up
-picks the sweep's block size and whether one direction's sweep is fused
-with the RK update (none, x or the last direction; --fused) by timing real
-steps; every choice gives the same bits.  Tuning steps beyond W are
-ordinary steps of the run.  Afterwards the
-run continues, untimed, to step 2200, where the state is compared bit for bit
-with the reference program's own run (tests/golden/golden_meta.json, entry
-ot512_2200steps).  It then runs on until the reference's own FP64 run stops
-(negative internal energy during step 2250, before t = 0.5); the stop must
-happen at the same step and cell.
+Timed region: K steps after W warm-up steps, bracketed by
+torch.cuda.synchronize().  The warm-up picks the sweep's block size and
+whether one direction's sweep is fused with the RK update (none, x or the
+last direction; --fused) by timing real steps; every choice gives the same
+bits.  The same run is then repeated with the FP32-mixed Riemann solver
+(--no-mixed skips it): its zone-updates/s, the ratio to FP64, and the
+per-variable relative L1 of its state against the FP64 run at the same
+physical time go into the same line.
+
+roofline: one RK stage is the unit: the flux sweep launch (all directions,
+or the directions before the fused one) and the launch that applies the RK
+update (k_rk, or the fused sweep + RK).  Their average durations are timed
+with HIP events on the library's stream inside the timed region (only these
+launches carry events, about 9 us of completion wait each).  Algorithmic
+bytes are the compulsory FP64 state traffic per zone-update, shared out per
+stage: RK3 stage 1 reads U^n and writes U^1 (2*nvar*8 B per cell), stages 2
+and 3 read U^n and U^s and write U^s+1 (3*nvar*8 B), the CFL pass reads U
+once (nvar*8 B, fused into the last stage), the gravity source reads the
+potential each stage (8 B); averaged over the stages, 24*nvar+8 B per cell
+(224 B for MHD).  The whole step adds the gravity wave update: the source
+write (8 B) and 96 B per sub-step (step_roofline).  The bytes the kernels
+themselves move, intermediates included, are reported beside them.
+cpu_baseline: the CPU oracle (C/OpenMP restatement of the reference) on this
+host's cores over a bounded prefix of the same run.
 
 Other workloads (--workload):
-  polytrope256  3D 256^3 hydro + Cartesian hyperbolic self-gravity, n=1
-                polytrope (M=1, R=0.3, G=1, gamma=2) in [-1,1]^3, 200 steps;
-                reports mass and energy drift
+  ot512         2D Orszag-Tang MHD vortex 512^2, periodic, courant 0.9, run
+                toward t=0.5 (checks bitwise parity with the reference
+                program's run at step 2200 and its stop at step 2250)
   kh384         3D 384^3 MHD Kelvin-Helmholtz, periodic, 100 steps
   serving       continuous-batching ensemble: seeded Poisson arrivals (50/s)
                 of independent 2D/3D MHD Riemann or blast problems, grids
                 log-uniform from 64^2 to 192^3 cells, each to t = 0.1, admitted
                 against an HBM budget into one batch of persistent kernels;
                 value = aggregate zone-updates/s, with p50/p99 time-to-solution
-With --precision fp32mix and --vs-fp64, the same workload is also run in FP64
-and the per-variable relative L1 sum|q-q64|dV / sum|q64|dV is reported.
-
-roofline: the dominant kernel is the fused direction sweep.  Its average
-launch duration is timed right after the timed region: back-to-back sweep
-launches of the same state on the library's stream between two HIP events
-(the timed region itself carries no per-dispatch events, which would add a
-completion wait to every launch).  Its algorithmic bytes per launch are
-dim*(nw+nv)*8 per cell: primitives read once, one flux divergence written per
-direction.
-cpu_baseline: the CPU oracle (C/OpenMP restatement of the reference) on this
-host's cores over a bounded prefix of the same run.
 """
 from __future__ import annotations
 
@@ -65,6 +69,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 NAMES = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3", "phi")
+METRIC = "zone-updates/s (FP64 vs FP32-mixed, L1≤1e-5 vs FP64); p50/p99 ensemble latency"
 
 
 def parse():
@@ -72,19 +77,23 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="ot512",
-                    choices=["ot512", "polytrope256", "kh384", "serving"])
+    ap.add_argument("--workload", default="polytrope256",
+                    choices=["polytrope256", "ot512", "kh384", "serving"])
     ap.add_argument("--requests", type=int, default=64, help="serving: number of requests")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0,
                     help="serving: HBM admission budget")
     ap.add_argument("--n", type=int, default=None, help="cells per side (default per workload)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32mix"])
-    ap.add_argument("--mag", action="store_true", help="polytrope: MHD variant with B=0")
-    ap.add_argument("--vs-fp64", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32mix"],
+                    help="precision of the timed run that gives `value`")
+    ap.add_argument("--hydro", action="store_true",
+                    help="polytrope: hydro (5 variables) instead of MHD with B=0")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the FP32-mixed run that follows the FP64 one")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="no per-launch events in the timed region (no roofline)")
     ap.add_argument("--serving-lanes", default="50000,500000",
                     help="serving: cell-count bounds of the concurrently served size classes "
                          "(comma separated; empty = one batch)")
@@ -114,21 +123,32 @@ def copy_calibration():
                                            "write_bytes": n * 8}}))
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "pmc_traffic.json")
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 
-def measured_traffic(workload, precision, kernel):
+def _profile_table(name):
+    try:
+        with open(os.path.join(PROFILES, name)) as f:
+            return json.load(f).get("entries", {})
+    except (OSError, ValueError):
+        return {}
+
+
+def measured_traffic(key, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprof counter
     runs (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration), or None
     when no measurement of this workload exists."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            tab = json.load(f)
-    except (OSError, ValueError):
-        return None
-    e = tab.get("entries", {}).get(f"{workload}/{precision}/{kernel}")
+    e = _profile_table("pmc_traffic.json").get(f"{key}/{kernel}")
     return None if e is None else e.get("bytes_per_launch")
+
+
+def measured_valu(key, kernel):
+    """VALU-busy fraction of `kernel` from the committed SQ counter run:
+    SQ_INSTS_VALU x 4 cycles (a wave64 instruction on a 16-lane SIMD) over
+    the SIMD-cycles of the launch, or None."""
+    e = _profile_table("pmc_sq.json").get(f"{key}/{kernel}")
+    return None if e is None else {k: e[k] for k in ("valu_busy", "valu_insts_per_wave",
+                                                     "vgprs", "waves_per_simd") if k in e}
 
 
 # steps the warm-up may spend on launch tuning beyond --warmup (the run's
@@ -142,13 +162,20 @@ def setup(a, precision):
     if a.workload == "ot512":
         return P.make_problem("orszagtang_xy", n=a.n or 512, precision=precision)
     if a.workload == "polytrope256":
-        return P.make_problem("polytrope3d", n=a.n or 256, steps=limit, mag_on=a.mag,
+        return P.make_problem("polytrope3d", n=a.n or 256, steps=limit, mag_on=not a.hydro,
                               precision=precision)
     return P.make_problem("kh3d", n=a.n or 384, steps=limit, precision=precision)
 
 
+def workload_key(a, prm):
+    if a.workload == "polytrope256":
+        return "polytrope256" + ("mhd" if prm.mag_on else "hydro")
+    return a.workload
+
+
 def cpu_baseline(a, seconds: float):
-    """Oracle (C/OpenMP restatement of the reference) on the host cores."""
+    """Oracle (C/OpenMP restatement of the reference) on the host cores,
+    over the first steps of the same run (bounded to about `seconds`)."""
     from hormone_amd import driver as D
     from oracle import pyoracle
 
@@ -158,10 +185,10 @@ def cpu_baseline(a, seconds: float):
     eng = pyoracle.Oracle(**D.desc_fields(prm, g))
     pyoracle.set_threads(0)
     D.load_initial_state(eng, prm, g, st)
-    t0 = time.perf_counter()
-    D.run(eng, prm, g, max_steps=2)
-    per = (time.perf_counter() - t0) / 2
+    r = D.run(eng, prm, g, max_steps=2)
+    per = r.wall_s / 2
     steps = max(2, min(3000, int(seconds / max(per, 1e-6))))
+    del eng
     eng2 = pyoracle.Oracle(**D.desc_fields(prm, g))
     pyoracle.set_threads(0)
     D.load_initial_state(eng2, prm, g, st)
@@ -183,8 +210,9 @@ def cpu_baseline(a, seconds: float):
 
 
 class Runner:
-    """Drives one engine through the reference's main loop, step by step or
-    with the device-resident loop, keeping the clock on the host."""
+    """Drives one engine through the reference's main loop with the
+    device-resident loop (hg_run, or hg_run_gravity with self-gravity),
+    keeping the clock on the host."""
 
     def __init__(self, eng, prm, g):
         self.eng, self.prm, self.g = eng, prm, g
@@ -192,70 +220,56 @@ class Runner:
         self.damp = math.exp(-prm.alpha9wave * prm.courant)
         self.tim, self.tn = 0.0, 0
         self.grvtime, self.cgrav_old = 0.0, 0.0
-        self.nsub = []
+        self.nsub = []  # (steps, gravity sub-steps) per loop call
+
+    def _loop(self, left, t_out, dt_out, t_end, outstyle, endstyle, tnlim, tn_out):
+        from hormone_amd import native
+        eng = self.eng
+        n0 = self.tn
+        try:
+            if self.prm.gravswitch == 3:
+                (self.tim, self.tn, t_out, reason, self.grvtime, self.cgrav_old,
+                 ns) = eng.run_loop_gravity(self.tim, self.tn, t_out, dt_out, t_end, outstyle,
+                                            endstyle, tnlim, tn_out, left, self.damp,
+                                            self.grvtime, self.cgrav_old)
+                self.nsub.append((self.tn - n0, ns))
+            else:
+                self.tim, self.tn, t_out, reason = eng.run_loop(
+                    self.tim, self.tn, t_out, dt_out, t_end, outstyle, endstyle, tnlim,
+                    tn_out, left, self.damp)
+        except native.HgError as err:
+            self.tn, self.tim = err.tn, err.time
+            raise
+        return t_out, reason
 
     def advance(self, k):
         """k more steps; False when the run's end is reached."""
         from hormone_amd import native
-        prm, eng = self.prm, self.eng
-        if prm.gravswitch == 3:
-            for _ in range(k):
-                dt, ch, cg, dtg = eng.timestep(self.tim, self.t_out, self.t_end, prm.outstyle,
-                                               prm.endstyle, prm.fixed_dt, grav=True)
-                if self.tn == 0:
-                    self.grvtime = 0.0
-                else:
-                    self.grvtime, self.cgrav_old, _, ns = eng.gravity(
-                        self.tim, dt, self.grvtime, cg, self.cgrav_old, dtg)
-                    self.nsub.append(ns)
-                eng.hydro_step(dt, ch)
-                if prm.mag_on and self.g.dim > 1:
-                    eng.phidamp(self.damp)
-                self.tim += dt
-                self.tn += 1
-            return True
+        prm = self.prm
         left = k
         while left > 0:
             n0 = self.tn
-            try:
-                self.tim, self.tn, self.t_out, reason = eng.run_loop(
-                    self.tim, self.tn, self.t_out, self.dt_out, self.t_end, prm.outstyle,
-                    prm.endstyle, prm.tnlim, prm.tn_out, left, self.damp)
-            except native.HgError as err:
-                self.tn, self.tim = err.tn, err.time
-                raise
+            self.t_out, reason = self._loop(left, self.t_out, self.dt_out, self.t_end,
+                                            prm.outstyle, prm.endstyle, prm.tnlim, prm.tn_out)
             left -= self.tn - n0
             if reason == native.STOP_END:
                 return left <= 0
         return True
 
+    def advance_to(self, t_target):
+        """Step until the clock reaches t_target exactly: the end condition
+        becomes endstyle 1 (time >= t_end), whose dt clipping
+        (timestep.f90:85) lands the last step on t_target; output stops are
+        off."""
+        from hormone_amd import native
+        while self.tim < t_target:
+            _, reason = self._loop(1 << 40, t_target, 0.0, t_target, 0, 1, 1 << 40, 1)
+            if reason == native.STOP_END:
+                return
 
-def advance_to(runner, t_target):
-    """Step a Runner until its clock reaches t_target exactly: the end
-    condition becomes endstyle 1 (time >= t_end), whose dt clipping
-    (timestep.f90:85) lands the last step on t_target; output stops are off."""
-    from hormone_amd import native
-    prm, eng = runner.prm, runner.eng
-    if prm.gravswitch == 3:
-        while runner.tim < t_target:
-            dt, ch, cg, dtg = eng.timestep(runner.tim, t_target, t_target, 0, 1, 0.0, grav=True)
-            if runner.tn == 0:
-                runner.grvtime = 0.0
-            else:
-                runner.grvtime, runner.cgrav_old, _, ns = eng.gravity(
-                    runner.tim, dt, runner.grvtime, cg, runner.cgrav_old, dtg)
-            eng.hydro_step(dt, ch)
-            if prm.mag_on and runner.g.dim > 1:
-                eng.phidamp(runner.damp)
-            runner.tim += dt
-            runner.tn += 1
-        return
-    while runner.tim < t_target:
-        runner.tim, runner.tn, _, reason = eng.run_loop(
-            runner.tim, runner.tn, t_target, 0.0, t_target, 0, 1, 1 << 40, 1, 1 << 40,
-            runner.damp)
-        if reason == native.STOP_END:
-            return
+    def mean_nsub(self, since=0):
+        st = sum(s for s, _ in self.nsub[since:])
+        return sum(n for _, n in self.nsub[since:]) / st if st else None
 
 
 def sums(D, eng, g, prm):
@@ -269,6 +283,211 @@ def sums(D, eng, g, prm):
     if prm.gravswitch == 3:
         e += float((s[D.F_D] * 0.5 * s[D.F_GRVPHI] * dv).sum())
     return m, e
+
+
+def tune(a, eng, run):
+    """Warm-up with launch tuning; returns the tuning report and the fused
+    form (0 separate, 1 last direction, 2 x)."""
+    tuned = None
+    if a.sweep_waves:
+        eng.set_launch(a.sweep_waves)
+        run.advance(a.warmup)
+    else:
+        # autotune the sweep's block size on warm-up steps (launch shapes do
+        # not change results), then finish the warm-up with the winner
+        from hormone_amd.batch import tune_single
+        per = max(1, a.warmup // 5)
+        best, times = tune_single(eng, run.advance, steps=per)
+        tuned = {"sweep_block_waves": best, "us_per_launch": {str(k): round(v, 2)
+                                                              for k, v in times.items()}}
+        left = a.warmup - per * len(times)
+        if left > 0:
+            run.advance(left)
+    fused = 0
+    names = {0: "separate", 1: "last", 2: "x"}
+    if a.fused in ("last", "x"):
+        fused = {"last": 1, "x": 2}[a.fused]
+        eng.set_fused(fused)
+        tuned = dict(tuned or {})
+        tuned["fused_sweep"] = a.fused
+    elif a.fused == "auto":
+        from hormone_amd.batch import tune_fused
+        fused, ftimes = tune_fused(eng, run.advance, steps=max(2, min(8, a.warmup // 4)))
+        tuned = dict(tuned or {})
+        tuned["fused_sweep"] = names[fused]
+        tuned["ms_per_step"] = {names[k]: round(v * 1e3, 3) for k, v in ftimes.items()}
+    return tuned, fused
+
+
+def stage_bytes(nv, rktype, grav):
+    """Algorithmic bytes per cell of one RK stage, averaged over the
+    stages: the conserved state read and written by the RK update, the CFL
+    read in the last stage and the potential read by the gravity source."""
+    tot = 2 * nv * 8 + (rktype - 1) * 3 * nv * 8 + nv * 8
+    return tot / rktype + (8.0 if grav else 0.0)
+
+
+def kernel_bytes(fused, nv, nw, dim, mag):
+    """Bytes per cell the stage's own launches move (their reads and writes,
+    intermediates included): the sweep reads the primitives once per
+    direction and writes one divergence per direction; the RK update reads
+    U, U0 and every divergence and writes U and the primitive fields that do
+    not alias U; the fused launch reads its direction's primitives instead
+    of its divergence.  Averaged over the stages (stage 1 writes U0 instead
+    of reading it: the same count)."""
+    aliased = 5 if mag else 2
+    wr = nw - aliased
+    if fused:
+        sweep = (dim - 1) * (nw + nv) * 8
+        rk = (nw + (dim - 1) * nv + 3 * nv + wr) * 8
+    else:
+        sweep = dim * (nw + nv) * 8
+        rk = (dim * nv + 3 * nv + wr) * 8
+    return sweep, rk
+
+
+def timed_run(a, precision, want_roofline):
+    """Set up, warm up, tune, and time a.steps steps; returns a dict."""
+    import torch
+
+    from hormone_amd import driver as D
+    from hormone_amd import native
+
+    prm, g, st = setup(a, precision)
+    eng = native.Hydro(**D.desc_fields(prm, g))
+    D.load_initial_state(eng, prm, g, st)
+    run = Runner(eng, prm, g)
+    m0, e0 = sums(D, eng, g, prm)
+    tuned, fused = tune(a, eng, run)
+    if want_roofline:
+        # per-launch events on the stage's two launches only
+        eng.profile(True, kinds=(native.PROF_SWEEP, native.PROF_RK))
+    k0 = len(run.nsub)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ok = run.advance(a.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert ok, "the run ended inside the timed region; lower --steps"
+    out = {"eng": eng, "prm": prm, "g": g, "run": run, "wall": wall, "tuned": tuned,
+           "fused": fused, "m0": m0, "e0": e0,
+           "zus": a.steps * g.ncells / wall, "nsub": run.mean_nsub(k0)}
+    if want_roofline:
+        out["sweep"] = eng.profile_read(native.PROF_SWEEP)
+        out["rk"] = eng.profile_read(native.PROF_RK)
+        eng.profile(False)
+        if prm.gravswitch == 3:  # gravity stages over a short window after
+            eng.profile(True, kinds=(native.PROF_GRAV,))
+            run.advance(min(5, a.steps))
+            out["grav"] = eng.profile_read(native.PROF_GRAV)
+            eng.profile(False)
+    return out
+
+
+def roofline(a, r):
+    """The stage roofline of a timed run (see the module docstring)."""
+    eng, prm, g = r["eng"], r["prm"], r["g"]
+    nv = eng.ufnmax()
+    dim = g.dim
+    nw = 10 if prm.mag_on else 6
+    grav = prm.gravswitch == 3
+    (ms_s, n_s, _), (ms_r, n_r, _) = r["sweep"], r["rk"]
+    if not n_r:
+        return None
+    us_s = ms_s / n_s * 1e3 if n_s else 0.0
+    us_r = ms_r / n_r * 1e3
+    fused = r["fused"]
+    sweep_k = "k_sweep" + ("_mix" if prm.precision == "fp32mix" else "")
+    rk_k = "k_sweep_rk" if fused else "k_rk"
+    b_alg = stage_bytes(nv, prm.rktype, grav) * g.ncells
+    sb, rb = kernel_bytes(fused, nv, nw, dim, prm.mag_on)
+    us = us_s + us_r
+    ach = b_alg / (us * 1e-6) / 1e9
+    key = f"{workload_key(a, prm)}/{prm.precision}"
+    tr_s, tr_r = measured_traffic(key, sweep_k), measured_traffic(key, rk_k)
+    traffic = tr_s + tr_r if (tr_s is not None and tr_r is not None) else None
+    form = {0: "separate RK update", 1: "last direction fused with RK",
+            2: "x fused with RK"}[fused]
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": f"one RK stage: {sweep_k} + {rk_k} ({form})",
+            "algorithmic_bytes_per_launch": b_alg,
+            "algorithmic_bytes_per_cell": stage_bytes(nv, prm.rktype, grav),
+            "kernel_own_bytes_per_launch": (sb + rb) * g.ncells,
+            "traffic_over_algorithmic": round(traffic / b_alg, 3) if traffic else None,
+            "avg_launch_us": {sweep_k: round(us_s, 2), rk_k: round(us_r, 2)},
+            "launches_timed": {sweep_k: n_s, rk_k: n_r},
+            "timing": "HIP events on the library stream, inside the timed region",
+            "note": "flux sweep limited by FP64 VALU issue and latency (bit-exact IEEE "
+                    "division/square-root chains), not HBM",
+            "share_of_timed_region": round(us * 1e-6 * prm.rktype * a.steps / r["wall"], 3)}
+    valu = measured_valu(key, sweep_k)
+    if valu:
+        roof["sweep_valu"] = valu
+    if "grav" in r and r["grav"][1]:
+        gms, gn, gb = r["grav"]
+        roof["grav_stage_avg_launch_us"] = round(gms / gn * 1e3, 2)
+        # 96 B per cell and sub-step over its two stage launches
+        roof["grav_achieved_GBs"] = round(48.0 * g.ncells / (gms / gn * 1e-3) / 1e9, 1)
+    return roof
+
+
+def step_roofline(r):
+    prm, g, eng = r["prm"], r["g"], r["eng"]
+    b_zu = 72.0 * eng.ufnmax()
+    if prm.gravswitch == 3 and r["nsub"]:
+        b_zu += 24.0 + 8.0 + 96.0 * r["nsub"]
+    return {"bytes_per_zone_update": b_zu, "achieved": round(r["zus"] * b_zu / 1e9, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(r["zus"] * b_zu / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def compare_runs(mine, ref):
+    """Relative L1 per variable of run `mine` against run `ref` at the same
+    physical time: the run that is behind steps to the other's clock."""
+    from hormone_amd import driver as D
+    rm, rr = mine["run"], ref["run"]
+    if rm.tim < rr.tim:
+        rm.advance_to(rr.tim)
+    elif rr.tim < rm.tim:
+        rr.advance_to(rm.tim)
+    prm, g = ref["prm"], ref["g"]
+    fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+    a = D.fetch(mine["eng"], g, fields)
+    b = D.fetch(ref["eng"], g, fields + (D.F_P,))
+    out = {"rel_l1": D.rel_l1_errors(a, {f: b[f] for f in fields}),
+           "time": rr.tim, "steps": {"fp32mix": rm.tn, "fp64": rr.tn}}
+    cs = np.sqrt(prm.gamma * b[D.F_P] / b[D.F_D])
+    out["vel_l1_over_cs"] = float(sum(np.abs(a[f] - b[f]).sum() for f in
+                                      (D.F_V1, D.F_V2, D.F_V3)) / (3.0 * cs.sum()))
+    out["reference_norms"] = D.reference_norms(a, {f: b[f] for f in fields}, g, prm.gamma,
+                                               bool(prm.mag_on))
+    return out
+
+
+def ot512_parity(run):
+    from hormone_amd import driver as D
+    from hormone_amd import native
+    from tests.helpers import meta
+    eng, g = run.eng, run.g
+    m = meta()
+    run.advance(2200 - run.tn)
+    out = D.fetch(eng, g, D.SNAP_FIELDS_MAG)
+    e = m.get("ot512_2200steps")
+    same = e is not None and run.tn == e["tn"] and run.tim == e["time"] and all(
+        hashlib.sha256(np.ascontiguousarray(out[f], "<f8").tobytes()).hexdigest()
+        == e["fields"][nm]["sha256"] for f, nm in zip(D.SNAP_FIELDS_MAG, NAMES))
+    parity = {"step": run.tn, "bitwise_equal_to_reference_run_at_step_2200": bool(same)}
+    fail = m.get("ot512_t05_failure")
+    try:
+        run.advance(1 << 30)
+        parity["reference_stop_reproduced"] = False
+    except native.HgError as err:
+        parity["stop"] = {"code": err.code, "completed_steps": err.tn, "cell": list(err.cell)}
+        parity["reference_stop_reproduced"] = bool(
+            fail and err.code == native.HG_ERR_NEG_EINT
+            and err.tn == fail["completed_steps"] and list(err.cell) == fail["cell"])
+    return parity
 
 
 def serving_bench(a):
@@ -321,18 +540,14 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return
-    if a.workload == "serving":
-        if a.gpus != 1:
-            print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-                  f"(the sharded path is a design only).", file=sys.stderr)
-            sys.exit(2)
-        serving_bench(a)
-        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               f"(the sharded path is a design only).", file=sys.stderr)
         sys.exit(2)
-    defaults = {"ot512": (2000, 20), "polytrope256": (190, 10), "kh384": (90, 10)}
+    if a.workload == "serving":
+        serving_bench(a)
+        return
+    defaults = {"ot512": (2000, 20), "polytrope256": (150, 10), "kh384": (90, 10)}
     if a.steps is None:
         a.steps = defaults[a.workload][0]
         if a.workload == "ot512" and a.precision == "fp32mix":
@@ -342,180 +557,57 @@ def main():
             a.steps = 1000
     if a.warmup is None:
         a.warmup = defaults[a.workload][1]
-    import torch
-
     from hormone_amd import driver as D
-    from hormone_amd import native
 
-    prm, g, st = setup(a, a.precision)
-    eng = native.Hydro(**D.desc_fields(prm, g))
-    D.load_initial_state(eng, prm, g, st)
-    run = Runner(eng, prm, g)
-    m0, e0 = sums(D, eng, g, prm)
-
-    tuned = None
-    if a.sweep_waves:
-        eng.set_launch(a.sweep_waves)
-        run.advance(a.warmup)
-    else:
-        # autotune the sweep's block size on warm-up steps (launch shapes do
-        # not change results), then finish the warm-up with the winner
-        from hormone_amd.batch import tune_single
-        per = max(1, a.warmup // 5)
-        best, times = tune_single(eng, run.advance, steps=per)
-        tuned = {"sweep_block_waves": best, "us_per_launch": {str(k): round(v, 2)
-                                                              for k, v in times.items()}}
-        left = a.warmup - per * len(times)
-        if left > 0:
-            run.advance(left)
-    fused = False
-    names = {0: "separate", 1: "last", 2: "x"}
-    if a.fused in ("last", "x"):
-        fused = {"last": 1, "x": 2}[a.fused]
-        eng.set_fused(fused)
-        tuned = dict(tuned or {})
-        tuned["fused_sweep"] = a.fused
-    elif a.fused == "auto":
-        from hormone_amd.batch import tune_fused
-        fused, ftimes = tune_fused(eng, run.advance, steps=max(2, min(8, a.warmup // 4)))
-        tuned = dict(tuned or {})
-        tuned["fused_sweep"] = names[fused]
-        tuned["ms_per_step"] = {names[k]: round(v * 1e3, 3) for k, v in ftimes.items()}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ok = run.advance(a.steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    assert ok, "the run ended inside the timed region; lower --steps"
-    zus = a.steps * g.ncells / wall
-    roof = None
-    if not a.no_profile:
-        # The timed region carries no per-dispatch events: on this stack each
-        # one adds a completion wait of about 9 us to its launch (15% of an
-        # OT 512^2 step).  The sweep's own duration is timed right after, as
-        # back-to-back launches on the same state and stream between two HIP
-        # events (the sweep only rewrites the flux divergences); the other
-        # kernels over a short window with per-dispatch events.
-        _, _, bpl = eng.profile_read(native.PROF_SWEEP)
-        est = eng.time_sweep(2)
-        nrep = int(min(2000, max(20, 0.25e3 / max(est / 2, 1e-3))))
-        ms = eng.time_sweep(nrep)
-        avg_s = ms / 1e3 / nrep
-        ach = bpl / avg_s / 1e9
-        eng.profile(True)
-        win = min(20, a.steps)
-        run.advance(win)
-        ev_ms, ev_n, _ = eng.profile_read(native.PROF_SWEEP)
-        rk_ms, rk_n, rk_b = eng.profile_read(native.PROF_RK)
-        gr_ms, gr_n, gr_b = eng.profile_read(native.PROF_GRAV)
-        eng.profile(False)
-        kname = "k_sweep" + ("_mix" if a.precision == "fp32mix" else "")
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(a.workload, a.precision, kname),
-                "kernel": kname,
-                "kernel_note": "all solved directions in one launch per RK stage; limited by "
-                               "FP64 VALU latency (IEEE division/square-root chains), not HBM",
-                "avg_launch_us": round(avg_s * 1e6, 2), "launches": nrep,
-                "timing": "back-to-back launches between two HIP events, after the timed region",
-                "algorithmic_bytes_per_launch": bpl,
-                "share_of_timed_region": round(avg_s * a.steps * prm.rktype / wall, 3)}
-        if fused:
-            # the timed region ran one direction's sweep fused with the RK
-            # update (k_sweep_rk) and the other directions in k_sweep; the
-            # replay above times the all-direction k_sweep
-            roof["timed_region_form"] = {1: "last", 2: "x"}[fused] + " direction fused with RK"
-            roof.pop("share_of_timed_region")
-        if ev_n:  # fused: the sweep of the other directions only
-            roof["other_directions_sweep_avg_launch_us" if fused else "event_avg_launch_us"] = \
-                round(ev_ms / ev_n * 1e3, 2)
-        if rk_n and fused:  # the RK events time the fused sweep + RK launch
-            roof["fused_sweep_rk_avg_launch_us"] = round(rk_ms / rk_n * 1e3, 2)
-        elif rk_n:
-            roof["rk_avg_launch_us"] = round(rk_ms / rk_n * 1e3, 2)
-            roof["rk_achieved_GBs"] = round(rk_b / (rk_ms / rk_n / 1e3) / 1e9, 1)
-        if gr_n:
-            roof["grav_avg_launch_us"] = round(gr_ms / gr_n * 1e3, 2)
-            roof["grav_achieved_GBs"] = round(gr_b / (gr_ms / gr_n / 1e3) / 1e9, 1)
-    parity = None
-    extra = {}
-    # whole-step figure: compulsory FP64 state traffic per zone-update (RK3
-    # reads/writes the conserved state: 72 B x nvar; with hyperbolic gravity
-    # + 24 B potential reads + 8 B source + 96 B per gravity sub-step)
-    b_zu = 72.0 * eng.ufnmax()
-    if run.nsub:
-        b_zu += 24.0 + 8.0 + 96.0 * float(np.mean(run.nsub[-a.steps:]))
-    extra["step_roofline"] = {"bytes_per_zone_update": b_zu,
-                              "achieved": round(zus * b_zu / 1e9, 1), "peak": HBM_PEAK_GBS,
-                              "unit": "GB/s", "frac": round(zus * b_zu / 1e9 / HBM_PEAK_GBS, 4)}
-    if a.workload == "ot512" and a.precision == "fp64" and not a.no_check and (a.n or 512) == 512:
-        from tests.helpers import meta
-        m = meta()
-        run.advance(2200 - run.tn)
-        out = D.fetch(eng, g, D.SNAP_FIELDS_MAG)
-        e = m.get("ot512_2200steps")
-        same = e is not None and run.tn == e["tn"] and run.tim == e["time"] and all(
-            hashlib.sha256(np.ascontiguousarray(out[f], "<f8").tobytes()).hexdigest()
-            == e["fields"][nm]["sha256"] for f, nm in zip(D.SNAP_FIELDS_MAG, NAMES))
-        parity = {"step": run.tn, "bitwise_equal_to_reference_run_at_step_2200": bool(same)}
-        fail = m.get("ot512_t05_failure")
-        try:
-            run.advance(1 << 30)
-            parity["reference_stop_reproduced"] = False
-        except native.HgError as err:
-            parity["stop"] = {"code": err.code, "completed_steps": err.tn, "cell": list(err.cell)}
-            parity["reference_stop_reproduced"] = bool(
-                fail and err.code == native.HG_ERR_NEG_EINT
-                and err.tn == fail["completed_steps"] and list(err.cell) == fail["cell"])
-    if a.workload != "ot512" or a.precision != "fp64":
-        m1, e1 = sums(D, eng, g, prm)
-        extra["mass_drift"] = (m1 - m0) / m0
-        extra["energy_drift"] = (e1 - e0) / abs(e0)
-        if run.nsub:
-            extra["gravity_substeps_mean"] = float(np.mean(run.nsub))
-    if a.vs_fp64 and a.precision == "fp32mix":
-        fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
-        mine = D.fetch(eng, g, fields)
-        n_done = run.tn
-        del eng
-        prm64, g64, st64 = setup(a, "fp64")
-        eng64 = native.Hydro(**D.desc_fields(prm64, g64))
-        D.load_initial_state(eng64, prm64, g64, st64)
-        # compare at the same physical time: the mixed run's dt sequence
-        # differs slightly (its CFL speeds carry FP32 rounding), so the FP64
-        # run is stepped to the mixed run's clock (last dt clipped onto it)
-        r64 = Runner(eng64, prm64, g64)
-        advance_to(r64, run.tim)
-        ref = D.fetch(eng64, g64, fields + (D.F_P,))
-        extra["rel_l1_vs_fp64"] = D.rel_l1_errors(mine, ref)
-        extra["reference_norms_vs_fp64"] = D.reference_norms(
-            mine, {f: ref[f] for f in fields}, g64, prm.gamma, bool(prm.mag_on))
-        # velocity error in units of the local sound speed (the relevant
-        # scale for a near-hydrostatic flow, where |v| itself is noise)
-        cs = np.sqrt(prm.gamma * ref[D.F_P] / ref[D.F_D])
-        extra["vel_l1_over_cs"] = float(sum(np.abs(mine[f] - ref[f]).sum() for f in
-                                            (D.F_V1, D.F_V2, D.F_V3)) / (3.0 * cs.sum()))
-        extra["rel_l1_steps"] = n_done
-        extra["rel_l1_time"] = run.tim
-        extra["fp64_steps_to_same_time"] = r64.tn
-    cpu = None if a.no_cpu else cpu_baseline(a, a.cpu_seconds)
-    wl = {"ot512": f"orszagtang_xy {g.ax[0].n}x{g.ax[1].n} MHD+GLM, MUSCL+HLLD, RK3, "
-                   f"courant 0.9 (run toward t=0.5)",
-          "polytrope256": f"polytrope {g.ax[0].n}^3 {'MHD B=0' if prm.mag_on else 'hydro'} + "
-                          f"hyperbolic self-gravity, gamma 2, {prm.tnlim} steps",
-          "kh384": f"Kelvin-Helmholtz {g.ax[0].n}^3 MHD, periodic, {prm.tnlim} steps"}
+    r = timed_run(a, a.precision, not a.no_profile)
+    prm, g, eng = r["prm"], r["g"], r["eng"]
     line = {
-        "metric": f"zone-updates/s ({a.precision.upper()}, {a.workload})",
-        "value": zus, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "metric": METRIC, "value": r["zus"], "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": r["wall"] / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f32+f64",
-        "data": "synthetic (closed-form initial state)",
-        "config": {"workload": wl[a.workload], "cells": g.ncells, "precision": a.precision},
-        "roofline": roof, "cpu_baseline": cpu, "parity": parity,
-        "launch_tuning": tuned,
+        "data": "synthetic (closed-form initial state, seeded noise)",
     }
-    line.update(extra)
+    wl = {"ot512": f"orszagtang_xy {g.ax[0].n}x{g.ax[1].n} MHD+GLM, MUSCL+HLLD, RK3, "
+                   f"courant 0.9 (run toward t=0.5)",
+          "polytrope256": f"n=1 polytrope {g.ax[0].n}^3 "
+                          f"{'MHD (B=0, 9 variables)' if prm.mag_on else 'hydro'} + "
+                          f"Cartesian hyperbolic self-gravity, gamma 2, MUSCL+HLLD, RK3, "
+                          f"courant 0.9",
+          "kh384": f"Kelvin-Helmholtz {g.ax[0].n}^3 MHD, periodic, MUSCL+HLLD, RK3"}
+    line["config"] = {"workload": wl[a.workload], "cells": g.ncells,
+                      "precision": a.precision}
+    line["roofline"] = None if a.no_profile else roofline(a, r)
+    line["step_roofline"] = step_roofline(r)
+    line["cpu_baseline"] = None
+    line["launch_tuning"] = r["tuned"]
+    if r["nsub"] is not None:
+        line["gravity_substeps_mean"] = r["nsub"]
+    if a.workload == "ot512" and a.precision == "fp64" and not a.no_check and \
+            (a.n or 512) == 512:
+        line["parity"] = ot512_parity(r["run"])
+    else:
+        m1, e1 = sums(D, eng, g, prm)
+        line["mass_drift"] = (m1 - r["m0"]) / r["m0"]
+        line["energy_drift"] = (e1 - r["e0"]) / abs(r["e0"])
+    if a.precision == "fp64" and not a.no_mixed and a.workload != "ot512":
+        mix = timed_run(a, "fp32mix", not a.no_profile)
+        m1, e1 = sums(D, mix["eng"], mix["g"], mix["prm"])
+        cmp = compare_runs(mix, r)
+        line["fp32mix"] = {
+            "value": mix["zus"], "ms_per_step": mix["wall"] / a.steps * 1e3,
+            "speedup_vs_fp64": mix["zus"] / r["zus"],
+            "rel_l1_vs_fp64": cmp["rel_l1"], "vel_l1_over_cs": cmp["vel_l1_over_cs"],
+            "reference_norms_vs_fp64": cmp["reference_norms"],
+            "compared_at": {"time": cmp["time"], "steps": cmp["steps"]},
+            "mass_drift": (m1 - mix["m0"]) / mix["m0"],
+            "energy_drift": (e1 - mix["e0"]) / abs(mix["e0"]),
+            "roofline": None if a.no_profile else roofline(a, mix),
+            "launch_tuning": mix["tuned"]}
+        del mix
+    del r, eng
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(a, a.cpu_seconds)
     print(json.dumps(line))
 
 

@@ -31,6 +31,7 @@ struct hg_ctx {
   double dtg_unit = 0.0;      // tools.f90:154-184 (gravswitch 3)
   // profiling (the reference's profiler categories, profiler.f90:5-51)
   bool prof = false;
+  unsigned prof_mask = 0;  // categories that carry events (bit HG_PROF_*)
   double prof_ms[HG_PROF_N] = {0};
   long long prof_n[HG_PROF_N] = {0};
   std::vector<hipEvent_t> ev;   // event pool
@@ -502,7 +503,7 @@ extern "C" int hg_download(hg_ctx *c, int32_t field, double *a) {
 // Events for the next timed launch of the given kind (nullptr when profiling
 // is off); the launch records both on its own dispatch.
 static const KEv *prof_ev(hg_ctx *c, int kind) {
-  if (!c->prof) return nullptr;
+  if (!c->prof || !(c->prof_mask & (1u << kind))) return nullptr;
   if ((size_t)(2 * c->ev_used + 2) > c->ev.size()) {
     for (int q = 0; q < 64; ++q) {
       hipEvent_t e;
@@ -544,6 +545,8 @@ extern "C" int hg_profile(hg_ctx *c, int32_t enable) {
   HIPCHK(hipStreamSynchronize(c->stream));
   prof_collect(c);
   c->prof = enable != 0;
+  c->prof_mask = (enable & HG_PROF_ONLY) ? (unsigned)enable & ((1u << HG_PROF_N) - 1)
+                                         : (1u << HG_PROF_N) - 1;
   for (int k = 0; k < HG_PROF_N; ++k) {
     c->prof_ms[k] = 0.0;
     c->prof_n[k] = 0;
@@ -618,6 +621,9 @@ extern "C" int hg_time_sweep(hg_ctx *c, int32_t launches, double *ms_total) {
 }
 
 // ---- the hot-path entry points ------------------------------------------------
+static void grav_speeds(const hg_ctx *c, double dt, double cfmax, double *cgrav,
+                        double *dtgrav);
+
 extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
                            int32_t outstyle, int32_t endstyle, double fixed_dt, double *dt_out,
                            double *ch_out, double *cgrav_out, double *dtgrav_out) {
@@ -641,16 +647,7 @@ extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
   if (dt_out) *dt_out = c->h_sc->dt;
   if (ch_out) *ch_out = c->h_sc->ch;
   double cgrav = 0.0, dtgrav = 0.0;
-  if (c->D.grav) {  // timestep.f90:92-97
-    const double dt = c->h_sc->dt;
-    cgrav = c->desc.HGfac * c->h_sc->cfmax;
-    if (c->desc.maxtngrv > 0) {
-      const double lim = c->desc.hgcfl * c->dtg_unit * (double)c->desc.maxtngrv / dt;
-      cgrav = lim < cgrav ? lim : cgrav;
-    }
-    const double b = c->desc.hgcfl * c->dtg_unit / cgrav;
-    dtgrav = b < dt ? b : dt;
-  }
+  if (c->D.grav) grav_speeds(c, c->h_sc->dt, c->h_sc->cfmax, &cgrav, &dtgrav);
   if (cgrav_out) *cgrav_out = cgrav;
   if (dtgrav_out) *dtgrav_out = dtgrav;
   return HG_OK;
@@ -659,15 +656,15 @@ extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
 // gravity_hyperbolic (gravity_hyperbolic.f90:19-50) with gsrc = rho
 // (gravity.f90:114-150): tngrav = ceiling((time+dt-grvtime)/dtgrav) RK
 // sub-steps of the wave equation, then grvtime and cgrav_old advance.
-extern "C" int hg_gravity(hg_ctx *c, double time, double dt, double *grvtime, double cgrav,
-                          double *cgrav_old, double *dtgrav, int32_t *nsub) {
-  if (!c || !grvtime || !cgrav_old || !dtgrav) return HG_ERR_ARG;
-  if (!c->D.grav) return HG_ERR_STATE;
-  HIPCHK(hipSetDevice(c->device));
-  int st = ensure_eint(c);
-  if (st) return st;
+// The sub-steps of one gravity call, enqueued on the context's stream:
+// tngrav, the sub-step length and every sub-step's psi damping factor are
+// host arithmetic on (time, dt, grvtime, cgrav, cgrav_old, dtgrav), in the
+// reference's order.  Returns tngrav, or 0 when time + dt does not lie
+// beyond grvtime.
+static int enqueue_gravity(hg_ctx *c, double time, double dt, double *grvtime, double cgrav,
+                           double *cgrav_old, double *dtgrav) {
   const int tngrav = (int)std::ceil((time + dt - *grvtime) / *dtgrav);
-  if (tngrav < 1) return HG_ERR_ARG;
+  if (tngrav < 1) return 0;
   *dtgrav = (time + dt - *grvtime) / (double)tngrav;
   const int gk = c->desc.grktype;
   for (int l = 1; l <= tngrav; ++l) {
@@ -683,6 +680,32 @@ extern "C" int hg_gravity(hg_ctx *c, double time, double dt, double *grvtime, do
     *grvtime = *grvtime + *dtgrav;
     *cgrav_old = cgrav;
   }
+  return tngrav;
+}
+
+// cgrav and dtgrav of timestep.f90:92-97 from the step's dt and the
+// largest fast speed cfmax.
+static void grav_speeds(const hg_ctx *c, double dt, double cfmax, double *cgrav,
+                        double *dtgrav) {
+  double cg = c->desc.HGfac * cfmax;
+  if (c->desc.maxtngrv > 0) {
+    const double lim = c->desc.hgcfl * c->dtg_unit * (double)c->desc.maxtngrv / dt;
+    cg = lim < cg ? lim : cg;
+  }
+  const double b = c->desc.hgcfl * c->dtg_unit / cg;
+  *cgrav = cg;
+  *dtgrav = b < dt ? b : dt;
+}
+
+extern "C" int hg_gravity(hg_ctx *c, double time, double dt, double *grvtime, double cgrav,
+                          double *cgrav_old, double *dtgrav, int32_t *nsub) {
+  if (!c || !grvtime || !cgrav_old || !dtgrav) return HG_ERR_ARG;
+  if (!c->D.grav) return HG_ERR_STATE;
+  HIPCHK(hipSetDevice(c->device));
+  int st = ensure_eint(c);
+  if (st) return st;
+  const int tngrav = enqueue_gravity(c, time, dt, grvtime, cgrav, cgrav_old, dtgrav);
+  if (tngrav < 1) return HG_ERR_ARG;
   HIPCHK(hipGetLastError());
   if (nsub) *nsub = tngrav;
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -690,16 +713,33 @@ extern "C" int hg_gravity(hg_ctx *c, double time, double dt, double *grvtime, do
   return HG_OK;
 }
 
+// The part of RK stage 1 that needs neither the step's dt nor the gravity
+// potential of this step: the flux sweep (of every direction, or of the
+// directions before the fused one) and the species pass.  It only reads the
+// primitives and ch, which the previous step's end has already set.
+static void enqueue_stage1_sweep(hg_ctx *c) {
+  const Dev &D = c->D;
+  if (c->fused) {
+    RKArgs a{};
+    launch_fused_stage(D, 1, a, c->stream, nullptr, nullptr, 1);
+    return;
+  }
+  launch_sweep(D, 1, c->stream);
+  launch_species(D, c->stream);
+}
+
 // The RK stages of one step.  Returns the number of CFL partials the last
 // stage wrote (fuse_tail): one per 256 cells, or one per block of the fused
-// last sweep.
-static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
+// last sweep.  sweep1_done: stage 1's sweep was enqueued beforehand
+// (enqueue_stage1_sweep).
+static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp, bool sweep1_done = false) {
   const Dev &D = c->D;
   const int rk = c->desc.rktype;
   int npart = D.nblk_cfl;
   for (int stage = 1; stage <= rk; ++stage) {
     double faco, fact, facn;
     runge_coeff(stage, rk, &faco, &fact, &facn);
+    const bool pre = sweep1_done && stage == 1;
     if (c->fused) {
       const bool last = stage == rk;
       RKArgs a{faco, fact, facn, damp, fuse_tail && last, fuse_tail && last && damp > 0.0,
@@ -707,21 +747,23 @@ static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp) {
       // a pair for the sweep of the directions before the fused one only
       // when there are such directions (every pair taken must be recorded)
       KEv k0{}, k1{};
-      const long pre = D.fdir == 0 ? D.woff[2] - D.woff[1] + D.waves[2] : D.woff[D.fdir];
-      const KEv *e0 = pre > 0 ? prof_ev_into(c, HG_PROF_SWEEP, k0) : nullptr;
+      const long npre = D.fdir == 0 ? D.woff[2] - D.woff[1] + D.waves[2] : D.woff[D.fdir];
+      const KEv *e0 = npre > 0 && !pre ? prof_ev_into(c, HG_PROF_SWEEP, k0) : nullptr;
       const KEv *e1 = prof_ev_into(c, HG_PROF_RK, k1);
       Dev Df = D;
       for (int v = 0; v < D.nv; ++v) Df.Uo[v] = c->alt_U[v];
       for (int w = 0; w < D.nw; ++w) Df.Wo[w] = c->alt_W[w];
-      const int g = launch_fused_stage(Df, stage, a, c->stream, e0, e1);
+      const int g = launch_fused_stage(Df, stage, a, c->stream, e0, e1, pre ? 2 : 0);
       // the new state is in the other set: it becomes the current one
       for (int v = 0; v < D.nv; ++v) std::swap(c->D.U[v], c->alt_U[v]);
       for (int w = 0; w < D.nw; ++w) std::swap(c->D.W[w], c->alt_W[w]);
       if (last && fuse_tail) npart = g;
       continue;
     }
-    launch_sweep(D, stage, c->stream, prof_ev(c, HG_PROF_SWEEP));
-    launch_species(D, c->stream);
+    if (!pre) {
+      launch_sweep(D, stage, c->stream, prof_ev(c, HG_PROF_SWEEP));
+      launch_species(D, c->stream);
+    }
     const bool last = stage == rk;
     launch_rk(D, stage, faco, fact, facn, fuse_tail && last, fuse_tail && last && damp > 0.0,
               damp, c->stream, prof_ev(c, HG_PROF_RK));
@@ -758,7 +800,7 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
                       int32_t *stop_reason) {
   if (!c || !time || !tn || !t_out || max_steps < 1) return HG_ERR_ARG;
   if (outstyle == 2 && tn_out < 1) return HG_ERR_ARG;
-  if (c->D.grav) return HG_ERR_UNSUPPORTED;  // gravity runs use the per-step entry points
+  if (c->D.grav) return HG_ERR_UNSUPPORTED;  // gravity runs use hg_run_gravity
   HIPCHK(hipSetDevice(c->device));
   int st = ensure_eint(c);
   if (st) return st;
@@ -805,6 +847,89 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
   *tn = c->h_sc->tn;
   *t_out = c->h_sc->t_out;
   c->ch = c->h_sc->ch;
+  if (stop_reason) *stop_reason = reason;
+  if (reason == STOP_ERROR) return check_error(c);
+  return HG_OK;
+}
+
+// The main loop with hyperbolic self-gravity (hormone.f90:124-179 with
+// gravswitch 3): per step the device's previous step end has computed dt,
+// ch and the largest fast speed; the host reads them (one pinned copy and
+// an event, not a stream synchronisation), evaluates cgrav, dtgrav, tngrav
+// and the damping factors exactly as hg_timestep and hg_gravity do, and
+// enqueues the gravity sub-steps, the RK stages and the step end.  Stage
+// 1's flux sweep needs neither dt nor the new potential, so it is enqueued
+// before the host waits: the GPU sweeps while the host reads the scalars.
+extern "C" int hg_run_gravity(hg_ctx *c, double *time, int64_t *tn, double *t_out,
+                              double dt_out, double t_end, int32_t outstyle, int32_t endstyle,
+                              int64_t tnlim, int64_t tn_out, int64_t max_steps,
+                              double damp_factor, double *grvtime, double *cgrav_old,
+                              int64_t *nsub_total, int32_t *stop_reason) {
+  if (!c || !time || !tn || !t_out || !grvtime || !cgrav_old || max_steps < 1)
+    return HG_ERR_ARG;
+  if (outstyle == 2 && tn_out < 1) return HG_ERR_ARG;
+  if (!c->D.grav) return HG_ERR_STATE;
+  HIPCHK(hipSetDevice(c->device));
+  int st = ensure_eint(c);
+  if (st) return st;
+  const Dev &D = c->D;
+  const bool damp = D.mag && D.dim > 1 && damp_factor > 0.0;
+  // stage 1's sweep runs while the host waits; it carries no profiling
+  // events, so every timed launch is one of a step that ran
+  const bool overlap = true;
+  hipEvent_t ready;
+  HIPCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  auto fail = [&](int code) {
+    hipStreamSynchronize(c->stream);
+    hipEventDestroy(ready);
+    return code;
+  };
+  auto post = [&]() -> bool {  // scalars to the host, then the next sweep
+    if (hipMemcpyAsync(c->h_sc, D.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipEventRecord(ready, c->stream) != hipSuccess)
+      return false;
+    if (overlap) enqueue_stage1_sweep(c);
+    return hipGetLastError() == hipSuccess;
+  };
+  launch_set_clock(D, *time, *t_out, t_end, dt_out, outstyle, endstyle, *tn, tnlim, tn_out,
+                   max_steps, c->stream);
+  launch_cfl(D, c->stream, prof_ev(c, HG_PROF_CFL));
+  launch_cfl_final(D, c->stream);
+  if (!post()) return fail(HG_ERR_DEVICE);
+  int64_t nsum = 0;
+  int reason = STOP_NONE;
+  for (;;) {
+    if (hipEventSynchronize(ready) != hipSuccess) return fail(HG_ERR_DEVICE);
+    if (c->prof) prof_collect(c);
+    const Scalars s = *c->h_sc;
+    if (s.stop != STOP_NONE) {
+      reason = s.stop;
+      break;
+    }
+    double cgrav, dtgrav;
+    grav_speeds(c, s.dt, s.cfmax, &cgrav, &dtgrav);
+    // gravity.f90:41-60: the potential of step 0 is the initial one
+    if (s.tn == 0) {
+      *grvtime = 0.0;
+    } else {
+      const int ns = enqueue_gravity(c, s.time, s.dt, grvtime, cgrav, cgrav_old, &dtgrav);
+      if (ns < 1) return fail(HG_ERR_STATE);
+      nsum += ns;
+    }
+    Dev De = D;  // the step end reduces the partials the last stage wrote
+    De.nblk_cfl = enqueue_stages(c, true, damp ? damp_factor : 0.0, overlap);
+    launch_step_end_cfl(De, c->stream);
+    if (hipGetLastError() != hipSuccess || !post()) return fail(HG_ERR_DEVICE);
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
+  hipEventDestroy(ready);
+  prof_collect(c);
+  *time = c->h_sc->time;
+  *tn = c->h_sc->tn;
+  *t_out = c->h_sc->t_out;
+  c->ch = c->h_sc->ch;
+  if (nsub_total) *nsub_total = nsum;
   if (stop_reason) *stop_reason = reason;
   if (reason == STOP_ERROR) return check_error(c);
   return HG_OK;
@@ -942,12 +1067,13 @@ extern "C" int hg_last_error(const hg_ctx *c, int32_t *i, int32_t *j, int32_t *k
 }
 
 extern "C" int hg_kat(int32_t which, int32_t n, const double *in, double *out) {
-  static const int nin[6] = {25, 5, 8, 5, 2, 5}, nout[6] = {9, 1, 1, 1, 2, 4};
-  if (which < 0 || which > 5 || n < 0 || !in || !out) return HG_ERR_ARG;
+  static const int nin[8] = {25, 5, 8, 5, 2, 5, 3, 9}, nout[8] = {9, 1, 1, 1, 2, 4, 2, 1};
+  const int w = which >= 16 ? which - 16 : which;
+  if (w < 0 || w > 7 || n < 0 || !in || !out) return HG_ERR_ARG;
   if (n == 0) return HG_OK;
   double *din = nullptr, *dout = nullptr;
-  const size_t bi = sizeof(double) * (size_t)n * nin[which];
-  const size_t bo = sizeof(double) * (size_t)n * nout[which];
+  const size_t bi = sizeof(double) * (size_t)n * nin[w];
+  const size_t bo = sizeof(double) * (size_t)n * nout[w];
   if (hipMalloc((void **)&din, bi) != hipSuccess) return HG_ERR_NOMEM;
   if (hipMalloc((void **)&dout, bo) != hipSuccess) {
     hipFree(din);
@@ -981,8 +1107,11 @@ struct hg_batch {
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // pinned host copies of the members' scalars after the last step, in
+  // member order; sc_valid[q] is 0 until member q has been stepped
   Scalars *h_sc = nullptr;
   size_t cap_hsc = 0;
+  std::vector<char> sc_valid;
   // launch shapes: sweep block size (wavefronts per block) and grid sizes of
   // the persistent sweep / rk launches (0 = one item per wavefront / block)
   int sweep_block_waves = 4;
@@ -1025,6 +1154,23 @@ extern "C" int hg_batch_destroy(hg_batch *b) {
   return HG_OK;
 }
 
+// Room for `need` member records in the pinned host array, keeping the
+// records already there.
+static int batch_hsc_reserve(hg_batch *b, size_t need) {
+  if (need <= b->cap_hsc) return HG_OK;
+  const size_t cap = need * 2;
+  Scalars *p = nullptr;
+  if (hipHostMalloc((void **)&p, cap * sizeof(Scalars), hipHostMallocDefault) != hipSuccess)
+    return HG_ERR_NOMEM;
+  if (b->h_sc) {
+    std::memcpy(p, b->h_sc, b->cap_hsc * sizeof(Scalars));
+    hipHostFree(b->h_sc);
+  }
+  b->h_sc = p;
+  b->cap_hsc = cap;
+  return HG_OK;
+}
+
 extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, double dt_out,
                             double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
                             int64_t tn_out) {
@@ -1039,7 +1185,9 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
        c->desc.rktype != b->rktype))
     return HG_ERR_UNSUPPORTED;  // one batch runs one kernel variant
   HIPCHK(hipSetDevice(b->device));
-  int st = ensure_eint(c);
+  int st = batch_hsc_reserve(b, b->members.size() + 1);
+  if (st) return st;
+  st = ensure_eint(c);
   if (st) return st;
   HIPCHK(hipStreamSynchronize(c->stream));
   // the box's clock and its first CFL partials, on the batch stream
@@ -1053,6 +1201,7 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
   b->mixed = c->D.mixed;
   b->rktype = c->desc.rktype;
   b->members.push_back(c);
+  b->sc_valid.push_back(0);
   c->in_batch = true;
   b->dirty = true;
   return HG_OK;
@@ -1064,8 +1213,10 @@ extern "C" int hg_batch_remove(hg_batch *b, hg_ctx *c) {
     if (b->members[q] == c) {
       HIPCHK(hipStreamSynchronize(b->stream));
       b->members.erase(b->members.begin() + q);
+      b->sc_valid.erase(b->sc_valid.begin() + q);
       c->in_batch = false;
-      // keep the cached member states aligned with the member list
+      // keep the cached member states aligned with the member list (the
+      // array holds a record per member since hg_batch_add)
       for (size_t r = q; r < b->members.size(); ++r) b->h_sc[r] = b->h_sc[r + 1];
       b->dirty = true;
       return HG_OK;
@@ -1135,16 +1286,7 @@ static int batch_rebuild(hg_batch *b) {
   b->B.lds_max = nb ? b->members[0]->D.lds_max : 0;
   b->B.citems = b->d_citems;
   b->B.ncitems = (long)ci.size();
-  if (b->cap_hsc < (nb ? nb : 1)) {
-    if (b->h_sc) hipHostFree(b->h_sc);
-    b->h_sc = nullptr;
-    b->cap_hsc = (nb ? nb : 1) * 2;
-    if (hipHostMalloc((void **)&b->h_sc, b->cap_hsc * sizeof(Scalars), hipHostMallocDefault) !=
-        hipSuccess) {
-      b->cap_hsc = 0;
-      return HG_ERR_NOMEM;
-    }
-  }
+  if ((st = batch_hsc_reserve(b, nb ? nb : 1))) return st;
   if (b->use_graph && nb) {
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     batch_enqueue_step(b);
@@ -1179,6 +1321,7 @@ extern "C" int hg_batch_step(hg_batch *b, int32_t nsteps, int32_t *nrunning) {
   int run = 0;
   for (size_t q = 0; q < b->members.size(); ++q) {
     hg_ctx *c = b->members[q];
+    b->sc_valid[q] = 1;
     *c->h_sc = b->h_sc[q];
     c->ch = b->h_sc[q].ch;
     if (b->h_sc[q].stop == STOP_NONE) ++run;
@@ -1194,6 +1337,7 @@ extern "C" int hg_batch_member_state(hg_batch *b, hg_ctx *c, double *time, int64
   if (!b || !c) return HG_ERR_ARG;
   for (size_t q = 0; q < b->members.size(); ++q)
     if (b->members[q] == c) {
+      if (!b->sc_valid[q]) return HG_ERR_STATE;  // not stepped since it was added
       const Scalars &s = b->h_sc[q];
       if (time) *time = s.time;
       if (tn) *tn = s.tn;

@@ -1543,33 +1543,52 @@ __global__ __launch_bounds__(256) void k_phidamp(Dev D, double factor) {
 }
 
 // ---- known-answer evaluation of the device arithmetic -----------------------
+// SH: the forms the sweep runs, with divisions by a shared denominator (qdiv)
+template <bool SH>
+__device__ void kat_one(int which, long t, const double *in, double *out) {
+  if (which == 0) {  // hlldflux: 25 inputs (hlldflux.f90:19-21 order), 9 outputs
+    const double *a = in + 25 * t;
+    double f[9];
+    hlld_d<SH>(f, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11],
+               a[12], a[13], a[14], a[15], a[16], a[17], a[18], a[19], a[20], a[21], a[22]);
+    for (int s = 0; s < 9; ++s) out[9 * t + s] = f[s];
+  } else if (which == 1) {  // get_cf: d cs b1 b2 b3
+    const double *a = in + 5 * t;
+    out[t] = get_cf_d(mkden<SH>(a[0]), a[1], a[2], a[3], a[4]);
+  } else if (which == 2) {  // modified_mc: u0 u1 u2 xm x0 xp xim xi0
+    const double *a = in + 8 * t;
+    const double xm = a[3], x0 = a[4], xp = a[5], xim = a[6], xi0 = a[7];
+    const double dxp = xp - x0;
+    const double cf = dxp / (xi0 - x0), cb = (x0 - xm) / (x0 - xim);
+    out[t] = modified_mc_d(a[0], a[1], a[2], dxp, cf, cb, mkden<SH>(x0 - xm), mkden<SH>(dxp));
+  } else if (which == 3) {  // minmod: u0 u1 u2 dxa dxb
+    const double *a = in + 5 * t;
+    out[t] = minmod_d(a[0], a[1], a[2], a[3], a[4], mkden<SH>(a[3] + a[4]));
+  } else if (which == 4) {  // division and square root: a b -> a/b, sqrt(a)
+    const double *a = in + 2 * t;
+    out[2 * t] = a[0] / a[1];
+    out[2 * t + 1] = sqrt(a[0]);
+  } else if (which == 5) {  // division by a shared denominator: b a0 a1 a2 a3 -> ai/b
+    const double *a = in + 5 * t;
+    const Den64 Db = mkden<true>(a[0]);
+    for (int s = 0; s < 4; ++s) out[4 * t + s] = qdiv(Db, a[1 + s]);
+  } else if (which == 6) {  // eos_p_cs as the face states use it: d eint gamma -> p cs
+    const double *a = in + 3 * t;
+    const double p = (a[2] - 1.0) * a[1];
+    out[2 * t] = p;
+    out[2 * t + 1] = sqrt(qdiv(mkden<SH>(a[0]), a[2] * p));
+  } else if (which == 7) {  // get_eint: etot d v1 v2 v3 b1 b2 b3 mag_on -> eint
+    const double *a = in + 9 * t;
+    out[t] = get_eint(a[8] > 0.5, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]);
+  }
+}
+
+// which 0..7 as hg_kat; + 16: the sweep's shared-denominator form
 __global__ void k_kat(int which, int n, const double *in, double *out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  if (which == 0) {  // hlldflux: 25 inputs (hlldflux.f90:19-21 order), 9 outputs
-    const double *a = in + 25 * (long)t;
-    double f[9];
-    hlld(f, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11],
-         a[12], a[13], a[14], a[15], a[16], a[17], a[18], a[19], a[20], a[21], a[22]);
-    for (int s = 0; s < 9; ++s) out[9 * (long)t + s] = f[s];
-  } else if (which == 1) {  // get_cf: d cs b1 b2 b3
-    const double *a = in + 5 * (long)t;
-    out[t] = get_cf(a[0], a[1], a[2], a[3], a[4]);
-  } else if (which == 2) {  // modified_mc: u0 u1 u2 xm x0 xp xim xi0
-    const double *a = in + 8 * (long)t;
-    out[t] = modified_mc(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]);
-  } else if (which == 3) {  // minmod: u0 u1 u2 dxa dxb
-    const double *a = in + 5 * (long)t;
-    out[t] = minmod(a[0], a[1], a[2], a[3], a[4]);
-  } else if (which == 4) {  // division and square root: a b -> a/b, sqrt(a)
-    const double *a = in + 2 * (long)t;
-    out[2 * (long)t] = a[0] / a[1];
-    out[2 * (long)t + 1] = sqrt(a[0]);
-  } else if (which == 5) {  // division by a shared denominator: b a0 a1 a2 a3 -> ai/b
-    const double *a = in + 5 * (long)t;
-    const Den64 Db = mkden<true>(a[0]);
-    for (int s = 0; s < 4; ++s) out[4 * (long)t + s] = qdiv(Db, a[1 + s]);
-  }
+  if (which >= 16) kat_one<true>(which - 16, t, in, out);
+  else kat_one<false>(which, t, in, out);
 }
 
 // ---- launchers ----------------------------------------------------------------
@@ -1723,7 +1742,7 @@ static void sweep_rk_dir(const Dev &D, int stage, const RKArgs &rk, unsigned g, 
 }
 
 int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
-                       const KEv *ev, const KEv *ev_fused) {
+                       const KEv *ev, const KEv *ev_fused, int part) {
   const int L = D.fdir;
   const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
   const int bs = 64 * bw;
@@ -1739,7 +1758,7 @@ int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
     Do.woff[1] = 0;
     nwo = Do.woff[2] + D.waves[2];
   }
-  if (nwo > 0) {
+  if (nwo > 0 && part != 2) {
     const unsigned g0 = nblk(nwo, bw);
     if (D.mag) {
       if (D.limiter == LIM_FLAT) sweep_variant<true, LIM_FLAT>(Do, stage, g0, bs, tile, s, ev);
@@ -1752,6 +1771,7 @@ int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
     }
   }
   const unsigned g = nblk(D.waves[L], bw);
+  if (part == 1) return (int)g;
   if (L == 0) sweep_rk_dir<0>(D, stage, rk, g, bs, 0, s, ev_fused);
   else if (L == 1) sweep_rk_dir<1>(D, stage, rk, g, bs, tile, s, ev_fused);
   else sweep_rk_dir<2>(D, stage, rk, g, bs, tile, s, ev_fused);

@@ -169,9 +169,12 @@ struct RKArgs {
 };
 // the fused path of one RK stage: the sweeps of the other directions, then
 // D.fdir's sweep with the RK update into D.Uo/D.Wo; returns the number
-// of CFL partials the fused launch wrote (rk.cfl)
+// of CFL partials the fused launch wrote (rk.cfl).  part 1 enqueues only the
+// other directions' sweep (it needs neither dt nor the gravity potential),
+// part 2 only the fused launch, 0 both.
 int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
-                       const KEv *ev_sweep = nullptr, const KEv *ev_fused = nullptr);
+                       const KEv *ev_sweep = nullptr, const KEv *ev_fused = nullptr,
+                       int part = 0);
 void launch_species(const Dev &D, hipStream_t s);
 size_t sweep_tile_bytes(const Dev &D, int bw);
 // cfl: also the next step's CFL partials; damp: phidamp

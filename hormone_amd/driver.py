@@ -158,6 +158,9 @@ def run(eng, prm: P.Params, g: Grid, max_steps: int | None = None,
     if (device_loop and hasattr(eng, "run_loop") and prm.fixed_dt <= 0 and not record_dt
             and prm.gravswitch == 0):
         return _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out, start)
+    if (device_loop and hasattr(eng, "run_loop_gravity") and prm.fixed_dt <= 0
+            and not record_dt and prm.gravswitch == 3):
+        return _run_device_gravity(eng, prm, g, max_steps, t_end, dt_out, t_out, start)
     tn, tim = int(start[0]), float(start[1])
     damp = math.exp(-prm.alpha9wave * prm.courant)  # source.f90:253
     dts = []
@@ -219,6 +222,29 @@ def _run_device(eng, prm, g, max_steps, t_end, dt_out, t_out, start=(0, 0.0)) ->
             break
     wall = _time.perf_counter() - t0
     return RunResult(tn, tim, wall, tn)
+
+
+def _run_device_gravity(eng, prm, g, max_steps, t_end, dt_out, t_out,
+                        start=(0, 0.0)) -> RunResult:
+    damp = math.exp(-prm.alpha9wave * prm.courant) if (prm.mag_on and g.dim > 1) else 0.0
+    tn, tim = int(start[0]), float(start[1])
+    grvtime, cgrav_old = 0.0, 0.0
+    if len(start) > 2:  # restart.f90:59
+        grvtime, cgrav_old = tim, float(start[2])
+    budget = max_steps if max_steps is not None else 1 << 62
+    nsub_total = 0
+    t0 = _time.perf_counter()
+    while True:
+        tim, tn, t_out, reason, grvtime, cgrav_old, ns = eng.run_loop_gravity(
+            tim, tn, t_out, dt_out, t_end, prm.outstyle, prm.endstyle, prm.tnlim, prm.tn_out,
+            budget - tn, damp, grvtime, cgrav_old)
+        nsub_total += ns
+        if reason != 2 or tn >= budget:
+            break
+    wall = _time.perf_counter() - t0
+    res = RunResult(tn, tim, wall, tn, cgrav_old=cgrav_old)
+    res.nsub_total = nsub_total
+    return res
 
 
 def fetch(eng, g: Grid, fields) -> dict:

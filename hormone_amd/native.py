@@ -109,6 +109,10 @@ def _load():
     lib.hg_run.argtypes = [vp, _dp, C.POINTER(C.c_int64), _dp, C.c_double, C.c_double,
                            C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int64,
                            C.c_double, C.POINTER(C.c_int32)]
+    lib.hg_run_gravity.argtypes = [vp, _dp, C.POINTER(C.c_int64), _dp, C.c_double,
+                                   C.c_double, C.c_int32, C.c_int32, C.c_int64, C.c_int64,
+                                   C.c_int64, C.c_double, _dp, _dp, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int32)]
     lib.hg_profile.argtypes = [vp, C.c_int32]
     lib.hg_profile_read.argtypes = [vp, C.c_int32, _dp, C.POINTER(C.c_int64), _dp]
     lib.hg_kat.argtypes = [C.c_int32, C.c_int32, _dp, _dp]
@@ -136,13 +140,14 @@ def exported_symbols():
     """Entry points declared in include/hormone_hip.h."""
     return ["hg_version", "hg_create", "hg_destroy", "hg_ufnmax", "hg_upload",
             "hg_download", "hg_timestep", "hg_hydro_step", "hg_phidamp",
-            "hg_last_error", "hg_run", "hg_profile", "hg_profile_read", "hg_kat",
+            "hg_last_error", "hg_run", "hg_run_gravity", "hg_profile", "hg_profile_read", "hg_kat",
             "hg_gravity", "hg_set_launch", "hg_time_sweep", "hg_set_fused",
             "hg_set_ghosts", "hg_set_boundary_flux"]
 
 
 STOP_END, STOP_OUTPUT, STOP_MAXSTEPS, STOP_ERROR = 1, 2, 3, 4
 PROF_SWEEP, PROF_RK, PROF_CFL, PROF_GRAV = 0, 1, 2, 3
+PROF_ONLY = 0x100
 
 
 class Hydro:
@@ -237,11 +242,29 @@ class Hydro:
         self._check(st, "hg_run", tn=n.value, time=t.value)
         return t.value, n.value, to.value, r.value
 
+    def run_loop_gravity(self, time, tn, t_out, dt_out, t_end, outstyle, endstyle, tnlim,
+                         tn_out, max_steps, damp_factor, grvtime, cgrav_old):
+        """Device-resident main loop with hyperbolic gravity (hg_run_gravity);
+        returns (time, tn, t_out, reason, grvtime, cgrav_old, nsub_total)."""
+        t, n, to, r = C.c_double(time), C.c_int64(tn), C.c_double(t_out), C.c_int32(0)
+        gt, co, ns = C.c_double(grvtime), C.c_double(cgrav_old), C.c_int64(0)
+        st = self._lib.hg_run_gravity(self.h, C.byref(t), C.byref(n), C.byref(to), dt_out,
+                                      t_end, outstyle, endstyle, tnlim, tn_out, max_steps,
+                                      damp_factor, C.byref(gt), C.byref(co), C.byref(ns),
+                                      C.byref(r))
+        self._check(st, "hg_run_gravity", tn=n.value, time=t.value)
+        return t.value, n.value, to.value, r.value, gt.value, co.value, ns.value
+
     def set_launch(self, sweep_block_waves: int):
         self._check(self._lib.hg_set_launch(self.h, sweep_block_waves), "hg_set_launch")
 
-    def profile(self, enable=True):
-        self._check(self._lib.hg_profile(self.h, int(enable)), "hg_profile")
+    def profile(self, enable=True, kinds=None):
+        """Per-launch event timing: every category, or only `kinds`
+        (PROF_* values); enable=False turns it off."""
+        mode = int(bool(enable))
+        if enable and kinds is not None:
+            mode = PROF_ONLY | sum(1 << k for k in kinds)
+        self._check(self._lib.hg_profile(self.h, mode), "hg_profile")
 
     def set_fused(self, on=True):
         """Fuse one sweep direction with the RK update (bit-identical):

@@ -222,8 +222,19 @@ def _serve_lane(lane: int, queue: list, sh: _Shared, t0: float, steps_per_round:
                         sh.tuned.append((lane, tuned))
             try:
                 batch.step(steps_per_round)
-            except HgError:
-                pass
+            except HgError as err:
+                # the batch step itself failed (a launch, a graph rebuild or
+                # its memory): every request in flight fails with it and
+                # leaves the batch, so the lane never spins on a broken batch
+                for key, (req, eng, g) in list(active.items()):
+                    req.error = f"batch step: {err}"
+                    req.finished = time.perf_counter() - t0
+                    batch.remove(eng)
+                    eng.close()
+                    with sh.lock:
+                        sh.used -= device_bytes(req)
+                    del active[key]
+                continue
             with sh.lock:
                 sh.events.append({"t": round(time.perf_counter() - t0, 4), "lane": lane,
                                   "active": len(active),

@@ -33,8 +33,10 @@ extern "C" {
 enum {
   HG_OK = 0,
   HG_ERR_ARG = 1,         /* bad argument / descriptor                       */
-  HG_ERR_UNSUPPORTED = 2, /* crdnt/=0, eostype/=0, compswitch 1 or >2, bc 4/5/9/10,
-                             gravswitch other than 0 and 3                   */
+  HG_ERR_UNSUPPORTED = 2, /* crdnt/=0, eostype/=0, compswitch 1 or >2, a
+                             boundary code a face does not take (see
+                             hg_desc.bc), bc 9/10 with species, gravswitch
+                             other than 0 and 3                              */
   HG_ERR_DEVICE = 3,      /* a HIP runtime call failed                       */
   HG_ERR_FACE_EOS = 4,    /* numflux.f90:579-593 error_flux: rho<=0 or
                              e_int<=0 at a reconstructed face state          */
@@ -51,8 +53,11 @@ enum { HG_LIMITER_FLAT = 0, HG_LIMITER_MINMOD = 1, HG_LIMITER_MODIFIED_MC = 2 };
 /* precision of the arithmetic on the path */
 enum { HG_PREC_FP64 = 0, HG_PREC_FP32MIX = 1 };
 
-/* boundary codes accepted (boundarycondition.f90:30-38): 0 periodic,
-   1 reflective, 2 outgoing, 3 free */
+/* boundary codes accepted (boundarycondition.f90:30-38 and the faces' own
+   select blocks): 0 periodic, 1 reflective, 2 outgoing, 3 free everywhere;
+   9 Dirichlet and 10 flux everywhere (not with species); 4 linear and 5
+   linear + outgoing at x1 outer, x3 inner and x3 outer (scalar code 5 not
+   at x3 outer).  See hg_desc.bc. */
 
 /* fields for hg_upload / hg_download.  Primitive fields are the reference's
    physval arrays (modules.f90:106-138); HG_F_U + n (n = 0..ufnmax-1) is the
@@ -187,18 +192,41 @@ int hg_phidamp(hg_ctx *ctx, double factor);
    Returns when the run ends, an output is due (the caller writes it and calls
    again), max_steps steps have been taken, or an error occurs; *stop_reason
    is 1, 2, 3 or 4 respectively.  time, tn and t_out are updated in place.
-   Gives the same state, bit for bit, as the per-step entry points. */
+   Gives the same state, bit for bit, as the per-step entry points.
+   Contexts with gravswitch 3 use hg_run_gravity (HG_ERR_UNSUPPORTED here). */
 int hg_run(hg_ctx *ctx, double *time, int64_t *tn, double *t_out, double dt_out,
            double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
            int64_t tn_out, int64_t max_steps, double damp_factor,
            int32_t *stop_reason);
 
+/* hg_run for gravswitch 3: the iteration of hormone.f90:124-179 with the
+   gravity call between timestep and hydro_step -- timestep (with cgrav and
+   dtgrav, timestep.f90:92-97), at tn > 0 gravity_hyperbolic (tngrav
+   sub-steps, as hg_gravity; at tn == 0 grvtime = 0, gravity.f90:41-60),
+   hydro step, phidamp, clock and end/output checks as hg_run.  grvtime and
+   cgrav_old are the gravmod module values, updated in place; *nsub_total
+   (may be NULL) receives the sub-steps taken in this call.  The host reads
+   one small record of device scalars per step (no stream synchronisation);
+   the step's first flux sweep is already running while it does.  Same
+   state, bit for bit, as hg_timestep + hg_gravity + hg_hydro_step +
+   hg_phidamp. */
+int hg_run_gravity(hg_ctx *ctx, double *time, int64_t *tn, double *t_out, double dt_out,
+                   double t_end, int32_t outstyle, int32_t endstyle, int64_t tnlim,
+                   int64_t tn_out, int64_t max_steps, double damp_factor, double *grvtime,
+                   double *cgrav_old, int64_t *nsub_total, int32_t *stop_reason);
+
 /* Kernel timing with HIP events on the context's stream, by category
    (cf. the reference's profiler, profiler.f90:5-51).  hg_profile(ctx, 1)
-   resets and enables; hg_profile_read returns the summed milliseconds, the
-   number of launches and the algorithmic HBM bytes of one launch. */
+   resets and enables every category; hg_profile(ctx, HG_PROF_ONLY | mask)
+   only those whose bit (1 << HG_PROF_*) is in mask (each timed launch waits
+   for its completion signal, so timing fewer launches disturbs a run less);
+   hg_profile(ctx, 0) disables.  hg_profile_read returns the summed
+   milliseconds, the number of launches and the HBM bytes one launch of the
+   kernel itself moves (its own reads and writes, intermediates included).
+   The sweep category times the flux sweeps (k_sweep); the RK category the
+   RK update, or the fused sweep + RK launch when a direction is fused. */
 enum { HG_PROF_SWEEP = 0, HG_PROF_RK = 1, HG_PROF_CFL = 2, HG_PROF_GRAV = 3,
-       HG_PROF_N = 4 };
+       HG_PROF_N = 4, HG_PROF_ONLY = 0x100 };
 int hg_profile(hg_ctx *ctx, int32_t enable);
 int hg_profile_read(const hg_ctx *ctx, int32_t kind, double *ms_total,
                     int64_t *launches, double *bytes_per_launch);
@@ -292,8 +320,13 @@ int hg_last_error(const hg_ctx *ctx, int32_t *i, int32_t *j, int32_t *k,
    cfl..ch,erl,err -> 9 outputs tmpflux(1:9)), 1 get_cf (d cs b1 b2 b3 -> cf),
    2 modified_mc (u(1:3) x(1:3) xi(1:2) -> slope), 3 minmod (u(1:3) dx(1:2)
    -> slope), 4 IEEE a/b and sqrt(a) (a b -> a/b sqrt(a)), 5 the sweep's
-   division by a shared denominator (b a0 a1 a2 a3 -> a0/b .. a3/b).  n
-   records, row-major.  For tests of bit-exactness against the reference. */
+   division by a shared denominator (b a0 a1 a2 a3 -> a0/b .. a3/b), 6
+   eos_p_cs as the face states evaluate it (eos.f90:195-242 with erad = 0:
+   d eint gamma -> p cs), 7 get_eint (eos.f90:846-865: etot d v1 v2 v3 b1
+   b2 b3 mag_on -> eint).  which + 16: the same routine in the form the
+   sweep kernels run it, with divisions by a shared denominator (kinds 0-3
+   and 6).  n records, row-major.  For tests of bit-exactness against the
+   reference. */
 int hg_kat(int32_t which, int32_t n, const double *in, double *out);
 
 #ifdef __cplusplus

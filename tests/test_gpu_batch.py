@@ -86,3 +86,43 @@ def test_serving_results_equal_alone_and_report(hiplib, lanes):
     for r in reqs[:4]:
         tn, tm, dg = S.run_alone(r)
         assert r.steps == tn and r.digest == dg, f"request {r.rid} differs in the mix"
+
+
+def test_batch_remove_and_state_before_any_step(hiplib):
+    """A member removed, or asked for its state, before the batch has
+    stepped: no stale host record is read (HG_ERR_STATE for the state) and
+    the members that stay run on to their alone result."""
+    from hormone_amd.native import HG_ERR_STATE, HgError
+    reqs = small_requests(3, seed=31)
+    b = Batch()
+    engs = []
+    for r in reqs:
+        prm, g, st = S.setup(r)
+        eng = Hydro(**D.desc_fields(prm, g))
+        D.load_initial_state(eng, prm, g, st)
+        t_end, dt_out, t_out = prm.times()
+        b.add(eng, t_end=t_end, t_out=t_out, dt_out=dt_out)
+        engs.append((eng, g, r))
+    with pytest.raises(HgError) as ei:
+        b.state(engs[0][0])
+    assert ei.value.code == HG_ERR_STATE
+    b.remove(engs[0][0])
+    b.step(2)
+    # added after the last step: no record yet
+    prm, g, st = S.setup(reqs[0])
+    late = Hydro(**D.desc_fields(prm, g))
+    D.load_initial_state(late, prm, g, st)
+    t_end, dt_out, t_out = prm.times()
+    b.add(late, t_end=t_end, t_out=t_out, dt_out=dt_out)
+    with pytest.raises(HgError) as ei:
+        b.state(late)
+    assert ei.value.code == HG_ERR_STATE
+    b.remove(late)
+    while b.step(8):
+        pass
+    for eng, g, r in engs[1:]:
+        t, n, stop = b.state(eng)
+        tn, tm, dg = S.run_alone(r)
+        assert stop == 1 and n == tn and t == tm
+        b.remove(eng)
+        assert S.state_digest(eng, g) == dg
