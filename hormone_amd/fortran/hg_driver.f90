@@ -74,12 +74,12 @@ program hg_driver
                     gis,gie,gjs,gje,gks,gke
  namelist /gpucon/ simtype,outfile,initfile,is_test,device,gconst,fused
 
- real(8),allocatable,target:: x1(:),xi1(:),dx1(:),dxi1(:),idx1(:),idxi1(:)
- real(8),allocatable,target:: x2(:),xi2(:),dx2(:),dxi2(:),idx2(:),idxi2(:)
- real(8),allocatable,target:: x3(:),xi3(:),dx3(:),dxi3(:),idx3(:),idxi3(:)
- real(8),allocatable,target:: d(:,:,:),p(:,:,:),v1(:,:,:),v2(:,:,:),v3(:,:,:), &
-                              e(:,:,:),b1(:,:,:),b2(:,:,:),b3(:,:,:),phi(:,:,:), &
-                              eint(:,:,:),u(:,:,:,:),grvphi(:,:,:),grvpsi(:,:,:)
+ real(8),allocatable:: x1(:),xi1(:),dx1(:),dxi1(:),idx1(:),idxi1(:)
+ real(8),allocatable:: x2(:),xi2(:),dx2(:),dxi2(:),idx2(:),idxi2(:)
+ real(8),allocatable:: x3(:),xi3(:),dx3(:),dxi3(:),idx3(:),idxi3(:)
+ real(8),allocatable:: d(:,:,:),p(:,:,:),v1(:,:,:),v2(:,:,:),v3(:,:,:), &
+                       e(:,:,:),b1(:,:,:),b2(:,:,:),b3(:,:,:),phi(:,:,:), &
+                       eint(:,:,:),u(:,:,:,:),grvphi(:,:,:),grvpsi(:,:,:)
  type(hg_desc):: desc
  type(c_ptr):: ctx
  integer(c_int):: st
@@ -157,12 +157,6 @@ program hg_driver
  end if
 
  desc%is = is ; desc%ie = ie ; desc%js = js ; desc%je = je ; desc%ks = ks ; desc%ke = ke
- desc%x1 = c_loc(x1) ; desc%xi1 = c_loc(xi1) ; desc%dx1 = c_loc(dx1)
- desc%dxi1 = c_loc(dxi1) ; desc%idx1 = c_loc(idx1) ; desc%idxi1 = c_loc(idxi1)
- desc%x2 = c_loc(x2) ; desc%xi2 = c_loc(xi2) ; desc%dx2 = c_loc(dx2)
- desc%dxi2 = c_loc(dxi2) ; desc%idx2 = c_loc(idx2) ; desc%idxi2 = c_loc(idxi2)
- desc%x3 = c_loc(x3) ; desc%xi3 = c_loc(xi3) ; desc%dx3 = c_loc(dx3)
- desc%dxi3 = c_loc(dxi3) ; desc%idx3 = c_loc(idx3) ; desc%idxi3 = c_loc(idxi3)
  desc%crdnt = crdnt ; desc%eostype = eostype ; desc%compswitch = compswitch
  desc%mag_on = merge(1,0,mag_on) ; desc%rktype = rktype
  select case(trim(flux_limiter))
@@ -177,29 +171,30 @@ program hg_driver
  desc%maxtngrv = maxtngrv ; desc%precision = HG_PREC_FP64 ; desc%device = device
  desc%spn = spn
 
- st = hg_create(desc, ctx)
+ st = hg_create_cartesian(desc, ctx, x1, xi1, dx1, dxi1, idx1, idxi1, &
+                          x2, xi2, dx2, dxi2, idx2, idxi2, x3, xi3, dx3, dxi3, idx3, idxi3)
  call hg_check(ctx, st, 'hg_create')
  if(fused/=0)then
   st = hg_set_fused(ctx, int(fused, c_int32_t)) ; call hg_check(ctx, st, 'hg_set_fused')
  end if
- st = hg_upload(ctx, HG_F_D, c_loc(d))   ; call hg_check(ctx, st, 'upload')
- st = hg_upload(ctx, HG_F_V1, c_loc(v1)) ; call hg_check(ctx, st, 'upload')
- st = hg_upload(ctx, HG_F_V2, c_loc(v2)) ; call hg_check(ctx, st, 'upload')
- st = hg_upload(ctx, HG_F_V3, c_loc(v3)) ; call hg_check(ctx, st, 'upload')
- st = hg_upload(ctx, HG_F_E, c_loc(e))   ; call hg_check(ctx, st, 'upload')
+ st = hg_upload(ctx, HG_F_D, d)   ; call hg_check(ctx, st, 'upload')
+ st = hg_upload(ctx, HG_F_V1, v1) ; call hg_check(ctx, st, 'upload')
+ st = hg_upload(ctx, HG_F_V2, v2) ; call hg_check(ctx, st, 'upload')
+ st = hg_upload(ctx, HG_F_V3, v3) ; call hg_check(ctx, st, 'upload')
+ st = hg_upload(ctx, HG_F_E, e)   ; call hg_check(ctx, st, 'upload')
  if(mag_on)then
-  st = hg_upload(ctx, HG_F_B1, c_loc(b1)) ; call hg_check(ctx, st, 'upload')
-  st = hg_upload(ctx, HG_F_B2, c_loc(b2)) ; call hg_check(ctx, st, 'upload')
-  st = hg_upload(ctx, HG_F_B3, c_loc(b3)) ; call hg_check(ctx, st, 'upload')
-  st = hg_upload(ctx, HG_F_PHI, c_loc(phi)) ; call hg_check(ctx, st, 'upload')
+  st = hg_upload(ctx, HG_F_B1, b1) ; call hg_check(ctx, st, 'upload')
+  st = hg_upload(ctx, HG_F_B2, b2) ; call hg_check(ctx, st, 'upload')
+  st = hg_upload(ctx, HG_F_B3, b3) ; call hg_check(ctx, st, 'upload')
+  st = hg_upload(ctx, HG_F_PHI, phi) ; call hg_check(ctx, st, 'upload')
  end if
  do n = 1, ufnmax
-  st = hg_upload(ctx, HG_F_U+n-1, c_loc(u(is-2,js-2,ks-2,n)))
+  st = hg_upload(ctx, HG_F_U+n-1, u(:,:,:,n))
   call hg_check(ctx, st, 'upload')
  end do
  if(gravswitch==3)then
-  st = hg_upload(ctx, HG_F_GRVPHI, c_loc(grvphi)) ; call hg_check(ctx, st, 'upload')
-  st = hg_upload(ctx, HG_F_GRVPSI, c_loc(grvpsi)) ; call hg_check(ctx, st, 'upload')
+  st = hg_upload(ctx, HG_F_GRVPHI, grvphi) ; call hg_check(ctx, st, 'upload')
+  st = hg_upload(ctx, HG_F_GRVPSI, grvpsi) ; call hg_check(ctx, st, 'upload')
  end if
 
  ! main loop, hormone.f90:124-179 (hydro only)
@@ -237,16 +232,16 @@ program hg_driver
   end select
  end do main_loop
 
- st = hg_download(ctx, HG_F_D, c_loc(d))   ; call hg_check(ctx, st, 'download')
- st = hg_download(ctx, HG_F_V1, c_loc(v1)) ; call hg_check(ctx, st, 'download')
- st = hg_download(ctx, HG_F_V2, c_loc(v2)) ; call hg_check(ctx, st, 'download')
- st = hg_download(ctx, HG_F_V3, c_loc(v3)) ; call hg_check(ctx, st, 'download')
- st = hg_download(ctx, HG_F_E, c_loc(e))   ; call hg_check(ctx, st, 'download')
+ st = hg_download(ctx, HG_F_D, d)   ; call hg_check(ctx, st, 'download')
+ st = hg_download(ctx, HG_F_V1, v1) ; call hg_check(ctx, st, 'download')
+ st = hg_download(ctx, HG_F_V2, v2) ; call hg_check(ctx, st, 'download')
+ st = hg_download(ctx, HG_F_V3, v3) ; call hg_check(ctx, st, 'download')
+ st = hg_download(ctx, HG_F_E, e)   ; call hg_check(ctx, st, 'download')
  if(mag_on)then
-  st = hg_download(ctx, HG_F_B1, c_loc(b1)) ; call hg_check(ctx, st, 'download')
-  st = hg_download(ctx, HG_F_B2, c_loc(b2)) ; call hg_check(ctx, st, 'download')
-  st = hg_download(ctx, HG_F_B3, c_loc(b3)) ; call hg_check(ctx, st, 'download')
-  st = hg_download(ctx, HG_F_PHI, c_loc(phi)) ; call hg_check(ctx, st, 'download')
+  st = hg_download(ctx, HG_F_B1, b1) ; call hg_check(ctx, st, 'download')
+  st = hg_download(ctx, HG_F_B2, b2) ; call hg_check(ctx, st, 'download')
+  st = hg_download(ctx, HG_F_B3, b3) ; call hg_check(ctx, st, 'download')
+  st = hg_download(ctx, HG_F_PHI, phi) ; call hg_check(ctx, st, 'download')
  end if
  st = hg_destroy(ctx)
 

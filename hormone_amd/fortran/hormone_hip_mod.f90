@@ -6,7 +6,10 @@
 ! hyperbolic solver (src/main/gravity.f90:18-64), hg_hydro_step replaces
 ! hydro_step (src/main/hydro.f90:15-52) and hg_phidamp replaces phidamp
 ! (src/main/source.f90:239-260); hg_run runs the hydro-only loop on the
-! device.
+! device, hg_run_gravity the loop with hyperbolic gravity.  Field arrays are
+! passed by reference (the reference's allocatable module arrays need no
+! TARGET attribute); an array section such as u(:,:,:,n) or spc(n,:,:,:)
+! is passed as a contiguous copy where it is not contiguous itself.
 module hormone_hip_mod
  use iso_c_binding
  implicit none
@@ -20,7 +23,7 @@ module hormone_hip_mod
  integer(c_int),parameter:: HG_F_D=0, HG_F_V1=1, HG_F_V2=2, HG_F_V3=3, HG_F_E=4, &
                             HG_F_B1=5, HG_F_B2=6, HG_F_B3=7, HG_F_PHI=8, &
                             HG_F_EINT=9, HG_F_P=10, HG_F_GRVPHI=11, HG_F_GRVPSI=12, &
-                            HG_F_U=16
+                            HG_F_U=16, HG_F_SPC=32
 
  ! mirrors struct hg_desc field by field
  type,bind(C):: hg_desc
@@ -57,18 +60,18 @@ module hormone_hip_mod
   end function hg_ufnmax
 
   function hg_upload(ctx, field, padded) bind(C, name='hg_upload') result(st)
-   import:: c_int, c_int32_t, c_ptr
+   import:: c_int, c_int32_t, c_ptr, c_double
    type(c_ptr),value:: ctx
    integer(c_int32_t),value:: field
-   type(c_ptr),value:: padded
+   real(c_double),intent(in):: padded(*)
    integer(c_int):: st
   end function hg_upload
 
   function hg_download(ctx, field, padded) bind(C, name='hg_download') result(st)
-   import:: c_int, c_int32_t, c_ptr
+   import:: c_int, c_int32_t, c_ptr, c_double
    type(c_ptr),value:: ctx
    integer(c_int32_t),value:: field
-   type(c_ptr),value:: padded
+   real(c_double),intent(inout):: padded(*)
    integer(c_int):: st
   end function hg_download
 
@@ -108,6 +111,23 @@ module hormone_hip_mod
    integer(c_int):: st
   end function hg_gravity
 
+  ! the main loop with hyperbolic gravity on the device (hormone.f90:124-179
+  ! with gravswitch 3); grvtime and cgrav_old are the gravmod values
+  function hg_run_gravity(ctx, time, tn, t_out, dt_out, t_end, outstyle, endstyle, tnlim, &
+                          tn_out, max_steps, damp_factor, grvtime, cgrav_old, nsub_total, &
+                          stop_reason) bind(C, name='hg_run_gravity') result(st)
+   import:: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+   type(c_ptr),value:: ctx
+   real(c_double),intent(inout):: time, t_out, grvtime, cgrav_old
+   integer(c_int64_t),intent(inout):: tn
+   real(c_double),value:: dt_out, t_end, damp_factor
+   integer(c_int32_t),value:: outstyle, endstyle
+   integer(c_int64_t),value:: tnlim, tn_out, max_steps
+   integer(c_int64_t),intent(out):: nsub_total
+   integer(c_int32_t),intent(out):: stop_reason
+   integer(c_int):: st
+  end function hg_run_gravity
+
   ! hydro-only main loop on the device (hormone.f90:124-179 without gravity)
   function hg_run(ctx, time, tn, t_out, dt_out, t_end, outstyle, endstyle, tnlim, &
                   tn_out, max_steps, damp_factor, stop_reason) &
@@ -143,10 +163,10 @@ module hormone_hip_mod
   ! ghost values of the faces with codes 4, 5, 9, 10 (for Dirichlet faces:
   ! the d0, p0, v10.. arrays, after each dirichletbound)
   function hg_set_ghosts(ctx, field, padded) bind(C, name='hg_set_ghosts') result(st)
-   import:: c_int, c_int32_t, c_ptr
+   import:: c_int, c_int32_t, c_ptr, c_double
    type(c_ptr),value:: ctx
    integer(c_int32_t),value:: field
-   type(c_ptr),value:: padded
+   real(c_double),intent(in):: padded(*)
    integer(c_int):: st
   end function hg_set_ghosts
 
@@ -170,6 +190,35 @@ module hormone_hip_mod
  end interface
 
 contains
+
+ ! hg_create for the Cartesian grid module's coordinate arrays: the caller
+ ! passes x1(is-2), xi1(is-2), ... (or the whole arrays, which start there)
+ ! by reference, so its arrays need no TARGET attribute; the library copies
+ ! them during hg_create and keeps no pointer to them.
+ function hg_create_cartesian(desc, ctx, x1, xi1, dx1, dxi1, idx1, idxi1, &
+                              x2, xi2, dx2, dxi2, idx2, idxi2, &
+                              x3, xi3, dx3, dxi3, idx3, idxi3) result(st)
+  type(hg_desc),intent(inout):: desc
+  type(c_ptr),intent(out):: ctx
+  real(c_double),target,intent(in):: x1(*), xi1(*), dx1(*), dxi1(*), idx1(*), idxi1(*)
+  real(c_double),target,intent(in):: x2(*), xi2(*), dx2(*), dxi2(*), idx2(*), idxi2(*)
+  real(c_double),target,intent(in):: x3(*), xi3(*), dx3(*), dxi3(*), idx3(*), idxi3(*)
+  integer(c_int):: st
+  desc%x1 = c_loc(x1) ; desc%xi1 = c_loc(xi1) ; desc%dx1 = c_loc(dx1)
+  desc%dxi1 = c_loc(dxi1) ; desc%idx1 = c_loc(idx1) ; desc%idxi1 = c_loc(idxi1)
+  desc%x2 = c_loc(x2) ; desc%xi2 = c_loc(xi2) ; desc%dx2 = c_loc(dx2)
+  desc%dxi2 = c_loc(dxi2) ; desc%idx2 = c_loc(idx2) ; desc%idxi2 = c_loc(idxi2)
+  desc%x3 = c_loc(x3) ; desc%xi3 = c_loc(xi3) ; desc%dx3 = c_loc(dx3)
+  desc%dxi3 = c_loc(dxi3) ; desc%idx3 = c_loc(idx3) ; desc%idxi3 = c_loc(idxi3)
+  st = hg_create(desc, ctx)
+  ! the addresses are not valid past this call
+  desc%x1 = c_null_ptr ; desc%xi1 = c_null_ptr ; desc%dx1 = c_null_ptr
+  desc%dxi1 = c_null_ptr ; desc%idx1 = c_null_ptr ; desc%idxi1 = c_null_ptr
+  desc%x2 = c_null_ptr ; desc%xi2 = c_null_ptr ; desc%dx2 = c_null_ptr
+  desc%dxi2 = c_null_ptr ; desc%idx2 = c_null_ptr ; desc%idxi2 = c_null_ptr
+  desc%x3 = c_null_ptr ; desc%xi3 = c_null_ptr ; desc%dx3 = c_null_ptr
+  desc%dxi3 = c_null_ptr ; desc%idx3 = c_null_ptr ; desc%idxi3 = c_null_ptr
+ end function hg_create_cartesian
 
  ! Print the reference's message for a hot-path error and stop, as the
  ! reference does at numflux.f90:579-593 and eos.f90:833-839.

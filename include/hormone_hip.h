@@ -18,7 +18,9 @@
  * Array convention: a "padded" array is one 3D field laid out exactly like
  * the reference's allocations (src/main/allocations.f90:56-57,212): Fortran
  * order, x fastest, index ranges is-2:ie+2, js-2:je+2, ks-2:ke+2 (ghost width
- * 2 in every dimension, active or not).  Pass c_loc(a(is-2,js-2,ks-2)).
+ * 2 in every dimension, active or not): the address of a(is-2,js-2,ks-2),
+ * which a Fortran caller passes by reference (hormone_hip_mod.f90 declares
+ * it real(c_double) padded(*), so the array needs no TARGET attribute).
  */
 #ifndef HORMONE_HIP_H
 #define HORMONE_HIP_H
@@ -84,7 +86,9 @@ typedef struct hg_desc {
      solved when its range holds more than one cell (checksetup.f90:41-44). */
   int32_t is, ie, js, je, ks, ke;
   /* Cartesian coordinate arrays of the grid module (gridset.f90:43-158),
-     each covering indices is-2..ie+2 (resp. j, k): pass c_loc(x1(is-2)). */
+     each covering indices is-2..ie+2 (resp. j, k): the address of
+     x1(is-2); read during hg_create only (hormone_hip_mod.f90's
+     hg_create_cartesian sets them from arrays passed by reference). */
   const double *x1, *xi1, *dx1, *dxi1, *idx1, *idxi1;
   const double *x2, *xi2, *dx2, *dxi2, *idx2, *idxi2;
   const double *x3, *xi3, *dx3, *dxi3, *idx3, *idxi3;
