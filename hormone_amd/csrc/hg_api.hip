@@ -279,8 +279,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
       if (hipMemsetAsync(D.gphi[q], 0, gb, c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
     }
     if ((st = dev_alloc(c, fb, (void **)&D.gpsi))) return fail(st);
-    if ((st = dev_alloc(c, fb, (void **)&D.gphi0))) return fail(st);
-    if ((st = dev_alloc(c, fb, (void **)&D.gpsi0))) return fail(st);
+    if ((st = dev_alloc(c, fb, (void **)&D.gpsi1))) return fail(st);
     if (hipMemsetAsync(D.gpsi, 0, fb, c->stream) != hipSuccess) return fail(HG_ERR_DEVICE);
     D.gcur = 0;
     // dtg_unit = min over cells of dtgrav_cell with cgrav = 1 (tools.f90:154-184)
@@ -572,8 +571,9 @@ extern "C" int hg_profile_read(const hg_ctx *c, int32_t kind, double *ms_total,
     case HG_PROF_CFL:
       bytes = cell * (double)(D.nw - (D.mag ? 2 : 1));
       break;
-    case HG_PROF_GRAV:  // read phi, psi, phi0, psi0, rho; write phi, psi
-      bytes = cell * 7.0;
+    case HG_PROF_GRAV:  // per stage, averaged: stage 1 reads phi, psi, rho and writes
+                        // phi1, psi1; stage 2 also reads phi, psi at the cell
+      bytes = cell * 6.0;
       break;
     default:
       bytes = 0.0;
@@ -675,7 +675,12 @@ static int enqueue_gravity(hg_ctx *c, double time, double dt, double *grvtime, d
       runge_coeff(g, gk, &faco, &fact, &facn);
       launch_grav_stage(c->D, g, faco, fact, facn, *dtgrav, cgrav, g == gk, damp, c->stream,
                         prof_ev(c, HG_PROF_GRAV));
+    }
+    // one stage leaves the new values in the second buffers (k_grav_stage);
+    // two write them back over the first
+    if (gk == 1) {
       c->D.gcur = 1 - c->D.gcur;
+      std::swap(c->D.gpsi, c->D.gpsi1);
     }
     *grvtime = *grvtime + *dtgrav;
     *cgrav_old = cgrav;
@@ -1066,14 +1071,18 @@ extern "C" int hg_last_error(const hg_ctx *c, int32_t *i, int32_t *j, int32_t *k
   return HG_OK;
 }
 
+// values per record of each hg_kat kind (kind + 16: the same counts)
+static const int kat_nin[8] = {25, 5, 8, 5, 2, 5, 3, 9}, kat_nout[8] = {9, 1, 1, 1, 2, 4, 2, 1};
+
+static int kat_kind(int32_t which) { return which >= 16 ? which - 16 : which; }
+
 extern "C" int hg_kat(int32_t which, int32_t n, const double *in, double *out) {
-  static const int nin[8] = {25, 5, 8, 5, 2, 5, 3, 9}, nout[8] = {9, 1, 1, 1, 2, 4, 2, 1};
-  const int w = which >= 16 ? which - 16 : which;
+  const int w = kat_kind(which);
   if (w < 0 || w > 7 || n < 0 || !in || !out) return HG_ERR_ARG;
   if (n == 0) return HG_OK;
   double *din = nullptr, *dout = nullptr;
-  const size_t bi = sizeof(double) * (size_t)n * nin[w];
-  const size_t bo = sizeof(double) * (size_t)n * nout[w];
+  const size_t bi = sizeof(double) * (size_t)n * kat_nin[w];
+  const size_t bo = sizeof(double) * (size_t)n * kat_nout[w];
   if (hipMalloc((void **)&din, bi) != hipSuccess) return HG_ERR_NOMEM;
   if (hipMalloc((void **)&dout, bo) != hipSuccess) {
     hipFree(din);
@@ -1090,6 +1099,7 @@ extern "C" int hg_kat(int32_t which, int32_t n, const double *in, double *out) {
   hipFree(dout);
   return st;
 }
+
 
 // =============================================================================
 // Batches: many independent boxes advanced by one set of launches per step.

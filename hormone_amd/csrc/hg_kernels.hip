@@ -1246,8 +1246,14 @@ __global__ __launch_bounds__(256) void k_pressure(Dev D) {
 }
 
 // ---- hyperbolic self-gravity stage (gravity_hyperbolic.f90:61-125) --------
-// Reads the potential from gphi[in] (ghost layer = Dirichlet boundary values)
-// and writes the new potential to gphi[1-in]; psi is updated in place.
+// The potential is stored twice with its ghost layer (Dirichlet boundary
+// values), psi twice without.  Stage 1 (grungen 1) reads phi0 from gphi[in]
+// and psi0 from gpsi and writes phi1 to gphi[1-in] and psi1 to gpsi1; stage
+// 2 reads phi1 and psi1 for the update and phi0, psi0 at the cell itself
+// for the RK combination, and writes phi2 and psi2 over phi0 and psi0 (each
+// thread overwrites only its own cell, which no other thread reads in this
+// stage).  So the stage-1 copies grvphiorg of the reference are never
+// written: 40 + 56 bytes per cell and sub-step.
 // lap_coeff for crdnt 0 (new; the reference has no Cartesian branch):
 // c0 = -((idx1(i)+idx1(i+1))*dxi2*dxi3 + (idx2(j)+idx2(j+1))*dxi1*dxi3 +
 // (idx3(k)+idx3(k+1))*dxi1*dxi2), c1(i) = dxi2*dxi3*idx1(i+1), c2, c3
@@ -1261,8 +1267,11 @@ __global__ __launch_bounds__(256) void k_grav_stage(Dev D, int grungen, int in, 
   if (m >= D.ncell) return;
   const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2),
             k = (int)(m / ((long)D.n1 * D.n2));
-  const double *ph = D.gphi[in];
-  double *out = D.gphi[1 - in];
+  const bool s1 = grungen == 1;
+  const double *ph = D.gphi[s1 ? in : 1 - in];  // the potential of this stage
+  double *out = D.gphi[s1 ? 1 - in : in];
+  const double *psi = s1 ? D.gpsi : D.gpsi1;
+  double *psi_out = s1 ? D.gpsi1 : D.gpsi;
   const long g = gidx(D, i, j, k);
   const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
   const double *X1 = D.ax[0].dxi, *X2 = D.ax[1].dxi, *X3 = D.ax[2].dxi;
@@ -1282,21 +1291,14 @@ __global__ __launch_bounds__(256) void k_grav_stage(Dev D, int grungen, int in, 
   }
   const double lapphi = lap / (X1[i] * X2[j] * X3[k]);
   const double hgsrc = D.fourpig * D.W[WD][m];
-  const double p = ph[g], q = D.gpsi[m];
-  double p0, q0;
-  if (grungen == 1) {
-    p0 = p;
-    q0 = q;
-    D.gphi0[m] = p;
-    D.gpsi0[m] = q;
-  } else {
-    p0 = D.gphi0[m];
-    q0 = D.gpsi0[m];
-  }
+  const double p = ph[g], q = psi[m];
+  // grvphiorg(:,:,:,1:2): at stage 1 the values just read
+  const double p0 = s1 ? p : out[g];
+  const double q0 = s1 ? q : psi_out[m];
   out[g] = faco * p0 + facn * p + fact * dtg * cg * q;
   double qn = faco * q0 + facn * q + fact * dtg * cg * (lapphi - hgsrc);
   if (DAMP) qn = qn * damp;
-  D.gpsi[m] = qn;
+  psi_out[m] = qn;
 }
 
 // ---- CFL condition as its own pass (the reference's timestep) ------------

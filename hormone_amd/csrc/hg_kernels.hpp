@@ -95,12 +95,13 @@ struct Dev {
   int lds_max;     // LDS bytes one block may use on this device
   int sweep_bw;    // sweep wavefronts per block (1, 2, 4, 8 or 16)
   // hyperbolic self-gravity (gravswitch 3): the potential is stored with a
-  // one-cell ghost layer ((n1+2)(n2+2)(n3+2), Dirichlet values), twice for
-  // the Jacobi-style stage update; psi and the stage-1 copies are interior.
+  // one-cell ghost layer ((n1+2)(n2+2)(n3+2), Dirichlet values), twice:
+  // gphi[gcur] holds the current potential, the other the stage-1 one; psi
+  // (interior) likewise in gpsi and gpsi1 (k_grav_stage).
   int mixed;       // HG_PREC_FP32MIX
   double damp;     // phidamp factor exp(-alpha9wave*courant), evaluated on the host
   int grav;
-  double *gphi[2], *gpsi, *gphi0, *gpsi0;
+  double *gphi[2], *gpsi, *gpsi1;
   int gcur;          // which gphi buffer holds the current potential
   double fourpig;    // 4*pi*G
 };

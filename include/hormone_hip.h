@@ -333,6 +333,7 @@ int hg_last_error(const hg_ctx *ctx, int32_t *i, int32_t *j, int32_t *k,
    reference. */
 int hg_kat(int32_t which, int32_t n, const double *in, double *out);
 
+
 #ifdef __cplusplus
 }
 #endif

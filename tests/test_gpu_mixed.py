@@ -43,42 +43,66 @@ def _mixed_vs_fp64(name, steps, kw, floor=False):
     r.advance(steps)
     mine = D.fetch(eng, g, _fields(prm))
     prm6, g6, e6, r6 = _start(name, "fp64", kw)
-    B.advance_to(r6, r.tim)
+    r6.advance_to(r.tim)
     ref = D.fetch(e6, g6, _fields(prm))
     out = {"mixed": D.rel_l1_errors(mine, ref)}
     if floor:
         prm7, g7, e7, r7 = _start(name, "fp64", kw, perturb=1e-14)
-        B.advance_to(r7, r.tim)
+        r7.advance_to(r.tim)
         out["floor"] = D.rel_l1_errors(D.fetch(e7, g7, _fields(prm)), ref)
     return out
+
+
+# The target: relative L1 <= 1e-5 per variable against FP64.
+TARGET = 1e-5
+# phi (the GLM cleaning field) is noise-level; its relative L1 is not a
+# meaningful accuracy figure and is not asserted.
+CHECKED = ("d", "v1", "v2", "v3", "e", "b1", "b2", "b3")
 
 
 @pytest.mark.parametrize("name,steps,kw", [
     ("kh3d", 60, dict(n=48, steps=60)),
     ("polytrope3d", 40, dict(n=48, steps=40)),
+    ("polytrope3d", 40, dict(n=48, steps=40, mag_on=True)),
+    ("sodshock_x", 300, dict(n=1024, courant=0.4)),
 ])
 def test_mixed_error_at_rounding_floor(hiplib, name, steps, kw):
-    print(f"case mixed floor {name}", flush=True)
+    """Every variable: the mixed error is within 3x what FP64 itself shows
+    under a 1e-14 density perturbation, and within the 1e-5 target wherever
+    that FP64 floor is below it.  Measured (mixed / floor): KH 48^3 60
+    steps d 1.9e-6/2.0e-6, v 4.0-7.9e-6/3.9-7.7e-6, e 3.3e-6/3.3e-6, B
+    4.3e-6-1.06e-5/4.2e-6-1.05e-5; polytrope 48^3 40 steps d 3.3e-4/1.7e-4,
+    v 2.4e-2/2.0e-2, e 1.9e-4/9.3e-5; Sod 1024 300 steps d 9.2e-5/1.1e-4,
+    v1 7.7e-4/1.9e-3, e 2.9e-5/7.1e-5."""
+    print(f"case mixed floor {name} {kw}", flush=True)
     res = _mixed_vs_fp64(name, steps, kw, floor=True)
     print(res, flush=True)
-    for q in ("d", "e"):
-        assert res["mixed"][q] <= 3.0 * res["floor"][q] + 1e-9, (q, res)
+    for q in CHECKED:
+        if q not in res["mixed"]:
+            continue
+        m, f = res["mixed"][q], res["floor"][q]
+        assert m <= 3.0 * f + 1e-12, (q, res)
+        if f < TARGET:
+            assert m <= TARGET, (q, res)
 
 
 @pytest.mark.parametrize("name,steps,kw,tol", [
-    ("orszagtang_xy", 20, dict(n=128), 2e-4),
-    ("sodshock_x", 300, dict(n=1024, courant=0.4), 1e-3),
+    # smooth early-time flow: measured 1.4e-10 .. 1.9e-10 per variable
+    ("orszagtang_xy", 20, dict(n=128), 1e-9),
     # linear boundary codes on the z faces (the mixed sweep's own
-    # instantiation for codes 4, 5, 9, 10)
+    # instantiation for codes 4, 5, 9, 10): measured 3.3e-10 .. 5.1e-10
     ("orszagtang_yz", 30, dict(n=64, bc=(0, 0, 0, 0, 4, 4, 0, 0, 0, 0, 4, 5), outstyle=2,
-                               endstyle=2, tnlim=40, tn_out=40), 1e-3),
+                               endstyle=2, tnlim=40, tn_out=40), 3e-9),
 ])
 def test_mixed_close_to_fp64(hiplib, name, steps, kw, tol):
+    """Smooth flows, before the limiters' switches amplify rounding: every
+    variable within `tol` (far inside the 1e-5 target)."""
     print(f"case mixed {name}", flush=True)
     res = _mixed_vs_fp64(name, steps, kw)
     print(res, flush=True)
-    for q in ("d", "e"):
-        assert res["mixed"][q] < tol, (q, res)
+    for q in CHECKED:
+        if q in res["mixed"]:
+            assert res["mixed"][q] < tol, (q, res)
 
 
 def test_mixed_conserves_mass_to_fp64_rounding(hiplib):
