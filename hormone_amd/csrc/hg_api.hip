@@ -81,6 +81,11 @@ static int validate(const hg_desc *d) {
     return HG_ERR_UNSUPPORTED;
   if (d->compswitch == 2 && (d->spn < 1 || d->spn > HG_SPN_MAX)) return HG_ERR_ARG;
   if (d->ie < d->is || d->je < d->js || d->ke < d->ks) return HG_ERR_ARG;
+  // the kernels index cells, sweep positions and the potential's padded
+  // grid in 32 bits (4.3e9 elements: far beyond one device's memory at
+  // ~50 FP64 fields per cell)
+  if ((double)(d->ie - d->is + 5) * (d->je - d->js + 5) * (d->ke - d->ks + 5) >= 4294967295.0)
+    return HG_ERR_UNSUPPORTED;
   if (d->rktype < 1 || d->rktype > 3) return HG_ERR_ARG;
   if (d->flux_limiter < 0 || d->flux_limiter > 2) return HG_ERR_ARG;
   for (int b = 0; b < 12; ++b)
@@ -179,6 +184,8 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
       A.idxi = dp[5];
       A.n = nn[a];
       A.nlines = (unsigned)(D.ncell / nn[a]);
+      // x lines form one sweep sequence, y and z lines one per column i
+      A.nsub = a == 0 ? A.nlines : A.nlines / (unsigned)D.n1;
       A.stride = a == 0 ? 1 : (a == 1 ? (long)D.n1 : (long)D.n1 * D.n2);
       A.sc_lo = desc->bc[2 * a];
       A.sc_hi = desc->bc[2 * a + 1];
@@ -254,7 +261,7 @@ extern "C" int hg_create(const hg_desc *desc, hg_ctx **out) {
   D.nblk_cfl = (int)((D.ncell + 255) / 256);
   D.sweep_bw = 4;
   for (int a = 0; a < 3; ++a)
-    D.waves[a] = D.solve[a] ? (long)((nn[a] + 59) / 60) * (D.ncell / nn[a]) : 0;
+    D.waves[a] = D.solve[a] ? sweep_waves(a == 0 ? 1 : D.n1, D.ax[a].nsub, nn[a]) : 0;
   // y and z wavefront ranges start on a multiple of 16 (the largest block),
   // so a sweep block never holds two directions
   D.woff[0] = 0;
@@ -533,6 +540,12 @@ static void prof_collect(hg_ctx *c) {
     if (hipEventElapsedTime(&ms, c->ev[2 * q], c->ev[2 * q + 1]) == hipSuccess) {
       c->prof_ms[c->ev_kind[q]] += ms;
       c->prof_n[c->ev_kind[q]] += 1;
+    } else {
+      // a pair that was never recorded is dropped, and so is its status: HIP
+      // keeps the last failed call's status for the next hipGetLastError,
+      // where it would surface as a failed launch (HG_ERR_DEVICE) of a later
+      // step
+      (void)hipGetLastError();
     }
   }
   c->ev_used = 0;
@@ -1375,3 +1388,4 @@ extern "C" int hg_batch_config(hg_batch *b, int32_t sweep_block_waves, int32_t s
   b->dirty = true;
   return HG_OK;
 }
+

@@ -5,12 +5,13 @@
 // difference), then one "rk" kernel (SSP-RK combination, primitive recovery,
 // e_int check).  The timestep kernel reduces the CFL condition.
 //
-// Sweep mapping: a wavefront owns a line segment of 64 consecutive cells
-// along the sweep direction (cells c0-2 .. c0+61); lane l holds cell c0-2+l.
-// Neighbour values travel by cross-lane shuffles: slopes need l-1 and l+1,
-// the face between l and l+1 needs the right state from l+1, and the flux
-// difference of cell l needs the face flux of l-1.  60 of the 64 lanes
-// produce a result.  Boundary ghosts are evaluated on the fly.
+// Sweep mapping: a wavefront owns 64 consecutive positions of a sequence of
+// lines laid end to end along the sweep direction, each line with its two
+// ghost cells at either end (lane_cell); lane l holds one cell.  Neighbour
+// values travel by cross-lane shuffles: slopes need l-1 and l+1, the face
+// between l and l+1 needs the right state from l+1, and the flux difference
+// of cell l needs the face flux of l-1.  Lanes 2..61 produce a result when
+// they hold an interior cell.  Boundary ghosts are evaluated on the fly.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -18,6 +19,16 @@
 #include "hg_physics.hpp"
 
 namespace hg {
+
+// Workgroups go to the 8 XCDs round robin (workgroup b to XCD b % 8), and
+// each XCD has its own L2.  xcd_block renumbers them so that each XCD takes
+// one contiguous range of the logical blocks (XCD x the x-th range; the
+// first nb % 8 ranges are one block longer): the z neighbours a stencil
+// block reads then sit in the same L2 instead of being fetched once per XCD.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (b >> 3);
+}
 
 
 struct Cell {
@@ -233,16 +244,71 @@ __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base
 // line base and segment (-1 for padding) for the stores.
 __device__ __forceinline__ int tile_ld(int bw) { return bw + 1; }
 
-// line base index and segment of wave wc of direction DIR
+// ---- packed sweep lines ----------------------------------------------------
+// The lines of a direction, each extended by its two ghost cells at either
+// end (n+4 positions), lie end to end in sequences: all x lines form one
+// sequence (row order j + k*n2), the y lines of column i one (k order), the
+// z lines of column i another (j order).  Wave w of the direction takes
+// column i = w % n1 (y, z; x has the one sequence) and positions
+// 60*s .. 60*s+63 of it, s = w / n1: lane r holds position 60*s + r.
+// Consecutive waves are neighbouring columns at the same positions, so the
+// rows of a y/z tile stay contiguous in HBM.  Lanes 2..61 of the waves hold
+// every position from 2 on exactly once, and the stencil c-2 .. c+2 of an
+// interior cell c lies inside its own padded line, so a wavefront no longer
+// stops at the end of a line: a 256-cell line costs 260/60 wavefronts
+// instead of 5.
+// lane_cell: the line of position r of wave w (numbered as A.nlines counts
+// them: x j + k*n2, y and z i + (k or j)*n1), its base element and the cell
+// c (-2 .. n+1) of the position; false past the sequence's end, where c is
+// n+1 of the last line and the lane stores nothing.
 template <int DIR>
-__device__ __forceinline__ void wave_line(const Dev &D, const Axis &A, unsigned wc,
-                                          long &base, int &seg) {
-  const unsigned nl = A.nlines;
-  const unsigned line = wc % nl;
-  seg = (int)(wc / nl);
-  if (DIR == 0) base = (long)line * D.n1;
-  else if (DIR == 1) base = (long)(line % (unsigned)D.n1) + (long)(line / (unsigned)D.n1) * D.n1 * D.n2;
-  else base = line;
+__device__ __forceinline__ bool lane_cell(const Dev &D, const Axis &A, long w, int r,
+                                          long &line, long &base, int &c) {
+  const unsigned n4 = (unsigned)A.n + 4u;
+  unsigned col = 0, s = (unsigned)w;
+  if (DIR != 0) {
+    col = (unsigned)w % (unsigned)D.n1;
+    s = (unsigned)w / (unsigned)D.n1;
+  }
+  const unsigned pos = s * (unsigned)SEG + (unsigned)r;
+  unsigned sub = pos / n4;
+  c = (int)(pos - sub * n4) - 2;
+  const bool in = sub < A.nsub;
+  if (!in) {
+    sub = A.nsub - 1;
+    c = A.n + 1;
+  }
+  if (DIR == 0) {
+    line = sub;
+    base = (long)sub * D.n1;
+  } else {
+    line = (long)col + (long)sub * D.n1;
+    base = DIR == 1 ? (long)col + (long)sub * D.n1 * D.n2 : line;
+  }
+  return in;
+}
+
+// cell-0 coordinates of a line (error reports)
+template <int DIR>
+__device__ __forceinline__ void line_origin(const Dev &D, long line, int &ci, int &cj, int &ck) {
+  if (DIR == 0) {
+    ci = 0; cj = (int)(line % D.n2); ck = (int)(line / D.n2);
+  } else if (DIR == 1) {
+    ci = (int)(line % D.n1); cj = 0; ck = (int)(line / D.n1);
+  } else {
+    ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
+  }
+}
+
+// (i, j, k) of element m of a field; 32-bit arithmetic (hg_create keeps
+// every box below 2^32 elements)
+__device__ __forceinline__ void cell_ijk(const Dev &D, long m, int &i, int &j, int &k) {
+  const unsigned u = (unsigned)m, n1 = (unsigned)D.n1, n2 = (unsigned)D.n2;
+  const unsigned q = u / n1;
+  const unsigned kk = q / n2;
+  i = (int)(u - q * n1);
+  j = (int)(q - kk * n2);
+  k = (int)kk;
 }
 
 template <int DIR, bool MAG, bool EXT = true>
@@ -254,19 +320,17 @@ __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave
     const int t = threadIdx.x, r = t / bw, col = t - r * bw;
     const long last = D.waves[DIR] - 1;
     const long wc = wave0 + col < last ? wave0 + col : last;
-    long bc;
-    int sg;
-    wave_line<DIR>(D, A, (unsigned)wc, bc, sg);
-    if (r == 0) {  // the column's line and segment for the stores (-1: padding)
+    long ln, bc;
+    int cr;
+    lane_cell<DIR>(D, A, wc, r, ln, bc, cr);
+    if (r == 0) {  // the column's wave for the stores (-1: padding)
       long *cinfo = reinterpret_cast<long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
-      cinfo[2 * col] = bc;
-      cinfo[2 * col + 1] = wave0 + col < D.waves[DIR] ? sg : -1;
+      cinfo[2 * col] = wave0 + col < D.waves[DIR] ? wc : -1;
     }
     double *p = lds + r * ld + col;
-    const int cr = sg * SEG - 2 + r;
     if (EXT && A.ext && (cr < 0 || cr >= A.n)) {
       double f[NWMAX];
-      raw_cell<DIR, MAG>(D, A, bc, (long)((unsigned)wc % A.nlines), cr, f);
+      raw_cell<DIR, MAG>(D, A, bc, ln, cr, f);
       const int nf = MAG ? NWMAX : WEINT + 1;
 #pragma unroll
       for (int k = 0; k < NWMAX; ++k)
@@ -293,6 +357,22 @@ __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave
 #pragma unroll
   for (int k = 0; k < NWMAX; ++k) f[k] = k < nf ? p[k * fs] : 0.0;
   return finish_cell<DIR, MAG>(D, A, c, f);
+}
+
+// Row r of tile column col on the way out: true with its element index m
+// when the row holds an interior cell that this column updates (rows 2..61
+// of a column that is not padding).
+template <int DIR, bool MAG>
+__device__ __forceinline__ bool tile_row_cell(const Dev &D, const Axis &A, const double *lds,
+                                              int fs, int r, int col, long &m) {
+  const long *cinfo = reinterpret_cast<const long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
+  const long wc = cinfo[2 * col];
+  if (wc < 0 || r < 2 || r > 61) return false;  // padding column, halo row
+  long ln, bc;
+  int cr;
+  lane_cell<DIR>(D, A, wc, r, ln, bc, cr);
+  m = bc + (long)cr * A.stride;
+  return cr >= 0 && cr < A.n;
 }
 
 // Flux-divergence stores of a line: du[v] is the value of conserved
@@ -323,13 +403,8 @@ __device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base,
   }
   __syncthreads();
   const int t = threadIdx.x, r = t / bw, col = t - r * bw;
-  const long *cinfo = reinterpret_cast<const long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
-  const long bc = cinfo[2 * col];
-  const int sg = (int)cinfo[2 * col + 1];
-  if (sg < 0) return;  // padding column
-  const int cr = sg * SEG - 2 + r;
-  if (r >= 2 && r <= 61 && cr >= 0 && cr < n) {
-    const long m = bc + (long)cr * A.stride;
+  long m;
+  if (tile_row_cell<DIR, MAG>(D, A, lds, fs, r, col, m)) {
     const double *p = lds + r * ld + col;
 #pragma unroll
     for (int v = 0; v < NVMAX; ++v)
@@ -515,29 +590,15 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
                                            double *dti = nullptr, double *cfm = nullptr) {
   const Axis &A = D.ax[DIR];
   const int n = A.n;
-  const long nlines = D.ncell / n;
   // a tiled block's padding wavefronts (past the direction's last wave) take
-  // part in the block's loads and barriers on a copy of the last line and
+  // part in the block's loads and barriers on a copy of the last wave and
   // store and report nothing
   const bool valid = !TILE || wave < D.waves[DIR];
   const long wave0 = wave - (threadIdx.x >> 6);  // first wave of the block (tiled)
   if (!valid) wave = D.waves[DIR] - 1;
-  const long line = wave % nlines;
-  const int seg = (int)(wave / nlines);
-  long base;
-  int ci, cj, ck;  // cell-0 coordinates of the line (for error reports)
-  if (DIR == 0) {
-    base = line * D.n1;
-    ci = 0; cj = (int)(line % D.n2); ck = (int)(line / D.n2);
-  } else if (DIR == 1) {
-    const long i = line % D.n1, k = line / D.n1;
-    base = i + k * (long)D.n1 * D.n2;
-    ci = (int)i; cj = 0; ck = (int)k;
-  } else {
-    base = line;
-    ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
-  }
-  const int c = seg * SEG - 2 + lane;
+  long line, base;
+  int c;
+  lane_cell<DIR>(D, A, wave, lane, line, base, c);
   const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, c, lds)
                       : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
   // d e m1 m2 m3 b1 b2 b3 and eint (slopes) / phi (face) of the cell
@@ -586,7 +647,8 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   // faces between lanes l and l+1 have valid slopes on both sides for l = 1..61
   const bool face_valid = valid && (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61 && !fxb;
   if (face_valid && (fe[1] <= 0.0 || fe[0] <= 0.0 || fe[3] <= 0.0 || fe[2] <= 0.0)) {
-    int ii = ci, jj = cj, kk = ck;
+    int ii, jj, kk;
+    line_origin<DIR>(D, line, ii, jj, kk);
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
     atomicMin(&D.sc->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
   }
@@ -689,29 +751,15 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
                                                double *dti = nullptr, double *cfm = nullptr) {
   const Axis &A = D.ax[DIR];
   const int n = A.n;
-  const long nlines = D.ncell / n;
   // a tiled block's padding wavefronts (past the direction's last wave) take
-  // part in the block's loads and barriers on a copy of the last line and
+  // part in the block's loads and barriers on a copy of the last wave and
   // store and report nothing
   const bool valid = !TILE || wave < D.waves[DIR];
   const long wave0 = wave - (threadIdx.x >> 6);  // first wave of the block (tiled)
   if (!valid) wave = D.waves[DIR] - 1;
-  const long line = wave % nlines;
-  const int seg = (int)(wave / nlines);
-  long base;
-  int ci, cj, ck;
-  if (DIR == 0) {
-    base = line * D.n1;
-    ci = 0; cj = (int)(line % D.n2); ck = (int)(line / D.n2);
-  } else if (DIR == 1) {
-    const long i = line % D.n1, k = line / D.n1;
-    base = i + k * (long)D.n1 * D.n2;
-    ci = (int)i; cj = 0; ck = (int)k;
-  } else {
-    base = line;
-    ci = (int)(line % D.n1); cj = (int)(line / D.n1); ck = 0;
-  }
-  const int c = seg * SEG - 2 + lane;
+  long line, base;
+  int c;
+  lane_cell<DIR>(D, A, wave, lane, line, base, c);
   const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, c, lds)
                       : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
   const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
@@ -791,7 +839,8 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   const bool fxb = EXT && A.ext && flux_face(A, c);
   const bool face_valid = valid && (c >= -1 && c <= n - 1) && lane >= 1 && lane <= 61 && !fxb;
   if (face_valid && (eil <= 0.0 || L[0] <= 0.0 || eir <= 0.0 || R_[0] <= 0.0)) {
-    int ii = ci, jj = cj, kk = ck;
+    int ii, jj, kk;
+    line_origin<DIR>(D, line, ii, jj, kk);
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
     atomicMin(&D.sc->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
   }
@@ -885,14 +934,9 @@ template <int DIR, int LIM>
 __device__ __forceinline__ void species_line(const Dev &D, long wave, int lane) {
   const Axis &A = D.ax[DIR];
   const int n = A.n;
-  const long nlines = D.ncell / n;
-  const long line = wave % nlines;
-  const int seg = (int)(wave / nlines);
-  long base;
-  if (DIR == 0) base = line * D.n1;
-  else if (DIR == 1) base = line % D.n1 + (line / D.n1) * (long)D.n1 * D.n2;
-  else base = line;
-  const int c = seg * SEG - 2 + lane;
+  long line, base;
+  int c;
+  lane_cell<DIR>(D, A, wave, lane, line, base, c);
   const int cc = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
   long ms, mv;
   cell_sources(A, base, c, ms, mv);
@@ -959,8 +1003,8 @@ template <bool MAG>
 __device__ __forceinline__ void cfl_cell(const Dev &D, long m, double d, double eint,
                                          double v1, double v2, double v3, double b1, double b2,
                                          double b3, double &dti, double &cfm) {
-  const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2),
-            k = (int)(m / ((long)D.n1 * D.n2));
+  int i, j, k;
+  cell_ijk(D, m, i, j, k);
   const double gam = D.gamma;
   const double p = (gam - 1.0) * eint;
   const double cs = sqrt(gam * p / d);
@@ -1026,8 +1070,8 @@ __device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, doubl
 // g_a = -(non-uniform central difference of phi) * rho with the stage-start
 // rho and v; src(imo_a) = g_a, src(iene) = g.v.
 __device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVMAX]) {
-  const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2),
-            k = (int)(m / ((long)D.n1 * D.n2));
+  int i, j, k;
+  cell_ijk(D, m, i, j, k);
   const double *ph = D.gphi[D.gcur];
   const long g = gidx(D, i, j, k);
   const long st[3] = {1, (long)(D.n1 + 2), (long)(D.n1 + 2) * (D.n2 + 2)};
@@ -1068,9 +1112,13 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
   if (GRAV) grav_source(D, m, src);
   double un[NVMAX];
   double dold = 0.0;  // stage-start density (primitive d = u(icnt))
+  // a bound known at compile time for hydro, so that the loop unrolls and
+  // un[] stays in registers
+  const int nv = MAG ? D.nv : 5;
 #pragma unroll
   for (int v = 0; v < NVMAX; ++v) {
-    if (v >= D.nv) break;
+    un[v] = 0.0;
+    if (v >= nv) continue;
     // inactive directions contribute idxi*(0-0) = +0 (allocations.f90:214-240)
     const double a = LD == 0 ? duL[v] : (D.solve[0] ? D.DU[0][v][m] : 0.0);
     const double b = LD == 1 ? duL[v] : (D.solve[1] ? D.DU[1][v][m] : 0.0);
@@ -1125,7 +1173,8 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
   const double eint = get_eint(MAG, e, d, v1, v2, v3, b1, b2, b3);
   Wo[WEINT][m] = eint;
   if (eint <= 0.0) {
-    const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
+    int i, j, k;
+    cell_ijk(D, m, i, j, k);
     atomicMin(&D.sc->err, errkey(stage, 1, 0, i, j, k, D));
   }
   if (CFL) cfl_cell<MAG>(D, m, d, eint, v1, v2, v3, b1, b2, b3, dti, cfm);
@@ -1179,17 +1228,14 @@ __device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base,
   }
   __syncthreads();
   const int t = threadIdx.x, r = t / bw, col = t - r * bw;
-  const long *cinfo = reinterpret_cast<const long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
-  const long bc = cinfo[2 * col];
-  const int sg = (int)cinfo[2 * col + 1];
-  const int cr = sg * SEG - 2 + r;
-  if (sg >= 0 && r >= 2 && r <= 61 && cr >= 0 && cr < n) {
+  long m;
+  if (tile_row_cell<DIR, MAG>(D, A, lds, fs, r, col, m)) {
     const double *p = lds + r * ld + col;
     double d2[NVMAX];
 #pragma unroll
     for (int v = 0; v < NVMAX; ++v) d2[v] = v < nv ? p[v * fs] : 0.0;
-    rk_cell_rt<MAG, DIR>(D, bc + (long)cr * A.stride, stage, rk.faco, rk.fact, rk.facn,
-                         rk.damp, rk.cfl, rk.dampf, rk.grav, dti, cfm, d2);
+    rk_cell_rt<MAG, DIR>(D, m, stage, rk.faco, rk.fact, rk.facn, rk.damp, rk.cfl, rk.dampf,
+                         rk.grav, dti, cfm, d2);
   }
 }
 
@@ -1240,7 +1286,8 @@ __global__ __launch_bounds__(256) void k_pressure(Dev D) {
                                D.W[WV3][m], b1, b2, b3);
   D.W[WEINT][m] = eint;
   if (eint <= 0.0) {
-    const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2), k = (int)(m / ((long)D.n1 * D.n2));
+    int i, j, k;
+    cell_ijk(D, m, i, j, k);
     atomicMin(&D.sc->err, errkey(0, 1, 0, i, j, k, D));
   }
 }
@@ -1263,10 +1310,16 @@ template <bool DAMP>
 __global__ __launch_bounds__(256) void k_grav_stage(Dev D, int grungen, int in, double faco,
                                                     double fact, double facn, double dtg,
                                                     double cg, double damp) {
-  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= D.ncell) return;
-  const int i = (int)(m % D.n1), j = (int)((m / D.n1) % D.n2),
-            k = (int)(m / ((long)D.n1 * D.n2));
+  // a block is blockDim.y rows of blockDim.x cells of one z plane; blocks
+  // are numbered x fastest, then y, then z, after the XCD renumbering
+  const unsigned gx = ((unsigned)D.n1 + blockDim.x - 1) / blockDim.x;
+  const unsigned gy = ((unsigned)D.n2 + blockDim.y - 1) / blockDim.y;
+  const unsigned L = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned t = L / gx;
+  const int i = (int)((L - t * gx) * blockDim.x + threadIdx.x);
+  const int j = (int)((t % gy) * blockDim.y + threadIdx.y), k = (int)(t / gy);
+  if (i >= D.n1 || j >= D.n2) return;
+  const long m = ((long)k * D.n2 + j) * D.n1 + i;
   const bool s1 = grungen == 1;
   const double *ph = D.gphi[s1 ? in : 1 - in];  // the potential of this stage
   double *out = D.gphi[s1 ? 1 - in : in];
@@ -1516,6 +1569,7 @@ __global__ __launch_bounds__(256) void k_bstep_end_cfl(Batch B) {
 
 __global__ void k_clear_error(Scalars *sc) { sc->err = NOERR; }
 
+
 __global__ void k_set_step(Scalars *sc, double dt, double ch) {
   sc->dt = dt;
   sc->ch = ch;
@@ -1612,10 +1666,10 @@ void runge_coeff(int stage, int rk, double *faco, double *fact, double *facn) {
 // Launch k<<<g, b, 0, s>>>(args...); with ev the dispatch itself carries the
 // start/stop events.
 template <typename... KA, typename... A>
-static inline void klaunch(void (*k)(KA...), unsigned g, unsigned b, hipStream_t s,
-                           const KEv *ev, A... args) {
+static inline void klaunch(void (*k)(KA...), dim3 g, dim3 b, hipStream_t s, const KEv *ev,
+                           A... args) {
   if (ev)
-    hipExtLaunchKernelGGL(k, dim3(g), dim3(b), 0, s, ev->start, ev->stop, 0, KA(args)...);
+    hipExtLaunchKernelGGL(k, g, b, 0, s, ev->start, ev->stop, 0, KA(args)...);
   else
     k<<<g, b, 0, s>>>(KA(args)...);
 }
@@ -1648,6 +1702,7 @@ void launch_pressure(const Dev &D, hipStream_t s) {
 }
 
 void launch_clear_error(const Dev &D, hipStream_t s) { k_clear_error<<<1, 1, 0, s>>>(D.sc); }
+
 
 void launch_set_step(const Dev &D, double dt, double ch, hipStream_t s) {
   k_set_step<<<1, 1, 0, s>>>(D.sc, dt, ch);
@@ -1804,12 +1859,16 @@ void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, b
 void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, double facn,
                        double dtg, double cg, bool damp, double damp_factor, hipStream_t s,
                        const KEv *ev) {
-  const unsigned g = nblk(D.ncell, 256);
+  // rows of 64 cells, 4 rows per block (one 256-cell row per block when the
+  // box has fewer than 4 rows)
+  const unsigned bx = D.n2 >= 4 ? 64 : 256, by = D.n2 >= 4 ? 4 : 1;
+  const unsigned g = nblk(D.n1, (int)bx) * nblk(D.n2, (int)by) * (unsigned)D.n3;
+  const dim3 b(bx, by);
   if (damp)
-    klaunch(k_grav_stage<true>, g, 256, s, ev, D, grungen, D.gcur, faco, fact, facn, dtg, cg,
+    klaunch(k_grav_stage<true>, g, b, s, ev, D, grungen, D.gcur, faco, fact, facn, dtg, cg,
             damp_factor);
   else
-    klaunch(k_grav_stage<false>, g, 256, s, ev, D, grungen, D.gcur, faco, fact, facn, dtg, cg,
+    klaunch(k_grav_stage<false>, g, b, s, ev, D, grungen, D.gcur, faco, fact, facn, dtg, cg,
             damp_factor);
 }
 

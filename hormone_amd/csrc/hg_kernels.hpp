@@ -24,6 +24,15 @@ enum { WD = 0, WV1, WV2, WV3, WE, WEINT, WB1, WB2, WB3, WPHI, NWMAX };
 enum { NVMAX = 9 };
 static constexpr int SEG = 60;  // cells a 64-lane sweep wavefront updates (2-cell halos)
 
+// Wavefronts of one direction's sweep: ncol sequences of nsub lines laid end
+// to end, each line with its 2 ghost cells at either end (n+4 positions);
+// wave s of a sequence holds positions 60*s .. 60*s+63 and updates those of
+// its lanes 2..61, so the last interior cell, at position nsub*(n+4)-3,
+// needs (nsub*(n+4)-5)/60 + 1 waves.
+__host__ __device__ inline long sweep_waves(long ncol, long nsub, int n) {
+  return ncol * ((nsub * (long)(n + 4) - 5) / SEG + 1);
+}
+
 // stop reasons of the device-resident main loop
 enum { STOP_NONE = 0, STOP_END = 1, STOP_OUTPUT = 2, STOP_MAXSTEPS = 3, STOP_ERROR = 4 };
 
@@ -48,6 +57,7 @@ struct Axis {
   const double *mmw;
   int n;
   unsigned nlines;   // lines along this axis (ncell / n)
+  unsigned nsub;     // lines laid end to end in one sweep sequence (lane_cell)
   long stride;       // element stride of this axis in the SoA arrays
   int sc_lo, sc_hi;  // scalar BC codes (bcNis, bcNos)
   int vc_lo, vc_hi;  // vector BC codes (bcNiv, bcNov)

@@ -126,3 +126,94 @@ def test_batch_remove_and_state_before_any_step(hiplib):
         assert stop == 1 and n == tn and t == tm
         b.remove(eng)
         assert S.state_digest(eng, g) == dg
+
+
+# Golden problems of the reference that a batch can hold (no species, no
+# output stop before the end): one batch per kernel variant, members of
+# mixed dimension and size.
+GOLDEN_BATCHES = {
+    "hydro-muscl": [("sodshock_x", {}, "sodshock_x"), ("sodshock_y", {}, "sodshock_y"),
+                    ("sodshock_z", {}, "sodshock_z"),
+                    ("sodshock_x", dict(n=1024, courant=0.4), "sod1024_c04")],
+    "hydro-flat": [("sodshock_x", {}, "sodshock_x_flat"), ("sodshock_y", {}, "sodshock_y_flat"),
+                   ("sodshock_z", {}, "sodshock_z_flat")],
+    "mhd-muscl": [("briowushock_x", {}, "briowushock_x"), ("briowushock_y", {}, "briowushock_y"),
+                  ("briowushock_z", {}, "briowushock_z"), ("orszagtang_xy", {}, "orszagtang_xy"),
+                  ("orszagtang_xz", {}, "orszagtang_xz"), ("orszagtang_yz", {}, "orszagtang_yz")],
+    "mhd-flat": [("briowushock_y", {}, "briowushock_y_flat"),
+                 ("orszagtang_xy", {}, "orszagtang_xy_flat"),
+                 ("orszagtang_yz", {}, "orszagtang_yz_flat")],
+}
+
+
+@pytest.mark.parametrize("variant", list(GOLDEN_BATCHES))
+def test_batch_of_golden_problems_matches_the_reference(hiplib, variant):
+    """The reference's own golden snapshots, run as members of one batch:
+    every member ends at the golden's step count and time with every field
+    equal to the reference's bit for bit."""
+    from types import SimpleNamespace
+
+    from hormone_amd import problems as P
+    from tests.helpers import NAMES, compare_to_meta, compare_to_npz, meta
+
+    print(f"case batch of goldens {variant}", flush=True)
+    flat = variant.endswith("flat")
+    b = Batch()
+    members = []
+    for name, kw, key in GOLDEN_BATCHES[variant]:
+        prm, g, st = P.make_problem(name, flux_limiter="flat" if flat else "modified_mc", **kw)
+        eng = Hydro(**D.desc_fields(prm, g))
+        D.load_initial_state(eng, prm, g, st)
+        t_end, dt_out, t_out = prm.times()
+        assert t_out == t_end  # no output stop changes dt before the end
+        b.add(eng, t_end=t_end, t_out=t_out, dt_out=dt_out, outstyle=prm.outstyle,
+              endstyle=prm.endstyle, tnlim=prm.tnlim, tn_out=prm.tn_out)
+        members.append((eng, g, prm, key))
+    while b.step(64):
+        pass
+    m = meta()
+    for eng, g, prm, key in members:
+        t, n, stop = b.state(eng)
+        assert stop == 1, (key, stop)
+        b.remove(eng)
+        fields = D.SNAP_FIELDS_MAG if prm.mag_on else D.SNAP_FIELDS
+        out = D.fetch(eng, g, fields)
+        state = {NAMES[i]: out[f] for i, f in enumerate(fields)}
+        res = SimpleNamespace(tn=n, time=t)
+        if key in m and "fields" in m[key] and "sha256" in next(iter(m[key]["fields"].values())):
+            compare_to_meta(res, state, m[key])
+        else:
+            compare_to_npz(res, state, key)
+    b.close()
+
+
+def test_serving_ensemble_at_full_size_range(hiplib):
+    """The serving ensemble over its full size range: a seeded draw of the
+    default generator (64^2 .. 192^3 cells) whose first request is the
+    smallest box (64^2) and whose last the largest (192^3).  Every request
+    served in the mix ends as it does alone (bit for bit, or with the same
+    error at the same step), and the report carries p50/p99."""
+    from hormone_amd.native import HgError
+
+    reqs = S.make_requests(6, rate=50.0, seed=2024)
+    reqs[0].dim, reqs[0].n = 2, 64
+    reqs[-1].dim, reqs[-1].n = 3, 192
+    cells = [r.cells for r in reqs]
+    assert min(cells) == 64 ** 2 and max(cells) == 192 ** 3
+    print("case serving full range, cells", cells, flush=True)
+    rep = S.serve(reqs, hbm_budget=64 * 2 ** 30, steps_per_round=4, tune=True,
+                  keep_digest=True)
+    summ = rep.summary()
+    print("serving full range:", {k: summ[k] for k in ("completed", "failed", "p50_s", "p99_s",
+                                                       "aggregate_zone_updates_per_s")},
+          flush=True)
+    assert summ["completed"] + summ["failed"] == len(reqs)
+    assert summ["completed"] >= 1 and summ["p99_s"] >= summ["p50_s"] > 0.0
+    for r in reqs:
+        print(f"  alone: request {r.rid} ({r.dim}D {r.n}, {r.kind})", flush=True)
+        if r.error is None:
+            tn, tm, dg = S.run_alone(r)
+            assert r.steps == tn and r.digest == dg, f"request {r.rid} differs in the mix"
+        else:
+            with pytest.raises(HgError):
+                S.run_alone(r)
