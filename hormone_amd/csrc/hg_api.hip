@@ -683,18 +683,11 @@ static int enqueue_gravity(hg_ctx *c, double time, double dt, double *grvtime, d
   for (int l = 1; l <= tngrav; ++l) {
     const double damp =
         std::exp(-(1.0 - *cgrav_old / cgrav + c->desc.alphagrv * c->desc.hgcfl));
-    for (int g = 1; g <= gk; ++g) {
-      double faco, fact, facn;
-      runge_coeff(g, gk, &faco, &fact, &facn);
-      launch_grav_stage(c->D, g, faco, fact, facn, *dtgrav, cgrav, g == gk, damp, c->stream,
-                        prof_ev(c, HG_PROF_GRAV));
-    }
-    // one stage leaves the new values in the second buffers (k_grav_stage);
-    // two write them back over the first
-    if (gk == 1) {
-      c->D.gcur = 1 - c->D.gcur;
-      std::swap(c->D.gpsi, c->D.gpsi1);
-    }
+    // one launch per sub-step (both stages of grktype 2 in one); the new
+    // values are in the second buffers
+    launch_grav_substep(c->D, gk, *dtgrav, cgrav, damp, c->stream, prof_ev(c, HG_PROF_GRAV));
+    c->D.gcur = 1 - c->D.gcur;
+    std::swap(c->D.gpsi, c->D.gpsi1);
     *grvtime = *grvtime + *dtgrav;
     *cgrav_old = cgrav;
   }

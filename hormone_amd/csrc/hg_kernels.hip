@@ -23,12 +23,13 @@ namespace hg {
 // Workgroups go to the 8 XCDs round robin (workgroup b to XCD b % 8), and
 // each XCD has its own L2.  xcd_block renumbers them so that each XCD takes
 // one contiguous range of the logical blocks (XCD x the x-th range; the
-// first nb % 8 ranges are one block longer): the z neighbours a stencil
-// block reads then sit in the same L2 instead of being fetched once per XCD.
+// first nb % 8 ranges are one block longer): data that neighbouring blocks
+// share then sits in one L2 instead of being fetched once per XCD.
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
   const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (b >> 3);
 }
+
 
 
 struct Cell {
@@ -923,6 +924,30 @@ __global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
   }
 }
 
+// ---- one direction per launch ---------------------------------------------
+// The same sweep as k_sweep / k_sweep_mix for one direction only (no
+// species, no boundary codes 4/5/9/10: the fused forms that use it refuse
+// those).  One direction's code is about a third of the all-direction
+// kernel's, well inside the instruction cache.  The y and z blocks are
+// renumbered XCD-aware: neighbouring blocks hold neighbouring tile columns,
+// whose rows share 128-byte lines (a row of an 8-wavefront tile is 64
+// bytes), so on one XCD the second half of a line is an L2 hit instead of a
+// second fetch by another XCD.  Measured on the 256^3 MHD polytrope: y+z
+// sweep 2244 -> 2052 us; the same renumbering of the streaming x blocks (the
+// fused x sweep) and of the gravity sub-steps made them 3-6% slower.
+template <int DIR, bool MAG, int LIM, bool TILE, bool MIX>
+__global__ __launch_bounds__(1024) void k_sweep_dir(Dev D, int stage) {
+  extern __shared__ double lds[];
+  if (D.sc->stop) return;
+  const int lane = threadIdx.x & 63;
+  const unsigned b = DIR == 0 ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+  const long w = (long)b * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // tiled: a partly filled last block runs its padding wavefronts too
+  if (TILE ? w - (threadIdx.x >> 6) >= D.waves[DIR] : w >= D.waves[DIR]) return;
+  if (MIX) sweep_line_mix<DIR, MAG, LIM, TILE, false, false>(D, w, lane, stage, lds);
+  else sweep_line<DIR, MAG, LIM, TILE, false, false, false>(D, w, lane, stage, lds);
+}
+
 // ---- advected species (compswitch 2) -----------------------------------------
 // One wavefront per 64-cell segment of a line, as the sweeps: the species'
 // limited slopes (interpolation.f90:135-141; the e_int guard does not reset
@@ -1292,66 +1317,133 @@ __global__ __launch_bounds__(256) void k_pressure(Dev D) {
   }
 }
 
-// ---- hyperbolic self-gravity stage (gravity_hyperbolic.f90:61-125) --------
-// The potential is stored twice with its ghost layer (Dirichlet boundary
-// values), psi twice without.  Stage 1 (grungen 1) reads phi0 from gphi[in]
-// and psi0 from gpsi and writes phi1 to gphi[1-in] and psi1 to gpsi1; stage
-// 2 reads phi1 and psi1 for the update and phi0, psi0 at the cell itself
-// for the RK combination, and writes phi2 and psi2 over phi0 and psi0 (each
-// thread overwrites only its own cell, which no other thread reads in this
-// stage).  So the stage-1 copies grvphiorg of the reference are never
-// written: 40 + 56 bytes per cell and sub-step.
+// ---- hyperbolic self-gravity sub-step (gravity_hyperbolic.f90:61-125) -----
+// The potential is stored twice with its ghost layer (the Dirichlet boundary
+// values, the same in both buffers; no kernel writes a ghost), psi twice
+// without.  A sub-step reads phi from gphi[in] and psi from gpsi and writes
+// the new values to gphi[1-in] and gpsi1; the host then swaps the buffers.
 // lap_coeff for crdnt 0 (new; the reference has no Cartesian branch):
 // c0 = -((idx1(i)+idx1(i+1))*dxi2*dxi3 + (idx2(j)+idx2(j+1))*dxi1*dxi3 +
 // (idx3(k)+idx3(k+1))*dxi1*dxi2), c1(i) = dxi2*dxi3*idx1(i+1), c2, c3
-// alike, normalised by the cell volume.  The last stage also applies the
-// psi damping factor.
-template <bool DAMP>
-__global__ __launch_bounds__(256) void k_grav_stage(Dev D, int grungen, int in, double faco,
-                                                    double fact, double facn, double dtg,
-                                                    double cg, double damp) {
-  // a block is blockDim.y rows of blockDim.x cells of one z plane; blocks
-  // are numbered x fastest, then y, then z, after the XCD renumbering
+// alike, normalised by the cell volume (get_lapphi_hgsrc, :230-280).
+
+// Cell (i, j, k) and element m of a gravity launch: a block is blockDim.y
+// rows of blockDim.x cells of one z plane, blocks numbered x fastest, then
+// y, then z.  False past the box.
+__device__ __forceinline__ bool grav_cell(const Dev &D, int &i, int &j, int &k, long &m) {
   const unsigned gx = ((unsigned)D.n1 + blockDim.x - 1) / blockDim.x;
   const unsigned gy = ((unsigned)D.n2 + blockDim.y - 1) / blockDim.y;
-  const unsigned L = xcd_block(blockIdx.x, gridDim.x);
-  const unsigned t = L / gx;
-  const int i = (int)((L - t * gx) * blockDim.x + threadIdx.x);
-  const int j = (int)((t % gy) * blockDim.y + threadIdx.y), k = (int)(t / gy);
-  if (i >= D.n1 || j >= D.n2) return;
-  const long m = ((long)k * D.n2 + j) * D.n1 + i;
-  const bool s1 = grungen == 1;
-  const double *ph = D.gphi[s1 ? in : 1 - in];  // the potential of this stage
-  double *out = D.gphi[s1 ? 1 - in : in];
-  const double *psi = s1 ? D.gpsi : D.gpsi1;
-  double *psi_out = s1 ? D.gpsi1 : D.gpsi;
-  const long g = gidx(D, i, j, k);
-  const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
+  const unsigned t = blockIdx.x / gx;
+  i = (int)((blockIdx.x - t * gx) * blockDim.x + threadIdx.x);
+  j = (int)((t % gy) * blockDim.y + threadIdx.y);
+  k = (int)(t / gy);
+  m = ((long)k * D.n2 + j) * D.n1 + i;
+  return i < D.n1 && j < D.n2;
+}
+
+// The Laplacian of the potential at cell (i, j, k) from its value there (v)
+// and at its x, y, z neighbours (xm, xp, ...; the unsolved directions' are
+// not read).  One operation order for every caller.
+__device__ __forceinline__ double grav_lapphi(const Dev &D, int i, int j, int k, double v,
+                                              double xm, double xp, double ym, double yp,
+                                              double zm, double zp) {
   const double *X1 = D.ax[0].dxi, *X2 = D.ax[1].dxi, *X3 = D.ax[2].dxi;
   const double *I1 = D.ax[0].idx, *I2 = D.ax[1].idx, *I3 = D.ax[2].idx;
   const double c0 = -((I1[i] + I1[i + 1]) * X2[j] * X3[k] + (I2[j] + I2[j + 1]) * X1[i] * X3[k] +
                       (I3[k] + I3[k + 1]) * X1[i] * X2[j]);
-  double lap = c0 * ph[g];
-  lap = lap + X2[j] * X3[k] * I1[i] * ph[g - 1];
-  lap = lap + X2[j] * X3[k] * I1[i + 1] * ph[g + 1];
+  double lap = c0 * v;
+  lap = lap + X2[j] * X3[k] * I1[i] * xm;
+  lap = lap + X2[j] * X3[k] * I1[i + 1] * xp;
   if (D.n2 > 1) {
-    lap = lap + X1[i] * X3[k] * I2[j] * ph[g - sy];
-    lap = lap + X1[i] * X3[k] * I2[j + 1] * ph[g + sy];
+    lap = lap + X1[i] * X3[k] * I2[j] * ym;
+    lap = lap + X1[i] * X3[k] * I2[j + 1] * yp;
   }
   if (D.n3 > 1) {
-    lap = lap + X1[i] * X2[j] * I3[k] * ph[g - sz];
-    lap = lap + X1[i] * X2[j] * I3[k + 1] * ph[g + sz];
+    lap = lap + X1[i] * X2[j] * I3[k] * zm;
+    lap = lap + X1[i] * X2[j] * I3[k + 1] * zp;
   }
-  const double lapphi = lap / (X1[i] * X2[j] * X3[k]);
+  return lap / (X1[i] * X2[j] * X3[k]);
+}
+
+// One stage of the wave update with its RK combination (hyperbolic_gravity_
+// step, :73-116): phi = faco*phi0 + facn*phi + fact*dtg*c*psi, psi =
+// faco*psi0 + facn*psi + fact*dtg*c*(lap(phi) - 4 pi G rho).
+__device__ __forceinline__ double grav_phi(double faco, double facn, double fact, double dtg,
+                                           double cg, double p0, double p, double q) {
+  return faco * p0 + facn * p + fact * dtg * cg * q;
+}
+__device__ __forceinline__ double grav_psi(double faco, double facn, double fact, double dtg,
+                                           double cg, double q0, double q, double lapphi,
+                                           double hgsrc) {
+  return faco * q0 + facn * q + fact * dtg * cg * (lapphi - hgsrc);
+}
+
+// grktype 1: the single stage, with the damping factor (:114).
+__global__ __launch_bounds__(256) void k_grav_stage1(Dev D, int in, double faco, double fact,
+                                                     double facn, double dtg, double cg,
+                                                     double damp) {
+  int i, j, k;
+  long m;
+  if (!grav_cell(D, i, j, k, m)) return;
+  const double *ph = D.gphi[in];
+  const long g = gidx(D, i, j, k);
+  const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
+  const double lapphi = grav_lapphi(D, i, j, k, ph[g], ph[g - 1], ph[g + 1], ph[g - sy],
+                                    ph[g + sy], ph[g - sz], ph[g + sz]);
   const double hgsrc = D.fourpig * D.W[WD][m];
-  const double p = ph[g], q = psi[m];
-  // grvphiorg(:,:,:,1:2): at stage 1 the values just read
-  const double p0 = s1 ? p : out[g];
-  const double q0 = s1 ? q : psi_out[m];
-  out[g] = faco * p0 + facn * p + fact * dtg * cg * q;
-  double qn = faco * q0 + facn * q + fact * dtg * cg * (lapphi - hgsrc);
-  if (DAMP) qn = qn * damp;
-  psi_out[m] = qn;
+  const double p = ph[g], q = D.gpsi[m];
+  D.gphi[1 - in][g] = grav_phi(faco, facn, fact, dtg, cg, p, p, q);
+  D.gpsi1[m] = grav_psi(faco, facn, fact, dtg, cg, q, q, lapphi, hgsrc) * damp;
+}
+
+// grktype 2: both stages in one launch.  Stage 1's potential update has no
+// stencil (phi1 = faco*phi0 + facn*phi0 + fact*dtg*c*psi0 with the stage-1
+// coefficients), so phi1 at the cell and at its six neighbours follows from
+// phi0 and psi0 there (at a ghost it is the boundary value, which stage 1
+// leaves alone), psi1 at the cell from the Laplacian of phi0, and stage 2
+// runs at once.  Each value is the same expression on the same operands as
+// in two separate stage launches, so the result is the same bit for bit,
+// while phi0, psi0 and rho are read once and the new phi and psi written
+// once: 40 bytes per cell and sub-step against 96 for two launches (the
+// neighbours' phi0 and psi0 come from the caches).  Stage 2 applies the
+// damping factor.
+__global__ __launch_bounds__(256) void k_grav_substep2(Dev D, int in, double fo1, double ft1,
+                                                       double fn1, double fo2, double ft2,
+                                                       double fn2, double dtg, double cg,
+                                                       double damp) {
+  int i, j, k;
+  long m;
+  if (!grav_cell(D, i, j, k, m)) return;
+  const double *ph = D.gphi[in], *ps = D.gpsi;
+  const long g = gidx(D, i, j, k);
+  const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
+  const long my = D.n1, mz = (long)D.n1 * D.n2;
+  // stage 1
+  const double p = ph[g], q = ps[m];
+  const double pxm = ph[g - 1], pxp = ph[g + 1], pym = ph[g - sy], pyp = ph[g + sy];
+  const double pzm = ph[g - sz], pzp = ph[g + sz];
+  const double hgsrc = D.fourpig * D.W[WD][m];
+  const double lap0 = grav_lapphi(D, i, j, k, p, pxm, pxp, pym, pyp, pzm, pzp);
+  const double q1 = grav_psi(fo1, fn1, ft1, dtg, cg, q, q, lap0, hgsrc);
+  // phi1 at the cell and its neighbours (interior: the stage-1 update there)
+  auto phi1 = [&](double pn, bool inside, long mn) {
+    return inside ? grav_phi(fo1, fn1, ft1, dtg, cg, pn, pn, ps[mn]) : pn;
+  };
+  const double p1 = grav_phi(fo1, fn1, ft1, dtg, cg, p, p, q);
+  const double xm = phi1(pxm, i > 0, m - 1), xp = phi1(pxp, i < D.n1 - 1, m + 1);
+  double ym = 0.0, yp = 0.0, zm = 0.0, zp = 0.0;
+  if (D.n2 > 1) {
+    ym = phi1(pym, j > 0, m - my);
+    yp = phi1(pyp, j < D.n2 - 1, m + my);
+  }
+  if (D.n3 > 1) {
+    zm = phi1(pzm, k > 0, m - mz);
+    zp = phi1(pzp, k < D.n3 - 1, m + mz);
+  }
+  // stage 2 over grvphiorg = (phi0, psi0)
+  const double lap1 = grav_lapphi(D, i, j, k, p1, xm, xp, ym, yp, zm, zp);
+  D.gphi[1 - in][g] = grav_phi(fo2, fn2, ft2, dtg, cg, p, p1, q1);
+  D.gpsi1[m] = grav_psi(fo2, fn2, ft2, dtg, cg, q, q1, lap1, hgsrc) * damp;
 }
 
 // ---- CFL condition as its own pass (the reference's timestep) ------------
@@ -1798,23 +1890,64 @@ static void sweep_rk_dir(const Dev &D, int stage, const RKArgs &rk, unsigned g, 
   }
 }
 
+template <int DIR, bool MAG, int LIM>
+static void sweep_dir_launch(const Dev &D, int stage, int bw, size_t tile, hipStream_t s,
+                             const KEv *ev) {
+  const unsigned g = nblk(D.waves[DIR], bw);
+  const int bs = 64 * bw;
+  const bool t = DIR > 0 && tile;
+  if (D.mixed) {
+    if (t) klaunch_lds(k_sweep_dir<DIR, MAG, LIM, true, true>, g, bs, tile, s, ev, D, stage);
+    else klaunch_lds(k_sweep_dir<DIR, MAG, LIM, false, true>, g, bs, 0, s, ev, D, stage);
+  } else {
+    if (t) klaunch_lds(k_sweep_dir<DIR, MAG, LIM, true, false>, g, bs, tile, s, ev, D, stage);
+    else klaunch_lds(k_sweep_dir<DIR, MAG, LIM, false, false>, g, bs, 0, s, ev, D, stage);
+  }
+}
+
+template <bool MAG, int LIM>
+static void sweep_dirs_yz_v(const Dev &D, int stage, int bw, size_t tile, hipStream_t s,
+                            const KEv *ev) {
+  // one event pair around both launches: start on the first, stop on the last
+  const bool y = D.waves[1] > 0, z = D.waves[2] > 0;
+  KEv e1{}, e2{};
+  const KEv *p1 = nullptr, *p2 = nullptr;
+  if (ev) {
+    e1 = KEv{ev->start, z ? nullptr : ev->stop};
+    e2 = KEv{y ? nullptr : ev->start, ev->stop};
+    p1 = &e1;
+    p2 = &e2;
+  }
+  if (y) sweep_dir_launch<1, MAG, LIM>(D, stage, bw, tile, s, p1);
+  if (z) sweep_dir_launch<2, MAG, LIM>(D, stage, bw, tile, s, p2);
+}
+
+// The y and z sweeps of a stage as two launches of k_sweep_dir.
+static void sweep_dirs_yz(const Dev &D, int stage, int bw, size_t tile, hipStream_t s,
+                          const KEv *ev) {
+  if (D.mag) {
+    if (D.limiter == LIM_FLAT) sweep_dirs_yz_v<true, LIM_FLAT>(D, stage, bw, tile, s, ev);
+    else if (D.limiter == LIM_MINMOD) sweep_dirs_yz_v<true, LIM_MINMOD>(D, stage, bw, tile, s, ev);
+    else sweep_dirs_yz_v<true, LIM_MC>(D, stage, bw, tile, s, ev);
+  } else {
+    if (D.limiter == LIM_FLAT) sweep_dirs_yz_v<false, LIM_FLAT>(D, stage, bw, tile, s, ev);
+    else if (D.limiter == LIM_MINMOD) sweep_dirs_yz_v<false, LIM_MINMOD>(D, stage, bw, tile, s, ev);
+    else sweep_dirs_yz_v<false, LIM_MC>(D, stage, bw, tile, s, ev);
+  }
+}
+
 int launch_fused_stage(const Dev &D, int stage, const RKArgs &rk, hipStream_t s,
                        const KEv *ev, const KEv *ev_fused, int part) {
   const int L = D.fdir;
   const int bw = D.sweep_bw > 0 ? D.sweep_bw : 4;
   const int bs = 64 * bw;
   const size_t tile = sweep_tile_bytes(D, bw);
-  // the other directions first, as launch_sweep does them: those before L
-  // (L = 1, 2), or y and z with the x range of the wave numbering emptied
-  // (L = 0; the y/z ranges start on multiples of 16 wavefronts, so blocks
-  // still never mix directions)
+  // the other directions first: for L = 0, y and z in one launch each
+  // (k_sweep_dir, one event pair around both); for L = 1, 2 the directions
+  // before L in one all-direction launch over their wave ranges
+  if (L == 0 && part != 2) sweep_dirs_yz(D, stage, bw, tile, s, ev);
   Dev Do = D;
-  long nwo = D.woff[L];
-  if (L == 0) {
-    Do.woff[2] = D.woff[2] - D.woff[1];
-    Do.woff[1] = 0;
-    nwo = Do.woff[2] + D.waves[2];
-  }
+  const long nwo = L == 0 ? 0 : D.woff[L];
   if (nwo > 0 && part != 2) {
     const unsigned g0 = nblk(nwo, bw);
     if (D.mag) {
@@ -1856,20 +1989,25 @@ void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, b
   }
 }
 
-void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, double facn,
-                       double dtg, double cg, bool damp, double damp_factor, hipStream_t s,
-                       const KEv *ev) {
+// One gravity sub-step of RK order gk (1 or 2) with the damping factor;
+// the new phi and psi land in gphi[1-gcur] and gpsi1 (the caller swaps).
+void launch_grav_substep(const Dev &D, int gk, double dtg, double cg, double damp_factor,
+                         hipStream_t s, const KEv *ev) {
   // rows of 64 cells, 4 rows per block (one 256-cell row per block when the
   // box has fewer than 4 rows)
   const unsigned bx = D.n2 >= 4 ? 64 : 256, by = D.n2 >= 4 ? 4 : 1;
   const unsigned g = nblk(D.n1, (int)bx) * nblk(D.n2, (int)by) * (unsigned)D.n3;
   const dim3 b(bx, by);
-  if (damp)
-    klaunch(k_grav_stage<true>, g, b, s, ev, D, grungen, D.gcur, faco, fact, facn, dtg, cg,
-            damp_factor);
-  else
-    klaunch(k_grav_stage<false>, g, b, s, ev, D, grungen, D.gcur, faco, fact, facn, dtg, cg,
-            damp_factor);
+  double fo1, ft1, fn1;
+  runge_coeff(1, gk, &fo1, &ft1, &fn1);
+  if (gk == 1) {
+    klaunch(k_grav_stage1, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, dtg, cg, damp_factor);
+    return;
+  }
+  double fo2, ft2, fn2;
+  runge_coeff(2, gk, &fo2, &ft2, &fn2);
+  klaunch(k_grav_substep2, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg, cg,
+          damp_factor);
 }
 
 void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev) {

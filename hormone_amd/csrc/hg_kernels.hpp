@@ -107,7 +107,7 @@ struct Dev {
   // hyperbolic self-gravity (gravswitch 3): the potential is stored with a
   // one-cell ghost layer ((n1+2)(n2+2)(n3+2), Dirichlet values), twice:
   // gphi[gcur] holds the current potential, the other the stage-1 one; psi
-  // (interior) likewise in gpsi and gpsi1 (k_grav_stage).
+  // (interior) likewise in gpsi and gpsi1 (k_grav_substep2, k_grav_stage1).
   int mixed;       // HG_PREC_FP32MIX
   double damp;     // phidamp factor exp(-alpha9wave*courant), evaluated on the host
   int grav;
@@ -191,9 +191,8 @@ size_t sweep_tile_bytes(const Dev &D, int bw);
 // cfl: also the next step's CFL partials; damp: phidamp
 void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, bool cfl,
                bool damp, double damp_factor, hipStream_t s, const KEv *ev = nullptr);
-void launch_grav_stage(const Dev &D, int grungen, double faco, double fact, double facn,
-                       double dtg, double cg, bool damp, double damp_factor, hipStream_t s,
-                       const KEv *ev = nullptr);
+void launch_grav_substep(const Dev &D, int gk, double dtg, double cg, double damp_factor,
+                         hipStream_t s, const KEv *ev = nullptr);
 void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev = nullptr);
 void launch_cfl_final(const Dev &D, hipStream_t s);
 void launch_step_end_cfl(const Dev &D, hipStream_t s);  // step_end, then cfl_final
