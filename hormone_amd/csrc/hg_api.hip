@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1382,3 +1383,85 @@ extern "C" int hg_batch_config(hg_batch *b, int32_t sweep_block_waves, int32_t s
   return HG_OK;
 }
 
+// ---- allocation-free timestep (hg_timestep_noalloc) -------------------------
+// The device memory this entry point works in besides the caller's arrays:
+// one Scalars record (the clock the dt clipping reads, the CFL results and
+// the error key), then the CFL partials, two doubles per 256-cell block.
+// A file-scope device array, one copy per GPU, held until the process ends.
+static constexpr size_t NOALLOC_REC = 256;  // the Scalars record, padded
+static constexpr size_t NOALLOC_BYTES =
+    NOALLOC_REC + 2 * sizeof(double) * ((size_t)HG_NOALLOC_MAX_CELLS / 256);
+static_assert(sizeof(Scalars) <= NOALLOC_REC, "scratch record too small");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// The device address of `scratch` on each GPU, found once per device with
+// that device current (the host-side name is not a device address).
+static constexpr int NOALLOC_MAX_DEV = 64;
+static void *g_scratch_ptr[NOALLOC_MAX_DEV];
+static std::mutex g_scratch_mu;
+
+static int noalloc_scratch(int dev, unsigned char **ptr) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV) return HG_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HG_ERR_DEVICE;
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+  return HG_OK;
+}
+
+extern "C" int hg_timestep_noalloc(const hg_desc *desc, const double *const *prim,
+                                   const double *const *dxi, double time, double t_out,
+                                   double t_end, int32_t outstyle, int32_t endstyle,
+                                   double *out, void *stream) {
+  if (!desc || !prim || !dxi || !out) return HG_ERR_ARG;
+  if (desc->ie < desc->is || desc->je < desc->js || desc->ke < desc->ks) return HG_ERR_ARG;
+  const long n1 = desc->ie - desc->is + 1, n2 = desc->je - desc->js + 1,
+             n3 = desc->ke - desc->ks + 1;
+  if ((double)n1 * (double)n2 * (double)n3 > (double)HG_NOALLOC_MAX_CELLS)
+    return HG_ERR_SCRATCH;
+  const bool mag = desc->mag_on != 0;
+  for (int q = 0; q < (mag ? 8 : 5); ++q)
+    if (!prim[q]) return HG_ERR_ARG;
+  for (int a = 0; a < 3; ++a)
+    if (!dxi[a]) return HG_ERR_ARG;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  unsigned char *base = nullptr;
+  int st = noalloc_scratch(dev, &base);
+  if (st) return st;
+  // a Dev record of kernel arguments over the caller's arrays and the
+  // scratch (built on the host stack; nothing is allocated)
+  Dev D;
+  std::memset(&D, 0, sizeof(D));
+  D.n1 = (int)n1;
+  D.n2 = (int)n2;
+  D.n3 = (int)n3;
+  D.ncell = n1 * n2 * n3;
+  D.solve[0] = n1 > 1;
+  D.solve[1] = n2 > 1;
+  D.solve[2] = n3 > 1;
+  D.mag = mag;
+  D.gamma = desc->gamma;
+  D.courant = desc->courant;
+  D.W[WD] = const_cast<double *>(prim[0]);
+  for (int a = 0; a < 3; ++a) D.W[WV1 + a] = const_cast<double *>(prim[1 + a]);
+  D.W[WEINT] = const_cast<double *>(prim[4]);
+  if (mag)
+    for (int a = 0; a < 3; ++a) D.W[WB1 + a] = const_cast<double *>(prim[5 + a]);
+  for (int a = 0; a < 3; ++a) D.ax[a].dxi = dxi[a];
+  D.sc = reinterpret_cast<Scalars *>(base);
+  D.part = reinterpret_cast<double *>(base + NOALLOC_REC);
+  D.nblk_cfl = (int)((D.ncell + 255) / 256);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // every call starts from a cleared error key and its own clock
+  launch_clear_error(D, s);
+  launch_set_clock(D, time, t_out, t_end, 0.0, outstyle, endstyle, 0, 0, 1, 1, s);
+  launch_cfl(D, s);
+  launch_cfl_final(D, s);
+  launch_timestep_out(D, out, s);
+  HIPCHK(hipGetLastError());
+  return HG_OK;
+}

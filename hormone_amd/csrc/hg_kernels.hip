@@ -1661,6 +1661,13 @@ __global__ __launch_bounds__(256) void k_bstep_end_cfl(Batch B) {
 
 __global__ void k_clear_error(Scalars *sc) { sc->err = NOERR; }
 
+// dt, ch and the error status of a finished CFL reduction into a caller's
+// device array (the allocation-free timestep)
+__global__ void k_timestep_out(const Scalars *sc, double *out) {
+  out[0] = sc->dt;
+  out[1] = sc->ch;
+  out[2] = sc->err == NOERR ? 0.0 : (double)HG_ERR_NEG_EINT;
+}
 
 __global__ void k_set_step(Scalars *sc, double dt, double ch) {
   sc->dt = dt;
@@ -1795,6 +1802,9 @@ void launch_pressure(const Dev &D, hipStream_t s) {
 
 void launch_clear_error(const Dev &D, hipStream_t s) { k_clear_error<<<1, 1, 0, s>>>(D.sc); }
 
+void launch_timestep_out(const Dev &D, double *out, hipStream_t s) {
+  k_timestep_out<<<1, 1, 0, s>>>(D.sc, out);
+}
 
 void launch_set_step(const Dev &D, double dt, double ch, hipStream_t s) {
   k_set_step<<<1, 1, 0, s>>>(D.sc, dt, ch);

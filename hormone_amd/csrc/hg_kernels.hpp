@@ -161,6 +161,7 @@ void runge_coeff(int stage, int rktype, double *faco, double *fact, double *facn
 
 void launch_pressure(const Dev &D, hipStream_t s);
 void launch_clear_error(const Dev &D, hipStream_t s);
+void launch_timestep_out(const Dev &D, double *out, hipStream_t s);  // dt, ch, status
 void launch_set_step(const Dev &D, double dt, double ch, hipStream_t s);
 void launch_set_clock(const Dev &D, double time, double t_out, double t_end, double dt_out,
                       int outstyle, int endstyle, long long tn, long long tnlim,

@@ -18,6 +18,7 @@ LIB_PATH = os.environ.get("HORMONE_HIP_LIB") or os.path.join(HERE, "libhormone_h
 
 HG_OK, HG_ERR_ARG, HG_ERR_UNSUPPORTED, HG_ERR_DEVICE = 0, 1, 2, 3
 HG_ERR_FACE_EOS, HG_ERR_NEG_EINT, HG_ERR_NOMEM, HG_ERR_STATE = 4, 5, 6, 7
+HG_ERR_SCRATCH = 8
 PREC_FP64, PREC_FP32MIX = 0, 1
 
 _dp = C.POINTER(C.c_double)

@@ -45,7 +45,9 @@ enum {
   HG_ERR_NEG_EINT = 5,    /* eos.f90:833-839 internalenergy: e_int<=0 in a
                              cell after the RK update                        */
   HG_ERR_NOMEM = 6,       /* device allocation failed                        */
-  HG_ERR_STATE = 7        /* call order violated (e.g. step before upload)   */
+  HG_ERR_STATE = 7,       /* call order violated (e.g. step before upload)   */
+  HG_ERR_SCRATCH = 8      /* input larger than an allocation-free entry
+                             point's static scratch (hg_timestep_noalloc)   */
 };
 
 /* flux_limiter (settings: 'flat', 'minmod', 'modified_mc';
@@ -317,6 +319,35 @@ int hg_set_launch(hg_ctx *ctx, int32_t sweep_block_waves);
    last HG_ERR_FACE_EOS / HG_ERR_NEG_EINT. */
 int hg_last_error(const hg_ctx *ctx, int32_t *i, int32_t *j, int32_t *k,
                   int32_t *dir);
+
+/* hg_timestep without a context and without any device allocation.  No
+   counterpart in the reference: it is for callers that cannot allocate
+   device memory -- code under HIP graph capture, tight serving loops,
+   hosts that keep the state in their own device arrays.  The result is
+   hg_timestep's (timestep.f90:35-89 with the dt clipping of :74-85), bit for
+   bit, on the same state.
+   desc: extents (is..ke), mag_on, gamma and courant are read; nothing else.
+   prim: device pointers d, v1, v2, v3, eint and, with mag_on, b1, b2, b3:
+     interior fields, nx*ny*nz doubles each, x fastest.
+   dxi: device pointers to the interior cell widths dxi1(is:ie), dxi2(js:je),
+     dxi3(ks:ke).
+   out: device pointer to 3 doubles, written when the stream reaches them:
+     dt, ch (= the largest fast speed, cfmax) and 0, or HG_ERR_NEG_EINT when a
+     cell holds rho <= 0 or e_int <= 0 (dt is then meaningless).  cgrav and
+     dtgrav follow from ch on the host (HGfac*ch, timestep.f90:92-97).
+   stream: a hipStream_t (NULL: the null stream).  The call only enqueues
+     kernels; it allocates, frees and synchronises nothing, so it may be
+     captured into a graph.
+   Limit: nx*ny*nz <= HG_NOALLOC_MAX_CELLS, else HG_ERR_SCRATCH.  The working
+   memory is one static device array per GPU (HG_NOALLOC_MAX_CELLS/256
+   partial pairs plus a 256-byte record, 4 MiB) that every call on that GPU
+   shares and that keeps the last call's contents: calls that overlap on one
+   GPU (different streams or host threads at once) race. */
+enum { HG_NOALLOC_MAX_CELLS = 1 << 26 };
+int hg_timestep_noalloc(const hg_desc *desc, const double *const *prim,
+                        const double *const *dxi, double time, double t_out,
+                        double t_end, int32_t outstyle, int32_t endstyle,
+                        double *out, void *stream);
 
 /* Known-answer evaluation of the device arithmetic; no counterpart in the
    reference (its routines are module procedures without a C interface).
