@@ -23,8 +23,10 @@ physical time go into the same line.
 roofline: one RK stage is the unit: the flux sweep launch (all directions,
 or the directions before the fused one) and the launch that applies the RK
 update (k_rk, or the fused sweep + RK).  Their average durations are timed
-with HIP events on the library's stream inside the timed region (only these
-launches carry events, about 9 us of completion wait each).  Algorithmic
+with HIP events on the library's stream over the first eighth of the timed
+steps (only these launches and the gravity sub-steps carry events, about
+9 us of completion wait each; the rest of the timed region runs without
+them).  Algorithmic
 bytes are the compulsory FP64 state traffic per zone-update, shared out per
 stage: RK3 stage 1 reads U^n and writes U^1 (2*nvar*8 B per cell), stages 2
 and 3 read U^n and U^s and write U^s+1 (3*nvar*8 B), the CFL pass reads U
@@ -287,35 +289,29 @@ def sums(D, eng, g, prm):
 
 def tune(a, eng, run):
     """Warm-up with launch tuning; returns the tuning report and the fused
-    form (0 separate, 1 last direction, 2 x)."""
-    tuned = None
+    form (0 separate, 1 last direction, 2 x).  The form is chosen first
+    (whole steps of each at the default block size), then the sweep block
+    size by whole steps in that form; every choice gives the same bits, so
+    the tuning steps are ordinary steps of the run."""
+    from hormone_amd.batch import tune_block_steps, tune_fused
+    names = {0: "separate", 1: "last", 2: "x"}
+    tuned = {}
+    fused = 0
     if a.sweep_waves:
         eng.set_launch(a.sweep_waves)
-        run.advance(a.warmup)
-    else:
-        # autotune the sweep's block size on warm-up steps (launch shapes do
-        # not change results), then finish the warm-up with the winner
-        from hormone_amd.batch import tune_single
-        per = max(1, a.warmup // 5)
-        best, times = tune_single(eng, run.advance, steps=per)
-        tuned = {"sweep_block_waves": best, "us_per_launch": {str(k): round(v, 2)
-                                                              for k, v in times.items()}}
-        left = a.warmup - per * len(times)
-        if left > 0:
-            run.advance(left)
-    fused = 0
-    names = {0: "separate", 1: "last", 2: "x"}
     if a.fused in ("last", "x"):
         fused = {"last": 1, "x": 2}[a.fused]
         eng.set_fused(fused)
-        tuned = dict(tuned or {})
         tuned["fused_sweep"] = a.fused
     elif a.fused == "auto":
-        from hormone_amd.batch import tune_fused
         fused, ftimes = tune_fused(eng, run.advance, steps=max(2, min(8, a.warmup // 4)))
-        tuned = dict(tuned or {})
         tuned["fused_sweep"] = names[fused]
         tuned["ms_per_step"] = {names[k]: round(v * 1e3, 3) for k, v in ftimes.items()}
+    if not a.sweep_waves:
+        best, times = tune_block_steps(eng, run.advance, steps=max(2, a.warmup // 5))
+        tuned["sweep_block_waves"] = best
+        tuned["block_ms_per_step"] = {str(k): round(v * 1e3, 3) for k, v in times.items()}
+    run.advance(a.warmup)
     return tuned, fused
 
 
@@ -359,28 +355,33 @@ def timed_run(a, precision, want_roofline):
     run = Runner(eng, prm, g)
     m0, e0 = sums(D, eng, g, prm)
     tuned, fused = tune(a, eng, run)
-    if want_roofline:
-        # per-launch events on the stage's two launches only
-        eng.profile(True, kinds=(native.PROF_SWEEP, native.PROF_RK))
+    # Per-launch events on the stage's two launches (and the gravity
+    # sub-steps) over the first eighth of the timed steps: each timed launch
+    # waits for its completion signal (about 9 us), 15% of a 512^2 step, so
+    # the rest of the timed region runs without them.
+    kinds = (native.PROF_SWEEP, native.PROF_RK) + \
+        ((native.PROF_GRAV,) if prm.gravswitch == 3 else ())
+    nprof = max(1, a.steps // 8) if want_roofline else 0
+    out = {}
     k0 = len(run.nsub)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ok = run.advance(a.steps)
+    ok = True
+    if nprof:
+        eng.profile(True, kinds=kinds)
+        ok = run.advance(nprof)
+        out["sweep"] = eng.profile_read(native.PROF_SWEEP)
+        out["rk"] = eng.profile_read(native.PROF_RK)
+        if prm.gravswitch == 3:
+            out["grav"] = eng.profile_read(native.PROF_GRAV)
+        eng.profile(False)
+    ok = ok and run.advance(a.steps - nprof)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     assert ok, "the run ended inside the timed region; lower --steps"
-    out = {"eng": eng, "prm": prm, "g": g, "run": run, "wall": wall, "tuned": tuned,
-           "fused": fused, "m0": m0, "e0": e0,
-           "zus": a.steps * g.ncells / wall, "nsub": run.mean_nsub(k0)}
-    if want_roofline:
-        out["sweep"] = eng.profile_read(native.PROF_SWEEP)
-        out["rk"] = eng.profile_read(native.PROF_RK)
-        eng.profile(False)
-        if prm.gravswitch == 3:  # gravity stages over a short window after
-            eng.profile(True, kinds=(native.PROF_GRAV,))
-            run.advance(min(5, a.steps))
-            out["grav"] = eng.profile_read(native.PROF_GRAV)
-            eng.profile(False)
+    out.update({"eng": eng, "prm": prm, "g": g, "run": run, "wall": wall, "tuned": tuned,
+                "fused": fused, "m0": m0, "e0": e0, "profiled_steps": nprof,
+                "zus": a.steps * g.ncells / wall, "nsub": run.mean_nsub(k0)})
     return out
 
 
@@ -397,7 +398,10 @@ def roofline(a, r):
     us_s = ms_s / n_s * 1e3 if n_s else 0.0
     us_r = ms_r / n_r * 1e3
     fused = r["fused"]
-    sweep_k = "k_sweep" + ("_mix" if prm.precision == "fp32mix" else "")
+    # the fused-x form sweeps y and z with one one-direction launch each
+    # (k_sweep_dir), timed together by one event pair
+    sweep_k = ("k_sweep_dir(y,z)" if fused == 2 else
+               "k_sweep" + ("_mix" if prm.precision == "fp32mix" else ""))
     rk_k = "k_sweep_rk" if fused else "k_rk"
     b_alg = stage_bytes(nv, prm.rktype, grav) * g.ncells
     sb, rb = kernel_bytes(fused, nv, nw, dim, prm.mag_on)
@@ -417,7 +421,8 @@ def roofline(a, r):
             "traffic_over_algorithmic": round(traffic / b_alg, 3) if traffic else None,
             "avg_launch_us": {sweep_k: round(us_s, 2), rk_k: round(us_r, 2)},
             "launches_timed": {sweep_k: n_s, rk_k: n_r},
-            "timing": "HIP events on the library stream, inside the timed region",
+            "timing": f"HIP events on the library stream over the first {r['profiled_steps']} "
+                      f"of the {a.steps} timed steps",
             "note": "flux sweep limited by FP64 VALU issue and latency (bit-exact IEEE "
                     "division/square-root chains), not HBM",
             "share_of_timed_region": round(us * 1e-6 * prm.rktype * a.steps / r["wall"], 3)}
@@ -549,7 +554,10 @@ def main():
     if a.workload == "serving":
         serving_bench(a)
         return
-    defaults = {"ot512": (2000, 20), "polytrope256": (150, 10), "kh384": (90, 10)}
+    # OT 512^2: the timed window, the warm-up and the tuning steps (up to
+    # 21 per form and round, 21 per block size) stay before step 2200, where
+    # the reference's own run is checked
+    defaults = {"ot512": (1900, 20), "polytrope256": (150, 10), "kh384": (90, 10)}
     if a.steps is None:
         a.steps = defaults[a.workload][0]
         if a.workload == "ot512" and a.precision == "fp32mix":

@@ -15,6 +15,7 @@ Results are cached per shape class (rounded total cells and box count).
 from __future__ import annotations
 
 import ctypes as C
+import math
 import time
 
 from . import native
@@ -169,20 +170,29 @@ def autotune(batch: Batch, steps_per_trial: int = 2, use_cache: bool = True):
     return best, timings
 
 
-def tune_single(eng: native.Hydro, advance, candidates=(2, 4, 8, 16), steps: int = 4):
-    """Pick the sweep block size of one context by timing its sweep kernel
-    (HIP events) over a few real steps per candidate; ``advance(k)`` must
-    take k steps.  Returns (best, {candidate: microseconds per launch})."""
+def _time_host_steps(advance, steps: int, min_s: float = 0.02, cap: int = 20) -> float:
+    """Seconds per step over at least `steps` steps after one untimed step,
+    more (up to `cap`) when that would take under `min_s`: host-clock noise
+    dominates short measurements of small boxes."""
+    t0 = time.perf_counter()
+    advance(1)
+    est = time.perf_counter() - t0
+    n = max(steps, min(cap, int(math.ceil(min_s / max(est, 1e-6)))))
+    t0 = time.perf_counter()
+    advance(n)
+    return (time.perf_counter() - t0) / n
+
+
+def tune_block_steps(eng: native.Hydro, advance, candidates=(2, 4, 8, 16), steps: int = 2):
+    """Pick the sweep block size by timing whole steps (host clock) in the
+    form the context runs now, so a fused sweep + RK launch counts too;
+    ``advance(k)`` must take k steps and return after they finish.  Returns
+    (best, {candidate: s/step})."""
     timings = {}
     for bw in candidates:
         eng.set_launch(bw)
-        eng.profile(True)
-        advance(steps)
-        ms, n, _ = eng.profile_read(native.PROF_SWEEP)
-        eng.profile(False)
-        if n:
-            timings[bw] = ms / n * 1e3
-    best = min(timings, key=timings.get) if timings else 4
+        timings[bw] = _time_host_steps(advance, steps)
+    best = min(timings, key=timings.get)
     eng.set_launch(best)
     return best, timings
 
@@ -210,10 +220,7 @@ def tune_fused(eng: native.Hydro, advance, steps: int = 4):
     for _ in range(2):
         for m in modes:
             eng.set_fused(m)
-            advance(1)
-            t0 = time.perf_counter()
-            advance(steps)
-            dt = (time.perf_counter() - t0) / steps
+            dt = _time_host_steps(advance, steps)
             timings[m] = min(dt, timings.get(m, dt))
     best = min(timings, key=timings.get)
     eng.set_fused(best)

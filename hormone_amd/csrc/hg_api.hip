@@ -572,22 +572,30 @@ extern "C" int hg_profile_read(const hg_ctx *c, int32_t kind, double *ms_total,
   if (!c || kind < 0 || kind >= HG_PROF_N) return HG_ERR_ARG;
   const Dev &D = c->D;
   const double cell = (double)D.ncell * sizeof(double);
+  // the W fields that are not U (d, e and B alias U)
+  const int wr = D.nw - (D.mag ? 5 : 2);
+  // directions the sweep launches cover: every solved one, or all but the
+  // fused one
+  const int nsw = D.dim - (c->fused ? 1 : 0);
   double bytes = 0.0;
   switch (kind) {
     case HG_PROF_SWEEP:  // read the primitives once and write one divergence per direction
-      bytes = cell * (double)D.dim * (D.nw + D.nv);
+      bytes = cell * (double)nsw * (D.nw + D.nv);
       break;
-    case HG_PROF_RK: {  // read U, U0, DU_d; write U and the W fields that are
-                        // not U (stage 1 writes U0, reads no U0)
-      bytes = cell * (double)(D.nv * (2 + D.dim) + D.nw - (D.mag ? 5 : 2));
+    case HG_PROF_RK:
+      // read U, U0 and the swept divergences, write U and the W fields that
+      // are not U (stage 1 writes U0 and reads none: the same count); the
+      // fused launch also reads its direction's primitives instead of a
+      // divergence
+      bytes = cell * (double)(c->fused ? D.nw + nsw * D.nv + 3 * D.nv + wr
+                                       : D.dim * D.nv + 3 * D.nv + wr);
       break;
-    }
     case HG_PROF_CFL:
       bytes = cell * (double)(D.nw - (D.mag ? 2 : 1));
       break;
-    case HG_PROF_GRAV:  // per stage, averaged: stage 1 reads phi, psi, rho and writes
-                        // phi1, psi1; stage 2 also reads phi, psi at the cell
-      bytes = cell * 6.0;
+    case HG_PROF_GRAV:  // per sub-step launch (grktype 2: both stages in one): phi,
+                        // psi, rho read, phi and psi written
+      bytes = cell * 5.0;
       break;
     default:
       bytes = 0.0;
