@@ -45,6 +45,10 @@ struct hg_ctx {
   double *alt_U[NVMAX] = {}, *alt_W[NWMAX] = {};
   bool alt_ready = false;
   bool in_batch = false;
+  // hg_run_gravity: the gravity sub-steps run on their own stream, beside
+  // stage 1's flux sweep; gdone joins them back before stage 1's update
+  hipStream_t gstream = nullptr;
+  hipEvent_t gdone = nullptr;
 };
 
 #define HIPCHK(x)                                   \
@@ -352,7 +356,10 @@ extern "C" int hg_destroy(hg_ctx *c) {
   if (!c) return HG_OK;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gstream) hipStreamSynchronize(c->gstream);
   for (hipEvent_t e : c->ev) hipEventDestroy(e);
+  if (c->gdone) hipEventDestroy(c->gdone);
+  if (c->gstream) hipStreamDestroy(c->gstream);
   for (void *p : c->allocs) hipFree(p);
   if (c->h_sc) hipHostFree(c->h_sc);
   if (c->h_stage) hipHostFree(c->h_stage);
@@ -684,7 +691,7 @@ extern "C" int hg_timestep(hg_ctx *c, double time, double t_out, double t_end,
 // reference's order.  Returns tngrav, or 0 when time + dt does not lie
 // beyond grvtime.
 static int enqueue_gravity(hg_ctx *c, double time, double dt, double *grvtime, double cgrav,
-                           double *cgrav_old, double *dtgrav) {
+                           double *cgrav_old, double *dtgrav, hipStream_t stream) {
   const int tngrav = (int)std::ceil((time + dt - *grvtime) / *dtgrav);
   if (tngrav < 1) return 0;
   *dtgrav = (time + dt - *grvtime) / (double)tngrav;
@@ -694,7 +701,7 @@ static int enqueue_gravity(hg_ctx *c, double time, double dt, double *grvtime, d
         std::exp(-(1.0 - *cgrav_old / cgrav + c->desc.alphagrv * c->desc.hgcfl));
     // one launch per sub-step (both stages of grktype 2 in one); the new
     // values are in the second buffers
-    launch_grav_substep(c->D, gk, *dtgrav, cgrav, damp, c->stream, prof_ev(c, HG_PROF_GRAV));
+    launch_grav_substep(c->D, gk, *dtgrav, cgrav, damp, stream, prof_ev(c, HG_PROF_GRAV));
     c->D.gcur = 1 - c->D.gcur;
     std::swap(c->D.gpsi, c->D.gpsi1);
     *grvtime = *grvtime + *dtgrav;
@@ -724,7 +731,7 @@ extern "C" int hg_gravity(hg_ctx *c, double time, double dt, double *grvtime, do
   HIPCHK(hipSetDevice(c->device));
   int st = ensure_eint(c);
   if (st) return st;
-  const int tngrav = enqueue_gravity(c, time, dt, grvtime, cgrav, cgrav_old, dtgrav);
+  const int tngrav = enqueue_gravity(c, time, dt, grvtime, cgrav, cgrav_old, dtgrav, c->stream);
   if (tngrav < 1) return HG_ERR_ARG;
   HIPCHK(hipGetLastError());
   if (nsub) *nsub = tngrav;
@@ -880,6 +887,13 @@ extern "C" int hg_run(hg_ctx *c, double *time, int64_t *tn, double *t_out, doubl
 // enqueues the gravity sub-steps, the RK stages and the step end.  Stage
 // 1's flux sweep needs neither dt nor the new potential, so it is enqueued
 // before the host waits: the GPU sweeps while the host reads the scalars.
+// The gravity sub-steps go to a second stream, after the previous step's
+// end: they stream the potential through HBM while the sweep keeps the
+// VALUs busy, and stage 1's RK update waits for them.  They read only the
+// density of the state the sweep reads too (the update writes the other
+// buffer set) and write only the potential, which no sweep reads; the next
+// step's sub-steps start after this step's end, so no stage still reads
+// the potential they overwrite.
 extern "C" int hg_run_gravity(hg_ctx *c, double *time, int64_t *tn, double *t_out,
                               double dt_out, double t_end, int32_t outstyle, int32_t endstyle,
                               int64_t tnlim, int64_t tn_out, int64_t max_steps,
@@ -899,7 +913,12 @@ extern "C" int hg_run_gravity(hg_ctx *c, double *time, int64_t *tn, double *t_ou
   const bool overlap = true;
   hipEvent_t ready;
   HIPCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  if (!c->gstream) {
+    HIPCHK(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->gdone, hipEventDisableTiming));
+  }
   auto fail = [&](int code) {
+    hipStreamSynchronize(c->gstream);
     hipStreamSynchronize(c->stream);
     hipEventDestroy(ready);
     return code;
@@ -933,8 +952,13 @@ extern "C" int hg_run_gravity(hg_ctx *c, double *time, int64_t *tn, double *t_ou
     if (s.tn == 0) {
       *grvtime = 0.0;
     } else {
-      const int ns = enqueue_gravity(c, s.time, s.dt, grvtime, cgrav, cgrav_old, &dtgrav);
+      hipStream_t gs = c->gstream;
+      if (hipStreamWaitEvent(gs, ready, 0) != hipSuccess) return fail(HG_ERR_DEVICE);
+      const int ns = enqueue_gravity(c, s.time, s.dt, grvtime, cgrav, cgrav_old, &dtgrav, gs);
       if (ns < 1) return fail(HG_ERR_STATE);
+      if (hipEventRecord(c->gdone, gs) != hipSuccess ||
+          hipStreamWaitEvent(c->stream, c->gdone, 0) != hipSuccess)
+        return fail(HG_ERR_DEVICE);
       nsum += ns;
     }
     Dev De = D;  // the step end reduces the partials the last stage wrote

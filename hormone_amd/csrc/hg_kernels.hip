@@ -245,6 +245,42 @@ __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base
 // line base and segment (-1 for padding) for the stores.
 __device__ __forceinline__ int tile_ld(int bw) { return bw + 1; }
 
+// The fused launch's lane path keeps its own cell's primitives in LDS
+// between the load and the RK update: d, e and B are U[0], U[4] and
+// U[5..7] (the same buffers) and d, v feed the gravity source, so the
+// update reads none of them from memory again, and they hold no registers
+// across the flux computation.  Layout [field][thread], fields d e v1 v2
+// v3 (b1 b2 b3); each thread reads back only what it wrote.
+__host__ __device__ inline int park_fields(bool mag) { return mag ? 8 : 5; }
+template <bool MAG>
+__device__ __forceinline__ void park_cell(double *lds, const Cell &q) {
+  double *p = lds + threadIdx.x;
+  const int bs = blockDim.x;
+  p[0] = q.d;
+  p[bs] = q.e;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p[(2 + a) * bs] = q.v[a];
+  if (MAG) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[(5 + a) * bs] = q.b[a];
+  }
+}
+template <bool MAG>
+__device__ __forceinline__ Cell parked_cell(const double *lds) {
+  const double *p = lds + threadIdx.x;
+  const int bs = blockDim.x;
+  Cell q;
+  q.d = p[0];
+  q.e = p[bs];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q.v[a] = p[(2 + a) * bs];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) q.b[a] = MAG ? p[(5 + a) * bs] : 0.0;
+  q.eint = 0.0;
+  q.phi = 0.0;
+  return q;
+}
+
 // ---- packed sweep lines ----------------------------------------------------
 // The lines of a direction, each extended by its two ghost cells at either
 // end (n+4 positions), lie end to end in sequences: all x lines form one
@@ -602,6 +638,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   lane_cell<DIR>(D, A, wave, lane, line, base, c);
   const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, c, lds)
                       : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
+  if (FUSED && !TILE) park_cell<MAG>(lds, q);
   // d e m1 m2 m3 b1 b2 b3 and eint (slopes) / phi (face) of the cell
   const double qv[9] = {q.d, q.e, q.d * q.v[0], q.d * q.v[1], q.d * q.v[2],
                         q.b[0], q.b[1], q.b[2], q.eint};
@@ -763,6 +800,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   lane_cell<DIR>(D, A, wave, lane, line, base, c);
   const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, c, lds)
                       : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
+  if (FUSED && !TILE) park_cell<MAG>(lds, q);
   const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
   const double qd[8] = {q.d, q.e, m0, m1, m2, q.b[0], q.b[1], q.b[2]};
   const int cc = c < -2 ? -2 : (c > n + 1 ? n + 1 : c);
@@ -1094,14 +1132,17 @@ __device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, doubl
 // Gravity source (source.f90:14-85 Cartesian, get_fieldforce :177-228):
 // g_a = -(non-uniform central difference of phi) * rho with the stage-start
 // rho and v; src(imo_a) = g_a, src(iene) = g.v.
-__device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVMAX]) {
+// qc: the cell's current primitives when the caller holds them (the fused
+// launch's lane path), else read from W.
+__device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVMAX],
+                                            const Cell *qc = nullptr) {
   int i, j, k;
   cell_ijk(D, m, i, j, k);
   const double *ph = D.gphi[D.gcur];
   const long g = gidx(D, i, j, k);
   const long st[3] = {1, (long)(D.n1 + 2), (long)(D.n1 + 2) * (D.n2 + 2)};
   const int q3[3] = {i, j, k};
-  const double d = D.W[WD][m];
+  const double d = qc ? qc->d : D.W[WD][m];
   double gr[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -1117,24 +1158,28 @@ __device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVM
   src[1] = gr[0];
   src[2] = gr[1];
   src[3] = gr[2];
-  src[4] = gr[0] * D.W[WV1][m] + gr[1] * D.W[WV2][m] + gr[2] * D.W[WV3][m];
+  if (qc) src[4] = gr[0] * qc->v[0] + gr[1] * qc->v[1] + gr[2] * qc->v[2];
+  else src[4] = gr[0] * D.W[WV1][m] + gr[1] * D.W[WV2][m] + gr[2] * D.W[WV3][m];
 }
 
 // LD >= 0: direction LD's flux divergence comes from duL[] (the fused last
 // sweep) instead of memory.  The flags are plain arguments so that one
 // inlined copy serves every variant; k_rk passes constants.
+// qc: the cell's current primitives (parked_cell) when the caller holds
+// them: d, e and B are then not read from U again.
 template <bool MAG, int LD = -1>
 __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, double faco,
                                            double fact, double facn, double damp, bool CFL,
                                            bool DAMP, bool GRAV, double &dti, double &cfm,
-                                           const double *duL = nullptr) {
+                                           const double *duL = nullptr,
+                                           const Cell *qc = nullptr) {
   // the fused form writes the new state to the other buffer set (the
   // launch's neighbours still read this one); k_rk updates in place
   double *const *Uo = LD >= 0 ? D.Uo : D.U;
   double *const *Wo = LD >= 0 ? D.Wo : D.W;
   const double ft = fact * D.sc->dt;
   double src[NVMAX];
-  if (GRAV) grav_source(D, m, src);
+  if (GRAV) grav_source(D, m, src, qc);
   double un[NVMAX];
   double dold = 0.0;  // stage-start density (primitive d = u(icnt))
   // a bound known at compile time for hydro, so that the loop unrolls and
@@ -1149,7 +1194,11 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
     const double b = LD == 1 ? duL[v] : (D.solve[1] ? D.DU[1][v][m] : 0.0);
     const double c = LD == 2 ? duL[v] : (D.solve[2] ? D.DU[2][v][m] : 0.0);
     const double acc = ((a + b) + c) + (GRAV ? src[v] : 0.0);  // + src
-    const double u = D.U[v][m];
+    double u;
+    if (qc && v == 0) u = qc->d;
+    else if (qc && v == 4) u = qc->e;
+    else if (qc && MAG && v >= 5 && v <= 7) u = qc->b[v - 5];
+    else u = D.U[v][m];
     double u0;
     if (stage == 1) {
       u0 = u;
@@ -1236,10 +1285,12 @@ __device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base,
                                          double &cfm) {
   const int n = A.n;
   const int nv = MAG ? (D.i9wv >= 0 ? 9 : 8) : 5;
-  if (!TILE) {
-    if (lane >= 2 && lane <= 61 && c >= 0 && c < n)
+  if (!TILE) {  // the lane's own cell, its primitives parked in LDS
+    if (lane >= 2 && lane <= 61 && c >= 0 && c < n) {
+      const Cell qc = parked_cell<MAG>(lds);
       rk_cell_rt<MAG, DIR>(D, base + (long)c * A.stride, stage, rk.faco, rk.fact, rk.facn,
-                           rk.damp, rk.cfl, rk.dampf, rk.grav, dti, cfm, du);
+                           rk.damp, rk.cfl, rk.dampf, rk.grav, dti, cfm, du, &qc);
+    }
     return;
   }
   const int bw = blockDim.x >> 6, w = threadIdx.x >> 6;
@@ -1882,8 +1933,10 @@ static void sweep_rk_variant(const Dev &D, int stage, const RKArgs &rk, unsigned
       return;
     }
   }
-  if (D.mixed) klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false, true>, g, bs, 0, s, ev, D, stage, rk);
-  else klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false>, g, bs, 0, s, ev, D, stage, rk);
+  // the lane path parks each thread's primitives (park_cell)
+  const size_t park = (size_t)park_fields(MAG) * sizeof(double) * (size_t)bs;
+  if (D.mixed) klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false, true>, g, bs, park, s, ev, D, stage, rk);
+  else klaunch_lds(k_sweep_rk<DIR, MAG, LIM, false>, g, bs, park, s, ev, D, stage, rk);
 }
 
 template <int DIR>
@@ -2003,9 +2056,19 @@ void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, b
 // the new phi and psi land in gphi[1-gcur] and gpsi1 (the caller swaps).
 void launch_grav_substep(const Dev &D, int gk, double dtg, double cg, double damp_factor,
                          hipStream_t s, const KEv *ev) {
-  // rows of 64 cells, 4 rows per block (one 256-cell row per block when the
-  // box has fewer than 4 rows)
-  const unsigned bx = D.n2 >= 4 ? 64 : 256, by = D.n2 >= 4 ? 4 : 1;
+  // 256 threads per block: the widest of 256, 128 or 64 cells that divides
+  // a row, times 1, 2 or 4 rows (fewer rows in a block: fewer rows that
+  // neighbouring blocks both read); 64 x 4 when none divides, one 256-cell
+  // row per block when the box has fewer than 4 rows
+  unsigned bx = D.n2 >= 4 ? 64 : 256, by = D.n2 >= 4 ? 4 : 1;
+  if (D.n2 >= 4) {
+    for (unsigned w = 256; w >= 64; w /= 2)
+      if (D.n1 % (int)w == 0) {
+        bx = w;
+        by = 256 / w;
+        break;
+      }
+  }
   const unsigned g = nblk(D.n1, (int)bx) * nblk(D.n2, (int)by) * (unsigned)D.n3;
   const dim3 b(bx, by);
   double fo1, ft1, fn1;
