@@ -1286,15 +1286,21 @@ static int grow(T **p, size_t *cap, size_t need) {
 
 static void batch_enqueue_step(hg_batch *b) {
   const Batch &B = b->B;
-  const long sw = (B.nsitems + b->sweep_block_waves - 1) / b->sweep_block_waves;  // chunks
-  const unsigned gs = b->sweep_grid ? (unsigned)std::min<long>(b->sweep_grid, sw) : (unsigned)sw;
+  // per direction: chunks of one item per wavefront, or the persistent grid
+  unsigned gs[3];
+  for (int d = 0; d < 3; ++d) {
+    const long sw = (B.dend[d] - B.dbeg[d] + b->sweep_block_waves - 1) / b->sweep_block_waves;
+    gs[d] = b->sweep_grid ? (unsigned)std::min<long>(b->sweep_grid, sw) : (unsigned)sw;
+  }
   const unsigned gr = b->rk_grid ? (unsigned)std::min<long>(b->rk_grid, B.ncitems)
                                  : (unsigned)B.ncitems;
   for (int stage = 1; stage <= b->rktype; ++stage) {
     double faco, fact, facn;
     runge_coeff(stage, b->rktype, &faco, &fact, &facn);
-    launch_bsweep_blocks(B, b->mag, b->lim, b->mixed, stage, gs, b->sweep_block_waves,
-                         b->stream);
+    for (int d = 0; d < 3; ++d)
+      if (gs[d] > 0)
+        launch_bsweep_dir(B, d, b->mag, b->lim, b->mixed, stage, gs[d], b->sweep_block_waves,
+                          b->stream);
     launch_brk(B, b->mag, stage, faco, fact, facn, stage == b->rktype, gr, b->stream);
   }
   launch_bstep_end_cfl(B, b->stream);
@@ -1308,12 +1314,18 @@ static int batch_rebuild(hg_batch *b) {
   for (size_t q = 0; q < nb; ++q) {
     const Dev &D = b->members[q]->D;
     boxes[q] = D;
-    // the box's wavefronts as numbered in its own sweep launch (padded
-    // direction ranges); the count is a multiple of 16, so every box's range
-    // starts on a multiple of 16 and aligned chunks never mix boxes
-    const long nwv = D.woff[2] + (D.waves[2] + 15) / 16 * 16;
-    for (long w = 0; w < nwv; ++w) si.push_back(make_int2((int)q, (int)w));
     for (long k = 0; k < D.nblk_cfl; ++k) ci.push_back(make_int2((int)q, (int)k));
+  }
+  // direction by direction, each box's wavefronts of the direction padded
+  // to a multiple of 16, so every box's range starts on a multiple of 16
+  // and aligned chunks never mix boxes
+  for (int d = 0; d < 3; ++d) {
+    b->B.dbeg[d] = (long)si.size();
+    for (size_t q = 0; q < nb; ++q) {
+      const long nwv = (b->members[q]->D.waves[d] + 15) / 16 * 16;
+      for (long w = 0; w < nwv; ++w) si.push_back(make_int2((int)q, (int)w));
+    }
+    b->B.dend[d] = (long)si.size();
   }
   int st;
   if ((st = grow(&b->d_boxes, &b->cap_boxes, nb ? nb : 1))) return st;

@@ -1611,64 +1611,42 @@ __device__ __forceinline__ void step_end(const Dev &D) {
 
 
 // ---- batched boxes: persistent kernels over flattened work-lists ---------
-// Work items are (box, wavefront) with each box's wavefronts numbered as in
-// a single-box sweep launch (x from 0, y from woff[1], z from woff[2]; the
-// ranges padded to multiples of 16 with items that do nothing), and every
-// box's item range starts on a multiple of 16.  A block takes its items in
-// aligned chunks of one per wavefront, so a chunk never mixes boxes or
-// directions and its y/z lines can share an LDS tile (tile_ok: the launch
-// carries the LDS; the box decides by its x extent).
-template <bool MAG, int LIM, bool MIX>
-__device__ __forceinline__ void bsweep_item(const Dev &D, long w, int lane, int stage,
-                                            bool tiled, double *lds) {
-  if (w < D.woff[1]) {
-    if (w < D.waves[0]) {
-      if (MIX) sweep_line_mix<0, MAG, LIM, false, false, false>(D, w, lane, stage);
-      else sweep_line<0, MAG, LIM, false, false, false, false>(D, w, lane, stage);
-    }
-    return;
-  }
-  // tiled: padding waves of a partly filled chunk join its barriers
-  const long wid = threadIdx.x >> 6;
-  if (w < D.woff[2]) {
-    w -= D.woff[1];
-    if (tiled ? w - wid >= D.waves[1] : w >= D.waves[1]) return;
-    if (tiled) {
-      if (MIX) sweep_line_mix<1, MAG, LIM, true, false, false>(D, w, lane, stage, lds);
-      else sweep_line<1, MAG, LIM, true, false, false, false>(D, w, lane, stage, lds);
-    } else {
-      if (MIX) sweep_line_mix<1, MAG, LIM, false, false, false>(D, w, lane, stage);
-      else sweep_line<1, MAG, LIM, false, false, false, false>(D, w, lane, stage);
-    }
-    return;
-  }
-  w -= D.woff[2];
-  if (tiled ? w - wid >= D.waves[2] : w >= D.waves[2]) return;
-  if (tiled) {
-    if (MIX) sweep_line_mix<2, MAG, LIM, true, false, false>(D, w, lane, stage, lds);
-    else sweep_line<2, MAG, LIM, true, false, false, false>(D, w, lane, stage, lds);
-  } else {
-    if (MIX) sweep_line_mix<2, MAG, LIM, false, false, false>(D, w, lane, stage);
-    else sweep_line<2, MAG, LIM, false, false, false, false>(D, w, lane, stage);
-  }
-}
-
-template <bool MAG, int LIM, bool MIX>
-__global__ __launch_bounds__(1024) void k_bsweep(Batch B, int stage, int tile_ok) {
+// One launch per direction (as the single-box x-fused form, a third of the
+// all-direction code each, within the instruction cache).  Its work items
+// are (box, wavefront of the direction), every box's range padded to a
+// multiple of 16 items that do nothing past the box's last wavefront.  A
+// block takes its items in aligned chunks of one per wavefront, so a chunk
+// never mixes boxes and its y/z lines can share an LDS tile (tile_ok: the
+// launch carries the LDS); y and z chunks go to the XCDs in contiguous
+// ranges (xcd_block), as the single-box y/z launches.
+template <int DIR, bool MAG, int LIM, bool MIX>
+__global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage, int tile_ok) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x & 63;
   const int bw = blockDim.x >> 6;
-  const long nchunk = (B.nsitems + bw - 1) / bw;
-  for (long g = blockIdx.x; g < nchunk; g += gridDim.x) {
-    const long it = g * bw + (threadIdx.x >> 6);
-    if (it >= B.nsitems) continue;  // only in a last, untiled chunk
+  const long wid = threadIdx.x >> 6;
+  const long ibeg = B.dbeg[DIR], nit = B.dend[DIR] - ibeg;
+  const long nchunk = (nit + bw - 1) / bw;
+  const bool tiled = DIR > 0 && tile_ok && bw >= 4;
+  const unsigned b0 = DIR == 0 ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+  for (long g = b0; g < nchunk; g += gridDim.x) {
+    const long it = g * bw + wid;
+    if (it >= nit) continue;  // only in a last, untiled chunk
     // the work item is the same for the whole wavefront: make that visible
     // so the box record is read with scalar loads
-    const int2 x = B.sitems[it];
+    const int2 x = B.sitems[ibeg + it];
     const Dev &D = B.boxes[__builtin_amdgcn_readfirstlane(x.x)];
     if (D.sc->stop) continue;  // uniform over the chunk (one box)
-    const bool tiled = tile_ok && bw >= 4;
-    bsweep_item<MAG, LIM, MIX>(D, __builtin_amdgcn_readfirstlane(x.y), lane, stage, tiled, lds);
+    const long w = __builtin_amdgcn_readfirstlane(x.y);
+    // tiled: padding waves of a partly filled chunk join its barriers
+    if (tiled ? w - wid >= D.waves[DIR] : w >= D.waves[DIR]) continue;
+    if (tiled) {
+      if (MIX) sweep_line_mix<DIR, MAG, LIM, true, false, false>(D, w, lane, stage, lds);
+      else sweep_line<DIR, MAG, LIM, true, false, false, false>(D, w, lane, stage, lds);
+    } else {
+      if (MIX) sweep_line_mix<DIR, MAG, LIM, false, false, false>(D, w, lane, stage);
+      else sweep_line<DIR, MAG, LIM, false, false, false, false>(D, w, lane, stage);
+    }
   }
 }
 
@@ -2103,31 +2081,39 @@ void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
   k_phidamp<<<nblk(D.ncell, 256), 256, 0, s>>>(D, factor);
 }
 
-template <bool MAG, int LIM>
-static void bsweep_mix(const Batch &B, int mixed, int stage, unsigned g, int bs,
-                       hipStream_t s) {
-  // LDS for the y/z tile when it fits (the boxes' own x extents decide)
+template <int DIR, bool MAG, int LIM>
+static void bsweep_dir_mix(const Batch &B, int mixed, int stage, unsigned g, int bs,
+                           hipStream_t s) {
+  // LDS for the y/z tile when it fits
   const int bw = bs / 64;
   const size_t nf = MAG ? NWMAX : WEINT + 1;
   size_t tile = (nf * 64 * (size_t)(bw + 1) + 2 * (size_t)bw) * sizeof(double);
-  if (bw < 4 || tile > (size_t)B.lds_max) tile = 0;
+  if (DIR == 0 || bw < 4 || tile > (size_t)B.lds_max) tile = 0;
   const int ok = tile > 0 ? 1 : 0;
-  if (mixed) k_bsweep<MAG, LIM, true><<<g, bs, tile, s>>>(B, stage, ok);
-  else k_bsweep<MAG, LIM, false><<<g, bs, tile, s>>>(B, stage, ok);
+  if (mixed) k_bsweep_dir<DIR, MAG, LIM, true><<<g, bs, tile, s>>>(B, stage, ok);
+  else k_bsweep_dir<DIR, MAG, LIM, false><<<g, bs, tile, s>>>(B, stage, ok);
 }
 
-void launch_bsweep_blocks(const Batch &B, int mag, int lim, int mixed, int stage,
-                          unsigned g, int block_waves, hipStream_t s) {
-  const int bs = 64 * block_waves;
+template <int DIR>
+static void bsweep_dir_v(const Batch &B, int mag, int lim, int mixed, int stage, unsigned g,
+                         int bs, hipStream_t s) {
   if (mag) {
-    if (lim == LIM_FLAT) bsweep_mix<true, LIM_FLAT>(B, mixed, stage, g, bs, s);
-    else if (lim == LIM_MINMOD) bsweep_mix<true, LIM_MINMOD>(B, mixed, stage, g, bs, s);
-    else bsweep_mix<true, LIM_MC>(B, mixed, stage, g, bs, s);
+    if (lim == LIM_FLAT) bsweep_dir_mix<DIR, true, LIM_FLAT>(B, mixed, stage, g, bs, s);
+    else if (lim == LIM_MINMOD) bsweep_dir_mix<DIR, true, LIM_MINMOD>(B, mixed, stage, g, bs, s);
+    else bsweep_dir_mix<DIR, true, LIM_MC>(B, mixed, stage, g, bs, s);
   } else {
-    if (lim == LIM_FLAT) bsweep_mix<false, LIM_FLAT>(B, mixed, stage, g, bs, s);
-    else if (lim == LIM_MINMOD) bsweep_mix<false, LIM_MINMOD>(B, mixed, stage, g, bs, s);
-    else bsweep_mix<false, LIM_MC>(B, mixed, stage, g, bs, s);
+    if (lim == LIM_FLAT) bsweep_dir_mix<DIR, false, LIM_FLAT>(B, mixed, stage, g, bs, s);
+    else if (lim == LIM_MINMOD) bsweep_dir_mix<DIR, false, LIM_MINMOD>(B, mixed, stage, g, bs, s);
+    else bsweep_dir_mix<DIR, false, LIM_MC>(B, mixed, stage, g, bs, s);
   }
+}
+
+void launch_bsweep_dir(const Batch &B, int dir, int mag, int lim, int mixed, int stage,
+                       unsigned g, int block_waves, hipStream_t s) {
+  const int bs = 64 * block_waves;
+  if (dir == 0) bsweep_dir_v<0>(B, mag, lim, mixed, stage, g, bs, s);
+  else if (dir == 1) bsweep_dir_v<1>(B, mag, lim, mixed, stage, g, bs, s);
+  else bsweep_dir_v<2>(B, mag, lim, mixed, stage, g, bs, s);
 }
 
 void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, double facn,

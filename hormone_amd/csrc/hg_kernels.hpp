@@ -143,15 +143,20 @@ __host__ __device__ inline unsigned long long errkey(int stage, int phase, int d
 struct Batch {
   const Dev *boxes;
   int nbox;
-  const int2 *sitems;  // (box, wavefront) in 16-aligned per-box ranges
+  // (box, wavefront of the direction) items, direction by direction: those
+  // of direction d are sitems[dbeg[d] .. dend[d]), each box's range padded
+  // to a multiple of 16 items
+  const int2 *sitems;
   long nsitems;
+  long dbeg[3], dend[3];
   int lds_max;         // LDS bytes one block may use
   const int2 *citems;
   long ncitems;
 };
 
-void launch_bsweep_blocks(const Batch &B, int mag, int lim, int mixed, int stage,
-                          unsigned grid, int block_waves, hipStream_t s);
+// one direction's sweep over the batch (k_bsweep_dir)
+void launch_bsweep_dir(const Batch &B, int dir, int mag, int lim, int mixed, int stage,
+                       unsigned grid, int block_waves, hipStream_t s);
 void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, double facn,
                 bool last, unsigned grid, hipStream_t s);
 void launch_bstep_end_cfl(const Batch &B, hipStream_t s);  // per box: step_end, cfl_final
