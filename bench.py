@@ -99,6 +99,8 @@ def parse():
     ap.add_argument("--serving-lanes", default="50000,500000",
                     help="serving: cell-count bounds of the concurrently served size classes "
                          "(comma separated; empty = one batch)")
+    ap.add_argument("--batch-form", default="fused", choices=["fused", "separate"],
+                    help="serving: the batches' x-fused form or the separate RK update")
     ap.add_argument("--sweep-waves", type=int, default=0,
                     help="sweep wavefronts per block (0: autotune during warm-up)")
     ap.add_argument("--fused", default="auto", choices=["auto", "last", "x", "off"],
@@ -511,7 +513,7 @@ def serving_bench(a):
     torch.cuda.synchronize()
     lanes = tuple(int(x) for x in a.serving_lanes.split(",") if x) if a.serving_lanes else ()
     rep = S.serve(reqs, hbm_budget=a.hbm_budget_gib * 2 ** 30, steps_per_round=4,
-                  digest_ids=check, lanes=lanes)
+                  digest_ids=check, lanes=lanes, batch_fused=a.batch_form == "fused")
     torch.cuda.synchronize()
     sm = rep.summary()
     done = [r for r in reqs if r.finished is not None and r.error is None]
@@ -528,7 +530,7 @@ def serving_bench(a):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded request generator)",
         "config": {"workload": f"serving: {a.requests} Poisson(50/s) MHD Riemann/blast requests, "
                                f"64^2..192^3 cells, t_end 0.1, HBM budget {a.hbm_budget_gib:g} GiB",
-                   "requests": a.requests,
+                   "requests": a.requests, "batch_form": a.batch_form,
                    "size_classes": [f"<= {b} cells" for b in lanes] + ["larger"]},
         "roofline": None, "cpu_baseline": None,
         "p50_time_to_solution_s": sm["p50_s"], "p99_time_to_solution_s": sm["p99_s"],

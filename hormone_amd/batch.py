@@ -39,6 +39,8 @@ def _bind():
     L.hg_batch_member_state.argtypes = [vp, vp, _dp, C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int32)]
     L.hg_batch_config.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.hg_batch_set_fused.argtypes = [vp, C.c_int32]
+    L.hg_batch_form.argtypes = [vp, C.POINTER(C.c_int32)]
     _bound = True
     return L
 
@@ -97,6 +99,21 @@ class Batch:
         if st:
             raise native.HgError(st, "hg_batch_member_state")
         return t.value, n.value, s.value
+
+    def set_fused(self, on: bool = True):
+        """x sweep with the RK update (the default, when every member sweeps
+        x) or the separate RK update; the same bits either way."""
+        st = self._lib.hg_batch_set_fused(self.h, int(bool(on)))
+        if st:
+            raise native.HgError(st, "hg_batch_set_fused")
+
+    def fused(self) -> bool:
+        """Whether the launches were last built in the x-fused form."""
+        f = C.c_int32()
+        st = self._lib.hg_batch_form(self.h, C.byref(f))
+        if st:
+            raise native.HgError(st, "hg_batch_form")
+        return bool(f.value)
 
     def configure(self, sweep_block_waves=4, sweep_grid=0, rk_grid=0, use_graph=True):
         st = self._lib.hg_batch_config(self.h, sweep_block_waves, sweep_grid, rk_grid,

@@ -45,6 +45,10 @@ struct hg_ctx {
   double *alt_U[NVMAX] = {}, *alt_W[NWMAX] = {};
   bool alt_ready = false;
   bool in_batch = false;
+  // the batch it is in, and whether its sets were swapped on the host since
+  // the batch last built its device records (batch_member_current)
+  struct hg_batch *batch = nullptr;
+  int bswap = 0;
   // hg_run_gravity: the gravity sub-steps run on their own stream, beside
   // stage 1's flux sweep; gdone joins them back before stage 1's update
   hipStream_t gstream = nullptr;
@@ -411,8 +415,11 @@ static int upload_grvphi(hg_ctx *c, const double *a) {
   return HG_OK;
 }
 
+static int batch_member_current(hg_ctx *c);
+
 extern "C" int hg_upload(hg_ctx *c, int32_t field, const double *a) {
   if (!c || !a) return HG_ERR_ARG;
+  if (int st = batch_member_current(c)) return st;
   if (field == HG_F_GRVPHI || field == HG_F_GRVPSI) {
     if (!c->D.grav) return HG_ERR_ARG;
     HIPCHK(hipSetDevice(c->device));
@@ -468,6 +475,7 @@ static int ensure_eint(hg_ctx *c) {
 
 extern "C" int hg_download(hg_ctx *c, int32_t field, double *a) {
   if (!c || !a) return HG_ERR_ARG;
+  if (int st = batch_member_current(c)) return st;
   HIPCHK(hipSetDevice(c->device));
   const Dev &D = c->D;
   if (!D.mag && field >= HG_F_B1 && field <= HG_F_PHI) {
@@ -1043,6 +1051,45 @@ extern "C" int hg_set_boundary_flux(hg_ctx *c, int32_t face, const double *flux)
   return HG_OK;
 }
 
+// The fused form's buffers: the second state set (the fused launch reads
+// one and writes the other, then the two swap; d, e and B alias U in both,
+// as in hg_create; a phi the update does not write, 1D MHD, is shared) and
+// one CFL partial per block of direction L's fused launch (one wavefront
+// per block at most).
+static int ensure_fused_buffers(hg_ctx *c, int L) {
+  Dev &D = c->D;
+  if (!c->alt_ready) {
+    const size_t fb = sizeof(double) * (size_t)D.ncell;
+    for (int v = 0; v < D.nv; ++v) {
+      int st = dev_alloc(c, fb, (void **)&c->alt_U[v]);
+      if (st) return st;
+    }
+    for (int w = 0; w < D.nw; ++w) {
+      if (w == WD) c->alt_W[w] = c->alt_U[0];
+      else if (w == WE) c->alt_W[w] = c->alt_U[4];
+      else if (D.mag && w >= WB1 && w <= WB3) c->alt_W[w] = c->alt_U[5 + (w - WB1)];
+      else if (w == WPHI && D.i9wv < 0) c->alt_W[w] = D.W[w];
+      else if (int st = dev_alloc(c, fb, (void **)&c->alt_W[w])) return st;
+    }
+    c->alt_ready = true;
+  }
+  if (D.waves[L] > c->part_cap) {
+    double *p = nullptr;
+    int st = dev_alloc(c, sizeof(double) * 2 * (size_t)D.waves[L], (void **)&p);
+    if (st) return st;
+    D.part = p;
+    c->part_cap = D.waves[L];
+  }
+  return HG_OK;
+}
+
+// The current state into the other buffer set's pointers (the fused form's
+// new state becomes the current one).
+static void swap_state_sets(hg_ctx *c) {
+  for (int v = 0; v < c->D.nv; ++v) std::swap(c->D.U[v], c->alt_U[v]);
+  for (int w = 0; w < c->D.nw; ++w) std::swap(c->D.W[w], c->alt_W[w]);
+}
+
 extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
   if (!c) return HG_ERR_ARG;
   HIPCHK(hipSetDevice(c->device));
@@ -1068,33 +1115,8 @@ extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
   for (int a = 0; a < 3; ++a)
     if (D.ax[a].ext) return HG_ERR_UNSUPPORTED;  // fused kernels carry codes 0-3 only
   if (c->in_batch) return HG_ERR_UNSUPPORTED;
-  // the second buffer set: the fused launch reads one and writes the
-  // other, then the two swap (d, e and B alias U in both, as in hg_create;
-  // a phi the update does not write, 1D MHD, is shared)
-  if (!c->alt_ready) {
-    const size_t fb = sizeof(double) * (size_t)D.ncell;
-    for (int v = 0; v < D.nv; ++v) {
-      int st = dev_alloc(c, fb, (void **)&c->alt_U[v]);
-      if (st) return st;
-    }
-    for (int w = 0; w < D.nw; ++w) {
-      if (w == WD) c->alt_W[w] = c->alt_U[0];
-      else if (w == WE) c->alt_W[w] = c->alt_U[4];
-      else if (D.mag && w >= WB1 && w <= WB3) c->alt_W[w] = c->alt_U[5 + (w - WB1)];
-      else if (w == WPHI && D.i9wv < 0) c->alt_W[w] = D.W[w];
-      else if (int st = dev_alloc(c, fb, (void **)&c->alt_W[w])) return st;
-    }
-    c->alt_ready = true;
-  }
-  // one CFL partial per block of the fused launch (one wavefront per block
-  // at most)
-  if (D.waves[L] > c->part_cap) {
-    double *p = nullptr;
-    int st = dev_alloc(c, sizeof(double) * 2 * (size_t)D.waves[L], (void **)&p);
-    if (st) return st;
-    D.part = p;
-    c->part_cap = D.waves[L];
-  }
+  int st = ensure_fused_buffers(c, L);
+  if (st) return st;
   D.fdir = L;
   c->fused = true;
   return HG_OK;
@@ -1165,6 +1187,13 @@ struct hg_batch {
   // the persistent sweep / rk launches (0 = one item per wavefront / block)
   int sweep_block_waves = 4;
   unsigned sweep_grid = 0, rk_grid = 0;
+  // the x-fused form (hg_batch_set_fused): taken when every member sweeps
+  // x; d_bpar[r] is the state set of rec_members[r], the member order of
+  // the device records (the order of the last rebuild)
+  bool want_fused = true, fused = false;
+  int *d_bpar = nullptr;
+  size_t cap_bpar = 0;
+  std::vector<hg_ctx *> rec_members;
 };
 
 static void batch_drop_graph(hg_batch *b) {
@@ -1172,6 +1201,46 @@ static void batch_drop_graph(hg_batch *b) {
   if (b->graph) hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
+}
+
+// After steps of the x-fused form a member's state may sit in its second
+// buffer set (d_bpar[r] = 1): swap the context's sets back so that its own
+// record (hg_ctx::D) holds the state again.  `only`: just that member (it
+// leaves the batch); nullptr: every member of the device records.
+static int batch_normalize(hg_batch *b, hg_ctx *only) {
+  if (!b->fused || b->rec_members.empty()) return HG_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<int> par(b->rec_members.size());
+  HIPCHK(hipMemcpy(par.data(), b->d_bpar, par.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (size_t r = 0; r < par.size(); ++r) {
+    hg_ctx *c = b->rec_members[r];
+    if (!c || (only && c != only)) continue;
+    if (par[r] ^ c->bswap) swap_state_sets(c);
+    c->bswap = 0;
+    b->rec_members[r] = nullptr;  // its records are stale until the rebuild
+  }
+  return HG_OK;
+}
+
+// A member's own record (hg_ctx::D) onto the set that holds its state now,
+// for hg_download / hg_upload between batch steps; the device records are
+// left as they are (bswap remembers the host-side swap).
+static int batch_member_current(hg_ctx *c) {
+  hg_batch *b = c->batch;
+  if (!b || !b->fused) return HG_OK;
+  for (size_t r = 0; r < b->rec_members.size(); ++r)
+    if (b->rec_members[r] == c) {
+      HIPCHK(hipSetDevice(b->device));
+      HIPCHK(hipStreamSynchronize(b->stream));
+      int par = 0;
+      HIPCHK(hipMemcpy(&par, b->d_bpar + r, sizeof(int), hipMemcpyDeviceToHost));
+      if (par ^ c->bswap) {
+        swap_state_sets(c);
+        c->bswap ^= 1;
+      }
+      return HG_OK;
+    }
+  return HG_OK;  // added since the last build: its state is in its own set
 }
 
 extern "C" int hg_batch_create(int32_t device, hg_batch **out) {
@@ -1198,9 +1267,15 @@ extern "C" int hg_batch_destroy(hg_batch *b) {
   if (b->d_sitems) hipFree(b->d_sitems);
   if (b->d_citems) hipFree(b->d_citems);
   if (b->h_sc) hipHostFree(b->h_sc);
+  const int st = batch_normalize(b, nullptr);  // members left in the batch
+  for (hg_ctx *c : b->members) {  // ordinary contexts again
+    c->in_batch = false;
+    c->batch = nullptr;
+  }
+  if (b->d_bpar) hipFree(b->d_bpar);
   hipStreamDestroy(b->stream);
   delete b;
-  return HG_OK;
+  return st;
 }
 
 // Room for `need` member records in the pinned host array, keeping the
@@ -1252,6 +1327,8 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
   b->members.push_back(c);
   b->sc_valid.push_back(0);
   c->in_batch = true;
+  c->batch = b;
+  c->bswap = 0;
   b->dirty = true;
   return HG_OK;
 }
@@ -1261,9 +1338,12 @@ extern "C" int hg_batch_remove(hg_batch *b, hg_ctx *c) {
   for (size_t q = 0; q < b->members.size(); ++q)
     if (b->members[q] == c) {
       HIPCHK(hipStreamSynchronize(b->stream));
+      const int st = batch_normalize(b, c);
+      if (st) return st;
       b->members.erase(b->members.begin() + q);
       b->sc_valid.erase(b->sc_valid.begin() + q);
       c->in_batch = false;
+      c->batch = nullptr;
       // keep the cached member states aligned with the member list (the
       // array holds a record per member since hg_batch_add)
       for (size_t r = q; r < b->members.size(); ++r) b->h_sc[r] = b->h_sc[r + 1];
@@ -1297,24 +1377,64 @@ static void batch_enqueue_step(hg_batch *b) {
   for (int stage = 1; stage <= b->rktype; ++stage) {
     double faco, fact, facn;
     runge_coeff(stage, b->rktype, &faco, &fact, &facn);
+    const bool last = stage == b->rktype;
+    if (B.fused) {  // y, z, then x with the RK update
+      for (int d = 1; d < 3; ++d)
+        if (gs[d] > 0)
+          launch_bsweep_dir(B, d, b->mag, b->lim, b->mixed, stage, gs[d], b->sweep_block_waves,
+                            b->stream);
+      launch_bsweep_rk(B, b->mag, b->lim, b->mixed, stage, faco, fact, facn, last, gs[0],
+                       b->sweep_block_waves, b->stream);
+      continue;
+    }
     for (int d = 0; d < 3; ++d)
       if (gs[d] > 0)
         launch_bsweep_dir(B, d, b->mag, b->lim, b->mixed, stage, gs[d], b->sweep_block_waves,
                           b->stream);
-    launch_brk(B, b->mag, stage, faco, fact, facn, stage == b->rktype, gr, b->stream);
+    launch_brk(B, b->mag, stage, faco, fact, facn, last, gr, b->stream);
   }
   launch_bstep_end_cfl(B, b->stream);
 }
 
 static int batch_rebuild(hg_batch *b) {
   batch_drop_graph(b);
+  int st = batch_normalize(b, nullptr);
+  if (st) return st;
   const size_t nb = b->members.size();
-  std::vector<Dev> boxes(nb);
+  // the x-fused form when every member sweeps x
+  bool fused = b->want_fused && nb > 0;
+  for (hg_ctx *c : b->members) fused = fused && c->D.waves[0] > 0;
+  for (size_t q = 0; fused && q < nb; ++q)
+    if ((st = ensure_fused_buffers(b->members[q], 0))) return st;
+  std::vector<Dev> boxes(fused ? 2 * nb : nb);
   std::vector<int2> si, ci;
   for (size_t q = 0; q < nb; ++q) {
-    const Dev &D = b->members[q]->D;
-    boxes[q] = D;
-    for (long k = 0; k < D.nblk_cfl; ++k) ci.push_back(make_int2((int)q, (int)k));
+    hg_ctx *c = b->members[q];
+    const Dev &D = c->D;
+    if (!fused) {
+      boxes[q] = D;
+      for (long k = 0; k < D.nblk_cfl; ++k) ci.push_back(make_int2((int)q, (int)k));
+      continue;
+    }
+    // record 2q reads the member's own set and writes the second, 2q+1 the
+    // reverse; the last stage's CFL partials are one per chunk of x
+    // wavefronts
+    Dev r0 = D, r1 = D;
+    for (int v = 0; v < D.nv; ++v) {
+      r0.Uo[v] = c->alt_U[v];
+      r1.U[v] = c->alt_U[v];
+      r1.Uo[v] = D.U[v];
+    }
+    for (int w = 0; w < D.nw; ++w) {
+      r0.Wo[w] = c->alt_W[w];
+      r1.W[w] = c->alt_W[w];
+      r1.Wo[w] = D.W[w];
+    }
+    r0.fdir = r1.fdir = 0;
+    r0.nblk_cfl = r1.nblk_cfl =
+        (int)((D.waves[0] + b->sweep_block_waves - 1) / b->sweep_block_waves);
+    boxes[2 * q] = r0;
+    boxes[2 * q + 1] = r1;
   }
   // direction by direction, each box's wavefronts of the direction padded
   // to a multiple of 16, so every box's range starts on a multiple of 16
@@ -1327,13 +1447,17 @@ static int batch_rebuild(hg_batch *b) {
     }
     b->B.dend[d] = (long)si.size();
   }
-  int st;
-  if ((st = grow(&b->d_boxes, &b->cap_boxes, nb ? nb : 1))) return st;
+  if ((st = grow(&b->d_boxes, &b->cap_boxes, boxes.size() ? boxes.size() : 1))) return st;
   if ((st = grow(&b->d_sitems, &b->cap_s, si.size() ? si.size() : 1))) return st;
   if ((st = grow(&b->d_citems, &b->cap_c, ci.size() ? ci.size() : 1))) return st;
+  if ((st = grow(&b->d_bpar, &b->cap_bpar, nb ? nb : 1))) return st;
   if (nb) {
-    HIPCHK(hipMemcpyAsync(b->d_boxes, boxes.data(), nb * sizeof(Dev), hipMemcpyHostToDevice,
+    // every member's state is in its own set now (batch_normalize)
+    const std::vector<int> zero(nb, 0);
+    HIPCHK(hipMemcpyAsync(b->d_bpar, zero.data(), nb * sizeof(int), hipMemcpyHostToDevice,
                           b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_boxes, boxes.data(), boxes.size() * sizeof(Dev),
+                          hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_sitems, si.data(), si.size() * sizeof(int2),
                           hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_citems, ci.data(), ci.size() * sizeof(int2),
@@ -1347,6 +1471,11 @@ static int batch_rebuild(hg_batch *b) {
   b->B.lds_max = nb ? b->members[0]->D.lds_max : 0;
   b->B.citems = b->d_citems;
   b->B.ncitems = (long)ci.size();
+  b->B.fused = fused ? 1 : 0;
+  b->B.rkodd = b->rktype & 1;
+  b->B.bpar = b->d_bpar;
+  b->fused = fused;
+  b->rec_members = b->members;
   if ((st = batch_hsc_reserve(b, nb ? nb : 1))) return st;
   if (b->use_graph && nb) {
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1424,6 +1553,24 @@ extern "C" int hg_batch_config(hg_batch *b, int32_t sweep_block_waves, int32_t s
   b->rk_grid = (unsigned)rk_grid;
   b->use_graph = use_graph != 0;
   b->dirty = true;
+  return HG_OK;
+}
+
+// The x-fused form for the batch (on = 1, the default: taken when every
+// member sweeps x; every member then holds a second copy of its state) or
+// the separate RK update (on = 0).  Results are the same bits either way.
+extern "C" int hg_batch_set_fused(hg_batch *b, int32_t on) {
+  if (!b || (on != 0 && on != 1)) return HG_ERR_ARG;
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->want_fused = on != 0;
+  b->dirty = true;
+  return HG_OK;
+}
+
+extern "C" int hg_batch_form(const hg_batch *b, int32_t *fused) {
+  if (!b || !fused) return HG_ERR_ARG;
+  *fused = b->fused ? 1 : 0;
   return HG_OK;
 }
 

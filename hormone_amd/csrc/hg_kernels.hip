@@ -32,6 +32,17 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
 
 
 
+// A pointer read out of device memory (a batch's box record) is generic to
+// the compiler, which then loads and stores through it with flat
+// instructions; passing it through the global address space lets it emit
+// global ones (with a scalar base).  The state, divergence and coordinate
+// arrays are all device (global) memory.  A no-op for pointers the compiler
+// already knows to be global (kernel arguments).
+template <class T>
+__device__ __forceinline__ T *gptr(T *p) {
+  return (T *)(__attribute__((address_space(1))) T *)p;
+}
+
 struct Cell {
   double d, v[3], e, eint, b[3], phi;
 };
@@ -133,8 +144,8 @@ __device__ __forceinline__ void ghost_ext(const Dev &D, const Axis &A, long base
   const double *G = A.G[lo ? 0 : 1] + (long)layer * nl + line;  // field g at G[g * 2 * nl]
   const long me = base + (long)(lo ? 0 : n - 1) * A.stride;      // edge cell
   const long mn = base + (long)(lo ? 1 : n - 2) * A.stride;      // its inner neighbour
-  const double r1 = A.dx[lo ? 0 : n - 1], r1d = A.dx[lo ? 1 : n - 2];
-  const double r2 = A.dx[lo ? -1 : n], r2d = A.dx[lo ? 0 : n - 1];
+  const double r1 = gptr(A.dx)[lo ? 0 : n - 1], r1d = gptr(A.dx)[lo ? 1 : n - 2];
+  const double r2 = gptr(A.dx)[lo ? -1 : n], r2d = gptr(A.dx)[lo ? 0 : n - 1];
   auto lin = [&](double qe, double qn, double &q1, double &q2) {
     if (lo) {
       q1 = qe - (qn - qe) * r1 / r1d;
@@ -147,13 +158,13 @@ __device__ __forceinline__ void ghost_ext(const Dev &D, const Axis &A, long base
   if (sc == 4 || sc == 5) {
     const double gm1 = D.gamma - 1.0;
     double q1, q2;
-    const double de = D.W[WD][me];
-    lin(de, D.W[WD][mn], q1, q2);
+    const double de = gptr(D.W[WD])[me];
+    lin(de, gptr(D.W[WD])[mn], q1, q2);
     f[WD] = q2 <= 0.0 ? de : (layer ? q2 : q1);
-    const double pe = gm1 * D.W[WEINT][me];
-    lin(pe, gm1 * D.W[WEINT][mn], q1, q2);
+    const double pe = gm1 * gptr(D.W[WEINT])[me];
+    lin(pe, gm1 * gptr(D.W[WEINT])[mn], q1, q2);
     f[WEINT] = q2 <= 0.0 ? pe : (layer ? q2 : q1);
-    if (MAG) f[WPHI] = D.W[WPHI][me];
+    if (MAG) f[WPHI] = gptr(D.W[WPHI])[me];
   } else if (sc >= 9) {
     f[WD] = G[GF_D * 2 * nl];
     f[WEINT] = G[GF_P * 2 * nl];
@@ -163,10 +174,10 @@ __device__ __forceinline__ void ghost_ext(const Dev &D, const Axis &A, long base
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       double q1, q2;
-      lin(D.W[WV1 + a][me], D.W[WV1 + a][mn], q1, q2);
+      lin(gptr(D.W[WV1 + a])[me], gptr(D.W[WV1 + a])[mn], q1, q2);
       f[WV1 + a] = layer ? q2 : q1;
       if (MAG) {
-        lin(D.W[WB1 + a][me], D.W[WB1 + a][mn], q1, q2);
+        lin(gptr(D.W[WB1 + a])[me], gptr(D.W[WB1 + a])[mn], q1, q2);
         f[WB1 + a] = layer ? q2 : q1;
       }
     }
@@ -175,7 +186,7 @@ __device__ __forceinline__ void ghost_ext(const Dev &D, const Axis &A, long base
       if (MAG) f[WB2] = G[GF_B2 * 2 * nl];
     }
     if (vc == 5) {
-      const double vn = D.W[WV1 + DIR][me];
+      const double vn = gptr(D.W[WV1 + DIR])[me];
       f[WV1 + DIR] = f[WV1 + DIR] * (lo ? 0.5 - sgn(0.5, vn) : 0.5 + sgn(0.5, vn));
     }
   } else if (vc >= 9) {
@@ -193,14 +204,14 @@ __device__ __forceinline__ void raw_cell(const Dev &D, const Axis &A, long base,
                                          int c, double f[NWMAX]) {
   long ms, mv;
   cell_sources(A, base, c, ms, mv);
-  f[WD] = D.W[WD][ms];
-  f[WE] = D.W[WE][ms];
-  f[WEINT] = D.W[WEINT][ms];
+  f[WD] = gptr(D.W[WD])[ms];
+  f[WE] = gptr(D.W[WE])[ms];
+  f[WEINT] = gptr(D.W[WEINT])[ms];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) f[WV1 + a] = D.W[WV1 + a][mv];
+  for (int a = 0; a < 3; ++a) f[WV1 + a] = gptr(D.W[WV1 + a])[mv];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) f[WB1 + a] = MAG ? D.W[WB1 + a][mv] : 0.0;
-  f[WPHI] = MAG ? D.W[WPHI][ms] : 0.0;
+  for (int a = 0; a < 3; ++a) f[WB1 + a] = MAG ? gptr(D.W[WB1 + a])[mv] : 0.0;
+  f[WPHI] = MAG ? gptr(D.W[WPHI])[ms] : 0.0;
   if (A.ext && (c < 0 || c >= A.n)) ghost_ext<DIR, MAG>(D, A, base, line, c, f);
 }
 
@@ -217,15 +228,15 @@ __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base
   long ms, mv;
   cell_sources(A, base, c, ms, mv);
   double f[NWMAX];
-  f[WD] = D.W[WD][ms];
-  f[WE] = D.W[WE][ms];
-  f[WEINT] = D.W[WEINT][ms];
+  f[WD] = gptr(D.W[WD])[ms];
+  f[WE] = gptr(D.W[WE])[ms];
+  f[WEINT] = gptr(D.W[WEINT])[ms];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) f[WV1 + a] = D.W[WV1 + a][mv];
+  for (int a = 0; a < 3; ++a) f[WV1 + a] = gptr(D.W[WV1 + a])[mv];
   if (MAG) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) f[WB1 + a] = D.W[WB1 + a][mv];
-    f[WPHI] = D.W[WPHI][ms];
+    for (int a = 0; a < 3; ++a) f[WB1 + a] = gptr(D.W[WB1 + a])[mv];
+    f[WPHI] = gptr(D.W[WPHI])[ms];
   }
   return finish_cell<DIR, MAG>(D, A, c, f);
 }
@@ -375,15 +386,15 @@ __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave
     } else {
       long ms, mv;
       cell_sources(A, bc, cr, ms, mv);
-      p[WD * fs] = D.W[WD][ms];
-      p[WE * fs] = D.W[WE][ms];
-      p[WEINT * fs] = D.W[WEINT][ms];
+      p[WD * fs] = gptr(D.W[WD])[ms];
+      p[WE * fs] = gptr(D.W[WE])[ms];
+      p[WEINT * fs] = gptr(D.W[WEINT])[ms];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) p[(WV1 + a) * fs] = D.W[WV1 + a][mv];
+      for (int a = 0; a < 3; ++a) p[(WV1 + a) * fs] = gptr(D.W[WV1 + a])[mv];
       if (MAG) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) p[(WB1 + a) * fs] = D.W[WB1 + a][mv];
-        p[WPHI * fs] = D.W[WPHI][ms];
+        for (int a = 0; a < 3; ++a) p[(WB1 + a) * fs] = gptr(D.W[WB1 + a])[mv];
+        p[WPHI * fs] = gptr(D.W[WPHI])[ms];
       }
     }
   }
@@ -425,7 +436,7 @@ __device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base,
       const long m = base + (long)c * A.stride;
 #pragma unroll
       for (int v = 0; v < NVMAX; ++v)
-        if (v < nv) DU[v][m] = du[v];
+        if (v < nv) gptr(DU[v])[m] = du[v];
     }
     return;
   }
@@ -445,7 +456,7 @@ __device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base,
     const double *p = lds + r * ld + col;
 #pragma unroll
     for (int v = 0; v < NVMAX; ++v)
-      if (v < nv) DU[v][m] = p[v * fs];
+      if (v < nv) gptr(DU[v])[m] = p[v * fs];
   }
 }
 
@@ -657,9 +668,9 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
       qr[s] = dn1(qv[s]);
     }
     const int cl = cc > -1 ? cc : -1, cr = cc < n ? cc : n;  // keep indices in range
-    const double xm = A.x[cl - 1], x0 = A.x[cl], xp = A.x[cr + 1];
-    const double xim = A.xi[cl - 1], xi0 = A.xi[cl];
-    const double ida = A.idx[cl], idb = A.idx[cr + 1];
+    const double xm = gptr(A.x)[cl - 1], x0 = gptr(A.x)[cl], xp = gptr(A.x)[cr + 1];
+    const double xim = gptr(A.xi)[cl - 1], xi0 = gptr(A.xi)[cl];
+    const double ida = gptr(A.idx)[cl], idb = gptr(A.idx)[cr + 1];
     cell_slopes<MAG, LIM, true>(qv, ql, qr, xm, x0, xp, xim, xi0, ida, idb, S);
   }
 
@@ -676,8 +687,8 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   for (int s = 0; s < 8; ++s) RS[s] = (!MAG && s >= 5) ? 0.0 : dn1(S[s]);
 
   const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;  // clamp face index
-  const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
-  const double gam = D.gamma, ch = D.sc->ch;
+  const double dx1 = gptr(A.xi)[cf_] - gptr(A.x)[cf_], dx2 = gptr(A.x)[cf_ + 1] - gptr(A.xi)[cf_];
+  const double gam = D.gamma, ch = gptr(D.sc)->ch;
   double t[9], fe[4];
   face_flux<DIR, MAG, true>(L, S, R_, RS, dx1, dx2, gam, ch, t, fe);
   const bool fxb = EXT && A.ext && flux_face(A, c);
@@ -688,7 +699,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
     int ii, jj, kk;
     line_origin<DIR>(D, line, ii, jj, kk);
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
-    atomicMin(&D.sc->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
+    atomicMin(&gptr(D.sc)->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
   }
 
   // ---- flux difference of cell c (rungekutta.f90:42-55, one direction) --
@@ -697,7 +708,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   for (int s = 0; s < 9; ++s) tm[s] = up1(t[s]);
   // equation order: icnt imo1 imo2 imo3 iene img1 img2 img3 i9wv
   constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
-  const double ix = A.idxi[cc];
+  const double ix = gptr(A.idxi)[cc];
   double du[NVMAX];
   du[0] = ix * (tm[0] - t[0]);
   du[1 + a_] = ix * (tm[1] - t[1]);
@@ -727,7 +738,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
 template <bool MAG, int LIM, bool TILE, bool SPC = false, bool EXT = false>
 __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
   extern __shared__ double lds[];
-  if (D.sc->stop) return;
+  if (gptr(D.sc)->stop) return;
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave < D.woff[1]) {
@@ -811,9 +822,9 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     for (int s = 0; s < 8; ++s) S[s] = 0.0;
   } else {
     const int cl = cc > -1 ? cc : -1, cr = cc < n ? cc : n;
-    const double4 geo = A.limg[cl];
-    const double ida = A.idx[cl], idb = A.idx[cr + 1], mw = A.mmw[cl];
-    const double hl = A.x[cl] - A.xi[cl - 1], hr = A.xi[cl] - A.x[cl];
+    const double4 geo = gptr(A.limg)[cl];
+    const double ida = gptr(A.idx)[cl], idb = gptr(A.idx)[cr + 1], mw = gptr(A.mmw)[cl];
+    const double hl = gptr(A.x)[cl] - gptr(A.xi)[cl - 1], hr = gptr(A.xi)[cl] - gptr(A.x)[cl];
     double fl[8], fr[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
@@ -853,7 +864,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   }
   const double rphi = MAG ? dn1(q.phi) : 0.0;
   const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;
-  const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
+  const double dx1 = gptr(A.xi)[cf_] - gptr(A.x)[cf_], dx2 = gptr(A.x)[cf_ + 1] - gptr(A.xi)[cf_];
   constexpr int a_ = DIR, b_ = (DIR + 1) % 3, c_ = (DIR + 2) % 3;
   double L[8], R_[8];
 #pragma unroll
@@ -881,7 +892,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     int ii, jj, kk;
     line_origin<DIR>(D, line, ii, jj, kk);
     if (DIR == 0) ii = c; else if (DIR == 1) jj = c; else kk = c;
-    atomicMin(&D.sc->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
+    atomicMin(&gptr(D.sc)->err, errkey(stage, 0, DIR + 1, ii, jj, kk, D));
   }
   const double gm1 = D.gamma - 1.0;
   const double pl = gm1 * eil, pr = gm1 * eir;
@@ -899,7 +910,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
     cfr = csr;
   }
   // GLM face values (hlldflux.f90:37-38) in FP64
-  const double ch = D.sc->ch;
+  const double ch = gptr(D.sc)->ch;
   double b1 = 0.0, phi = 0.0;
   if (MAG) {
     b1 = 0.5 * ((L[5 + a_] + R_[5 + a_]) - (rphi - q.phi) * rcp64(ch));
@@ -920,7 +931,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
 #pragma unroll
   for (int s = 0; s < 9; ++s) tm[s] = (!MAG && s >= 5) ? 0.0 : up1(t[s]);
   // equation order: icnt imo1 imo2 imo3 iene img1 img2 img3 i9wv
-  const double ix = A.idxi[cc];
+  const double ix = gptr(A.idxi)[cc];
   double du[NVMAX];
   du[0] = ix * (tm[0] - t[0]);
   du[1 + a_] = ix * (tm[1] - t[1]);
@@ -942,7 +953,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
 template <bool MAG, int LIM, bool TILE, bool SPC = false, bool EXT = false>
 __global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
   extern __shared__ double lds[];
-  if (D.sc->stop) return;
+  if (gptr(D.sc)->stop) return;
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave < D.woff[1]) {
@@ -976,7 +987,7 @@ __global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
 template <int DIR, bool MAG, int LIM, bool TILE, bool MIX>
 __global__ __launch_bounds__(1024) void k_sweep_dir(Dev D, int stage) {
   extern __shared__ double lds[];
-  if (D.sc->stop) return;
+  if (gptr(D.sc)->stop) return;
   const int lane = threadIdx.x & 63;
   const unsigned b = DIR == 0 ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
   const long w = (long)b * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1004,15 +1015,15 @@ __device__ __forceinline__ void species_line(const Dev &D, long wave, int lane) 
   long ms, mv;
   cell_sources(A, base, c, ms, mv);
   const int cl = cc > -1 ? cc : -1, cr = cc < n ? cc : n;  // as the hydro slopes
-  const double xm = A.x[cl - 1], x0 = A.x[cl], xp = A.x[cr + 1];
-  const double xim = A.xi[cl - 1], xi0 = A.xi[cl];
-  const double ida = A.idx[cl], idb = A.idx[cr + 1];
+  const double xm = gptr(A.x)[cl - 1], x0 = gptr(A.x)[cl], xp = gptr(A.x)[cr + 1];
+  const double xim = gptr(A.xi)[cl - 1], xi0 = gptr(A.xi)[cl];
+  const double ida = gptr(A.idx)[cl], idb = gptr(A.idx)[cr + 1];
   const double dxp = xp - x0;
   const double cf = dxp / (xi0 - x0), cb = (x0 - xm) / (x0 - xim);
   const auto Dm = mkden<true>(LIM == LIM_MINMOD ? ida + idb : x0 - xm);
   const auto Dp = mkden<true>(dxp);
   const int cf_ = cc < n - 1 ? (cc > -1 ? cc : -1) : n - 1;  // face index, as the sweep
-  const double dx1 = A.xi[cf_] - A.x[cf_], dx2 = A.x[cf_ + 1] - A.xi[cf_];
+  const double dx1 = gptr(A.xi)[cf_] - gptr(A.x)[cf_], dx2 = gptr(A.x)[cf_ + 1] - gptr(A.xi)[cf_];
   const long N = D.ncell;
   double sl[HG_SPN_MAX], sr[HG_SPN_MAX];
   double suml = 0.0, sumr = 0.0;  // sum(spcl(1:spn)), sum(spcr(1:spn))
@@ -1035,7 +1046,7 @@ __device__ __forceinline__ void species_line(const Dev &D, long wave, int lane) 
   const double f0 = face ? D.FM[DIR][line * (n + 1) + (c + 1)] : 0.0;
   const double sg = sgn(0.5, f0);
   const double fix = 1.0 / (suml * (0.5 + sg) + sumr * (0.5 - sg));
-  const double ix = A.idxi[cc];
+  const double ix = gptr(A.idxi)[cc];
   const bool own = lane >= 2 && lane <= 61 && c >= 0 && c < n;
   const long m = base + (long)c * A.stride;
 #pragma unroll
@@ -1049,7 +1060,7 @@ __device__ __forceinline__ void species_line(const Dev &D, long wave, int lane) 
 
 template <int LIM>
 __global__ __launch_bounds__(256) void k_species(Dev D) {
-  if (D.sc->stop) return;
+  if (gptr(D.sc)->stop) return;
   const int lane = threadIdx.x & 63;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave < D.woff[1]) {
@@ -1088,7 +1099,7 @@ __device__ __forceinline__ void cfl_cell(const Dev &D, long m, double d, double 
                o3 = D.solve[2] ? 1.0 : 0.0;
   dti = dvol / (cf1 * o1 * (sa1 + sa1) + cf2 * o2 * (sa2 + sa2) + cf3 * o3 * (sa3 + sa3));
   cfm = mx3(cf1, cf2, cf3);
-  if (eint <= 0.0 || d <= 0.0) atomicMin(&D.sc->err, errkey(0, 1, 0, i, j, k, D));
+  if (eint <= 0.0 || d <= 0.0) atomicMin(&gptr(D.sc)->err, errkey(0, 1, 0, i, j, k, D));
 }
 
 __device__ __forceinline__ double wave_min(double x) {
@@ -1120,8 +1131,8 @@ __device__ __forceinline__ void cfl_block_reduce(const Dev &D, double dti, doubl
       a = mn2(a, smin[q]);
       b = mx2(b, smax[q]);
     }
-    D.part[2 * blk] = a;
-    D.part[2 * blk + 1] = b;
+    gptr(D.part)[2 * blk] = a;
+    gptr(D.part)[2 * blk + 1] = b;
   }
 }
 
@@ -1142,7 +1153,7 @@ __device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVM
   const long g = gidx(D, i, j, k);
   const long st[3] = {1, (long)(D.n1 + 2), (long)(D.n1 + 2) * (D.n2 + 2)};
   const int q3[3] = {i, j, k};
-  const double d = qc ? qc->d : D.W[WD][m];
+  const double d = qc ? qc->d : gptr(D.W[WD])[m];
   double gr[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -1159,7 +1170,7 @@ __device__ __forceinline__ void grav_source(const Dev &D, long m, double src[NVM
   src[2] = gr[1];
   src[3] = gr[2];
   if (qc) src[4] = gr[0] * qc->v[0] + gr[1] * qc->v[1] + gr[2] * qc->v[2];
-  else src[4] = gr[0] * D.W[WV1][m] + gr[1] * D.W[WV2][m] + gr[2] * D.W[WV3][m];
+  else src[4] = gr[0] * gptr(D.W[WV1])[m] + gr[1] * gptr(D.W[WV2])[m] + gr[2] * gptr(D.W[WV3])[m];
 }
 
 // LD >= 0: direction LD's flux divergence comes from duL[] (the fused last
@@ -1177,7 +1188,7 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
   // launch's neighbours still read this one); k_rk updates in place
   double *const *Uo = LD >= 0 ? D.Uo : D.U;
   double *const *Wo = LD >= 0 ? D.Wo : D.W;
-  const double ft = fact * D.sc->dt;
+  const double ft = fact * gptr(D.sc)->dt;
   double src[NVMAX];
   if (GRAV) grav_source(D, m, src, qc);
   double un[NVMAX];
@@ -1190,24 +1201,24 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
     un[v] = 0.0;
     if (v >= nv) continue;
     // inactive directions contribute idxi*(0-0) = +0 (allocations.f90:214-240)
-    const double a = LD == 0 ? duL[v] : (D.solve[0] ? D.DU[0][v][m] : 0.0);
-    const double b = LD == 1 ? duL[v] : (D.solve[1] ? D.DU[1][v][m] : 0.0);
-    const double c = LD == 2 ? duL[v] : (D.solve[2] ? D.DU[2][v][m] : 0.0);
+    const double a = LD == 0 ? duL[v] : (D.solve[0] ? gptr(D.DU[0][v])[m] : 0.0);
+    const double b = LD == 1 ? duL[v] : (D.solve[1] ? gptr(D.DU[1][v])[m] : 0.0);
+    const double c = LD == 2 ? duL[v] : (D.solve[2] ? gptr(D.DU[2][v])[m] : 0.0);
     const double acc = ((a + b) + c) + (GRAV ? src[v] : 0.0);  // + src
     double u;
     if (qc && v == 0) u = qc->d;
     else if (qc && v == 4) u = qc->e;
     else if (qc && MAG && v >= 5 && v <= 7) u = qc->b[v - 5];
-    else u = D.U[v][m];
+    else u = gptr(D.U[v])[m];
     double u0;
     if (stage == 1) {
       u0 = u;
-      D.U0[v][m] = u;
+      gptr(D.U0[v])[m] = u;
     } else {
-      u0 = D.U0[v][m];
+      u0 = gptr(D.U0[v])[m];
     }
     un[v] = faco * u0 + facn * u + ft * acc;
-    Uo[v][m] = un[v];
+    gptr(Uo[v])[m] = un[v];
     if (v == 0) dold = u;
   }
   // species (rungekutta.f90:61-85): spcorg = spc*uorg(icnt) at stage 1;
@@ -1234,22 +1245,22 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
   const double v1 = un[1] / d, v2 = un[2] / d, v3 = un[3] / d;
   const double e = un[4];
   // W[d], W[e] and W[B] are U[0], U[4] and U[5..7] (same buffers)
-  Wo[WV1][m] = v1;
-  Wo[WV2][m] = v2;
-  Wo[WV3][m] = v3;
+  gptr(Wo[WV1])[m] = v1;
+  gptr(Wo[WV2])[m] = v2;
+  gptr(Wo[WV3])[m] = v3;
   double b1 = 0, b2 = 0, b3 = 0;
   if (MAG) {
     b1 = un[5];
     b2 = un[6];
     b3 = un[7];
-    if (D.i9wv >= 0) Wo[WPHI][m] = DAMP ? un[8] * damp : un[8];
+    if (D.i9wv >= 0) gptr(Wo[WPHI])[m] = DAMP ? un[8] * damp : un[8];
   }
   const double eint = get_eint(MAG, e, d, v1, v2, v3, b1, b2, b3);
-  Wo[WEINT][m] = eint;
+  gptr(Wo[WEINT])[m] = eint;
   if (eint <= 0.0) {
     int i, j, k;
     cell_ijk(D, m, i, j, k);
-    atomicMin(&D.sc->err, errkey(stage, 1, 0, i, j, k, D));
+    atomicMin(&gptr(D.sc)->err, errkey(stage, 1, 0, i, j, k, D));
   }
   if (CFL) cfl_cell<MAG>(D, m, d, eint, v1, v2, v3, b1, b2, b3, dti, cfm);
 }
@@ -1264,7 +1275,7 @@ __device__ __forceinline__ void rk_cell(const Dev &D, long m, int stage, double 
 template <bool MAG, bool CFL, bool DAMP, bool GRAV>
 __global__ __launch_bounds__(256) void k_rk(Dev D, int stage, double faco, double fact,
                                             double facn, double damp) {
-  if (D.sc->stop) return;
+  if (gptr(D.sc)->stop) return;
   const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double dti = __builtin_inf(), cfm = 0.0;
   if (m < D.ncell)
@@ -1320,15 +1331,15 @@ __device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base,
 template <int DIR, bool MAG, int LIM, bool TILE, bool MIX = false>
 __global__ __launch_bounds__(1024) void k_sweep_rk(Dev D, int stage, RKArgs rk) {
   extern __shared__ double lds[];
-  if (D.sc->stop) {
+  if (gptr(D.sc)->stop) {
     // stages enqueued after the run stopped: the host still makes the
     // output set the current one, so hand the state over unchanged
     const long nt = (long)gridDim.x * blockDim.x;
     for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < D.ncell; m += nt) {
-      for (int v = 0; v < D.nv; ++v) D.Uo[v][m] = D.U[v][m];
+      for (int v = 0; v < D.nv; ++v) gptr(D.Uo[v])[m] = gptr(D.U[v])[m];
       for (int w = 0; w < D.nw; ++w) {  // d, e, B alias U; a shared phi stays
         const bool aliased = w == WD || w == WE || (MAG && w >= WB1 && w <= WB3);
-        if (!aliased && D.Wo[w] != D.W[w]) D.Wo[w][m] = D.W[w][m];
+        if (!aliased && D.Wo[w] != D.W[w]) gptr(D.Wo[w])[m] = gptr(D.W[w])[m];
       }
     }
     return;
@@ -1354,17 +1365,17 @@ __global__ __launch_bounds__(256) void k_pressure(Dev D) {
   if (m >= D.ncell) return;
   double b1 = 0, b2 = 0, b3 = 0;
   if (MAG) {
-    b1 = D.W[WB1][m];
-    b2 = D.W[WB2][m];
-    b3 = D.W[WB3][m];
+    b1 = gptr(D.W[WB1])[m];
+    b2 = gptr(D.W[WB2])[m];
+    b3 = gptr(D.W[WB3])[m];
   }
-  const double eint = get_eint(MAG, D.W[WE][m], D.W[WD][m], D.W[WV1][m], D.W[WV2][m],
-                               D.W[WV3][m], b1, b2, b3);
-  D.W[WEINT][m] = eint;
+  const double eint = get_eint(MAG, gptr(D.W[WE])[m], gptr(D.W[WD])[m], gptr(D.W[WV1])[m], gptr(D.W[WV2])[m],
+                               gptr(D.W[WV3])[m], b1, b2, b3);
+  gptr(D.W[WEINT])[m] = eint;
   if (eint <= 0.0) {
     int i, j, k;
     cell_ijk(D, m, i, j, k);
-    atomicMin(&D.sc->err, errkey(0, 1, 0, i, j, k, D));
+    atomicMin(&gptr(D.sc)->err, errkey(0, 1, 0, i, j, k, D));
   }
 }
 
@@ -1441,7 +1452,7 @@ __global__ __launch_bounds__(256) void k_grav_stage1(Dev D, int in, double faco,
   const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
   const double lapphi = grav_lapphi(D, i, j, k, ph[g], ph[g - 1], ph[g + 1], ph[g - sy],
                                     ph[g + sy], ph[g - sz], ph[g + sz]);
-  const double hgsrc = D.fourpig * D.W[WD][m];
+  const double hgsrc = D.fourpig * gptr(D.W[WD])[m];
   const double p = ph[g], q = D.gpsi[m];
   D.gphi[1 - in][g] = grav_phi(faco, facn, fact, dtg, cg, p, p, q);
   D.gpsi1[m] = grav_psi(faco, facn, fact, dtg, cg, q, q, lapphi, hgsrc) * damp;
@@ -1473,7 +1484,7 @@ __global__ __launch_bounds__(256) void k_grav_substep2(Dev D, int in, double fo1
   const double p = ph[g], q = ps[m];
   const double pxm = ph[g - 1], pxp = ph[g + 1], pym = ph[g - sy], pyp = ph[g + sy];
   const double pzm = ph[g - sz], pzp = ph[g + sz];
-  const double hgsrc = D.fourpig * D.W[WD][m];
+  const double hgsrc = D.fourpig * gptr(D.W[WD])[m];
   const double lap0 = grav_lapphi(D, i, j, k, p, pxm, pxp, pym, pyp, pzm, pzp);
   const double q1 = grav_psi(fo1, fn1, ft1, dtg, cg, q, q, lap0, hgsrc);
   // phi1 at the cell and its neighbours (interior: the stage-1 update there)
@@ -1505,11 +1516,11 @@ __global__ __launch_bounds__(256) void k_cfl(Dev D) {
   if (m < D.ncell) {
     double b1 = 0, b2 = 0, b3 = 0;
     if (MAG) {
-      b1 = D.W[WB1][m];
-      b2 = D.W[WB2][m];
-      b3 = D.W[WB3][m];
+      b1 = gptr(D.W[WB1])[m];
+      b2 = gptr(D.W[WB2])[m];
+      b3 = gptr(D.W[WB3])[m];
     }
-    cfl_cell<MAG>(D, m, D.W[WD][m], D.W[WEINT][m], D.W[WV1][m], D.W[WV2][m], D.W[WV3][m],
+    cfl_cell<MAG>(D, m, gptr(D.W[WD])[m], gptr(D.W[WEINT])[m], gptr(D.W[WV1])[m], gptr(D.W[WV2])[m], gptr(D.W[WV3])[m],
                   b1, b2, b3, dti, cfm);
   }
   cfl_block_reduce(D, dti, cfm, blockIdx.x);
@@ -1616,18 +1627,33 @@ __device__ __forceinline__ void step_end(const Dev &D) {
 // are (box, wavefront of the direction), every box's range padded to a
 // multiple of 16 items that do nothing past the box's last wavefront.  A
 // block takes its items in aligned chunks of one per wavefront, so a chunk
-// never mixes boxes and its y/z lines can share an LDS tile (tile_ok: the
+// never mixes boxes and its y/z lines can share an LDS tile (TILE: the
 // launch carries the LDS); y and z chunks go to the XCDs in contiguous
 // ranges (xcd_block), as the single-box y/z launches.
-template <int DIR, bool MAG, int LIM, bool MIX>
-__global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage, int tile_ok) {
+// Record of box q at RK stage `stage`: the box's own record, or in the
+// x-fused form the one that reads the set holding the state before this
+// stage's fused x launch.
+//
+// The records and bpar do not change during a sweep launch, but the compiler
+// cannot see that past the launch's own stores: read through the constant
+// address space, they become scalar loads that stay in SGPRs, as a
+// single-box launch's kernel-argument record does.
+__device__ __forceinline__ const Dev &batch_box(const Batch &B, int q, int stage) {
+  typedef const __attribute__((address_space(4))) Dev CDev;
+  typedef const __attribute__((address_space(4))) int CInt;
+  int r = q;
+  if (B.fused) r = 2 * q + ((*((CInt *)B.bpar + q) + stage - 1) & 1);
+  return *(const Dev *)((CDev *)B.boxes + r);
+}
+
+template <int DIR, bool MAG, int LIM, bool MIX, bool TILE>
+__global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x & 63;
   const int bw = blockDim.x >> 6;
   const long wid = threadIdx.x >> 6;
   const long ibeg = B.dbeg[DIR], nit = B.dend[DIR] - ibeg;
   const long nchunk = (nit + bw - 1) / bw;
-  const bool tiled = DIR > 0 && tile_ok && bw >= 4;
   const unsigned b0 = DIR == 0 ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
   for (long g = b0; g < nchunk; g += gridDim.x) {
     const long it = g * bw + wid;
@@ -1635,17 +1661,51 @@ __global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage, int til
     // the work item is the same for the whole wavefront: make that visible
     // so the box record is read with scalar loads
     const int2 x = B.sitems[ibeg + it];
-    const Dev &D = B.boxes[__builtin_amdgcn_readfirstlane(x.x)];
-    if (D.sc->stop) continue;  // uniform over the chunk (one box)
+    const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
+    if (gptr(D.sc)->stop) continue;  // uniform over the chunk (one box)
     const long w = __builtin_amdgcn_readfirstlane(x.y);
     // tiled: padding waves of a partly filled chunk join its barriers
-    if (tiled ? w - wid >= D.waves[DIR] : w >= D.waves[DIR]) continue;
-    if (tiled) {
-      if (MIX) sweep_line_mix<DIR, MAG, LIM, true, false, false>(D, w, lane, stage, lds);
-      else sweep_line<DIR, MAG, LIM, true, false, false, false>(D, w, lane, stage, lds);
-    } else {
-      if (MIX) sweep_line_mix<DIR, MAG, LIM, false, false, false>(D, w, lane, stage);
-      else sweep_line<DIR, MAG, LIM, false, false, false, false>(D, w, lane, stage);
+    if (TILE ? w - wid >= D.waves[DIR] : w >= D.waves[DIR]) continue;
+    if (MIX) sweep_line_mix<DIR, MAG, LIM, TILE, false, false>(D, w, lane, stage, lds);
+    else sweep_line<DIR, MAG, LIM, TILE, false, false, false>(D, w, lane, stage, lds);
+  }
+}
+
+// The x-fused form: x sweep and RK update of every box (rk_cell_rt through
+// fused_rk, as k_sweep_rk<0>), reading the record's state set and writing
+// the other.  The CFL partials of the last stage go one per chunk of the
+// box (its wavefronts w .. w+bw-1 give partial w/bw, as a single-box launch
+// numbers its blocks); the phidamp factor is the box's own.
+template <bool MAG, int LIM, bool MIX>
+__global__ __launch_bounds__(1024) void k_bsweep_rk(Batch B, int stage, double faco,
+                                                    double fact, double facn, int last) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63;
+  const int bw = blockDim.x >> 6;
+  const long wid = threadIdx.x >> 6;
+  const long ibeg = B.dbeg[0], nit = B.dend[0] - ibeg;
+  const long nchunk = (nit + bw - 1) / bw;
+  for (long g = blockIdx.x; g < nchunk; g += gridDim.x) {
+    // whole chunks only: every box's range is a multiple of 16 items and
+    // bw divides 16
+    const int2 x = B.sitems[ibeg + g * bw + wid];
+    const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
+    if (gptr(D.sc)->stop) continue;  // uniform over the chunk (one box)
+    const long w = __builtin_amdgcn_readfirstlane(x.y);
+    if (w - wid >= D.waves[0]) continue;  // a chunk of padding only
+    const RKArgs rk{faco, fact, facn, D.damp, last, last && MAG && D.i9wv >= 0, 0};
+    double dti = __builtin_inf(), cfm = 0.0;
+    if (w < D.waves[0]) {
+      if (MIX)
+        sweep_line_mix<0, MAG, LIM, false, false, false, true>(D, w, lane, stage, lds, &rk, &dti,
+                                                               &cfm);
+      else
+        sweep_line<0, MAG, LIM, false, false, true, false>(D, w, lane, stage, lds, &rk, &dti,
+                                                           &cfm);
+    }
+    if (last) {
+      cfl_block_reduce(D, dti, cfm, (w - wid) / bw);
+      __syncthreads();  // the reduction's shared slots are reused next chunk
     }
   }
 }
@@ -1655,8 +1715,8 @@ __global__ __launch_bounds__(256) void k_brk(Batch B, int stage, double faco, do
                                              double facn) {
   for (long it = blockIdx.x; it < B.ncitems; it += gridDim.x) {
     const int2 x = B.citems[it];
-    const Dev &D = B.boxes[__builtin_amdgcn_readfirstlane(x.x)];
-    if (D.sc->stop) continue;  // uniform over the block
+    const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
+    if (gptr(D.sc)->stop) continue;  // uniform over the block
     const long m = (long)x.y * 256 + threadIdx.x;
     double dti = __builtin_inf(), cfm = 0.0;
     if (m < D.ncell) {
@@ -1680,10 +1740,17 @@ __global__ __launch_bounds__(1024) void k_step_end_cfl(Dev D) {
   cfl_final(D);
 }
 
-// One block per box: close its step, then compute its next dt and ch.
+// One block per box: close its step, then compute its next dt and ch.  In
+// the x-fused form a box that ran this step (it was not stopped at its
+// start) has moved its state by one set per stage.
 __global__ __launch_bounds__(256) void k_bstep_end_cfl(Batch B) {
-  const Dev &D = B.boxes[blockIdx.x];
-  if (threadIdx.x == 0) step_end(D);
+  const int q = blockIdx.x;
+  const Dev &D = B.boxes[B.fused ? 2 * q : q];
+  if (threadIdx.x == 0) {
+    const bool ran = gptr(D.sc)->stop == STOP_NONE;
+    step_end(D);
+    if (B.fused && ran && B.rkodd) B.bpar[q] ^= 1;
+  }
   __syncthreads();
   cfl_final(D);
 }
@@ -1723,7 +1790,7 @@ __global__ void k_set_clock(Scalars *sc, double time, double t_out, double t_end
 __global__ __launch_bounds__(256) void k_phidamp(Dev D, double factor) {
   const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= D.ncell) return;
-  D.W[WPHI][m] = D.W[WPHI][m] * factor;
+  gptr(D.W[WPHI])[m] = gptr(D.W[WPHI])[m] * factor;
 }
 
 // ---- known-answer evaluation of the device arithmetic -----------------------
@@ -2084,14 +2151,21 @@ void launch_phidamp(const Dev &D, double factor, hipStream_t s) {
 template <int DIR, bool MAG, int LIM>
 static void bsweep_dir_mix(const Batch &B, int mixed, int stage, unsigned g, int bs,
                            hipStream_t s) {
-  // LDS for the y/z tile when it fits
+  // the y/z tile when it fits (a separate instantiation each way: one
+  // kernel holding both paths outgrows the instruction cache)
   const int bw = bs / 64;
   const size_t nf = MAG ? NWMAX : WEINT + 1;
   size_t tile = (nf * 64 * (size_t)(bw + 1) + 2 * (size_t)bw) * sizeof(double);
   if (DIR == 0 || bw < 4 || tile > (size_t)B.lds_max) tile = 0;
-  const int ok = tile > 0 ? 1 : 0;
-  if (mixed) k_bsweep_dir<DIR, MAG, LIM, true><<<g, bs, tile, s>>>(B, stage, ok);
-  else k_bsweep_dir<DIR, MAG, LIM, false><<<g, bs, tile, s>>>(B, stage, ok);
+  if constexpr (DIR > 0) {
+    if (tile) {
+      if (mixed) k_bsweep_dir<DIR, MAG, LIM, true, true><<<g, bs, tile, s>>>(B, stage);
+      else k_bsweep_dir<DIR, MAG, LIM, false, true><<<g, bs, tile, s>>>(B, stage);
+      return;
+    }
+  }
+  if (mixed) k_bsweep_dir<DIR, MAG, LIM, true, false><<<g, bs, 0, s>>>(B, stage);
+  else k_bsweep_dir<DIR, MAG, LIM, false, false><<<g, bs, 0, s>>>(B, stage);
 }
 
 template <int DIR>
@@ -2114,6 +2188,31 @@ void launch_bsweep_dir(const Batch &B, int dir, int mag, int lim, int mixed, int
   if (dir == 0) bsweep_dir_v<0>(B, mag, lim, mixed, stage, g, bs, s);
   else if (dir == 1) bsweep_dir_v<1>(B, mag, lim, mixed, stage, g, bs, s);
   else bsweep_dir_v<2>(B, mag, lim, mixed, stage, g, bs, s);
+}
+
+template <bool MAG, int LIM>
+static void bsweep_rk_mix(const Batch &B, int mixed, int stage, double faco, double fact,
+                          double facn, bool last, unsigned g, int bs, hipStream_t s) {
+  // the lane path parks each thread's primitives (park_cell)
+  const size_t park = (size_t)park_fields(MAG) * sizeof(double) * (size_t)bs;
+  const int l = last ? 1 : 0;
+  if (mixed) k_bsweep_rk<MAG, LIM, true><<<g, bs, park, s>>>(B, stage, faco, fact, facn, l);
+  else k_bsweep_rk<MAG, LIM, false><<<g, bs, park, s>>>(B, stage, faco, fact, facn, l);
+}
+
+void launch_bsweep_rk(const Batch &B, int mag, int lim, int mixed, int stage, double faco,
+                      double fact, double facn, bool last, unsigned g, int block_waves,
+                      hipStream_t s) {
+  const int bs = 64 * block_waves;
+  if (mag) {
+    if (lim == LIM_FLAT) bsweep_rk_mix<true, LIM_FLAT>(B, mixed, stage, faco, fact, facn, last, g, bs, s);
+    else if (lim == LIM_MINMOD) bsweep_rk_mix<true, LIM_MINMOD>(B, mixed, stage, faco, fact, facn, last, g, bs, s);
+    else bsweep_rk_mix<true, LIM_MC>(B, mixed, stage, faco, fact, facn, last, g, bs, s);
+  } else {
+    if (lim == LIM_FLAT) bsweep_rk_mix<false, LIM_FLAT>(B, mixed, stage, faco, fact, facn, last, g, bs, s);
+    else if (lim == LIM_MINMOD) bsweep_rk_mix<false, LIM_MINMOD>(B, mixed, stage, faco, fact, facn, last, g, bs, s);
+    else bsweep_rk_mix<false, LIM_MC>(B, mixed, stage, faco, fact, facn, last, g, bs, s);
+  }
 }
 
 void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, double facn,

@@ -152,11 +152,22 @@ struct Batch {
   int lds_max;         // LDS bytes one block may use
   const int2 *citems;
   long ncitems;
+  // the x-fused form (every member sweeps x with its RK update): boxes
+  // holds two records per box, 2q reading state set 0 and writing set 1,
+  // 2q+1 the reverse; bpar[q] is the set that holds box q's state at the
+  // start of its step (each stage moves it to the other set); rkodd: an
+  // odd number of stages per step
+  int fused, rkodd;
+  int *bpar;
 };
 
 // one direction's sweep over the batch (k_bsweep_dir)
 void launch_bsweep_dir(const Batch &B, int dir, int mag, int lim, int mixed, int stage,
                        unsigned grid, int block_waves, hipStream_t s);
+// the x-fused form's x sweep with the RK update over the batch (k_bsweep_rk)
+void launch_bsweep_rk(const Batch &B, int mag, int lim, int mixed, int stage, double faco,
+                      double fact, double facn, bool last, unsigned grid, int block_waves,
+                      hipStream_t s);
 void launch_brk(const Batch &B, int mag, int stage, double faco, double fact, double facn,
                 bool last, unsigned grid, hipStream_t s);
 void launch_bstep_end_cfl(const Batch &B, hipStream_t s);  // per box: step_end, cfl_final

@@ -119,10 +119,13 @@ def setup(req: Request):
     return prm, g, st
 
 
-def device_bytes(req: Request) -> int:
-    """HBM footprint of one MHD context (W 10 + U 9 + U0 9 + DU dim*9 fields)."""
+def device_bytes(req: Request, batch_fused: bool = True) -> int:
+    """HBM footprint of one MHD context: the 5 W fields that do not alias U,
+    U 9, U0 9, DU dim*9, and in an x-fused batch the second state set (U 9,
+    W 5)."""
     nv = 9
-    return 8 * req.cells * (10 + 2 * nv + req.dim * nv) + (1 << 20)
+    return 8 * req.cells * (5 + 2 * nv + req.dim * nv + (nv + 5 if batch_fused else 0)) + \
+        (1 << 20)
 
 
 def state_digest(eng, g) -> str:
@@ -180,11 +183,12 @@ class _Shared:
 
 
 def _serve_lane(lane: int, queue: list, sh: _Shared, t0: float, steps_per_round: int,
-                tune: bool, keep_digest: bool, digest_ids):
+                tune: bool, keep_digest: bool, digest_ids, batch_fused: bool = True):
     """One size class: its own Batch (hence its own stream and hipGraph),
     admitting its requests in arrival order as the shared budget allows."""
     qi = 0
     batch = Batch()
+    batch.set_fused(batch_fused)
     active = {}   # id(eng) -> (req, eng, g)
     tuned_class = None
     try:
@@ -192,7 +196,7 @@ def _serve_lane(lane: int, queue: list, sh: _Shared, t0: float, steps_per_round:
             now = time.perf_counter() - t0
             while qi < len(queue) and queue[qi].arrival <= now:
                 req = queue[qi]
-                need = device_bytes(req)
+                need = device_bytes(req, batch_fused)
                 with sh.lock:
                     # over the budget, wait for memory unless nothing at all
                     # is in flight (a request larger than the budget still runs)
@@ -232,7 +236,7 @@ def _serve_lane(lane: int, queue: list, sh: _Shared, t0: float, steps_per_round:
                     batch.remove(eng)
                     eng.close()
                     with sh.lock:
-                        sh.used -= device_bytes(req)
+                        sh.used -= device_bytes(req, batch_fused)
                     del active[key]
                 continue
             with sh.lock:
@@ -257,7 +261,7 @@ def _serve_lane(lane: int, queue: list, sh: _Shared, t0: float, steps_per_round:
                             req.digest = state_digest(eng, g)
                     eng.close()
                     with sh.lock:
-                        sh.used -= device_bytes(req)
+                        sh.used -= device_bytes(req, batch_fused)
                     del active[key]
     finally:
         batch.close()
@@ -265,7 +269,7 @@ def _serve_lane(lane: int, queue: list, sh: _Shared, t0: float, steps_per_round:
 
 def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int = 4,
           tune: bool = True, keep_digest: bool = False, digest_ids=None,
-          lanes=(50_000, 500_000)) -> ServeReport:
+          lanes=(50_000, 500_000), batch_fused: bool = True) -> ServeReport:
     """Continuous batching over arriving requests (arrival times are honoured
     in wall-clock time from the start of the call).  A final-state digest is
     kept for every request (keep_digest) or for the ids in digest_ids.
@@ -274,7 +278,9 @@ def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int
     Each class has its own batch -- own stream, hipGraph and host thread --
     so small requests advance at their own step rate instead of one step per
     step of the largest box in flight, while all classes share the GPU and
-    the HBM budget.  lanes=() serves everything from one batch."""
+    the HBM budget.  lanes=() serves everything from one batch.
+    batch_fused: the batches' x-fused form (hg_batch_set_fused) or the
+    separate RK update; the same bits either way."""
     bounds = sorted(lanes) + [math.inf]
     queues = [[] for _ in bounds]
     for r in sorted(requests, key=lambda r: r.arrival):
@@ -284,7 +290,8 @@ def serve(requests: list, hbm_budget: float = 64 * 2 ** 30, steps_per_round: int
 
     def worker(lane, queue):
         try:
-            _serve_lane(lane, queue, sh, t0, steps_per_round, tune, keep_digest, digest_ids)
+            _serve_lane(lane, queue, sh, t0, steps_per_round, tune, keep_digest, digest_ids,
+                        batch_fused)
         except BaseException as err:  # re-raised below
             errors.append(err)
 

@@ -288,8 +288,9 @@ int hg_time_sweep(hg_ctx *ctx, int32_t launches, double *ms_total);
    work-lists of (box, sweep wavefront) and (box, cell block) items, replayed
    from a captured hipGraph.  Each box keeps its own dt, clock and stop
    condition (the same as hg_run's), so its result is bit-identical to
-   running it alone.  While a context is in a batch, call nothing else on it;
-   after hg_batch_remove it is an ordinary context again. */
+   running it alone.  While a context is in a batch, call nothing else on it
+   but hg_download / hg_upload between steps; after hg_batch_remove (or
+   hg_batch_destroy) it is an ordinary context again. */
 typedef struct hg_batch hg_batch;
 int hg_batch_create(int32_t device, hg_batch **out);
 int hg_batch_destroy(hg_batch *batch);
@@ -309,6 +310,14 @@ int hg_batch_member_state(hg_batch *batch, hg_ctx *ctx, double *time, int64_t *t
    sizes (0 = one work item per wavefront / block) and hipGraph replay. */
 int hg_batch_config(hg_batch *batch, int32_t sweep_block_waves, int32_t sweep_grid,
                     int32_t rk_grid, int32_t use_graph);
+/* The batch's form: on = 1 (the default) sweeps x together with the RK
+   update whenever every member sweeps x (the single-box x-fused form of
+   hg_set_fused(ctx, 2); each member then holds a second copy of its state,
+   the stages alternating between the two), on = 0 the separate RK update.
+   The same bits either way.  hg_batch_form: *fused = the form the batch's
+   launches were last built for. */
+int hg_batch_set_fused(hg_batch *batch, int32_t on);
+int hg_batch_form(const hg_batch *batch, int32_t *fused);
 
 /* Launch shape of a context's sweep kernel: wavefronts per block (1, 2, 4,
    8 or 16; default 4).  Changes which wavefront computes which line, never

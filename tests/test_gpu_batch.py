@@ -143,6 +143,12 @@ GOLDEN_BATCHES = {
     "mhd-flat": [("briowushock_y", {}, "briowushock_y_flat"),
                  ("orszagtang_xy", {}, "orszagtang_xy_flat"),
                  ("orszagtang_yz", {}, "orszagtang_yz_flat")],
+    # every member sweeps x: the batch runs in the x-fused form
+    "hydro-muscl-x": [("sodshock_x", {}, "sodshock_x"),
+                      ("sodshock_x", dict(n=1024, courant=0.4), "sod1024_c04")],
+    "mhd-muscl-x": [("briowushock_x", {}, "briowushock_x"), ("orszagtang_xy", {}, "orszagtang_xy"),
+                    ("orszagtang_xz", {}, "orszagtang_xz")],
+    "mhd-flat-x": [("orszagtang_xy", {}, "orszagtang_xy_flat")],
 }
 
 
@@ -157,7 +163,7 @@ def test_batch_of_golden_problems_matches_the_reference(hiplib, variant):
     from tests.helpers import NAMES, compare_to_meta, compare_to_npz, meta
 
     print(f"case batch of goldens {variant}", flush=True)
-    flat = variant.endswith("flat")
+    flat = "flat" in variant
     b = Batch()
     members = []
     for name, kw, key in GOLDEN_BATCHES[variant]:
@@ -171,6 +177,8 @@ def test_batch_of_golden_problems_matches_the_reference(hiplib, variant):
         members.append((eng, g, prm, key))
     while b.step(64):
         pass
+    # the x-fused form exactly when every member sweeps x
+    assert b.fused() == all(g.ax[0].n > 1 for _, g, _, _ in members)
     m = meta()
     for eng, g, prm, key in members:
         t, n, stop = b.state(eng)
@@ -217,3 +225,38 @@ def test_serving_ensemble_at_full_size_range(hiplib):
         else:
             with pytest.raises(HgError):
                 S.run_alone(r)
+
+
+@pytest.mark.parametrize("shape", [(4, 0, 0), (2, 2048, 0), (16, 1024, 0)])
+def test_batch_fused_and_separate_forms_agree(hiplib, shape):
+    """The x-fused batch form (each stage's x sweep with its RK update,
+    members alternating between two state sets) ends every member bit for
+    bit where the separate form and the member alone end, including a
+    member removed while its state sits in the second set."""
+    reqs = small_requests(4, seed=17)
+    digests = {}
+    for fused in (True, False):
+        print(f"case batch form fused={fused} shape={shape}", flush=True)
+        b = Batch()
+        b.configure(*shape)
+        b.set_fused(fused)
+        engs = []
+        for r in reqs:
+            prm, g, st = S.setup(r)
+            eng = Hydro(**D.desc_fields(prm, g))
+            D.load_initial_state(eng, prm, g, st)
+            t_end, dt_out, t_out = prm.times()
+            b.add(eng, t_end=t_end, t_out=t_out, dt_out=dt_out)
+            engs.append((eng, g))
+        b.step(1)  # one step of three stages: an odd number of set changes
+        assert b.fused() == fused
+        eng0, g0 = engs[0]
+        b.remove(eng0)  # leaves with its state moved back to its own set
+        mid = S.state_digest(eng0, g0)
+        while b.step(8):
+            pass
+        digests[fused] = [mid] + [S.state_digest(e, g) for e, g in engs[1:]]
+        b.close()
+    assert digests[True] == digests[False]
+    for (eng, g), r, dg in zip(engs[1:], reqs[1:], digests[True][1:]):
+        assert S.run_alone(r)[2] == dg
