@@ -1646,6 +1646,14 @@ __device__ __forceinline__ const Dev &batch_box(const Batch &B, int q, int stage
   return *(const Dev *)((CDev *)B.boxes + r);
 }
 
+// Work item i of a list, the same for the whole wavefront: a scalar load
+// (the lists do not change during a launch either).
+__device__ __forceinline__ int2 batch_item(const int2 *list, long i) {
+  typedef const __attribute__((address_space(4))) int CInt;
+  const CInt *p = (CInt *)(list + i);
+  return make_int2(p[0], p[1]);
+}
+
 template <int DIR, bool MAG, int LIM, bool MIX, bool TILE>
 __global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage) {
   extern __shared__ double lds[];
@@ -1656,11 +1664,11 @@ __global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage) {
   const long nchunk = (nit + bw - 1) / bw;
   const unsigned b0 = DIR == 0 ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
   for (long g = b0; g < nchunk; g += gridDim.x) {
-    const long it = g * bw + wid;
+    // the wavefront's index in the block, visibly uniform, so that the work
+    // item and the box record are scalar loads
+    const long it = g * bw + __builtin_amdgcn_readfirstlane((int)wid);
     if (it >= nit) continue;  // only in a last, untiled chunk
-    // the work item is the same for the whole wavefront: make that visible
-    // so the box record is read with scalar loads
-    const int2 x = B.sitems[ibeg + it];
+    const int2 x = batch_item(B.sitems, ibeg + it);
     const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
     if (gptr(D.sc)->stop) continue;  // uniform over the chunk (one box)
     const long w = __builtin_amdgcn_readfirstlane(x.y);
@@ -1688,7 +1696,7 @@ __global__ __launch_bounds__(1024) void k_bsweep_rk(Batch B, int stage, double f
   for (long g = blockIdx.x; g < nchunk; g += gridDim.x) {
     // whole chunks only: every box's range is a multiple of 16 items and
     // bw divides 16
-    const int2 x = B.sitems[ibeg + g * bw + wid];
+    const int2 x = batch_item(B.sitems, ibeg + g * bw + __builtin_amdgcn_readfirstlane((int)wid));
     const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
     if (gptr(D.sc)->stop) continue;  // uniform over the chunk (one box)
     const long w = __builtin_amdgcn_readfirstlane(x.y);
@@ -1714,7 +1722,7 @@ template <bool MAG, bool LAST>
 __global__ __launch_bounds__(256) void k_brk(Batch B, int stage, double faco, double fact,
                                              double facn) {
   for (long it = blockIdx.x; it < B.ncitems; it += gridDim.x) {
-    const int2 x = B.citems[it];
+    const int2 x = batch_item(B.citems, it);
     const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
     if (gptr(D.sc)->stop) continue;  // uniform over the block
     const long m = (long)x.y * 256 + threadIdx.x;
