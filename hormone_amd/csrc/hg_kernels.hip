@@ -43,6 +43,12 @@ __device__ __forceinline__ T *gptr(T *p) {
   return (T *)(__attribute__((address_space(1))) T *)p;
 }
 
+// The wavefront's index within its block, visibly uniform: index arithmetic
+// on it (the packed-line divisions) then runs on the scalar unit.
+__device__ __forceinline__ int wave_in_block() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 struct Cell {
   double d, v[3], e, eint, b[3], phi;
 };
@@ -365,7 +371,8 @@ __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave
   const int bw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ld = tile_ld(bw), fs = 64 * ld;
   {  // cooperative load: thread t -> (row r, column col)
-    const int t = threadIdx.x, r = t / bw, col = t - r * bw;
+    // bw is a power of two (sweep blocks of 1, 2, 4, 8 or 16 wavefronts)
+    const int t = threadIdx.x, r = t >> __builtin_ctz(bw), col = t & (bw - 1);
     const long last = D.waves[DIR] - 1;
     const long wc = wave0 + col < last ? wave0 + col : last;
     long ln, bc;
@@ -450,7 +457,7 @@ __device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base,
       if (v < nv) p[v * fs] = du[v];
   }
   __syncthreads();
-  const int t = threadIdx.x, r = t / bw, col = t - r * bw;
+  const int t = threadIdx.x, r = t >> __builtin_ctz(bw), col = t & (bw - 1);
   long m;
   if (tile_row_cell<DIR, MAG>(D, A, lds, fs, r, col, m)) {
     const double *p = lds + r * ld + col;
@@ -642,7 +649,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   // part in the block's loads and barriers on a copy of the last wave and
   // store and report nothing
   const bool valid = !TILE || wave < D.waves[DIR];
-  const long wave0 = wave - (threadIdx.x >> 6);  // first wave of the block (tiled)
+  const long wave0 = wave - wave_in_block();  // first wave of the block (tiled)
   if (!valid) wave = D.waves[DIR] - 1;
   long line, base;
   int c;
@@ -740,7 +747,7 @@ __global__ __launch_bounds__(1024) void k_sweep(Dev D, int stage) {
   extern __shared__ double lds[];
   if (gptr(D.sc)->stop) return;
   const int lane = threadIdx.x & 63;
-  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
   if (wave < D.woff[1]) {
     if (wave < D.waves[0]) sweep_line<0, MAG, LIM, false, SPC, false, EXT>(D, wave, lane, stage);
     return;
@@ -804,7 +811,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   // part in the block's loads and barriers on a copy of the last wave and
   // store and report nothing
   const bool valid = !TILE || wave < D.waves[DIR];
-  const long wave0 = wave - (threadIdx.x >> 6);  // first wave of the block (tiled)
+  const long wave0 = wave - wave_in_block();  // first wave of the block (tiled)
   if (!valid) wave = D.waves[DIR] - 1;
   long line, base;
   int c;
@@ -955,7 +962,7 @@ __global__ __launch_bounds__(1024) void k_sweep_mix(Dev D, int stage) {
   extern __shared__ double lds[];
   if (gptr(D.sc)->stop) return;
   const int lane = threadIdx.x & 63;
-  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
   if (wave < D.woff[1]) {
     if (wave < D.waves[0]) sweep_line_mix<0, MAG, LIM, false, SPC, EXT>(D, wave, lane, stage);
     return;
@@ -990,7 +997,7 @@ __global__ __launch_bounds__(1024) void k_sweep_dir(Dev D, int stage) {
   if (gptr(D.sc)->stop) return;
   const int lane = threadIdx.x & 63;
   const unsigned b = DIR == 0 ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-  const long w = (long)b * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long w = (long)b * (blockDim.x >> 6) + wave_in_block();
   // tiled: a partly filled last block runs its padding wavefronts too
   if (TILE ? w - (threadIdx.x >> 6) >= D.waves[DIR] : w >= D.waves[DIR]) return;
   if (MIX) sweep_line_mix<DIR, MAG, LIM, TILE, false, false>(D, w, lane, stage, lds);
@@ -1062,7 +1069,7 @@ template <int LIM>
 __global__ __launch_bounds__(256) void k_species(Dev D) {
   if (gptr(D.sc)->stop) return;
   const int lane = threadIdx.x & 63;
-  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
   if (wave < D.woff[1]) {
     if (wave < D.waves[0]) species_line<0, LIM>(D, wave, lane);
   } else if (wave < D.woff[2]) {
@@ -1314,7 +1321,7 @@ __device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base,
       if (v < nv) p[v * fs] = du[v];
   }
   __syncthreads();
-  const int t = threadIdx.x, r = t / bw, col = t - r * bw;
+  const int t = threadIdx.x, r = t >> __builtin_ctz(bw), col = t & (bw - 1);
   long m;
   if (tile_row_cell<DIR, MAG>(D, A, lds, fs, r, col, m)) {
     const double *p = lds + r * ld + col;
@@ -1345,7 +1352,7 @@ __global__ __launch_bounds__(1024) void k_sweep_rk(Dev D, int stage, RKArgs rk) 
     return;
   }
   const int lane = threadIdx.x & 63;
-  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long wave = (long)blockIdx.x * (blockDim.x >> 6) + wave_in_block();
   double dti = __builtin_inf(), cfm = 0.0;
   if (TILE || wave < D.waves[DIR]) {
     if (MIX)
