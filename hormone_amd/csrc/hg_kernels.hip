@@ -1661,29 +1661,40 @@ __device__ __forceinline__ int2 batch_item(const int2 *list, long i) {
   return make_int2(p[0], p[1]);
 }
 
+// One chunk (bw work items, one per wavefront) of direction DIR's sweep.
 template <int DIR, bool MAG, int LIM, bool MIX, bool TILE>
-__global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void bsweep_chunk(const Batch &B, int stage, long g, double *lds) {
   const int lane = threadIdx.x & 63;
   const int bw = blockDim.x >> 6;
   const long wid = threadIdx.x >> 6;
   const long ibeg = B.dbeg[DIR], nit = B.dend[DIR] - ibeg;
-  const long nchunk = (nit + bw - 1) / bw;
+  // the wavefront's index in the block, visibly uniform, so that the work
+  // item and the box record are scalar loads
+  const long it = g * bw + __builtin_amdgcn_readfirstlane((int)wid);
+  if (it >= nit) return;  // only in a last, untiled chunk
+  const int2 x = batch_item(B.sitems, ibeg + it);
+  const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
+  if (gptr(D.sc)->stop) return;  // uniform over the chunk (one box)
+  const long w = __builtin_amdgcn_readfirstlane(x.y);
+  // tiled: padding waves of a partly filled chunk join its barriers
+  if (TILE ? w - wid >= D.waves[DIR] : w >= D.waves[DIR]) return;
+  if (MIX) sweep_line_mix<DIR, MAG, LIM, TILE, false, false>(D, w, lane, stage, lds);
+  else sweep_line<DIR, MAG, LIM, TILE, false, false, false>(D, w, lane, stage, lds);
+}
+
+// PERSIST: fewer blocks than chunks, each block loops over its chunks;
+// else one chunk per block (no loop state held across the sweep).
+template <int DIR, bool MAG, int LIM, bool MIX, bool TILE, bool PERSIST>
+__global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage) {
+  extern __shared__ double lds[];
   const unsigned b0 = DIR == 0 ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
-  for (long g = b0; g < nchunk; g += gridDim.x) {
-    // the wavefront's index in the block, visibly uniform, so that the work
-    // item and the box record are scalar loads
-    const long it = g * bw + __builtin_amdgcn_readfirstlane((int)wid);
-    if (it >= nit) continue;  // only in a last, untiled chunk
-    const int2 x = batch_item(B.sitems, ibeg + it);
-    const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
-    if (gptr(D.sc)->stop) continue;  // uniform over the chunk (one box)
-    const long w = __builtin_amdgcn_readfirstlane(x.y);
-    // tiled: padding waves of a partly filled chunk join its barriers
-    if (TILE ? w - wid >= D.waves[DIR] : w >= D.waves[DIR]) continue;
-    if (MIX) sweep_line_mix<DIR, MAG, LIM, TILE, false, false>(D, w, lane, stage, lds);
-    else sweep_line<DIR, MAG, LIM, TILE, false, false, false>(D, w, lane, stage, lds);
+  if (!PERSIST) {
+    bsweep_chunk<DIR, MAG, LIM, MIX, TILE>(B, stage, b0, lds);
+    return;
   }
+  const int bw = blockDim.x >> 6;
+  const long nchunk = (B.dend[DIR] - B.dbeg[DIR] + bw - 1) / bw;
+  for (long g = b0; g < nchunk; g += gridDim.x) bsweep_chunk<DIR, MAG, LIM, MIX, TILE>(B, stage, g, lds);
 }
 
 // The x-fused form: x sweep and RK update of every box (rk_cell_rt through
@@ -2172,15 +2183,26 @@ static void bsweep_dir_mix(const Batch &B, int mixed, int stage, unsigned g, int
   const size_t nf = MAG ? NWMAX : WEINT + 1;
   size_t tile = (nf * 64 * (size_t)(bw + 1) + 2 * (size_t)bw) * sizeof(double);
   if (DIR == 0 || bw < 4 || tile > (size_t)B.lds_max) tile = 0;
+  const bool persist = (long)g < (B.dend[DIR] - B.dbeg[DIR] + bw - 1) / bw;
   if constexpr (DIR > 0) {
     if (tile) {
-      if (mixed) k_bsweep_dir<DIR, MAG, LIM, true, true><<<g, bs, tile, s>>>(B, stage);
-      else k_bsweep_dir<DIR, MAG, LIM, false, true><<<g, bs, tile, s>>>(B, stage);
+      if (persist) {
+        if (mixed) k_bsweep_dir<DIR, MAG, LIM, true, true, true><<<g, bs, tile, s>>>(B, stage);
+        else k_bsweep_dir<DIR, MAG, LIM, false, true, true><<<g, bs, tile, s>>>(B, stage);
+      } else {
+        if (mixed) k_bsweep_dir<DIR, MAG, LIM, true, true, false><<<g, bs, tile, s>>>(B, stage);
+        else k_bsweep_dir<DIR, MAG, LIM, false, true, false><<<g, bs, tile, s>>>(B, stage);
+      }
       return;
     }
   }
-  if (mixed) k_bsweep_dir<DIR, MAG, LIM, true, false><<<g, bs, 0, s>>>(B, stage);
-  else k_bsweep_dir<DIR, MAG, LIM, false, false><<<g, bs, 0, s>>>(B, stage);
+  if (persist) {
+    if (mixed) k_bsweep_dir<DIR, MAG, LIM, true, false, true><<<g, bs, 0, s>>>(B, stage);
+    else k_bsweep_dir<DIR, MAG, LIM, false, false, true><<<g, bs, 0, s>>>(B, stage);
+  } else {
+    if (mixed) k_bsweep_dir<DIR, MAG, LIM, true, false, false><<<g, bs, 0, s>>>(B, stage);
+    else k_bsweep_dir<DIR, MAG, LIM, false, false, false><<<g, bs, 0, s>>>(B, stage);
+  }
 }
 
 template <int DIR>
