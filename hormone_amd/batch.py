@@ -150,7 +150,8 @@ def _time_steps(batch: Batch, cand, steps: int):
 def autotune(batch: Batch, steps_per_trial: int = 2, use_cache: bool = True):
     """Pick the launch shape of a batch by timing real steps of it, in three
     stages: sweep wavefronts per block (4, 8, 16) with one work item per
-    wavefront; then the persistent sweep grid; then the RK grid.  Every
+    wavefront; then the persistent sweep grid; then the RK grid (the
+    separate form only).  Every
     trial step is an ordinary step of the run (the shape never changes the
     arithmetic).  Returns (best (block waves, sweep grid, rk grid),
     {shape: seconds per step})."""
@@ -178,7 +179,8 @@ def autotune(batch: Batch, steps_per_trial: int = 2, use_cache: bool = True):
             if not trial((bw, sg, 0)):
                 break
         best = min(timings, key=timings.get)
-        for rg in (CUS * 4, CUS * 8):
+        # the x-fused form has no separate RK launch
+        for rg in (() if batch.fused() else (CUS * 4, CUS * 8)):
             if not trial((best[0], best[1], rg)):
                 break
     best = min(timings, key=timings.get) if timings else (4, 0, 0)

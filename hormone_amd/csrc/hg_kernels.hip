@@ -1703,37 +1703,49 @@ __global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage) {
 // box (its wavefronts w .. w+bw-1 give partial w/bw, as a single-box launch
 // numbers its blocks); the phidamp factor is the box's own.
 template <bool MAG, int LIM, bool MIX>
-__global__ __launch_bounds__(1024) void k_bsweep_rk(Batch B, int stage, double faco,
-                                                    double fact, double facn, int last) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void bsweep_rk_chunk(const Batch &B, int stage, long g, double faco,
+                                                double fact, double facn, int last,
+                                                double *lds) {
   const int lane = threadIdx.x & 63;
   const int bw = blockDim.x >> 6;
   const long wid = threadIdx.x >> 6;
-  const long ibeg = B.dbeg[0], nit = B.dend[0] - ibeg;
-  const long nchunk = (nit + bw - 1) / bw;
-  for (long g = blockIdx.x; g < nchunk; g += gridDim.x) {
-    // whole chunks only: every box's range is a multiple of 16 items and
-    // bw divides 16
-    const int2 x = batch_item(B.sitems, ibeg + g * bw + __builtin_amdgcn_readfirstlane((int)wid));
-    const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
-    if (gptr(D.sc)->stop) continue;  // uniform over the chunk (one box)
-    const long w = __builtin_amdgcn_readfirstlane(x.y);
-    if (w - wid >= D.waves[0]) continue;  // a chunk of padding only
-    const RKArgs rk{faco, fact, facn, D.damp, last, last && MAG && D.i9wv >= 0, 0};
-    double dti = __builtin_inf(), cfm = 0.0;
-    if (w < D.waves[0]) {
-      if (MIX)
-        sweep_line_mix<0, MAG, LIM, false, false, false, true>(D, w, lane, stage, lds, &rk, &dti,
-                                                               &cfm);
-      else
-        sweep_line<0, MAG, LIM, false, false, true, false>(D, w, lane, stage, lds, &rk, &dti,
-                                                           &cfm);
-    }
-    if (last) {
-      cfl_block_reduce(D, dti, cfm, (w - wid) / bw);
-      __syncthreads();  // the reduction's shared slots are reused next chunk
-    }
+  // whole chunks only: every box's range is a multiple of 16 items and bw
+  // divides 16
+  const int2 x =
+      batch_item(B.sitems, B.dbeg[0] + g * bw + __builtin_amdgcn_readfirstlane((int)wid));
+  const Dev &D = batch_box(B, __builtin_amdgcn_readfirstlane(x.x), stage);
+  if (gptr(D.sc)->stop) return;  // uniform over the chunk (one box)
+  const long w = __builtin_amdgcn_readfirstlane(x.y);
+  if (w - wid >= D.waves[0]) return;  // a chunk of padding only
+  const RKArgs rk{faco, fact, facn, D.damp, last, last && MAG && D.i9wv >= 0, 0};
+  double dti = __builtin_inf(), cfm = 0.0;
+  if (w < D.waves[0]) {
+    if (MIX)
+      sweep_line_mix<0, MAG, LIM, false, false, false, true>(D, w, lane, stage, lds, &rk, &dti,
+                                                             &cfm);
+    else
+      sweep_line<0, MAG, LIM, false, false, true, false>(D, w, lane, stage, lds, &rk, &dti,
+                                                         &cfm);
   }
+  if (last) {
+    cfl_block_reduce(D, dti, cfm, (w - wid) / bw);
+    __syncthreads();  // the reduction's shared slots are reused next chunk
+  }
+}
+
+// PERSIST as k_bsweep_dir.
+template <bool MAG, int LIM, bool MIX, bool PERSIST>
+__global__ __launch_bounds__(1024) void k_bsweep_rk(Batch B, int stage, double faco,
+                                                    double fact, double facn, int last) {
+  extern __shared__ double lds[];
+  if (!PERSIST) {
+    bsweep_rk_chunk<MAG, LIM, MIX>(B, stage, blockIdx.x, faco, fact, facn, last, lds);
+    return;
+  }
+  const int bw = blockDim.x >> 6;
+  const long nchunk = (B.dend[0] - B.dbeg[0] + bw - 1) / bw;
+  for (long g = blockIdx.x; g < nchunk; g += gridDim.x)
+    bsweep_rk_chunk<MAG, LIM, MIX>(B, stage, g, faco, fact, facn, last, lds);
 }
 
 template <bool MAG, bool LAST>
@@ -2233,8 +2245,14 @@ static void bsweep_rk_mix(const Batch &B, int mixed, int stage, double faco, dou
   // the lane path parks each thread's primitives (park_cell)
   const size_t park = (size_t)park_fields(MAG) * sizeof(double) * (size_t)bs;
   const int l = last ? 1 : 0;
-  if (mixed) k_bsweep_rk<MAG, LIM, true><<<g, bs, park, s>>>(B, stage, faco, fact, facn, l);
-  else k_bsweep_rk<MAG, LIM, false><<<g, bs, park, s>>>(B, stage, faco, fact, facn, l);
+  const bool persist = (long)g < (B.dend[0] - B.dbeg[0] + bs / 64 - 1) / (bs / 64);
+  if (persist) {
+    if (mixed) k_bsweep_rk<MAG, LIM, true, true><<<g, bs, park, s>>>(B, stage, faco, fact, facn, l);
+    else k_bsweep_rk<MAG, LIM, false, true><<<g, bs, park, s>>>(B, stage, faco, fact, facn, l);
+  } else {
+    if (mixed) k_bsweep_rk<MAG, LIM, true, false><<<g, bs, park, s>>>(B, stage, faco, fact, facn, l);
+    else k_bsweep_rk<MAG, LIM, false, false><<<g, bs, park, s>>>(B, stage, faco, fact, facn, l);
+  }
 }
 
 void launch_bsweep_rk(const Batch &B, int mag, int lim, int mixed, int stage, double faco,
