@@ -44,6 +44,10 @@ struct hg_ctx {
   // the second state buffer set of the fused form (W aliasing U as in D)
   double *alt_U[NVMAX] = {}, *alt_W[NWMAX] = {};
   bool alt_ready = false;
+  // the fused form's third state set: its conserved part is D.U0's arrays,
+  // its own W fields these (d, e, B alias U as in the other sets)
+  double *alt2_W[NWMAX] = {};
+  bool alt2_ready = false;
   bool in_batch = false;
   // the batch it is in, and whether its sets were swapped on the host since
   // the batch last built its device records (batch_member_current)
@@ -767,10 +771,30 @@ static void enqueue_stage1_sweep(hg_ctx *c) {
 // stage wrote (fuse_tail): one per 256 cells, or one per block of the fused
 // last sweep.  sweep1_done: stage 1's sweep was enqueued beforehand
 // (enqueue_stage1_sweep).
+// One state set: the conserved fields and the primitives (d, e and B of W
+// are the U arrays).
+struct StateSet {
+  double *U[NVMAX], *W[NWMAX];
+};
+
 static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp, bool sweep1_done = false) {
   const Dev &D = c->D;
   const int rk = c->desc.rktype;
   int npart = D.nblk_cfl;
+  // the fused form's three state sets (current, second, third)
+  StateSet cur{}, alt{}, third{};
+  if (c->fused) {
+    for (int v = 0; v < D.nv; ++v) {
+      cur.U[v] = c->D.U[v];
+      alt.U[v] = c->alt_U[v];
+      third.U[v] = c->D.U0[v];
+    }
+    for (int w = 0; w < D.nw; ++w) {
+      cur.W[w] = c->D.W[w];
+      alt.W[w] = c->alt_W[w];
+      third.W[w] = c->alt2_W[w];
+    }
+  }
   for (int stage = 1; stage <= rk; ++stage) {
     double faco, fact, facn;
     runge_coeff(stage, rk, &faco, &fact, &facn);
@@ -785,13 +809,24 @@ static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp, bool sweep1_do
       const long npre = D.fdir == 0 ? D.woff[2] - D.woff[1] + D.waves[2] : D.woff[D.fdir];
       const KEv *e0 = npre > 0 && !pre ? prof_ev_into(c, HG_PROF_SWEEP, k0) : nullptr;
       const KEv *e1 = prof_ev_into(c, HG_PROF_RK, k1);
+      // three state sets: stage 1 reads the current set and writes the
+      // second; each later stage writes whichever of the second and the
+      // third it does not read, and takes the step's starting values u0
+      // from the current set, which no stage of the step writes, so stage
+      // 1 stores no U0 copy
+      const StateSet &in = stage == 1 ? cur : (stage % 2 == 0 ? alt : third);
+      const StateSet &out = stage % 2 == 1 ? alt : third;
       Dev Df = D;
-      for (int v = 0; v < D.nv; ++v) Df.Uo[v] = c->alt_U[v];
-      for (int w = 0; w < D.nw; ++w) Df.Wo[w] = c->alt_W[w];
+      for (int v = 0; v < D.nv; ++v) {
+        Df.U[v] = in.U[v];
+        Df.Uo[v] = out.U[v];
+        Df.U0[v] = stage == 1 ? nullptr : cur.U[v];
+      }
+      for (int w = 0; w < D.nw; ++w) {
+        Df.W[w] = in.W[w];
+        Df.Wo[w] = out.W[w];
+      }
       const int g = launch_fused_stage(Df, stage, a, c->stream, e0, e1, pre ? 2 : 0);
-      // the new state is in the other set: it becomes the current one
-      for (int v = 0; v < D.nv; ++v) std::swap(c->D.U[v], c->alt_U[v]);
-      for (int w = 0; w < D.nw; ++w) std::swap(c->D.W[w], c->alt_W[w]);
       if (last && fuse_tail) npart = g;
       continue;
     }
@@ -802,6 +837,24 @@ static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp, bool sweep1_do
     const bool last = stage == rk;
     launch_rk(D, stage, faco, fact, facn, fuse_tail && last, fuse_tail && last && damp > 0.0,
               damp, c->stream, prof_ev(c, HG_PROF_RK));
+  }
+  if (c->fused) {
+    // the last stage's output becomes the current set; the step's starting
+    // set is free again and takes the role its slot had
+    const bool in_alt = rk % 2 == 1;
+    const StateSet &ncur = in_alt ? alt : third;
+    const StateSet &nalt = in_alt ? cur : alt;
+    const StateSet &nthird = in_alt ? third : cur;
+    for (int v = 0; v < D.nv; ++v) {
+      c->D.U[v] = ncur.U[v];
+      c->alt_U[v] = nalt.U[v];
+      c->D.U0[v] = nthird.U[v];
+    }
+    for (int w = 0; w < D.nw; ++w) {
+      c->D.W[w] = ncur.W[w];
+      c->alt_W[w] = nalt.W[w];
+      c->alt2_W[w] = nthird.W[w];
+    }
   }
   return npart;
 }
@@ -1117,6 +1170,18 @@ extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
   if (c->in_batch) return HG_ERR_UNSUPPORTED;
   int st = ensure_fused_buffers(c, L);
   if (st) return st;
+  // the third state set: U is D.U0's arrays, W its own fields
+  if (!c->alt2_ready) {
+    const size_t fb = sizeof(double) * (size_t)D.ncell;
+    for (int w = 0; w < D.nw; ++w) {
+      if (w == WD) c->alt2_W[w] = D.U0[0];
+      else if (w == WE) c->alt2_W[w] = D.U0[4];
+      else if (D.mag && w >= WB1 && w <= WB3) c->alt2_W[w] = D.U0[5 + (w - WB1)];
+      else if (w == WPHI && D.i9wv < 0) c->alt2_W[w] = D.W[w];
+      else if ((st = dev_alloc(c, fb, (void **)&c->alt2_W[w]))) return st;
+    }
+    c->alt2_ready = true;
+  }
   D.fdir = L;
   c->fused = true;
   return HG_OK;

@@ -1220,7 +1220,9 @@ __device__ __forceinline__ void rk_cell_rt(const Dev &D, long m, int stage, doub
     double u0;
     if (stage == 1) {
       u0 = u;
-      gptr(D.U0[v])[m] = u;
+      // (no U0: the fused form's three state sets keep the step's starting
+      // values in their own set)
+      if (D.U0[v]) gptr(D.U0[v])[m] = u;
     } else {
       u0 = gptr(D.U0[v])[m];
     }
