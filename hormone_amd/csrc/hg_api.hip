@@ -15,6 +15,12 @@
 
 using namespace hg;
 
+// One state set: the conserved fields and the primitives (d, e and B of W
+// are the U arrays).
+struct StateSet {
+  double *U[NVMAX], *W[NWMAX];
+};
+
 struct hg_ctx {
   hg_desc desc;
   Dev D;
@@ -49,10 +55,10 @@ struct hg_ctx {
   double *alt2_W[NWMAX] = {};
   bool alt2_ready = false;
   bool in_batch = false;
-  // the batch it is in, and whether its sets were swapped on the host since
-  // the batch last built its device records (batch_member_current)
+  // the batch it is in, and its three state sets as the batch's device
+  // records number them (set 0 is the one D held when they were built)
   struct hg_batch *batch = nullptr;
-  int bswap = 0;
+  StateSet bsets[3] = {};
   // hg_run_gravity: the gravity sub-steps run on their own stream, beside
   // stage 1's flux sweep; gdone joins them back before stage 1's update
   hipStream_t gstream = nullptr;
@@ -771,12 +777,6 @@ static void enqueue_stage1_sweep(hg_ctx *c) {
 // stage wrote (fuse_tail): one per 256 cells, or one per block of the fused
 // last sweep.  sweep1_done: stage 1's sweep was enqueued beforehand
 // (enqueue_stage1_sweep).
-// One state set: the conserved fields and the primitives (d, e and B of W
-// are the U arrays).
-struct StateSet {
-  double *U[NVMAX], *W[NWMAX];
-};
-
 static int enqueue_stages(hg_ctx *c, bool fuse_tail, double damp, bool sweep1_done = false) {
   const Dev &D = c->D;
   const int rk = c->desc.rktype;
@@ -1104,11 +1104,10 @@ extern "C" int hg_set_boundary_flux(hg_ctx *c, int32_t face, const double *flux)
   return HG_OK;
 }
 
-// The fused form's buffers: the second state set (the fused launch reads
-// one and writes the other, then the two swap; d, e and B alias U in both,
-// as in hg_create; a phi the update does not write, 1D MHD, is shared) and
-// one CFL partial per block of direction L's fused launch (one wavefront
-// per block at most).
+// The fused form's buffers: the second and third state sets (d, e and B
+// alias U in every set, as in hg_create; a phi the update does not write,
+// 1D MHD, is shared) and one CFL partial per block of direction L's fused
+// launch (one wavefront per block at most).
 static int ensure_fused_buffers(hg_ctx *c, int L) {
   Dev &D = c->D;
   if (!c->alt_ready) {
@@ -1133,14 +1132,53 @@ static int ensure_fused_buffers(hg_ctx *c, int L) {
     D.part = p;
     c->part_cap = D.waves[L];
   }
+  // the third state set: U is D.U0's arrays, W its own fields
+  if (!c->alt2_ready) {
+    const size_t fb = sizeof(double) * (size_t)D.ncell;
+    for (int w = 0; w < D.nw; ++w) {
+      if (w == WD) c->alt2_W[w] = D.U0[0];
+      else if (w == WE) c->alt2_W[w] = D.U0[4];
+      else if (D.mag && w >= WB1 && w <= WB3) c->alt2_W[w] = D.U0[5 + (w - WB1)];
+      else if (w == WPHI && D.i9wv < 0) c->alt2_W[w] = D.W[w];
+      else if (int st = dev_alloc(c, fb, (void **)&c->alt2_W[w])) return st;
+    }
+    c->alt2_ready = true;
+  }
   return HG_OK;
 }
 
-// The current state into the other buffer set's pointers (the fused form's
-// new state becomes the current one).
-static void swap_state_sets(hg_ctx *c) {
-  for (int v = 0; v < c->D.nv; ++v) std::swap(c->D.U[v], c->alt_U[v]);
-  for (int w = 0; w < c->D.nw; ++w) std::swap(c->D.W[w], c->alt_W[w]);
+// The context's three state sets (current, second, third) and back.
+static void get_sets(const hg_ctx *c, StateSet s[3]) {
+  for (int v = 0; v < c->D.nv; ++v) {
+    s[0].U[v] = c->D.U[v];
+    s[1].U[v] = c->alt_U[v];
+    s[2].U[v] = c->D.U0[v];
+  }
+  for (int w = 0; w < c->D.nw; ++w) {
+    s[0].W[w] = c->D.W[w];
+    s[1].W[w] = c->alt_W[w];
+    s[2].W[w] = c->alt2_W[w];
+  }
+}
+
+static void put_sets(hg_ctx *c, const StateSet &cur, const StateSet &alt,
+                     const StateSet &third) {
+  for (int v = 0; v < c->D.nv; ++v) {
+    c->D.U[v] = cur.U[v];
+    c->alt_U[v] = alt.U[v];
+    c->D.U0[v] = third.U[v];
+  }
+  for (int w = 0; w < c->D.nw; ++w) {
+    c->D.W[w] = cur.W[w];
+    c->alt_W[w] = alt.W[w];
+    c->alt2_W[w] = third.W[w];
+  }
+}
+
+// A batch member's own record onto the set p of its batch numbering (the
+// set that holds its state), the others after it.
+static void member_to_set(hg_ctx *c, int p) {
+  put_sets(c, c->bsets[p], c->bsets[(p + 1) % 3], c->bsets[(p + 2) % 3]);
 }
 
 extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
@@ -1170,18 +1208,6 @@ extern "C" int hg_set_fused(hg_ctx *c, int32_t on) {
   if (c->in_batch) return HG_ERR_UNSUPPORTED;
   int st = ensure_fused_buffers(c, L);
   if (st) return st;
-  // the third state set: U is D.U0's arrays, W its own fields
-  if (!c->alt2_ready) {
-    const size_t fb = sizeof(double) * (size_t)D.ncell;
-    for (int w = 0; w < D.nw; ++w) {
-      if (w == WD) c->alt2_W[w] = D.U0[0];
-      else if (w == WE) c->alt2_W[w] = D.U0[4];
-      else if (D.mag && w >= WB1 && w <= WB3) c->alt2_W[w] = D.U0[5 + (w - WB1)];
-      else if (w == WPHI && D.i9wv < 0) c->alt2_W[w] = D.W[w];
-      else if ((st = dev_alloc(c, fb, (void **)&c->alt2_W[w]))) return st;
-    }
-    c->alt2_ready = true;
-  }
   D.fdir = L;
   c->fused = true;
   return HG_OK;
@@ -1280,16 +1306,15 @@ static int batch_normalize(hg_batch *b, hg_ctx *only) {
   for (size_t r = 0; r < par.size(); ++r) {
     hg_ctx *c = b->rec_members[r];
     if (!c || (only && c != only)) continue;
-    if (par[r] ^ c->bswap) swap_state_sets(c);
-    c->bswap = 0;
+    member_to_set(c, par[r]);
     b->rec_members[r] = nullptr;  // its records are stale until the rebuild
   }
   return HG_OK;
 }
 
 // A member's own record (hg_ctx::D) onto the set that holds its state now,
-// for hg_download / hg_upload between batch steps; the device records are
-// left as they are (bswap remembers the host-side swap).
+// for hg_download / hg_upload between batch steps; the device records stay
+// as they are (they number the sets by bsets).
 static int batch_member_current(hg_ctx *c) {
   hg_batch *b = c->batch;
   if (!b || !b->fused) return HG_OK;
@@ -1299,10 +1324,7 @@ static int batch_member_current(hg_ctx *c) {
       HIPCHK(hipStreamSynchronize(b->stream));
       int par = 0;
       HIPCHK(hipMemcpy(&par, b->d_bpar + r, sizeof(int), hipMemcpyDeviceToHost));
-      if (par ^ c->bswap) {
-        swap_state_sets(c);
-        c->bswap ^= 1;
-      }
+      member_to_set(c, par);
       return HG_OK;
     }
   return HG_OK;  // added since the last build: its state is in its own set
@@ -1393,7 +1415,6 @@ extern "C" int hg_batch_add(hg_batch *b, hg_ctx *c, double time, double t_out, d
   b->sc_valid.push_back(0);
   c->in_batch = true;
   c->batch = b;
-  c->bswap = 0;
   b->dirty = true;
   return HG_OK;
 }
@@ -1467,11 +1488,11 @@ static int batch_rebuild(hg_batch *b) {
   if (st) return st;
   const size_t nb = b->members.size();
   // the x-fused form when every member sweeps x
-  bool fused = b->want_fused && nb > 0;
+  bool fused = b->want_fused && nb > 0 && b->rktype <= 3;
   for (hg_ctx *c : b->members) fused = fused && c->D.waves[0] > 0;
   for (size_t q = 0; fused && q < nb; ++q)
     if ((st = ensure_fused_buffers(b->members[q], 0))) return st;
-  std::vector<Dev> boxes(fused ? 2 * nb : nb);
+  std::vector<Dev> boxes(fused ? 9 * nb : nb);
   std::vector<int2> si, ci;
   for (size_t q = 0; q < nb; ++q) {
     hg_ctx *c = b->members[q];
@@ -1481,25 +1502,28 @@ static int batch_rebuild(hg_batch *b) {
       for (long k = 0; k < D.nblk_cfl; ++k) ci.push_back(make_int2((int)q, (int)k));
       continue;
     }
-    // record 2q reads the member's own set and writes the second, 2q+1 the
-    // reverse; the last stage's CFL partials are one per chunk of x
-    // wavefronts
-    Dev r0 = D, r1 = D;
-    for (int v = 0; v < D.nv; ++v) {
-      r0.Uo[v] = c->alt_U[v];
-      r1.U[v] = c->alt_U[v];
-      r1.Uo[v] = D.U[v];
-    }
-    for (int w = 0; w < D.nw; ++w) {
-      r0.Wo[w] = c->alt_W[w];
-      r1.W[w] = c->alt_W[w];
-      r1.Wo[w] = D.W[w];
-    }
-    r0.fdir = r1.fdir = 0;
-    r0.nblk_cfl = r1.nblk_cfl =
-        (int)((D.waves[0] + b->sweep_block_waves - 1) / b->sweep_block_waves);
-    boxes[2 * q] = r0;
-    boxes[2 * q + 1] = r1;
+    // record 9q + 3p + (s - 1): starting set p, stage s (set p+1, p+2, p+1
+    // written; u0 from set p after stage 1, which stores no U0); the last
+    // stage's CFL partials are one per chunk of x wavefronts
+    get_sets(c, c->bsets);
+    for (int p = 0; p < 3; ++p)
+      for (int st3 = 1; st3 <= 3; ++st3) {
+        const StateSet &in = c->bsets[st3 == 1 ? p : (st3 == 2 ? (p + 1) % 3 : (p + 2) % 3)];
+        const StateSet &out = c->bsets[st3 == 2 ? (p + 2) % 3 : (p + 1) % 3];
+        Dev r = D;
+        for (int v = 0; v < D.nv; ++v) {
+          r.U[v] = in.U[v];
+          r.Uo[v] = out.U[v];
+          r.U0[v] = st3 == 1 ? nullptr : c->bsets[p].U[v];
+        }
+        for (int w = 0; w < D.nw; ++w) {
+          r.W[w] = in.W[w];
+          r.Wo[w] = out.W[w];
+        }
+        r.fdir = 0;
+        r.nblk_cfl = (int)((D.waves[0] + b->sweep_block_waves - 1) / b->sweep_block_waves);
+        boxes[9 * q + 3 * p + (st3 - 1)] = r;
+      }
   }
   // direction by direction, each box's wavefronts of the direction padded
   // to a multiple of 16, so every box's range starts on a multiple of 16
@@ -1537,7 +1561,7 @@ static int batch_rebuild(hg_batch *b) {
   b->B.citems = b->d_citems;
   b->B.ncitems = (long)ci.size();
   b->B.fused = fused ? 1 : 0;
-  b->B.rkodd = b->rktype & 1;
+  b->B.rk = b->rktype;
   b->B.bpar = b->d_bpar;
   b->fused = fused;
   b->rec_members = b->members;

@@ -1640,8 +1640,7 @@ __device__ __forceinline__ void step_end(const Dev &D) {
 // launch carries the LDS); y and z chunks go to the XCDs in contiguous
 // ranges (xcd_block), as the single-box y/z launches.
 // Record of box q at RK stage `stage`: the box's own record, or in the
-// x-fused form the one that reads the set holding the state before this
-// stage's fused x launch.
+// x-fused form the one for the box's starting set and this stage.
 //
 // The records and bpar do not change during a sweep launch, but the compiler
 // cannot see that past the launch's own stores: read through the constant
@@ -1651,7 +1650,7 @@ __device__ __forceinline__ const Dev &batch_box(const Batch &B, int q, int stage
   typedef const __attribute__((address_space(4))) Dev CDev;
   typedef const __attribute__((address_space(4))) int CInt;
   int r = q;
-  if (B.fused) r = 2 * q + ((*((CInt *)B.bpar + q) + stage - 1) & 1);
+  if (B.fused) r = 9 * q + 3 * *((CInt *)B.bpar + q) + (stage - 1);
   return *(const Dev *)((CDev *)B.boxes + r);
 }
 
@@ -1782,14 +1781,15 @@ __global__ __launch_bounds__(1024) void k_step_end_cfl(Dev D) {
 
 // One block per box: close its step, then compute its next dt and ch.  In
 // the x-fused form a box that ran this step (it was not stopped at its
-// start) has moved its state by one set per stage.
+// start) now holds its state in the set its last stage wrote.
 __global__ __launch_bounds__(256) void k_bstep_end_cfl(Batch B) {
   const int q = blockIdx.x;
-  const Dev &D = B.boxes[B.fused ? 2 * q : q];
+  const Dev &D = B.boxes[B.fused ? 9 * q : q];
   if (threadIdx.x == 0) {
     const bool ran = gptr(D.sc)->stop == STOP_NONE;
     step_end(D);
-    if (B.fused && ran && B.rkodd) B.bpar[q] ^= 1;
+    // the last stage wrote set p+1 (one or three stages) or p+2 (two)
+    if (B.fused && ran) B.bpar[q] = (B.bpar[q] + (B.rk == 2 ? 2 : 1)) % 3;
   }
   __syncthreads();
   cfl_final(D);

@@ -152,12 +152,12 @@ struct Batch {
   int lds_max;         // LDS bytes one block may use
   const int2 *citems;
   long ncitems;
-  // the x-fused form (every member sweeps x with its RK update): boxes
-  // holds two records per box, 2q reading state set 0 and writing set 1,
-  // 2q+1 the reverse; bpar[q] is the set that holds box q's state at the
-  // start of its step (each stage moves it to the other set); rkodd: an
-  // odd number of stages per step
-  int fused, rkodd;
+  // the x-fused form (every member sweeps x with its RK update), with three
+  // state sets per box: bpar[q] is the set p that holds box q's state at
+  // the start of its step; stage 1 writes set p+1, stage 2 set p+2, stage 3
+  // set p+1 again (mod 3), the later stages reading u0 from set p.  boxes
+  // holds nine records per box, 9q + 3p + (stage - 1); rk: stages per step
+  int fused, rk;
   int *bpar;
 };
 

@@ -122,9 +122,9 @@ def setup(req: Request):
 def device_bytes(req: Request, batch_fused: bool = True) -> int:
     """HBM footprint of one MHD context: the 5 W fields that do not alias U,
     U 9, U0 9, DU dim*9, and in an x-fused batch the second state set (U 9,
-    W 5)."""
+    W 5) and the third set's own W fields (5; its U is U0's arrays)."""
     nv = 9
-    return 8 * req.cells * (5 + 2 * nv + req.dim * nv + (nv + 5 if batch_fused else 0)) + \
+    return 8 * req.cells * (5 + 2 * nv + req.dim * nv + (nv + 10 if batch_fused else 0)) + \
         (1 << 20)
 
 
