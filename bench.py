@@ -325,19 +325,20 @@ def stage_bytes(nv, rktype, grav):
     return tot / rktype + (8.0 if grav else 0.0)
 
 
-def kernel_bytes(fused, nv, nw, dim, mag):
+def kernel_bytes(fused, nv, nw, dim, mag, rktype=3):
     """Bytes per cell the stage's own launches move (their reads and writes,
-    intermediates included): the sweep reads the primitives once per
-    direction and writes one divergence per direction; the RK update reads
-    U, U0 and every divergence and writes U and the primitive fields that do
-    not alias U; the fused launch reads its direction's primitives instead
-    of its divergence.  Averaged over the stages (stage 1 writes U0 instead
-    of reading it: the same count)."""
+    intermediates included), averaged over the stages: the sweep reads the
+    primitives once per direction and writes one divergence per direction;
+    the RK update reads U, U0 and every divergence and writes U and the
+    primitive fields that do not alias U (stage 1 writes U0 instead of
+    reading it: the same count).  The fused launch reads its direction's
+    primitives instead of its divergence, and with its three state sets it
+    stores no U0: stage 1 neither reads nor writes one."""
     aliased = 5 if mag else 2
     wr = nw - aliased
     if fused:
         sweep = (dim - 1) * (nw + nv) * 8
-        rk = (nw + (dim - 1) * nv + 3 * nv + wr) * 8
+        rk = (nw + (dim - 1) * nv + 2 * nv + nv * (rktype - 1) / rktype + wr) * 8
     else:
         sweep = dim * (nw + nv) * 8
         rk = (dim * nv + 3 * nv + wr) * 8
@@ -406,7 +407,7 @@ def roofline(a, r):
                "k_sweep" + ("_mix" if prm.precision == "fp32mix" else ""))
     rk_k = "k_sweep_rk" if fused else "k_rk"
     b_alg = stage_bytes(nv, prm.rktype, grav) * g.ncells
-    sb, rb = kernel_bytes(fused, nv, nw, dim, prm.mag_on)
+    sb, rb = kernel_bytes(fused, nv, nw, dim, prm.mag_on, prm.rktype)
     us = us_s + us_r
     ach = b_alg / (us * 1e-6) / 1e9
     key = f"{workload_key(a, prm)}/{prm.precision}"

@@ -607,14 +607,18 @@ extern "C" int hg_profile_read(const hg_ctx *c, int32_t kind, double *ms_total,
     case HG_PROF_SWEEP:  // read the primitives once and write one divergence per direction
       bytes = cell * (double)nsw * (D.nw + D.nv);
       break;
-    case HG_PROF_RK:
+    case HG_PROF_RK: {
       // read U, U0 and the swept divergences, write U and the W fields that
       // are not U (stage 1 writes U0 and reads none: the same count); the
       // fused launch also reads its direction's primitives instead of a
-      // divergence
-      bytes = cell * (double)(c->fused ? D.nw + nsw * D.nv + 3 * D.nv + wr
-                                       : D.dim * D.nv + 3 * D.nv + wr);
+      // divergence, and with its three state sets stage 1 neither reads
+      // nor writes a U0 (averaged over the stages)
+      const int rk = c->desc.rktype;
+      bytes = cell * (c->fused ? D.nw + nsw * D.nv + 2.0 * D.nv +
+                                     (double)D.nv * (rk - 1) / rk + wr
+                               : (double)(D.dim * D.nv + 3 * D.nv + wr));
       break;
+    }
     case HG_PROF_CFL:
       bytes = cell * (double)(D.nw - (D.mag ? 2 : 1));
       break;
