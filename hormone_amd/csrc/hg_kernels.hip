@@ -1496,20 +1496,29 @@ __global__ __launch_bounds__(256) void k_grav_substep2(Dev D, int in, double fo1
   const double hgsrc = D.fourpig * gptr(D.W[WD])[m];
   const double lap0 = grav_lapphi(D, i, j, k, p, pxm, pxp, pym, pyp, pzm, pzp);
   const double q1 = grav_psi(fo1, fn1, ft1, dtg, cg, q, q, lap0, hgsrc);
-  // phi1 at the cell and its neighbours (interior: the stage-1 update there)
-  auto phi1 = [&](double pn, bool inside, long mn) {
-    return inside ? grav_phi(fo1, fn1, ft1, dtg, cg, pn, pn, ps[mn]) : pn;
+  // phi1 at the cell and its neighbours (interior: the stage-1 update there).
+  // Every neighbour's psi0 is loaded unconditionally, from the cell itself
+  // where the neighbour is a ghost (the value is then not used): a load under
+  // a branch waits for its data before the branch joins, which made the six
+  // of them six memory round trips one after another.
+  const bool ixm = i > 0, ixp = i < D.n1 - 1, iym = j > 0, iyp = j < D.n2 - 1;
+  const bool izm = k > 0, izp = k < D.n3 - 1;
+  const double qxm = ps[ixm ? m - 1 : m], qxp = ps[ixp ? m + 1 : m];
+  const double qym = ps[iym ? m - my : m], qyp = ps[iyp ? m + my : m];
+  const double qzm = ps[izm ? m - mz : m], qzp = ps[izp ? m + mz : m];
+  auto phi1 = [&](double pn, bool inside, double qn) {
+    return inside ? grav_phi(fo1, fn1, ft1, dtg, cg, pn, pn, qn) : pn;
   };
   const double p1 = grav_phi(fo1, fn1, ft1, dtg, cg, p, p, q);
-  const double xm = phi1(pxm, i > 0, m - 1), xp = phi1(pxp, i < D.n1 - 1, m + 1);
+  const double xm = phi1(pxm, ixm, qxm), xp = phi1(pxp, ixp, qxp);
   double ym = 0.0, yp = 0.0, zm = 0.0, zp = 0.0;
   if (D.n2 > 1) {
-    ym = phi1(pym, j > 0, m - my);
-    yp = phi1(pyp, j < D.n2 - 1, m + my);
+    ym = phi1(pym, iym, qym);
+    yp = phi1(pyp, iyp, qyp);
   }
   if (D.n3 > 1) {
-    zm = phi1(pzm, k > 0, m - mz);
-    zp = phi1(pzp, k < D.n3 - 1, m + mz);
+    zm = phi1(pzm, izm, qzm);
+    zp = phi1(pzp, izp, qzp);
   }
   // stage 2 over grvphiorg = (phi0, psi0)
   const double lap1 = grav_lapphi(D, i, j, k, p1, xm, xp, ym, yp, zm, zp);
