@@ -258,9 +258,26 @@ __device__ __forceinline__ Cell load_cell(const Dev &D, const Axis &A, long base
 // its own line and segment, so a block may span the end of an x row or of a
 // segment; columns past the last wave of the direction (padding of a
 // partial block) work on a copy of the last line and store nothing.  LDS
-// layout [field][cell row][line], rows padded to bw+1, then each column's
-// line base and segment (-1 for padding) for the stores.
+// layout [field][cell row][line], rows padded to bw+1, then the index table.
 __device__ __forceinline__ int tile_ld(int bw) { return bw + 1; }
+
+// The tile's index table, two arrays of 64*bw entries after the fields:
+// entry e = r*bw + col belongs to row r of column col, that is lane r of
+// the block's wavefront col, and also to thread e, which loads and stores
+// that row.  src[e] = the cell's source elements for the scalar and the
+// vector fields (a ghost's boundary source), dst[e] = (the element its flux
+// divergence goes to, or TILE_NONE, and its cell index c along the line).
+// The owning lane computes them once from its own lane_cell; the loading
+// and storing threads read them instead of repeating the packed-line
+// divisions.  Element indices fit in 32 bits (every box holds fewer than
+// 2^32 elements).
+constexpr unsigned TILE_NONE = 0xffffffffu;
+__device__ __forceinline__ uint2 *tile_src(double *lds, int nf, int bw) {
+  return reinterpret_cast<uint2 *>(lds + nf * 64 * tile_ld(bw));
+}
+__device__ __forceinline__ uint2 *tile_dst(double *lds, int nf, int bw) {
+  return tile_src(lds, nf, bw) + 64 * bw;
+}
 
 // The fused launch's lane path keeps its own cell's primitives in LDS
 // between the load and the RK update: d, e and B are U[0], U[4] and
@@ -366,33 +383,41 @@ __device__ __forceinline__ void cell_ijk(const Dev &D, long m, int &i, int &j, i
 }
 
 template <int DIR, bool MAG, bool EXT = true>
-__device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave0, int c,
-                                          double *lds) {
+__device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave0, long base,
+                                          int c, bool valid, double *lds) {
   const int bw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ld = tile_ld(bw), fs = 64 * ld;
-  {  // cooperative load: thread t -> (row r, column col)
+  const int nf = MAG ? NWMAX : WEINT + 1;
+  uint2 *const src = tile_src(lds, nf, bw), *const dst = tile_dst(lds, nf, bw);
+  {  // this lane's entry: lane r of column w
+    long ms, mv;
+    cell_sources(A, base, c, ms, mv);
+    const bool st = valid && lane >= 2 && lane <= 61 && c >= 0 && c < A.n;
+    const int e = lane * bw + w;
+    src[e] = make_uint2((unsigned)ms, (unsigned)mv);
+    dst[e] = make_uint2(st ? (unsigned)(base + (long)c * A.stride) : TILE_NONE, (unsigned)c);
+  }
+  __syncthreads();
+  {  // cooperative load: thread t -> (row r, column col), table entry t
     // bw is a power of two (sweep blocks of 1, 2, 4, 8 or 16 wavefronts)
     const int t = threadIdx.x, r = t >> __builtin_ctz(bw), col = t & (bw - 1);
-    const long last = D.waves[DIR] - 1;
-    const long wc = wave0 + col < last ? wave0 + col : last;
-    long ln, bc;
-    int cr;
-    lane_cell<DIR>(D, A, wc, r, ln, bc, cr);
-    if (r == 0) {  // the column's wave for the stores (-1: padding)
-      long *cinfo = reinterpret_cast<long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
-      cinfo[2 * col] = wave0 + col < D.waves[DIR] ? wc : -1;
-    }
+    const uint2 sv = src[t];
+    const int cr = (int)dst[t].y;
     double *p = lds + r * ld + col;
     if (EXT && A.ext && (cr < 0 || cr >= A.n)) {
+      // a ghost the boundary code computes (raw_cell needs its line)
+      const long last = D.waves[DIR] - 1;
+      const long wc = wave0 + col < last ? wave0 + col : last;
+      long ln, bc;
+      int cl;
+      lane_cell<DIR>(D, A, wc, r, ln, bc, cl);
       double f[NWMAX];
-      raw_cell<DIR, MAG>(D, A, bc, ln, cr, f);
-      const int nf = MAG ? NWMAX : WEINT + 1;
+      raw_cell<DIR, MAG>(D, A, bc, ln, cl, f);
 #pragma unroll
       for (int k = 0; k < NWMAX; ++k)
         if (k < nf) p[k * fs] = f[k];
     } else {
-      long ms, mv;
-      cell_sources(A, bc, cr, ms, mv);
+      const long ms = sv.x, mv = sv.y;
       p[WD * fs] = gptr(D.W[WD])[ms];
       p[WE * fs] = gptr(D.W[WE])[ms];
       p[WEINT * fs] = gptr(D.W[WEINT])[ms];
@@ -408,26 +433,18 @@ __device__ __forceinline__ Cell tile_cell(const Dev &D, const Axis &A, long wave
   __syncthreads();
   double f[NWMAX];
   const double *p = lds + lane * ld + w;
-  const int nf = MAG ? NWMAX : WEINT + 1;
 #pragma unroll
   for (int k = 0; k < NWMAX; ++k) f[k] = k < nf ? p[k * fs] : 0.0;
   return finish_cell<DIR, MAG>(D, A, c, f);
 }
 
-// Row r of tile column col on the way out: true with its element index m
-// when the row holds an interior cell that this column updates (rows 2..61
-// of a column that is not padding).
-template <int DIR, bool MAG>
-__device__ __forceinline__ bool tile_row_cell(const Dev &D, const Axis &A, const double *lds,
-                                              int fs, int r, int col, long &m) {
-  const long *cinfo = reinterpret_cast<const long *>(lds + (MAG ? NWMAX : WEINT + 1) * fs);
-  const long wc = cinfo[2 * col];
-  if (wc < 0 || r < 2 || r > 61) return false;  // padding column, halo row
-  long ln, bc;
-  int cr;
-  lane_cell<DIR>(D, A, wc, r, ln, bc, cr);
-  m = bc + (long)cr * A.stride;
-  return cr >= 0 && cr < A.n;
+// Thread t's row of the tile on the way out: true with its element index m
+// when the row holds an interior cell that its column updates.
+template <bool MAG>
+__device__ __forceinline__ bool tile_row_cell(double *lds, int bw, int t, long &m) {
+  const unsigned e = tile_dst(lds, MAG ? NWMAX : WEINT + 1, bw)[t].x;
+  m = e;
+  return e != TILE_NONE;
 }
 
 // Flux-divergence stores of a line: du[v] is the value of conserved
@@ -459,7 +476,7 @@ __device__ __forceinline__ void store_du(const Dev &D, const Axis &A, long base,
   __syncthreads();
   const int t = threadIdx.x, r = t >> __builtin_ctz(bw), col = t & (bw - 1);
   long m;
-  if (tile_row_cell<DIR, MAG>(D, A, lds, fs, r, col, m)) {
+  if (tile_row_cell<MAG>(lds, bw, t, m)) {
     const double *p = lds + r * ld + col;
 #pragma unroll
     for (int v = 0; v < NVMAX; ++v)
@@ -654,7 +671,7 @@ __device__ __forceinline__ void sweep_line(const Dev &D, long wave, int lane, in
   long line, base;
   int c;
   lane_cell<DIR>(D, A, wave, lane, line, base, c);
-  const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, c, lds)
+  const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, base, c, valid, lds)
                       : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
   if (FUSED && !TILE) park_cell<MAG>(lds, q);
   // d e m1 m2 m3 b1 b2 b3 and eint (slopes) / phi (face) of the cell
@@ -816,7 +833,7 @@ __device__ __forceinline__ void sweep_line_mix(const Dev &D, long wave, int lane
   long line, base;
   int c;
   lane_cell<DIR>(D, A, wave, lane, line, base, c);
-  const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, c, lds)
+  const Cell q = TILE ? tile_cell<DIR, MAG, EXT>(D, A, wave0, base, c, valid, lds)
                       : load_cell<DIR, MAG, EXT>(D, A, base, line, c);
   if (FUSED && !TILE) park_cell<MAG>(lds, q);
   const double m0 = q.d * q.v[0], m1 = q.d * q.v[1], m2 = q.d * q.v[2];
@@ -1325,7 +1342,7 @@ __device__ __forceinline__ void fused_rk(const Dev &D, const Axis &A, long base,
   __syncthreads();
   const int t = threadIdx.x, r = t >> __builtin_ctz(bw), col = t & (bw - 1);
   long m;
-  if (tile_row_cell<DIR, MAG>(D, A, lds, fs, r, col, m)) {
+  if (tile_row_cell<MAG>(lds, bw, t, m)) {
     const double *p = lds + r * ld + col;
     double d2[NVMAX];
 #pragma unroll
@@ -1704,7 +1721,13 @@ __global__ __launch_bounds__(1024) void k_bsweep_dir(Batch B, int stage) {
   }
   const int bw = blockDim.x >> 6;
   const long nchunk = (B.dend[DIR] - B.dbeg[DIR] + bw - 1) / bw;
-  for (long g = b0; g < nchunk; g += gridDim.x) bsweep_chunk<DIR, MAG, LIM, MIX, TILE>(B, stage, g, lds);
+  for (long g = b0; g < nchunk; g += gridDim.x) {
+    bsweep_chunk<DIR, MAG, LIM, MIX, TILE>(B, stage, g, lds);
+    // the next chunk rewrites the tile's index table, which other rows may
+    // still be reading for their stores (every return in bsweep_chunk is
+    // uniform over a tiled block)
+    if (TILE) __syncthreads();
+  }
 }
 
 // The x-fused form: x sweep and RK update of every box (rk_cell_rt through
@@ -1930,13 +1953,13 @@ static inline void klaunch_lds(void (*k)(KA...), unsigned g, unsigned b, size_t 
 }
 
 // Bytes of LDS the y/z tile of a sweep block needs (the fields of 64 cells
-// of bw lines, then each column's line base and segment), or 0 when the
+// of bw lines, then the index table, 16 bytes per cell), or 0 when the
 // block has fewer than 4 lines, no y/z direction is solved or the tile
 // would not fit.
 size_t sweep_tile_bytes(const Dev &D, int bw) {
   if (bw < 4 || !(D.solve[1] || D.solve[2])) return 0;
   const size_t nf = D.mag ? NWMAX : WEINT + 1;
-  const size_t bytes = (nf * 64 * (size_t)(bw + 1) + 2 * (size_t)bw) * sizeof(double);
+  const size_t bytes = (nf * 64 * (size_t)(bw + 1) + 128 * (size_t)bw) * sizeof(double);
   return bytes <= (size_t)D.lds_max ? bytes : 0;
 }
 
@@ -2204,7 +2227,7 @@ static void bsweep_dir_mix(const Batch &B, int mixed, int stage, unsigned g, int
   // kernel holding both paths outgrows the instruction cache)
   const int bw = bs / 64;
   const size_t nf = MAG ? NWMAX : WEINT + 1;
-  size_t tile = (nf * 64 * (size_t)(bw + 1) + 2 * (size_t)bw) * sizeof(double);
+  size_t tile = (nf * 64 * (size_t)(bw + 1) + 128 * (size_t)bw) * sizeof(double);
   if (DIR == 0 || bw < 4 || tile > (size_t)B.lds_max) tile = 0;
   const bool persist = (long)g < (B.dend[DIR] - B.dbeg[DIR] + bw - 1) / bw;
   if constexpr (DIR > 0) {
