@@ -1417,13 +1417,16 @@ __global__ __launch_bounds__(256) void k_pressure(Dev D) {
 
 // Cell (i, j, k) and element m of a gravity launch: a block is blockDim.y
 // rows of blockDim.x cells of one z plane, blocks numbered x fastest, then
-// y, then z.  False past the box.
+// y, then z.  False past the box.  ROW: one row per block (blockDim.y = 1),
+// so j and k are the same for the whole block and the y and z grid factors
+// of the Laplacian are scalar loads, not vector ones.
+template <bool ROW>
 __device__ __forceinline__ bool grav_cell(const Dev &D, int &i, int &j, int &k, long &m) {
   const unsigned gx = ((unsigned)D.n1 + blockDim.x - 1) / blockDim.x;
-  const unsigned gy = ((unsigned)D.n2 + blockDim.y - 1) / blockDim.y;
+  const unsigned gy = ROW ? (unsigned)D.n2 : ((unsigned)D.n2 + blockDim.y - 1) / blockDim.y;
   const unsigned t = blockIdx.x / gx;
   i = (int)((blockIdx.x - t * gx) * blockDim.x + threadIdx.x);
-  j = (int)((t % gy) * blockDim.y + threadIdx.y);
+  j = ROW ? (int)(t % gy) : (int)((t % gy) * blockDim.y + threadIdx.y);
   k = (int)(t / gy);
   m = ((long)k * D.n2 + j) * D.n1 + i;
   return i < D.n1 && j < D.n2;
@@ -1467,12 +1470,13 @@ __device__ __forceinline__ double grav_psi(double faco, double facn, double fact
 }
 
 // grktype 1: the single stage, with the damping factor (:114).
+template <bool ROW>
 __global__ __launch_bounds__(256) void k_grav_stage1(Dev D, int in, double faco, double fact,
                                                      double facn, double dtg, double cg,
                                                      double damp) {
   int i, j, k;
   long m;
-  if (!grav_cell(D, i, j, k, m)) return;
+  if (!grav_cell<ROW>(D, i, j, k, m)) return;
   const double *ph = D.gphi[in];
   const long g = gidx(D, i, j, k);
   const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
@@ -1495,13 +1499,14 @@ __global__ __launch_bounds__(256) void k_grav_stage1(Dev D, int in, double faco,
 // once: 40 bytes per cell and sub-step against 96 for two launches (the
 // neighbours' phi0 and psi0 come from the caches).  Stage 2 applies the
 // damping factor.
+template <bool ROW>
 __global__ __launch_bounds__(256) void k_grav_substep2(Dev D, int in, double fo1, double ft1,
                                                        double fn1, double fo2, double ft2,
                                                        double fn2, double dtg, double cg,
                                                        double damp) {
   int i, j, k;
   long m;
-  if (!grav_cell(D, i, j, k, m)) return;
+  if (!grav_cell<ROW>(D, i, j, k, m)) return;
   const double *ph = D.gphi[in], *ps = D.gpsi;
   const long g = gidx(D, i, j, k);
   const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
@@ -2173,31 +2178,37 @@ void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, b
 // the new phi and psi land in gphi[1-gcur] and gpsi1 (the caller swaps).
 void launch_grav_substep(const Dev &D, int gk, double dtg, double cg, double damp_factor,
                          hipStream_t s, const KEv *ev) {
-  // 256 threads per block: the widest of 256, 128 or 64 cells that divides
-  // a row, times 1, 2 or 4 rows (fewer rows in a block: fewer rows that
-  // neighbouring blocks both read); 64 x 4 when none divides, one 256-cell
-  // row per block when the box has fewer than 4 rows
-  unsigned bx = D.n2 >= 4 ? 64 : 256, by = D.n2 >= 4 ? 4 : 1;
-  if (D.n2 >= 4) {
-    for (unsigned w = 256; w >= 64; w /= 2)
-      if (D.n1 % (int)w == 0) {
-        bx = w;
-        by = 256 / w;
-        break;
-      }
+  // One row per block (ROW): the widest of 256, 128 or 64 cells that
+  // divides a row, or 256 cells when the box has fewer than 4 rows; else
+  // 64 x 4 cells.
+  unsigned bx = 256, by = 1;
+  bool row = D.n2 < 4;
+  for (unsigned w = 256; w >= 64 && !row; w /= 2)
+    if (D.n1 % (int)w == 0) {
+      bx = w;
+      row = true;
+    }
+  if (!row) {
+    bx = 64;
+    by = 4;
   }
   const unsigned g = nblk(D.n1, (int)bx) * nblk(D.n2, (int)by) * (unsigned)D.n3;
   const dim3 b(bx, by);
   double fo1, ft1, fn1;
   runge_coeff(1, gk, &fo1, &ft1, &fn1);
   if (gk == 1) {
-    klaunch(k_grav_stage1, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, dtg, cg, damp_factor);
+    if (row) klaunch(k_grav_stage1<true>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, dtg, cg, damp_factor);
+    else klaunch(k_grav_stage1<false>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, dtg, cg, damp_factor);
     return;
   }
   double fo2, ft2, fn2;
   runge_coeff(2, gk, &fo2, &ft2, &fn2);
-  klaunch(k_grav_substep2, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg, cg,
-          damp_factor);
+  if (row)
+    klaunch(k_grav_substep2<true>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg,
+            cg, damp_factor);
+  else
+    klaunch(k_grav_substep2<false>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg,
+            cg, damp_factor);
 }
 
 void launch_cfl(const Dev &D, hipStream_t s, const KEv *ev) {

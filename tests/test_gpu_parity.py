@@ -231,23 +231,26 @@ def test_gpu_gravity_laplacian_quadratic(hiplib):
     assert np.max(np.abs(lap - 12.0)) < 1e-9
 
 
-@pytest.mark.parametrize("mag_on", [False, True], ids=["hydro", "mhd"])
-def test_gpu_polytrope_gravity_matches_oracle(hiplib, mag_on):
+@pytest.mark.parametrize("mag_on,n,steps", [(False, 32, 12), (True, 32, 12), (True, 64, 4)],
+                         ids=["hydro32", "mhd32", "mhd64"])
+def test_gpu_polytrope_gravity_matches_oracle(hiplib, mag_on, n, steps):
     """3D hydro (or MHD with B=0) + hyperbolic gravity: every field and the
-    potential equal the oracle bit for bit after 12 steps at 32^3."""
+    potential equal the oracle bit for bit after 12 steps at 32^3 (gravity
+    blocks of 64 x 4 cells) and 4 steps at 64^3 (one 64-cell row per
+    block)."""
     from hormone_amd import driver as D
     from hormone_amd import problems as P
-    print(f"case polytrope32 mag_on={mag_on}", flush=True)
+    print(f"case polytrope{n} mag_on={mag_on}", flush=True)
     res = []
     for cls in (Hydro, Oracle):
-        prm, g, st = P.make_problem("polytrope3d", n=32, steps=12, mag_on=mag_on)
+        prm, g, st = P.make_problem("polytrope3d", n=n, steps=steps, mag_on=mag_on)
         eng = cls(**D.desc_fields(prm, g))
         D.load_initial_state(eng, prm, g, st)
         r = D.run(eng, prm, g)
         flds = (D.SNAP_FIELDS_MAG if mag_on else D.SNAP_FIELDS) + (D.F_GRVPHI, D.F_GRVPSI)
         res.append((r, D.fetch(eng, g, flds)))
     (r1, s1), (r2, s2) = res
-    assert r1.tn == r2.tn == 12 and r1.time == r2.time and r1.nsub == r2.nsub
+    assert r1.tn == r2.tn == steps and r1.time == r2.time and r1.nsub == r2.nsub
     for f in s2:
         assert np.array_equal(s1[f].view(np.uint64), s2[f].view(np.uint64)), f
 
