@@ -1419,13 +1419,15 @@ __global__ __launch_bounds__(256) void k_pressure(Dev D) {
 // rows of blockDim.x cells of one z plane, blocks numbered x fastest, then
 // y, then z.  False past the box.  ROW: one row per block (blockDim.y = 1),
 // so j and k are the same for the whole block and the y and z grid factors
-// of the Laplacian are scalar loads, not vector ones.
-template <bool ROW>
+// of the Laplacian are scalar loads, not vector ones.  CPT cells of the row
+// per thread (i .. i+CPT-1; the launcher has CPT*blockDim.x divide n1).
+template <bool ROW, int CPT = 1>
 __device__ __forceinline__ bool grav_cell(const Dev &D, int &i, int &j, int &k, long &m) {
-  const unsigned gx = ((unsigned)D.n1 + blockDim.x - 1) / blockDim.x;
+  const unsigned bc = blockDim.x * CPT;
+  const unsigned gx = ((unsigned)D.n1 + bc - 1) / bc;
   const unsigned gy = ROW ? (unsigned)D.n2 : ((unsigned)D.n2 + blockDim.y - 1) / blockDim.y;
   const unsigned t = blockIdx.x / gx;
-  i = (int)((blockIdx.x - t * gx) * blockDim.x + threadIdx.x);
+  i = (int)(((blockIdx.x - t * gx) * blockDim.x + threadIdx.x) * CPT);
   j = ROW ? (int)(t % gy) : (int)((t % gy) * blockDim.y + threadIdx.y);
   k = (int)(t / gy);
   m = ((long)k * D.n2 + j) * D.n1 + i;
@@ -1435,25 +1437,32 @@ __device__ __forceinline__ bool grav_cell(const Dev &D, int &i, int &j, int &k, 
 // The Laplacian of the potential at cell (i, j, k) from its value there (v)
 // and at its x, y, z neighbours (xm, xp, ...; the unsolved directions' are
 // not read).  One operation order for every caller.
+// grav_lap_f takes the grid factors themselves: x1 = dxi1(i), a1 = idx1(i),
+// b1 = idx1(i+1), and the same for y (j) and z (k).  The unsolved
+// directions' terms are computed and not taken (a select, not a branch, so
+// that no load is held back behind one).
+__device__ __forceinline__ double grav_lap_f(const Dev &D, double x1, double a1, double b1,
+                                             double x2, double a2, double b2, double x3,
+                                             double a3, double b3, double v, double xm,
+                                             double xp, double ym, double yp, double zm,
+                                             double zp) {
+  const double c0 = -((a1 + b1) * x2 * x3 + (a2 + b2) * x1 * x3 + (a3 + b3) * x1 * x2);
+  double lap = c0 * v;
+  lap = lap + x2 * x3 * a1 * xm;
+  lap = lap + x2 * x3 * b1 * xp;
+  const double ly = (lap + x1 * x3 * a2 * ym) + x1 * x3 * b2 * yp;
+  lap = D.n2 > 1 ? ly : lap;
+  const double lz = (lap + x1 * x2 * a3 * zm) + x1 * x2 * b3 * zp;
+  lap = D.n3 > 1 ? lz : lap;
+  return lap / (x1 * x2 * x3);
+}
 __device__ __forceinline__ double grav_lapphi(const Dev &D, int i, int j, int k, double v,
                                               double xm, double xp, double ym, double yp,
                                               double zm, double zp) {
   const double *X1 = D.ax[0].dxi, *X2 = D.ax[1].dxi, *X3 = D.ax[2].dxi;
   const double *I1 = D.ax[0].idx, *I2 = D.ax[1].idx, *I3 = D.ax[2].idx;
-  const double c0 = -((I1[i] + I1[i + 1]) * X2[j] * X3[k] + (I2[j] + I2[j + 1]) * X1[i] * X3[k] +
-                      (I3[k] + I3[k + 1]) * X1[i] * X2[j]);
-  double lap = c0 * v;
-  lap = lap + X2[j] * X3[k] * I1[i] * xm;
-  lap = lap + X2[j] * X3[k] * I1[i + 1] * xp;
-  if (D.n2 > 1) {
-    lap = lap + X1[i] * X3[k] * I2[j] * ym;
-    lap = lap + X1[i] * X3[k] * I2[j + 1] * yp;
-  }
-  if (D.n3 > 1) {
-    lap = lap + X1[i] * X2[j] * I3[k] * zm;
-    lap = lap + X1[i] * X2[j] * I3[k + 1] * zp;
-  }
-  return lap / (X1[i] * X2[j] * X3[k]);
+  return grav_lap_f(D, X1[i], I1[i], I1[i + 1], X2[j], I2[j], I2[j + 1], X3[k], I3[k],
+                    I3[k + 1], v, xm, xp, ym, yp, zm, zp);
 }
 
 // One stage of the wave update with its RK combination (hyperbolic_gravity_
@@ -1499,53 +1508,71 @@ __global__ __launch_bounds__(256) void k_grav_stage1(Dev D, int in, double faco,
 // once: 40 bytes per cell and sub-step against 96 for two launches (the
 // neighbours' phi0 and psi0 come from the caches).  Stage 2 applies the
 // damping factor.
-template <bool ROW>
+// CPT cells i .. i+CPT-1 of a row per thread (CPT = 2: the compiler merges
+// the two cells' overlapping and adjacent loads and their stores into
+// 16-byte ones, so a cell takes about half the vector memory instructions).
+// Element offsets are 64-bit and the neighbours' are added to them, so
+// that the adjacency is visible; the new values are stored after every cell
+// is computed (a store could alias a later load).
+template <bool ROW, int CPT>
 __global__ __launch_bounds__(256) void k_grav_substep2(Dev D, int in, double fo1, double ft1,
                                                        double fn1, double fo2, double ft2,
                                                        double fn2, double dtg, double cg,
                                                        double damp) {
   int i, j, k;
   long m;
-  if (!grav_cell<ROW>(D, i, j, k, m)) return;
+  if (!grav_cell<ROW, CPT>(D, i, j, k, m)) return;
   const double *ph = D.gphi[in], *ps = D.gpsi;
   const long g = gidx(D, i, j, k);
   const long sy = D.n1 + 2, sz = (long)(D.n1 + 2) * (D.n2 + 2);
   const long my = D.n1, mz = (long)D.n1 * D.n2;
-  // stage 1
-  const double p = ph[g], q = ps[m];
-  const double pxm = ph[g - 1], pxp = ph[g + 1], pym = ph[g - sy], pyp = ph[g + sy];
-  const double pzm = ph[g - sz], pzp = ph[g + sz];
-  const double hgsrc = D.fourpig * gptr(D.W[WD])[m];
-  const double lap0 = grav_lapphi(D, i, j, k, p, pxm, pxp, pym, pyp, pzm, pzp);
-  const double q1 = grav_psi(fo1, fn1, ft1, dtg, cg, q, q, lap0, hgsrc);
-  // phi1 at the cell and its neighbours (interior: the stage-1 update there).
   // Every neighbour's psi0 is loaded unconditionally, from the cell itself
   // where the neighbour is a ghost (the value is then not used): a load under
   // a branch waits for its data before the branch joins, which made the six
   // of them six memory round trips one after another.
-  const bool ixm = i > 0, ixp = i < D.n1 - 1, iym = j > 0, iyp = j < D.n2 - 1;
-  const bool izm = k > 0, izp = k < D.n3 - 1;
-  const double qxm = ps[ixm ? m - 1 : m], qxp = ps[ixp ? m + 1 : m];
-  const double qym = ps[iym ? m - my : m], qyp = ps[iyp ? m + my : m];
-  const double qzm = ps[izm ? m - mz : m], qzp = ps[izp ? m + mz : m];
-  auto phi1 = [&](double pn, bool inside, double qn) {
-    return inside ? grav_phi(fo1, fn1, ft1, dtg, cg, pn, pn, qn) : pn;
-  };
-  const double p1 = grav_phi(fo1, fn1, ft1, dtg, cg, p, p, q);
-  const double xm = phi1(pxm, ixm, qxm), xp = phi1(pxp, ixp, qxp);
-  double ym = 0.0, yp = 0.0, zm = 0.0, zp = 0.0;
-  if (D.n2 > 1) {
-    ym = phi1(pym, iym, qym);
-    yp = phi1(pyp, iyp, qyp);
+  const bool iym = j > 0, iyp = j < D.n2 - 1, izm = k > 0, izp = k < D.n3 - 1;
+  const long oym = iym ? -my : 0, oyp = iyp ? my : 0, ozm = izm ? -mz : 0, ozp = izp ? mz : 0;
+  const double *X1 = D.ax[0].dxi + i, *I1 = D.ax[0].idx + i;
+  const double x2 = D.ax[1].dxi[j], a2 = D.ax[1].idx[j], b2 = D.ax[1].idx[j + 1];
+  const double x3 = D.ax[2].dxi[k], a3 = D.ax[2].idx[k], b3 = D.ax[2].idx[k + 1];
+  const double *rho = gptr(D.W[WD]);
+  double pn[CPT], qn[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const long gc = g + c, mc = m + c;
+    const double x1 = X1[c], a1 = I1[c], b1 = I1[c + 1];
+    // stage 1
+    const double p = ph[gc], q = ps[mc];
+    const double pxm = ph[gc - 1], pxp = ph[gc + 1], pym = ph[gc - sy], pyp = ph[gc + sy];
+    const double pzm = ph[gc - sz], pzp = ph[gc + sz];
+    const double hgsrc = D.fourpig * rho[mc];
+    const double lap0 =
+        grav_lap_f(D, x1, a1, b1, x2, a2, b2, x3, a3, b3, p, pxm, pxp, pym, pyp, pzm, pzp);
+    const double q1 = grav_psi(fo1, fn1, ft1, dtg, cg, q, q, lap0, hgsrc);
+    // phi1 at the cell and its neighbours (interior: the stage-1 update there)
+    const bool ixm = c > 0 || i > 0, ixp = c < CPT - 1 || i + c < D.n1 - 1;
+    const double qxm = ps[ixm ? mc - 1 : mc], qxp = ps[ixp ? mc + 1 : mc];
+    const double qym = ps[mc + oym], qyp = ps[mc + oyp];
+    const double qzm = ps[mc + ozm], qzp = ps[mc + ozp];
+    auto phi1 = [&](double pv, bool inside, double qv) {
+      return inside ? grav_phi(fo1, fn1, ft1, dtg, cg, pv, pv, qv) : pv;
+    };
+    const double p1 = grav_phi(fo1, fn1, ft1, dtg, cg, p, p, q);
+    const double xm = phi1(pxm, ixm, qxm), xp = phi1(pxp, ixp, qxp);
+    // the unsolved directions' values are not used (grav_lap_f)
+    const double ym = phi1(pym, iym, qym), yp = phi1(pyp, iyp, qyp);
+    const double zm = phi1(pzm, izm, qzm), zp = phi1(pzp, izp, qzp);
+    // stage 2 over grvphiorg = (phi0, psi0)
+    const double lap1 =
+        grav_lap_f(D, x1, a1, b1, x2, a2, b2, x3, a3, b3, p1, xm, xp, ym, yp, zm, zp);
+    pn[c] = grav_phi(fo2, fn2, ft2, dtg, cg, p, p1, q1);
+    qn[c] = grav_psi(fo2, fn2, ft2, dtg, cg, q, q1, lap1, hgsrc) * damp;
   }
-  if (D.n3 > 1) {
-    zm = phi1(pzm, izm, qzm);
-    zp = phi1(pzp, izp, qzp);
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    D.gphi[1 - in][g + c] = pn[c];
+    D.gpsi1[m + c] = qn[c];
   }
-  // stage 2 over grvphiorg = (phi0, psi0)
-  const double lap1 = grav_lapphi(D, i, j, k, p1, xm, xp, ym, yp, zm, zp);
-  D.gphi[1 - in][g] = grav_phi(fo2, fn2, ft2, dtg, cg, p, p1, q1);
-  D.gpsi1[m] = grav_psi(fo2, fn2, ft2, dtg, cg, q, q1, lap1, hgsrc) * damp;
 }
 
 // ---- CFL condition as its own pass (the reference's timestep) ------------
@@ -2178,11 +2205,18 @@ void launch_rk(const Dev &D, int stage, double faco, double fact, double facn, b
 // the new phi and psi land in gphi[1-gcur] and gpsi1 (the caller swaps).
 void launch_grav_substep(const Dev &D, int gk, double dtg, double cg, double damp_factor,
                          hipStream_t s, const KEv *ev) {
-  // One row per block (ROW): the widest of 256, 128 or 64 cells that
-  // divides a row, or 256 cells when the box has fewer than 4 rows; else
-  // 64 x 4 cells.
+  // One row per block (ROW): two cells per thread when 2 x 256, 2 x 128 or
+  // 2 x 64 cells divide a row, else one when 256, 128 or 64 do, or 256 cells
+  // when the box has fewer than 4 rows; else 64 x 4 cells.
   unsigned bx = 256, by = 1;
+  int cpt = 1;
   bool row = D.n2 < 4;
+  for (unsigned w = 256; w >= 64 && !row; w /= 2)
+    if (D.n1 % (int)(2 * w) == 0) {
+      bx = w;
+      cpt = 2;
+      row = true;
+    }
   for (unsigned w = 256; w >= 64 && !row; w /= 2)
     if (D.n1 % (int)w == 0) {
       bx = w;
@@ -2192,22 +2226,27 @@ void launch_grav_substep(const Dev &D, int gk, double dtg, double cg, double dam
     bx = 64;
     by = 4;
   }
-  const unsigned g = nblk(D.n1, (int)bx) * nblk(D.n2, (int)by) * (unsigned)D.n3;
-  const dim3 b(bx, by);
   double fo1, ft1, fn1;
   runge_coeff(1, gk, &fo1, &ft1, &fn1);
-  if (gk == 1) {
+  if (gk == 1) {  // one cell per thread
+    const unsigned g = nblk(D.n1, (int)bx) * nblk(D.n2, (int)by) * (unsigned)D.n3;
+    const dim3 b(bx, by);
     if (row) klaunch(k_grav_stage1<true>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, dtg, cg, damp_factor);
     else klaunch(k_grav_stage1<false>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, dtg, cg, damp_factor);
     return;
   }
+  const unsigned g = nblk(D.n1, (int)bx * cpt) * nblk(D.n2, (int)by) * (unsigned)D.n3;
+  const dim3 b(bx, by);
   double fo2, ft2, fn2;
   runge_coeff(2, gk, &fo2, &ft2, &fn2);
-  if (row)
-    klaunch(k_grav_substep2<true>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg,
+  if (cpt == 2)
+    klaunch(k_grav_substep2<true, 2>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg,
+            cg, damp_factor);
+  else if (row)
+    klaunch(k_grav_substep2<true, 1>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg,
             cg, damp_factor);
   else
-    klaunch(k_grav_substep2<false>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg,
+    klaunch(k_grav_substep2<false, 1>, g, b, s, ev, D, D.gcur, fo1, ft1, fn1, fo2, ft2, fn2, dtg,
             cg, damp_factor);
 }
 

@@ -231,13 +231,14 @@ def test_gpu_gravity_laplacian_quadratic(hiplib):
     assert np.max(np.abs(lap - 12.0)) < 1e-9
 
 
-@pytest.mark.parametrize("mag_on,n,steps", [(False, 32, 12), (True, 32, 12), (True, 64, 4)],
-                         ids=["hydro32", "mhd32", "mhd64"])
+@pytest.mark.parametrize("mag_on,n,steps",
+                         [(False, 32, 12), (True, 32, 12), (True, 64, 4), (True, 128, 2)],
+                         ids=["hydro32", "mhd32", "mhd64", "mhd128"])
 def test_gpu_polytrope_gravity_matches_oracle(hiplib, mag_on, n, steps):
     """3D hydro (or MHD with B=0) + hyperbolic gravity: every field and the
     potential equal the oracle bit for bit after 12 steps at 32^3 (gravity
-    blocks of 64 x 4 cells) and 4 steps at 64^3 (one 64-cell row per
-    block)."""
+    blocks of 64 x 4 cells), 4 steps at 64^3 (one 64-cell row per block)
+    and 2 steps at 128^3 (one row per block, two cells per thread)."""
     from hormone_amd import driver as D
     from hormone_amd import problems as P
     print(f"case polytrope{n} mag_on={mag_on}", flush=True)
