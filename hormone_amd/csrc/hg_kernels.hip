@@ -1536,32 +1536,55 @@ __global__ __launch_bounds__(256) void k_grav_substep2(Dev D, int in, double fo1
   const double x2 = D.ax[1].dxi[j], a2 = D.ax[1].idx[j], b2 = D.ax[1].idx[j + 1];
   const double x3 = D.ax[2].dxi[k], a3 = D.ax[2].idx[k], b3 = D.ax[2].idx[k + 1];
   const double *rho = gptr(D.W[WD]);
-  double pn[CPT], qn[CPT];
+  // every load first, then one scheduling barrier: the compiler would
+  // otherwise spread them over several waits to save registers
+  double P[CPT][7], Q[CPT][7], R[CPT], XF[CPT][3];
 #pragma unroll
   for (int c = 0; c < CPT; ++c) {
     const long gc = g + c, mc = m + c;
-    const double x1 = X1[c], a1 = I1[c], b1 = I1[c + 1];
+    const bool ixm = c > 0 || i > 0, ixp = c < CPT - 1 || i + c < D.n1 - 1;
+    XF[c][0] = X1[c];
+    XF[c][1] = I1[c];
+    XF[c][2] = I1[c + 1];
+    P[c][0] = ph[gc];
+    P[c][1] = ph[gc - 1];
+    P[c][2] = ph[gc + 1];
+    P[c][3] = ph[gc - sy];
+    P[c][4] = ph[gc + sy];
+    P[c][5] = ph[gc - sz];
+    P[c][6] = ph[gc + sz];
+    Q[c][0] = ps[mc];
+    Q[c][1] = ps[ixm ? mc - 1 : mc];
+    Q[c][2] = ps[ixp ? mc + 1 : mc];
+    Q[c][3] = ps[mc + oym];
+    Q[c][4] = ps[mc + oyp];
+    Q[c][5] = ps[mc + ozm];
+    Q[c][6] = ps[mc + ozp];
+    R[c] = rho[mc];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  double pn[CPT], qn[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const double x1 = XF[c][0], a1 = XF[c][1], b1 = XF[c][2];
     // stage 1
-    const double p = ph[gc], q = ps[mc];
-    const double pxm = ph[gc - 1], pxp = ph[gc + 1], pym = ph[gc - sy], pyp = ph[gc + sy];
-    const double pzm = ph[gc - sz], pzp = ph[gc + sz];
-    const double hgsrc = D.fourpig * rho[mc];
+    const double p = P[c][0], q = Q[c][0];
+    const double pxm = P[c][1], pxp = P[c][2], pym = P[c][3], pyp = P[c][4];
+    const double pzm = P[c][5], pzp = P[c][6];
+    const double hgsrc = D.fourpig * R[c];
     const double lap0 =
         grav_lap_f(D, x1, a1, b1, x2, a2, b2, x3, a3, b3, p, pxm, pxp, pym, pyp, pzm, pzp);
     const double q1 = grav_psi(fo1, fn1, ft1, dtg, cg, q, q, lap0, hgsrc);
     // phi1 at the cell and its neighbours (interior: the stage-1 update there)
     const bool ixm = c > 0 || i > 0, ixp = c < CPT - 1 || i + c < D.n1 - 1;
-    const double qxm = ps[ixm ? mc - 1 : mc], qxp = ps[ixp ? mc + 1 : mc];
-    const double qym = ps[mc + oym], qyp = ps[mc + oyp];
-    const double qzm = ps[mc + ozm], qzp = ps[mc + ozp];
     auto phi1 = [&](double pv, bool inside, double qv) {
       return inside ? grav_phi(fo1, fn1, ft1, dtg, cg, pv, pv, qv) : pv;
     };
     const double p1 = grav_phi(fo1, fn1, ft1, dtg, cg, p, p, q);
-    const double xm = phi1(pxm, ixm, qxm), xp = phi1(pxp, ixp, qxp);
+    const double xm = phi1(pxm, ixm, Q[c][1]), xp = phi1(pxp, ixp, Q[c][2]);
     // the unsolved directions' values are not used (grav_lap_f)
-    const double ym = phi1(pym, iym, qym), yp = phi1(pyp, iyp, qyp);
-    const double zm = phi1(pzm, izm, qzm), zp = phi1(pzp, izp, qzp);
+    const double ym = phi1(pym, iym, Q[c][3]), yp = phi1(pyp, iyp, Q[c][4]);
+    const double zm = phi1(pzm, izm, Q[c][5]), zp = phi1(pzp, izp, Q[c][6]);
     // stage 2 over grvphiorg = (phi0, psi0)
     const double lap1 =
         grav_lap_f(D, x1, a1, b1, x2, a2, b2, x3, a3, b3, p1, xm, xp, ym, yp, zm, zp);
