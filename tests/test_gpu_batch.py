@@ -42,7 +42,10 @@ def test_batch_members_equal_alone(hiplib):
 def test_batch_launch_shapes_do_not_change_results(hiplib):
     reqs = small_requests(3, seed=5)
     digests = []
-    for shape in [(4, 0, 0), (1, 512, 1024), (2, 2048, 0), (16, 1024, 0), (8, 0, 2048)]:
+    # (4, 64, 0): a 64-block persistent grid, so every tiled block runs many
+    # chunks in a row (the barrier between chunks guards the tile's table)
+    for shape in [(4, 0, 0), (1, 512, 1024), (2, 2048, 0), (16, 1024, 0), (8, 0, 2048),
+                  (4, 64, 0)]:
         b = Batch()
         b.configure(*shape, use_graph=shape[0] != 1)
         engs = []
@@ -227,7 +230,7 @@ def test_serving_ensemble_at_full_size_range(hiplib):
                 S.run_alone(r)
 
 
-@pytest.mark.parametrize("shape", [(4, 0, 0), (2, 2048, 0), (16, 1024, 0)])
+@pytest.mark.parametrize("shape", [(4, 0, 0), (2, 2048, 0), (16, 1024, 0), (4, 64, 0)])
 def test_batch_fused_and_separate_forms_agree(hiplib, shape):
     """The x-fused batch form (each stage's x sweep with its RK update,
     members alternating between two state sets) ends every member bit for
