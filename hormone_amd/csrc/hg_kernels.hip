@@ -1591,11 +1591,11 @@ __global__ __launch_bounds__(256) void k_grav_substep2(Dev D, int in, double fo1
     pn[c] = grav_phi(fo2, fn2, ft2, dtg, cg, p, p1, q1);
     qn[c] = grav_psi(fo2, fn2, ft2, dtg, cg, q, q1, lap1, hgsrc) * damp;
   }
+  // each array's values together, so that adjacent stores merge
 #pragma unroll
-  for (int c = 0; c < CPT; ++c) {
-    D.gphi[1 - in][g + c] = pn[c];
-    D.gpsi1[m + c] = qn[c];
-  }
+  for (int c = 0; c < CPT; ++c) D.gpsi1[m + c] = qn[c];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) D.gphi[1 - in][g + c] = pn[c];
 }
 
 // ---- CFL condition as its own pass (the reference's timestep) ------------
